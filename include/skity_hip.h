@@ -1,0 +1,206 @@
+/*
+ * skity_hip.h -- C ABI of the MI355X (gfx950) DrawPath backend.
+ *
+ * This is the drop-in boundary for skity's software rasteriser.  A skity
+ * canvas draw (skity::Canvas::DrawPath, include/skity/render/canvas.hpp:365-424
+ * and src/render/canvas.cc:237-239, dispatched to SWCanvas::OnDrawPath in
+ * src/render/sw/sw_canvas.cc:357-411) becomes one path record in a
+ * hipr_scene; hipr_surface_draw_scene rasterises a whole ordered list of such
+ * draws onto a device-resident RGBA8 surface with hand-written HIP kernels.
+ *
+ * Conventions follow skity's own C wrapper (module/capi):
+ *   - opaque handles `typedef struct X_s* X` (module/capi/include/skity_c/
+ *     skity_base.h:34), destroy calls are safe on NULL;
+ *   - result codes equal skity_result (skity_base.h:47-56), extended with
+ *     device-side failures below -5;
+ *   - no exceptions cross the boundary, symbols are exported with
+ *     visibility("default") from a library built -fvisibility=hidden
+ *     (module/capi/CMakeLists.txt:28-37);
+ *   - the caller owns every host array until the call returns; the library
+ *     owns every device buffer it allocates.
+ *
+ * Enumerations use the numeric values of the reference's C++ enums so that a
+ * skity Path/Paint can be packed without translation tables.
+ */
+#ifndef SKITY_HIP_H_
+#define SKITY_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPR_API __attribute__((visibility("default")))
+
+/* skity_result values (module/capi/include/skity_c/skity_base.h:47-56). */
+typedef enum hipr_result {
+  HIPR_SUCCESS = 0,
+  HIPR_ERROR_INVALID_HANDLE = -1,
+  HIPR_ERROR_INVALID_ARGUMENT = -2,
+  HIPR_ERROR_INITIALIZATION_FAILED = -3,
+  HIPR_ERROR_OUT_OF_HOST_MEMORY = -4,
+  HIPR_ERROR_NOT_SUPPORTED = -5,
+  /* Extensions for the device backend. */
+  HIPR_ERROR_DEVICE = -6,               /* a HIP runtime call failed        */
+  HIPR_ERROR_OUT_OF_DEVICE_MEMORY = -7, /* device allocation failed         */
+  HIPR_ERROR_CAPACITY = -8              /* a fixed device limit was exceeded */
+} hipr_result;
+
+/* Path::Verb (include/skity/graphic/path.hpp:45-60). */
+enum { HIPR_VERB_MOVE = 0, HIPR_VERB_LINE = 1, HIPR_VERB_QUAD = 2,
+       HIPR_VERB_CONIC = 3, HIPR_VERB_CUBIC = 4, HIPR_VERB_CLOSE = 5 };
+/* Path::PathFillType (path.hpp:62-67). */
+enum { HIPR_FILL_WINDING = 0, HIPR_FILL_EVEN_ODD = 1 };
+/* Paint::Style / Cap / Join (include/skity/graphic/paint.hpp:46-105). */
+enum { HIPR_STYLE_FILL = 0, HIPR_STYLE_STROKE = 1, HIPR_STYLE_STROKE_AND_FILL = 2 };
+enum { HIPR_CAP_BUTT = 0, HIPR_CAP_ROUND = 1, HIPR_CAP_SQUARE = 2 };
+enum { HIPR_JOIN_MITER = 0, HIPR_JOIN_ROUND = 1, HIPR_JOIN_BEVEL = 2 };
+/* Shader::GradientType (include/skity/effect/shader.hpp:40-47). */
+enum { HIPR_GRADIENT_LINEAR = 2, HIPR_GRADIENT_RADIAL = 3, HIPR_GRADIENT_SWEEP = 4 };
+/* TileMode (include/skity/graphic/tile_mode.hpp:13-30). */
+enum { HIPR_TILE_CLAMP = 0, HIPR_TILE_REPEAT = 1, HIPR_TILE_MIRROR = 2, HIPR_TILE_DECAL = 3 };
+/* BlurStyle (include/skity/effect/mask_filter.hpp:15-20). Only kNormal is
+ * drawn by the device path. */
+enum { HIPR_BLUR_NORMAL = 1 };
+/* BlendMode: only kSrcOver is implemented on the device (value 3 in
+ * include/skity/graphic/blend_mode.hpp). */
+enum { HIPR_BLEND_SRC_OVER = 3 };
+
+#define HIPR_MAX_GRADIENT_STOPS 16
+
+/* One Paint, as SWCanvas reads it (src/render/sw/sw_canvas.cc:357-411,727-795).
+ * `color` is the Color4f the reference Paint stores (Paint::SetColor keeps
+ * Color4fFromColor(c), src/graphic/paint.cc:80-84); fill and stroke colour are
+ * the same value. */
+typedef struct hipr_paint {
+  float color[4];       /* r, g, b, a in [0,1] (unpremultiplied)            */
+  int32_t style;        /* HIPR_STYLE_*                                     */
+  float stroke_width;   /* Paint::GetStrokeWidth                            */
+  float miter_limit;    /* Paint::GetStrokeMiter (default 4)                */
+  int32_t cap;          /* HIPR_CAP_*                                       */
+  int32_t join;         /* HIPR_JOIN_*                                      */
+  int32_t gradient;     /* index into hipr_scene.gradients, or -1           */
+  float blur_radius;    /* MaskFilter::GetBlurRadius, 0 = no mask filter    */
+  int32_t blur_style;   /* HIPR_BLUR_*                                      */
+  int32_t blend;        /* HIPR_BLEND_*                                     */
+} hipr_paint;
+
+/* A gradient shader as Shader::AsGradient reports it
+ * (include/skity/effect/shader.hpp:49-65). For sweep gradients radius[0] is
+ * the bias and radius[1] the scale computed by Shader::MakeSweep
+ * (src/effect/shader.cc:88-105). local_matrix is the affine shader local
+ * matrix {scale_x, skew_x, trans_x, skew_y, scale_y, trans_y}. */
+typedef struct hipr_gradient {
+  int32_t type;        /* HIPR_GRADIENT_*                                    */
+  int32_t tile_mode;   /* HIPR_TILE_*                                        */
+  int32_t color_count; /* 2 .. HIPR_MAX_GRADIENT_STOPS                       */
+  int32_t has_offsets; /* 0: evenly spaced colours, 1: `offsets` is used     */
+  float points[4];     /* point[0].x, point[0].y, point[1].x, point[1].y     */
+  float radius[2];
+  float local_matrix[6];
+  float colors[HIPR_MAX_GRADIENT_STOPS][4]; /* r, g, b, a                     */
+  float offsets[HIPR_MAX_GRADIENT_STOPS];
+} hipr_gradient;
+
+/* An ordered list of DrawPath calls in structure-of-arrays form.  Path p owns
+ * verbs [verb_offsets[p], verb_offsets[p+1]), points (x,y pairs)
+ * [point_offsets[p], point_offsets[p+1]) and conic weights
+ * [weight_offsets[p], weight_offsets[p+1]).  The verb/point streams must be
+ * what skity's Path builder produces (every contour starts with a move).
+ * The canvas matrix is the identity. */
+typedef struct hipr_scene {
+  int32_t n_paths;
+  const uint8_t* verbs;
+  const int32_t* verb_offsets;   /* n_paths + 1 */
+  const float* points;           /* 2 floats per point */
+  const int32_t* point_offsets;  /* n_paths + 1, in points */
+  const float* weights;
+  const int32_t* weight_offsets; /* n_paths + 1 */
+  const uint8_t* fill_types;     /* n_paths, HIPR_FILL_* */
+  const hipr_paint* paints;      /* n_paths */
+  const hipr_gradient* gradients;
+  int32_t n_gradients;
+} hipr_scene;
+
+typedef struct hipr_context_s* hipr_context;
+typedef struct hipr_surface_s* hipr_surface;
+
+/* Per-draw statistics of the last hipr_surface_draw_scene call. */
+typedef struct hipr_stats {
+  int64_t paths;          /* draws processed                                */
+  int64_t raster_paths;   /* fill/stroke rasterisations (strokes count once)*/
+  int64_t edges;          /* line + quad edges built (after culling)        */
+  int64_t tiles;          /* 16x16 coverage tiles written                   */
+  int64_t tile_planes;    /* coverage planes written (>= tiles)             */
+  int64_t fallbacks;      /* paths re-run by the exact two-pass method      */
+  int64_t max_active;     /* largest active edge list seen                  */
+  double ms_total;        /* device time of the whole draw                  */
+} hipr_stats;
+
+/* Execution options.  mode 0 = plain exact two-pass (count, scan, write);
+ * mode 1 = pre-sized single pass from the caller-supplied per-path estimates
+ * (estimates != NULL), falling back to the exact two-pass method for every
+ * path whose estimate was too small. */
+typedef struct hipr_options {
+  int32_t mode;
+  const int32_t* edge_estimates;  /* n_paths or NULL */
+  const int32_t* tile_estimates;  /* n_paths or NULL */
+} hipr_options;
+
+/* --- context / surface ---------------------------------------------------
+ * Replaces Canvas::MakeSoftwareCanvas(Bitmap*) (src/render/sw/sw_canvas.cc:
+ * 146-156) and the GPUSurface/LockCanvas pattern
+ * (include/skity/gpu/gpu_surface.hpp:95-113). */
+HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out);
+HIPR_API void hipr_context_destroy(hipr_context ctx);
+/* A premultiplied RGBA8 target (bytes R,G,B,A), cleared to transparent.
+ * width or height 0 -> HIPR_ERROR_INVALID_ARGUMENT, mirroring
+ * MakeSoftwareCanvas returning null for an empty bitmap. */
+HIPR_API hipr_result hipr_surface_create(hipr_context ctx, int32_t width,
+                                         int32_t height, hipr_surface* out);
+HIPR_API void hipr_surface_destroy(hipr_surface surface);
+HIPR_API hipr_result hipr_surface_clear(hipr_surface surface, uint32_t rgba);
+/* Copy the surface (width*height*4 bytes) to host memory.
+ * Replaces GPUSurface::ReadPixels (gpu_surface.hpp:110). */
+HIPR_API hipr_result hipr_surface_read_pixels(hipr_surface surface,
+                                              uint8_t* dst, size_t dst_bytes);
+HIPR_API hipr_result hipr_surface_write_pixels(hipr_surface surface,
+                                               const uint8_t* src,
+                                               size_t src_bytes);
+
+/* --- drawing --------------------------------------------------------------
+ * Draw every path of `scene` in order, exactly as consecutive
+ * Canvas::DrawPath calls on a software canvas would (sw_canvas.cc:357-411).
+ * Asynchronous on the surface's stream; hipr_surface_finish waits. */
+HIPR_API hipr_result hipr_surface_draw_scene(hipr_surface surface,
+                                             const hipr_scene* scene,
+                                             const hipr_options* options);
+HIPR_API hipr_result hipr_surface_finish(hipr_surface surface);
+HIPR_API hipr_result hipr_surface_get_stats(hipr_surface surface,
+                                            hipr_stats* out);
+
+/* --- stage entry points used by the parity tests -------------------------
+ * Stage (a): flatten + edge build (Stroke::QuadPath, src/geometry/
+ * stroke.cc:914-962 and SWEdgeBuilder::BuildEdges, src/render/sw/
+ * sw_edge.cc:299-336) for every path of `scene`, returning the edge records
+ * (16 int32 per edge, layout in DESIGN.md) in emission order and per-path
+ * edge offsets (n_paths + 1). */
+HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx,
+                                            const hipr_scene* scene,
+                                            int32_t* edge_offsets,
+                                            int32_t* edges, int64_t edge_cap);
+
+/* StackBlur on a device-resident premultiplied RGBA8 image
+ * (SWStackBlur::Blur, src/render/sw/sw_stack_blur.cc:18-284): radius is
+ * round(MaskFilter radius), clamped to 254; radius <= 1 copies. */
+HIPR_API hipr_result hipr_stackblur(hipr_context ctx, const uint8_t* src_host,
+                                    uint8_t* dst_host, int32_t width,
+                                    int32_t height, int32_t radius);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif  /* SKITY_HIP_H_ */

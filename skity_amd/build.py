@@ -1,0 +1,60 @@
+"""Builds the native libraries in-tree.
+
+  skity_amd/libskity_hip.so    gfx950 kernels + C ABI (hipcc)
+  skity_amd/libskity_scene.so  seeded scene generator (g++)
+  oracle/liboracle.so          CPU oracle, test infrastructure (make)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-fvisibility=hidden",
+    # every float operation rounds separately, as in the reference build
+    "-ffp-contract=off",
+    "-Wno-unused-value", "-Wno-unused-result",
+    # packed fp32 instructions are not used on this device
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, cwd=cwd, check=True)
+
+
+def build_hip():
+    src = os.path.join(HERE, "csrc", "skity_hip.hip")
+    out = os.path.join(HERE, "libskity_hip.so")
+    run([HIPCC] + HIP_FLAGS + ["-o", out + ".tmp", src])
+    os.replace(out + ".tmp", out)
+
+
+def build_scene():
+    src = os.path.join(HERE, "csrc", "scene_gen.cpp")
+    out = os.path.join(HERE, "libskity_scene.so")
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+         "-o", out + ".tmp", src])
+    os.replace(out + ".tmp", out)
+
+
+def build_oracle():
+    run(["make", "-s", "-B", "liboracle.so"], cwd=os.path.join(ROOT, "oracle"))
+
+
+def build_all():
+    build_oracle()
+    build_scene()
+    build_hip()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "hip": build_hip, "scene": build_scene,
+         "oracle": build_oracle}[w]()

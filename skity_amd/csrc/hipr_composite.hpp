@@ -1,0 +1,296 @@
+// hipr_composite.hpp -- stage (b)/(c) output: shade and blend coverage tiles.
+//
+// One wavefront per 16x16 canvas tile; lane l owns the 4 pixels of tile row
+// l / 4, columns 4*(l % 4) .. +3 (one 16-byte RGBA8 load and store, coalesced
+// across the wave).  The tile walks the canvas's draws in order, 64 at a time:
+// each lane tests one draw (tile inside its visible tile range, map entry
+// present), a ballot collects the draws that touch the tile, and they are
+// applied in order.  Per pixel and cover this is SWSpanBrush::Brush/BrushH
+// (src/render/sw/sw_span_brush.cc:66-138): cover & global alpha, AlphaMulQ
+// when the cover is not 255, then SWRenderTarget::BlendPixel for kSrcOver
+// (sw_render_target.cc:12-141, blend_mode.cc:125-191).  The target keeps
+// skity's kRGBA byte order; every channel is computed independently, so the
+// packed u32 arithmetic is the reference's AlphaMulQ/PMSrcOver.
+#ifndef SKITY_HIP_COMPOSITE_HPP_
+#define SKITY_HIP_COMPOSITE_HPP_
+
+#include "hipr_walk.hpp"
+
+namespace hipr {
+
+enum PaintKind : int32_t { kPaintSolid = 0, kPaintGradient = 1, kPaintImage = 2 };
+
+struct GradDev {
+  int32_t type, tile, count, has_offsets;
+  float sx, kx, tx, ky, sy, ty;  // device -> unit space
+  float bias, scale;             // sweep parameters
+  float offsets[16];
+  float colors[16][4];
+};
+
+struct PaintDev {
+  int32_t kind;
+  uint32_t color;   // premultiplied, RGBA byte order (solid)
+  int32_t ga;       // global alpha byte (cover & ga)
+  int32_t grad;     // GradDev index
+  int32_t layer;    // image: layer canvas index
+  int32_t L, T, R, B;            // image rect (filter bounds)
+  float isx, itx, isy, ity;      // image: device -> unit (scale, translate)
+};
+
+__device__ __forceinline__ uint32_t AlphaMulQ(uint32_t c, uint32_t scale) {
+  const uint32_t mask = 0xFF00FF;
+  uint32_t rb = ((c & mask) * scale) >> 8;
+  uint32_t ag = ((c >> 8) & mask) * scale;
+  return (rb & mask) | (ag & ~mask);
+}
+__device__ __forceinline__ uint32_t BlendSrcOver(uint32_t src, uint32_t dst) {
+  uint32_t a = src >> 24;
+  if (a == 0) return dst;
+  if (a == 255) return src;
+  return src + AlphaMulQ(dst, 256 - a);
+}
+__device__ __forceinline__ uint32_t MulDiv255Round(uint32_t a, uint32_t b) {
+  uint32_t prod = a * b + 128;
+  return (prod + (prod >> 8)) >> 8;
+}
+__device__ __forceinline__ uint32_t UnitToByte(float v) {
+  float s = v * 255;
+  s = fminf(fmaxf(s, 0.f), 255.f);
+  return (uint32_t)(uint8_t)s;
+}
+// Color4fToColor then ColorToPMColor (color.cc:53-59, color_priv.cc:103-106),
+// packed in RGBA byte order.
+__device__ __forceinline__ uint32_t PremulRGBA(const float c[4]) {
+  uint32_t a = UnitToByte(c[3]), r = UnitToByte(c[0]), g = UnitToByte(c[1]),
+           b = UnitToByte(c[2]);
+  if (a != 255) {
+    r = MulDiv255Round(r, a);
+    g = MulDiv255Round(g, a);
+    b = MulDiv255Round(b, a);
+  }
+  return r | (g << 8) | (b << 16) | (a << 24);
+}
+
+// RemapFloatTile (sw_span_brush.cc:21-32).
+__device__ __forceinline__ float RemapTile(float t, int tile) {
+  if (tile == HIPR_TILE_CLAMP) {
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+  } else if (tile == HIPR_TILE_REPEAT) {
+    t = t - floorf(t);
+  } else if (tile == HIPR_TILE_MIRROR) {
+    float t1 = t - 1;
+    float t2 = (float)((double)t1 - 2 * floor((double)t1 * 0.5) - 1);
+    t = fabsf(t2);
+  }
+  return t;
+}
+
+// GradientColorBrush::LerpColor + CalculateColor for linear / radial / sweep
+// (sw_span_brush.cc:239-383).
+__device__ uint32_t GradientColor(const GradDev& g, int x, int y) {
+  float sx = x + 0.5f, sy = y + 0.5f;
+  float mx = sx * g.sx + sy * g.kx + g.tx;
+  float my = sx * g.ky + sy * g.sy + g.ty;
+  float t;
+  if (g.type == HIPR_GRADIENT_LINEAR) {
+    t = mx;
+  } else if (g.type == HIPR_GRADIENT_RADIAL) {
+    t = sqrtf(mx * mx + my * my);
+  } else {
+    float angle = atan2f(-my, -mx);
+    const float k1Over2Pi = 0.1591549430918f;
+    t = (float)(((double)(angle * k1Over2Pi) + 0.5 + (double)g.bias) * (double)g.scale);
+  }
+  if (fabsf(t) <= 1.f / 4096) {
+    t = 0.0f;
+  } else if (fabsf(t - 1.0f) <= 1.f / 4096) {
+    t = 1.0f;
+  }
+  float c[4];
+  if (g.tile == HIPR_TILE_DECAL && (t < 0.0f || t >= 1.0f)) return 0;
+  t = RemapTile(t, g.tile);
+  int n = g.count;
+  bool stops = g.has_offsets != 0;
+  float step = 1.f / (n - 1);
+  if (stops && t <= g.offsets[0]) {
+    for (int q = 0; q < 4; q++) c[q] = g.colors[0][q];
+    return PremulRGBA(c);
+  }
+  int i = 0;
+  float start = 0.f, end = 0.f;
+  int si = 0, ei = 1;
+  for (i = 0; i < n - 1; i++) {
+    if (stops) {
+      start = g.offsets[i];
+      end = g.offsets[i + 1];
+    } else {
+      start = step * i;
+      end = step * (i + 1);
+    }
+    if (t >= start && t <= end) {
+      si = i;
+      ei = i + 1;
+      break;
+    }
+  }
+  if (i == n - 1) {
+    for (int q = 0; q < 4; q++) c[q] = g.colors[n - 1][q];
+    return PremulRGBA(c);
+  }
+  float total = end - start, value = t - start;
+  float mix = 0.5f;
+  if (total > 0) mix = value / total;
+  for (int q = 0; q < 4; q++) c[q] = g.colors[si][q] * (1 - mix) + g.colors[ei][q] * mix;
+  return PremulRGBA(c);
+}
+
+struct CompositeParams {
+  const Canvas* canvases;
+  int32_t n_canvases;
+  int32_t total_tiles;
+  const Job* jobs;
+  const JobCov* cov;
+  const int64_t* map_base;
+  const int32_t* map0;
+  const int32_t* map1;
+  const int64_t* plane_base;
+  const uint8_t* planes;
+  const PaintDev* paints;
+  const GradDev* grads;
+};
+
+__device__ __forceinline__ uint32_t ApplyCover(uint32_t dst, uint32_t color, uint32_t cover) {
+  uint32_t src = cover != 255 ? AlphaMulQ(color, cover) : color;
+  return BlendSrcOver(src, dst);
+}
+
+__global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
+  const int lane = Lane();
+  int t = blockIdx.x;
+  if (t >= P.total_tiles) return;
+  // canvas lookup (canvases are few or sorted by tile_begin)
+  int lo = 0, hi = P.n_canvases - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (P.canvases[mid].tile_begin <= t) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  const Canvas cv = P.canvases[lo];
+  int lt = t - cv.tile_begin;
+  int tx = lt % cv.tiles_x, ty = lt / cv.tiles_x;
+  int px0 = tx * kTile + 4 * (lane & 3);
+  int py = ty * kTile + (lane >> 2);
+  bool row_ok = py < cv.height;
+  uint32_t pix[4];
+  uint32_t* rowp = (uint32_t*)(cv.pixels + ((int64_t)py * cv.width) * 4);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    int x = px0 + q;
+    pix[q] = (row_ok && x < cv.width) ? rowp[x] : 0;
+  }
+  for (int jb = cv.job_begin; jb < cv.job_end; jb += kWave) {
+    int j = jb + lane;
+    int s0 = -1, s1 = -1;
+    bool present = false;
+    if (j < cv.job_end) {
+      const Job job = P.jobs[j];
+      if (job.kind == kJobImage) {
+        const PaintDev pd = P.paints[job.paint];
+        int x0 = tx * kTile, y0 = ty * kTile;
+        present = pd.L < x0 + kTile && pd.R > x0 && pd.T < y0 + kTile && pd.B > y0;
+      } else {
+        const JobCov c = P.cov[j];
+        PixBounds pb{c.left, c.top, c.right, c.bottom, !(c.left < c.right && c.top < c.bottom)};
+        TileRange trg = JobTiles(pb, cv.width, cv.height);
+        if (trg.ntc > 0 && tx >= trg.tc0 && tx < trg.tc0 + trg.ntc && ty >= trg.tr0 &&
+            ty < trg.tr0 + trg.ntr) {
+          int64_t mi = P.map_base[j] + (int64_t)(ty - trg.tr0) * trg.ntc + (tx - trg.tc0);
+          s0 = P.map0[mi];
+          if (s0 >= 0) {
+            s1 = P.map1[mi];
+            present = true;
+          }
+        }
+      }
+    }
+    uint64_t mask = __ballot(present);
+    while (mask) {
+      int b = __ffsll((unsigned long long)mask) - 1;
+      mask &= mask - 1;
+      int jj = jb + b;
+      int q0 = __shfl(s0, b, kWave), q1 = __shfl(s1, b, kWave);
+      const Job job = P.jobs[jj];
+      const PaintDev pd = P.paints[job.paint];
+      if (job.kind == kJobImage) {
+        // PixmapBrush over the filter rect: cover 255 & global alpha.
+        const Canvas layer = P.canvases[pd.layer];
+        uint32_t ga = (uint32_t)pd.ga;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          int x = px0 + q;
+          if (!row_ok || x >= cv.width || x < pd.L || x >= pd.R || py < pd.T || py >= pd.B)
+            continue;
+          float u = (x + 0.5f) * pd.isx + (py + 0.5f) * 0.f + pd.itx;
+          float v = (x + 0.5f) * 0.f + (py + 0.5f) * pd.isy + pd.ity;
+          uint32_t color = 0;
+          if (!(u < 0.0f || u >= 1.0f || v < 0.0f || v >= 1.0f)) {
+            uint32_t ix = (uint32_t)(u * (float)(uint32_t)layer.width);
+            uint32_t iy = (uint32_t)(v * (float)(uint32_t)layer.height);
+            ix = ix > (uint32_t)layer.width - 1 ? (uint32_t)layer.width - 1 : ix;
+            iy = iy > (uint32_t)layer.height - 1 ? (uint32_t)layer.height - 1 : iy;
+            uint32_t s = ((const uint32_t*)layer.pixels)[(int64_t)iy * layer.width + ix];
+            // Color4fFromColor -> Color4fToColor round trip per channel.
+            uint32_t o = 0;
+#pragma unroll
+            for (int ch = 0; ch < 4; ch++) {
+              float f = (float)((s >> (8 * ch)) & 0xFF) / 255.f;
+              o |= UnitToByte(f) << (8 * ch);
+            }
+            color = o;
+          }
+          uint32_t alpha = 255u & ga;
+          pix[q] = ApplyCover(pix[q], color, alpha);
+        }
+        continue;
+      }
+      int64_t pbase = P.plane_base[jj];
+      const uint32_t* pl0 =
+          (const uint32_t*)(P.planes + (pbase + q0) * kTilePlaneBytes) + lane;
+      uint32_t cov0 = *pl0;
+      uint32_t cov1 = 0;
+      if (q1 >= 0) cov1 = *((const uint32_t*)(P.planes + (pbase + q1) * kTilePlaneBytes) + lane);
+      if (pd.kind == kPaintSolid) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
+          if (c0) pix[q] = ApplyCover(pix[q], pd.color, c0);
+          if (c1) pix[q] = ApplyCover(pix[q], pd.color, c1);
+        }
+      } else {
+        const GradDev& g = P.grads[pd.grad];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
+          if (c0 | c1) {
+            uint32_t color = GradientColor(g, px0 + q, py);
+            if (c0) pix[q] = ApplyCover(pix[q], color, c0);
+            if (c1) pix[q] = ApplyCover(pix[q], color, c1);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    int x = px0 + q;
+    if (row_ok && x < cv.width) rowp[x] = pix[q];
+  }
+}
+
+}  // namespace hipr
+
+#endif  // SKITY_HIP_COMPOSITE_HPP_

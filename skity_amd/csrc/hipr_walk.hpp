@@ -1,0 +1,928 @@
+// hipr_walk.hpp -- stage (b): the analytic-AA scanline walk of one path per
+// wavefront, producing 16x16 coverage tiles.
+//
+// This is WalkEdges (src/render/sw/sw_raster.cc:546-677) with
+// insert_new_edges / backward_insert_edge_based_on_x / check_intersection
+// (sw_raster.cc:159-247) re-expressed for a 64-lane wave.  The active edge
+// list lives in registers (element p in lane p % 64, slot p / 64) and every
+// sequential list operation of the reference is replaced by the parallel
+// operation with the same result:
+//   * winding sums and interval roles: wave prefix sums in list order;
+//   * the left edge of an interval / the previous right edge: max-scans;
+//   * the per-edge x re-insertion after a step keeps the survivors stably
+//     sorted by x (backward_insert_edge_based_on_x moves an edge behind the
+//     processed edges with larger x), and check_intersection compares each
+//     survivor with its predecessor among the already processed survivors;
+//   * insert_new_edges places the first new edge after the last active edge
+//     with x <= its x and every later one before the first following active
+//     edge with x >= its x; new edges arrive sorted by (upper_y, x, dx, index)
+//     (SortEdges, sw_raster.cc:679-704, with emission order breaking ties).
+// Coverage of a pixel row is collected as LDS deltas (hipr_blit.hpp) and, when
+// the row is complete, resolved into at most two covers per pixel (the
+// directly emitted span first, then the accumulated one), which are written
+// into the plane(s) of the 16x16 tile holding the pixel.
+#ifndef SKITY_HIP_WALK_HPP_
+#define SKITY_HIP_WALK_HPP_
+
+#include "hipr_blit.hpp"
+#include "hipr_common.hpp"
+
+namespace hipr {
+
+constexpr int kTilePlaneBytes = kTile * kTile;
+
+struct WalkParams {
+  const Job* jobs;
+  const int32_t* walk_list;
+  int32_t n_walk;
+  const JobGeo* geo;
+  const int64_t* edge_base;
+  EdgeRec* wedges;          // per job, sorted by (upper_y, index); mutated
+  const Canvas* canvases;
+  const int64_t* plane_base;
+  const int32_t* plane_cap;
+  uint8_t* planes;
+  const int64_t* map_base;
+  int32_t* map0;
+  int32_t* map1;
+  JobCov* cov;
+  int32_t lds_width;        // row buffer capacity in pixels
+};
+
+// Job pixel bounds as RastePath computes them (sw_raster.cc:741-754).
+struct PixBounds {
+  int l, t, r, b;
+  bool empty;
+};
+__host__ __device__ inline PixBounds JobPixBounds(const JobGeo& g) {
+  PixBounds pb;
+  bool ok = g.l < g.r && g.t < g.b;
+  float l = ok ? floorf(g.l) : 0.f, t = ok ? floorf(g.t) : 0.f;
+  float r = ok ? ceilf(g.r) : 0.f, b = ok ? ceilf(g.b) : 0.f;
+  pb.empty = !(l < r && t < b);
+  pb.l = (int)l;
+  pb.t = (int)t;
+  pb.r = (int)r;
+  pb.b = (int)b;
+  return pb;
+}
+// Visible tile range of a job on its canvas (inclusive); ntc = 0 if none.
+struct TileRange {
+  int tc0, tr0, ntc, ntr;
+};
+__host__ __device__ inline TileRange JobTiles(const PixBounds& pb, int cw, int ch) {
+  TileRange tr{0, 0, 0, 0};
+  if (pb.empty) return tr;
+  int x0 = pb.l > 0 ? pb.l : 0, x1 = pb.r < cw ? pb.r : cw;
+  int y0 = pb.t > 0 ? pb.t : 0, y1 = pb.b < ch ? pb.b : ch;
+  if (x0 >= x1 || y0 >= y1) return tr;
+  tr.tc0 = x0 >> 4;
+  tr.tr0 = y0 >> 4;
+  tr.ntc = ((x1 - 1) >> 4) - tr.tc0 + 1;
+  tr.ntr = ((y1 - 1) >> 4) - tr.tr0 + 1;
+  return tr;
+}
+
+// meta word of an active edge: bit 0 winding < 0, bit 1 quad, bits 2.. index
+__device__ __forceinline__ int MetaWinding(int m) { return (m & 1) ? -1 : 1; }
+__device__ __forceinline__ bool MetaQuad(int m) { return (m & 2) != 0; }
+__device__ __forceinline__ int MetaIndex(int m) { return (int)((uint32_t)m >> 2); }
+__device__ __forceinline__ int MakeMeta(int winding, bool quad, int idx) {
+  return (winding < 0 ? 1 : 0) | (quad ? 2 : 0) | (idx << 2);
+}
+
+struct WalkLds {
+  int32_t* dA;
+  int32_t* dD;
+  int32_t* slot0;  // per tile column of the current band
+  int32_t* slot1;
+  int32_t* sX;     // permutation scratch (CAP entries each)
+  int32_t* sDX;
+  int32_t* sDY;
+  int32_t* sLY;
+  int32_t* sM;
+};
+
+template <int K, int kMode>
+__global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
+  constexpr int CAP = K * kWave;
+  extern __shared__ int32_t lds[];
+  const int lane = Lane();
+  const int wi = blockIdx.x;
+  if (wi >= P.n_walk) return;
+  const int j = P.walk_list[wi];
+  const Job job = P.jobs[j];
+  const JobGeo geo = P.geo[j];
+  const Canvas cv = P.canvases[job.canvas];
+  PixBounds pb = JobPixBounds(geo);
+  TileRange tr = JobTiles(pb, cv.width, cv.height);
+  int W = pb.r - pb.l;
+  int flags = 0;
+  int n_tiles = 0, n_planes = 0, max_active = 0;
+  int nE = geo.edges;
+  if (pb.empty || tr.ntc == 0 || nE == 0 || W > P.lds_width) {
+    if (!pb.empty && W > P.lds_width) flags |= 16;
+    if (lane == 0) {
+      JobCov c{pb.l, pb.t, pb.r, pb.b, 0, 0, 0, flags};
+      P.cov[j] = c;
+    }
+    return;
+  }
+  WalkLds L;
+  L.dA = lds;
+  L.dD = L.dA + (P.lds_width + 1);
+  L.slot0 = L.dD + (P.lds_width + 1);
+  L.slot1 = L.slot0 + (P.lds_width / kTile + 2);
+  L.sX = L.slot1 + (P.lds_width / kTile + 2);
+  L.sDX = L.sX + CAP;
+  L.sDY = L.sDX + CAP;
+  L.sLY = L.sDY + CAP;
+  L.sM = L.sLY + CAP;
+  for (int i = lane; i <= W; i += kWave) {
+    L.dA[i] = 0;
+    L.dD[i] = 0;
+  }
+  for (int i = lane; i < tr.ntc; i += kWave) {
+    L.slot0[i] = -1;
+    L.slot1[i] = -1;
+  }
+  __syncthreads();
+
+  EdgeRec* E = P.wedges + P.edge_base[j];
+  const int64_t pbase = kMode == 1 ? P.plane_base[j] : 0;
+  const int pcap = kMode == 1 ? P.plane_cap[j] : 0;
+  uint8_t* planes = P.planes;
+  int32_t* map0 = kMode == 1 ? P.map0 + P.map_base[j] : nullptr;
+  int32_t* map1 = kMode == 1 ? P.map1 + P.map_base[j] : nullptr;
+
+  const Fixed left_clip = (Fixed)((uint32_t)pb.l << 16);
+  const Fixed right_clip = (Fixed)((uint32_t)pb.r << 16);
+  const int winding_mask = job.fill == HIPR_FILL_EVEN_ODD ? 1 : -1;
+  const int cx0 = pb.l > 0 ? pb.l : 0, cx1 = pb.r < cv.width ? pb.r : cv.width;
+  const int cy0 = pb.t > 0 ? pb.t : 0, cy1 = pb.b < cv.height ? pb.b : cv.height;
+  RowSink sink{L.dA, L.dD, pb.l};
+
+  // Active list registers.
+  Fixed X[K], DX[K], DY[K], LY[K];
+  int M[K];
+  int n = 0;
+  int next_slot = 0;  // tile planes allocated so far
+  int band = -1;      // current 16-row band
+  int dmin = 0x7FFFFFFF, dmax = -0x7FFFFFFF;  // dirty x range of this row
+
+  // ---- helpers ---------------------------------------------------------------
+  // Write the active list (m elements in lane/slot layout) from scratch.
+  auto load_from_scratch = [&](int m) {
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      int p = s * kWave + lane;
+      if (p < m) {
+        X[s] = L.sX[p];
+        DX[s] = L.sDX[p];
+        DY[s] = L.sDY[p];
+        LY[s] = L.sLY[p];
+        M[s] = L.sM[p];
+      }
+    }
+  };
+
+  // Finish a band: publish the tile slots of every column to the job map.
+  auto finish_band = [&]() {
+    if (band < 0) return;
+    int trow = band - tr.tr0;
+    for (int c = lane; c < tr.ntc; c += kWave) {
+      int s0 = L.slot0[c], s1 = L.slot1[c];
+      // Only planes inside this job's allocation are published; an overflow
+      // is reported through flags and the job is redone by the exact path.
+      if (kMode == 1 && s0 >= 0 && s0 < pcap) {
+        map0[(int64_t)trow * tr.ntc + c] = s0;
+        map1[(int64_t)trow * tr.ntc + c] = s1 < pcap ? s1 : -1;
+      }
+      L.slot0[c] = -1;
+      L.slot1[c] = -1;
+    }
+    __syncthreads();
+  };
+
+  // Resolve pixel row r (prefix-sum the deltas) and emit its tile rows.
+  auto flush_row = [&](int r) {
+    int lo = WaveMin(dmin), hi = WaveMax(dmax);
+    dmin = 0x7FFFFFFF;
+    dmax = -0x7FFFFFFF;
+    if (lo > hi) return;  // nothing drawn in this row
+    __syncthreads();
+    bool visible = r >= cy0 && r < cy1;
+    int rb = r >> 4;
+    if (visible && rb != band) {
+      finish_band();
+      band = rb;
+    }
+    // canvas-aligned 16-pixel chunks; lane owns 4 pixels
+    int base = (pb.l + lo) & ~15;
+    int end = pb.l + hi;  // exclusive pixel bound of the dirty range
+    int carryA = 0, carryD = 0;
+    // deltas left of `base` are zero: base <= pb.l + lo
+    for (int x0 = base; x0 <= end; x0 += 4 * kWave) {
+      int px = x0 + 4 * lane;
+      int32_t vA[4], vD[4];
+      int sA = 0, sD = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int i = px + q - pb.l;
+        int a = 0, d = 0;
+        if (i >= 0 && i <= W) {
+          a = L.dA[i];
+          d = L.dD[i];
+          L.dA[i] = 0;
+          L.dD[i] = 0;
+        }
+        sA += a;
+        sD += d;
+        vA[q] = sA;
+        vD[q] = sD;
+      }
+      int iA = WaveInclusiveSum(sA), iD = WaveInclusiveSum(sD);
+      int eA = iA - sA + carryA, eD = iD - sD + carryD;
+      carryA += __shfl(iA, kWave - 1, kWave);
+      carryD += __shfl(iD, kWave - 1, kWave);
+      uint32_t c0 = 0, c1 = 0;
+      bool stacked = false;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int a = vA[q] + eA, d = vD[q] + eD;
+        int cnt = d >> 16, dv = d & 0xFFFF;
+        int av = a > 255 ? 255 : a;
+        int first = cnt ? dv : av;
+        int second = (cnt && av) ? av : 0;
+        if (cnt >= 2) stacked = true;
+        int x = px + q;
+        if (!visible || x < cx0 || x >= cx1) {
+          first = 0;
+          second = 0;
+        }
+        c0 |= (uint32_t)(first & 0xFF) << (8 * q);
+        c1 |= (uint32_t)(second & 0xFF) << (8 * q);
+      }
+      if (stacked) flags |= 4;
+      if (!visible) continue;
+      // chunk (4 lanes = 16 px) status
+      uint64_t nz0 = __ballot(c0 != 0), nz1 = __ballot(c1 != 0);
+      int g = lane >> 2;
+      bool chunk0 = ((nz0 >> (4 * g)) & 0xF) != 0;
+      bool chunk1 = ((nz1 >> (4 * g)) & 0xF) != 0;
+      int tcol = (px >> 4) - tr.tc0;
+      bool in_cols = tcol >= 0 && tcol < tr.ntc;
+      bool leader = (lane & 3) == 0;
+      // allocate planes for chunks that become non-empty in this band
+      int cur0 = in_cols ? L.slot0[tcol] : -1;
+      int cur1 = in_cols ? L.slot1[tcol] : -1;
+      bool need0 = in_cols && leader && chunk0 && cur0 < 0;
+      bool need1 = in_cols && leader && chunk1 && cur1 < 0;
+      int req = (need0 ? 1 : 0) + (need1 ? 1 : 0);
+      int rincl = WaveInclusiveSum(req);
+      int first_slot = next_slot + rincl - req;
+      next_slot += __shfl(rincl, kWave - 1, kWave);
+      if (need0) {
+        L.slot0[tcol] = first_slot;
+        n_tiles++;
+      }
+      if (need1) L.slot1[tcol] = first_slot + (need0 ? 1 : 0);
+      __syncthreads();
+      cur0 = in_cols ? L.slot0[tcol] : -1;
+      cur1 = in_cols ? L.slot1[tcol] : -1;
+      if (kMode == 1) {
+        int sub = lane & 3;
+        int rowk = r & 15;
+        bool new0 = __shfl(need0 ? 1 : 0, g * 4, kWave) != 0;
+        bool new1 = __shfl(need1 ? 1 : 0, g * 4, kWave) != 0;
+        if (in_cols) {
+          if (new0 && cur0 >= 0) {
+            if (cur0 < pcap) {
+              uint32_t* pl = (uint32_t*)(planes + (pbase + cur0) * kTilePlaneBytes);
+              for (int k = 0; k < 16; k++) pl[k * 4 + sub] = 0;
+            }
+          }
+          if (new1 && cur1 >= 0) {
+            if (cur1 < pcap) {
+              uint32_t* pl = (uint32_t*)(planes + (pbase + cur1) * kTilePlaneBytes);
+              for (int k = 0; k < 16; k++) pl[k * 4 + sub] = 0;
+            }
+          }
+          if (chunk0 && cur0 >= 0) {
+            if (cur0 < pcap) {
+              uint32_t* pl = (uint32_t*)(planes + (pbase + cur0) * kTilePlaneBytes);
+              pl[rowk * 4 + sub] = c0;
+            } else {
+              flags |= 1;
+            }
+          }
+          if (chunk1 && cur1 >= 0) {
+            if (cur1 < pcap) {
+              uint32_t* pl = (uint32_t*)(planes + (pbase + cur1) * kTilePlaneBytes);
+              pl[rowk * 4 + sub] = c1;
+            } else {
+              flags |= 1;
+            }
+          }
+        }
+      }
+    }
+    // clear the end-of-row delta entry
+    if (lane == 0) {
+      L.dA[W] = 0;
+      L.dD[W] = 0;
+    }
+    __syncthreads();
+  };
+
+  // Insert `k` new edges already in scratch (sX.. at [CAP-k.. ) no: at 0..k-1
+  // of the n* arrays) -- implemented inline below.
+
+  // ---- pending edges ---------------------------------------------------------
+  int pp = 0;  // next pending edge (sorted position)
+  auto pending_upper = [&](int idx) -> Fixed {
+    return idx < nE ? E[idx].upper_y : kMaxS32;
+  };
+
+  // Gathers the new edges with upper_y <= y into scratch slots
+  // [CAP/2 ...] -- uses separate regions of the scratch arrays: survivors are
+  // reloaded before new edges are staged, so the whole scratch is free.
+  // Returns the number of new edges (sorted by x, dx, index).
+  auto gather_new = [&](Fixed y, int* out_k) -> bool {
+    int k = 0;
+    for (;;) {
+      int idx = pp + k + lane;
+      Fixed uy = idx < nE ? E[idx].upper_y : kMaxS32;
+      bool ready = uy <= y;
+      uint64_t b = __ballot(ready);
+      int cnt = __popcll(b);
+      if (k + cnt > CAP) return false;
+      if (ready) {
+        EdgeRec e = E[idx];
+        int p = k + lane;
+        L.sX[p] = e.x;
+        L.sDX[p] = e.dx;
+        L.sDY[p] = e.dy;
+        L.sLY[p] = e.lower_y;
+        L.sM[p] = MakeMeta(EdgeWinding(e.packed), EdgeIsQuad(e.packed), idx);
+      }
+      k += cnt;
+      if (cnt < kWave) break;
+    }
+    *out_k = k;
+    return true;
+  };
+
+  // ---- initial activation (sw_raster.cc:554-563) -------------------------------
+  Fixed y = Max32(E[0].upper_y, IntToFixed(pb.t));
+  Fixed nny = kMaxS32;
+  {
+    int k;
+    if (!gather_new(y, &k)) {
+      flags |= 2;
+      k = 0;
+    }
+    __syncthreads();
+    // sort the k new edges by (x, dx, index): rank by counting
+    // (the sorted-by-upper_y input keeps index order within equal keys)
+    int rk[K];
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      int p = s * kWave + lane;
+      rk[s] = 0;
+      if (p < k) {
+        Fixed xi = L.sX[p], dxi = L.sDX[p];
+        int r = 0;
+        for (int q = 0; q < k; q++) {
+          Fixed xq = L.sX[q], dxq = L.sDX[q];
+          bool less = xq < xi || (xq == xi && (dxq < dxi || (dxq == dxi && q < p)));
+          r += less ? 1 : 0;
+        }
+        rk[s] = r;
+      }
+    }
+    Fixed tX[K], tDX[K], tDY[K], tLY[K];
+    int tM[K];
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      int p = s * kWave + lane;
+      if (p < k) {
+        tX[s] = L.sX[p];
+        tDX[s] = L.sDX[p];
+        tDY[s] = L.sDY[p];
+        tLY[s] = L.sLY[p];
+        tM[s] = L.sM[p];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      int p = s * kWave + lane;
+      if (p < k) {
+        L.sX[rk[s]] = tX[s];
+        L.sDX[rk[s]] = tDX[s];
+        L.sDY[rk[s]] = tDY[s];
+        L.sLY[rk[s]] = tLY[s];
+        L.sM[rk[s]] = tM[s];
+      }
+    }
+    __syncthreads();
+    load_from_scratch(k);
+    n = k;
+    pp = k;
+    int mn = kMaxS32;
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      int p = s * kWave + lane;
+      if (p < n && LY[s] > y) mn = Min32(mn, LY[s]);
+    }
+    nny = WaveMin(mn);
+    Fixed nu = pending_upper(pp);
+    if (nu > y && nu < nny) nny = nu;
+    __syncthreads();
+  }
+
+  const Fixed stop_fixed = IntToFixed(pb.b);
+  int guard = 0;
+  for (;;) {
+    if (++guard > (1 << 26)) {  // cannot happen: y strictly increases
+      flags |= 32;
+      break;
+    }
+    max_active = Max32(max_active, n);
+    Fixed next_y = Min32(nny, FixedCeilToFixed(y + 1));
+    int y_shift = 0;
+    if ((next_y - y) & (kFixed1 >> 2)) {
+      y_shift = 2;
+      next_y = y + (kFixed1 >> 2);
+    } else if ((next_y - y) & (kFixed1 >> 1)) {
+      y_shift = 1;
+    }
+    const Alpha full = (Alpha)FixedRoundToInt(0xFF * (next_y - y));
+    const int row = y >> 16;
+    const int S = (n + kWave - 1) / kWave;
+
+    // ---- phases 1-3: roles, trapezoids --------------------------------------
+    int carry_w = 0;
+    int carry_in = 0;
+    int carry_lidx = -1;
+    Fixed carry_lxo = 0, carry_lxn = 0, carry_ldy = 0;
+    int carry_ridx = -1;
+    Fixed carry_rxo = 0, carry_rxn = 0;
+    Fixed XN[K];
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      if (s >= S) break;
+      int p = s * kWave + lane;
+      bool valid = p < n;
+      int wv = valid ? MetaWinding(M[s]) : 0;
+      int incl = WaveInclusiveSum(wv) + carry_w;
+      int in = (valid && (incl & winding_mask) != 0) ? 1 : 0;
+      int prev_in = __shfl_up(in, 1, kWave);
+      if (lane == 0) prev_in = carry_in;
+      bool is_left = valid && in && !prev_in;
+      bool is_right = valid && !in && prev_in;
+      Fixed xo = X[s];
+      Fixed xn = valid ? xo + (DX[s] >> y_shift) : 0;
+      XN[s] = xn;
+      // nearest left edge at or before p
+      int lsel = WaveInclusiveMax(is_left ? lane : -1);
+      Fixed lxo = __shfl(xo, lsel < 0 ? 0 : lsel, kWave);
+      Fixed lxn = __shfl(xn, lsel < 0 ? 0 : lsel, kWave);
+      Fixed ldy = __shfl(DY[s], lsel < 0 ? 0 : lsel, kWave);
+      if (lsel < 0) {
+        lxo = carry_lxo;
+        lxn = carry_lxn;
+        ldy = carry_ldy;
+      }
+      // previous right edge strictly before p
+      int rsel_incl = WaveInclusiveMax(is_right ? lane : -1);
+      int rsel = __shfl_up(rsel_incl, 1, kWave);
+      if (lane == 0) rsel = -1;
+      Fixed rxo = __shfl(xo, rsel < 0 ? 0 : rsel, kWave);
+      Fixed rxn = __shfl(xn, rsel < 0 ? 0 : rsel, kWave);
+      bool has_prev_right = rsel >= 0 || carry_ridx >= 0;
+      if (rsel < 0) {
+        rxo = carry_rxo;
+        rxn = carry_rxn;
+      }
+      // next element (old x, dx); lane 63's successor is lane 0 of slot s+1
+      Fixed nxo = __shfl_down(xo, 1, kWave), ndx = __shfl_down(DX[s], 1, kWave);
+      int next_p = p + 1;
+      if (s + 1 < K) {
+        Fixed n0x = __shfl(X[s + 1 < K ? s + 1 : s], 0, kWave);
+        Fixed n0dx = __shfl(DX[s + 1 < K ? s + 1 : s], 0, kWave);
+        if (lane == kWave - 1) {
+          nxo = n0x;
+          ndx = n0dx;
+        }
+      }
+      bool next_active = next_p < n;
+      if (is_right) {
+        Fixed left = Max32(lxo, left_clip);
+        Fixed right = Min32(right_clip, xo);
+        Fixed next_left = Max32(left_clip, lxn);
+        Fixed next_right = Min32(right_clip, xn);
+        int prev_right = has_prev_right
+                             ? FixedCeilToInt(Max32(Min32(right_clip, rxo), rxn))
+                             : FixedFloorToInt(left_clip);
+        bool too_close_prev =
+            prev_right > FixedFloorToInt(left) || prev_right > FixedFloorToInt(lxn);
+        bool too_close_next = next_active && (xn + kFixed1 >= nxo - Abs32(ndx));
+        bool no_real = full == 0xFF && (too_close_prev || too_close_next);
+        BlitTrapezoidRow(&sink, left, right, next_left, next_right, ldy, DY[s], full,
+                         no_real);
+        int lo = FixedFloorToInt(Min32(Min32(left, next_left), Min32(right, next_right))) - pb.l;
+        int hi = FixedCeilToInt(Max32(Max32(left, next_left), Max32(right, next_right))) - pb.l;
+        dmin = Min32(dmin, lo);
+        dmax = Max32(dmax, hi);
+      }
+      // carries
+      carry_w = __shfl(incl, kWave - 1, kWave);
+      carry_in = __shfl(in, kWave - 1, kWave);
+      int lastl = __shfl(lsel, kWave - 1, kWave);
+      if (lastl >= 0) {
+        carry_lidx = s * kWave + lastl;
+        carry_lxo = __shfl(xo, lastl, kWave);
+        carry_lxn = __shfl(xn, lastl, kWave);
+        carry_ldy = __shfl(DY[s], lastl, kWave);
+      }
+      int lastr = __shfl(rsel_incl, kWave - 1, kWave);
+      if (lastr >= 0) {
+        carry_ridx = s * kWave + lastr;
+        carry_rxo = __shfl(xo, lastr, kWave);
+        carry_rxn = __shfl(xn, lastr, kWave);
+      }
+    }
+    // open interval at the end of the list (right edge culled away)
+    if (carry_in) {
+      flags |= 8;
+      // The reference tests edges_too_close(left_edge->prev, left_edge);
+      // with a closed path the winding sum is 0 here, so this is not reached
+      // on the parity scenes.  The left edge's list predecessor is not
+      // tracked; the span is emitted as accumulated coverage.
+      Fixed left = Max32(carry_lxo, left_clip);
+      if (lane == 0) {
+        BlitTrapezoidRow(&sink, left, right_clip, Max32(left_clip, carry_lxn), right_clip,
+                         carry_ldy, 0, full, full == 0xFF);
+        int lo = FixedFloorToInt(Min32(left, Max32(left_clip, carry_lxn))) - pb.l;
+        dmin = Min32(dmin, lo);
+        dmax = Max32(dmax, W);
+      }
+    }
+    __syncthreads();
+
+    // ---- phase 4: GoY and quad updates -----------------------------------------
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      if (s >= S) break;
+      int p = s * kWave + lane;
+      if (p < n) {
+        X[s] = XN[s];
+        if (LY[s] <= next_y && MetaQuad(M[s])) {
+          // while (lower_y <= next_y) { KeepContinuous; UpdateQuad } with
+          // KeepContinuous taking the edge's current x and y
+          // (sw_raster.cc:628-638, sw_edge.cc:294-297).
+          int idx = MetaIndex(M[s]);
+          EdgeRec e = E[idx];
+          Fixed sy = next_y;
+          Fixed cx = X[s];
+          while (LY[s] <= next_y && EdgeCurveCount(e.packed) > 0) {
+            e.snapped_x = cx;
+            if (!UpdateQuad(&e, sy)) break;
+            cx = e.x;
+            sy = e.upper_y;
+            DX[s] = e.dx;
+            DY[s] = e.dy;
+            LY[s] = e.lower_y;
+          }
+          X[s] = cx;
+          M[s] = MakeMeta(EdgeWinding(e.packed), true, idx);
+          E[idx] = e;
+        }
+      }
+    }
+
+    // ---- phases 5-6: survivors, stable x order, intersection checks ----------
+    int m = 0;
+    bool flag_isect = false;
+    {
+      int cnt_carry = 0;
+      bool sorted = true;
+      Fixed prevX_carry = kMinS32;
+      bool has_prev_carry = false;
+      int C[K];
+      bool SV[K];
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        if (s >= S) break;
+        int p = s * kWave + lane;
+        bool sv = p < n && LY[s] > next_y;
+        SV[s] = sv;
+        int inc = WaveInclusiveSum(sv ? 1 : 0);
+        C[s] = cnt_carry + inc - (sv ? 1 : 0);
+        // previous survivor's x
+        int psel_incl = WaveInclusiveMax(sv ? lane : -1);
+        int psel = __shfl_up(psel_incl, 1, kWave);
+        if (lane == 0) psel = -1;
+        Fixed px = __shfl(X[s], psel < 0 ? 0 : psel, kWave);
+        Fixed pdx = __shfl(DX[s], psel < 0 ? 0 : psel, kWave);
+        bool hp = psel >= 0;
+        if (!hp) {
+          px = prevX_carry;
+          hp = has_prev_carry;
+        }
+        if (sv && hp && X[s] < px) sorted = false;
+        (void)pdx;
+        int lastp = __shfl(psel_incl, kWave - 1, kWave);
+        if (lastp >= 0) {
+          prevX_carry = __shfl(X[s], lastp, kWave);
+          has_prev_carry = true;
+        }
+        cnt_carry += __shfl(inc, kWave - 1, kWave);
+      }
+      m = cnt_carry;
+      bool all_sorted = __ballot(!sorted) == 0;
+      // write survivors to scratch in survivor order
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        if (s >= S) break;
+        if (SV[s]) {
+          int c = C[s];
+          L.sX[c] = X[s];
+          L.sDX[c] = DX[s];
+          L.sDY[c] = DY[s];
+          L.sLY[c] = LY[s];
+          L.sM[c] = M[s];
+        }
+      }
+      __syncthreads();
+      if (all_sorted) {
+        // predecessor = previous survivor
+        bool f = false;
+        for (int c = lane + 1; c < m; c += kWave) {
+          if (L.sX[c - 1] + L.sDX[c - 1] > L.sX[c] + L.sDX[c]) f = true;
+        }
+        flag_isect = __ballot(f) != 0;
+        load_from_scratch(m);
+      } else {
+        // stable rank by x and predecessor among earlier survivors
+        int rk[K];
+        bool f = false;
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          int c = s * kWave + lane;
+          rk[s] = 0;
+          if (c < m) {
+            Fixed xi = L.sX[c];
+            int r = 0;
+            int pred = -1;
+            Fixed pxv = kMinS32;
+            for (int q = 0; q < m; q++) {
+              Fixed xq = L.sX[q];
+              r += (xq < xi || (xq == xi && q < c)) ? 1 : 0;
+              if (q < c && xq <= xi && xq >= pxv) {
+                pxv = xq;
+                pred = q;
+              }
+            }
+            rk[s] = r;
+            if (pred >= 0 && L.sX[pred] + L.sDX[pred] > xi + L.sDX[c]) f = true;
+          }
+        }
+        flag_isect = __ballot(f) != 0;
+        Fixed tX[K], tDX[K], tDY[K], tLY[K];
+        int tM[K];
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          int c = s * kWave + lane;
+          if (c < m) {
+            tX[s] = L.sX[c];
+            tDX[s] = L.sDX[c];
+            tDY[s] = L.sDY[c];
+            tLY[s] = L.sLY[c];
+            tM[s] = L.sM[c];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          int c = s * kWave + lane;
+          if (c < m) {
+            L.sX[rk[s]] = tX[s];
+            L.sDX[rk[s]] = tDX[s];
+            L.sDY[rk[s]] = tDY[s];
+            L.sLY[rk[s]] = tLY[s];
+            L.sM[rk[s]] = tM[s];
+          }
+        }
+        __syncthreads();
+        load_from_scratch(m);
+      }
+      __syncthreads();
+    }
+    n = m;
+    {
+      int mn = kMaxS32;
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        int p = s * kWave + lane;
+        if (p < n) mn = Min32(mn, LY[s]);
+      }
+      nny = WaveMin(mn);
+      if (flag_isect) nny = next_y + (kFixed1 >> kAcc);
+    }
+
+    // ---- row completion ------------------------------------------------------
+    y = next_y;
+    if ((y & 0xFFFF) == 0) flush_row(row);
+    if (y >= stop_fixed) break;
+
+    // ---- phase 7: insert new edges (insert_new_edges) --------------------------
+    Fixed nu = pending_upper(pp);
+    if (nu <= y) {
+      int k;
+      if (!gather_new(y, &k) || n + k > CAP) {
+        flags |= 2;
+        break;
+      }
+      __syncthreads();
+      // sort the new edges by (x, dx, sorted position)
+      int rk[K];
+      Fixed tX[K], tDX[K], tDY[K], tLY[K];
+      int tM[K];
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        int p = s * kWave + lane;
+        rk[s] = 0;
+        if (p < k) {
+          Fixed xi = L.sX[p], dxi = L.sDX[p];
+          int r = 0;
+          for (int q = 0; q < k; q++) {
+            Fixed xq = L.sX[q], dxq = L.sDX[q];
+            r += (xq < xi || (xq == xi && (dxq < dxi || (dxq == dxi && q < p)))) ? 1 : 0;
+          }
+          rk[s] = r;
+          tX[s] = L.sX[p];
+          tDX[s] = L.sDX[p];
+          tDY[s] = L.sDY[p];
+          tLY[s] = L.sLY[p];
+          tM[s] = L.sM[p];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        int p = s * kWave + lane;
+        if (p < k) {
+          L.sX[rk[s]] = tX[s];
+          L.sDX[rk[s]] = tDX[s];
+          L.sDY[rk[s]] = tDY[s];
+          L.sLY[rk[s]] = tLY[s];
+          L.sM[rk[s]] = tM[s];
+        }
+      }
+      __syncthreads();
+      // insertion positions: P1 = #{a : x_a <= x_N1};
+      // Pi = max(P(i-1), #{a : x_a < x_Ni}).
+      // The new edges are at scratch[0..k); positions go to sLY? keep in regs
+      int Pcur = 0;
+      bool fnew = false;
+      int posN[K];
+#pragma unroll
+      for (int s = 0; s < K; s++) posN[s] = 0;
+      for (int i = 0; i < k; i++) {
+        Fixed xi = L.sX[i];
+        int cnt = 0;
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          int p = s * kWave + lane;
+          bool v = p < n && (i == 0 ? X[s] <= xi : X[s] < xi);
+          cnt += __popcll(__ballot(v));
+        }
+        int Pi = i == 0 ? cnt : Max32(Pcur, cnt);
+        // predecessor at insertion: N(i-1) if it sits right before, else A[Pi-1]
+        Fixed predx = 0, preddx = 0;
+        bool has_pred = false;
+        if (i > 0 && Pi == Pcur) {
+          predx = L.sX[i - 1];
+          preddx = L.sDX[i - 1];
+          has_pred = true;
+        } else if (Pi > 0) {
+          int q = Pi - 1;
+          int s = q / kWave, ln = q % kWave;
+          Fixed ax = 0, adx = 0;
+#pragma unroll
+          for (int t = 0; t < K; t++) {
+            Fixed vx = __shfl(X[t], ln, kWave), vdx = __shfl(DX[t], ln, kWave);
+            if (t == s) {
+              ax = vx;
+              adx = vdx;
+            }
+          }
+          predx = ax;
+          preddx = adx;
+          has_pred = true;
+        }
+        if (has_pred && predx + preddx > L.sX[i] + L.sDX[i]) fnew = true;
+        if (i % kWave == lane) {
+#pragma unroll
+          for (int s = 0; s < K; s++)
+            if (s == i / kWave) posN[s] = Pi + i;
+        }
+        Pcur = Pi;
+      }
+      if (fnew) nny = y + (kFixed1 >> kAcc);
+      // lower_y of new edges
+      {
+        int mn = kMaxS32;
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          int p = s * kWave + lane;
+          if (p < k) mn = Min32(mn, L.sLY[p]);
+        }
+        mn = WaveMin(mn);
+        if (mn > y && mn < nny) nny = mn;
+      }
+      // merge: A[q] moves to q + #{i : P_i <= q}; N_i goes to P_i + i
+      // P_i are stored as posN - i; count with a loop over i
+      Fixed nX[K], nDX[K], nDY[K], nLY[K];
+      int nM[K];
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        int p = s * kWave + lane;
+        if (p < k) {
+          nX[s] = L.sX[p];
+          nDX[s] = L.sDX[p];
+          nDY[s] = L.sDY[p];
+          nLY[s] = L.sLY[p];
+          nM[s] = L.sM[p];
+        }
+      }
+      // P_i broadcast through LDS (reuse sLY region after reading)
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        int p = s * kWave + lane;
+        if (p < k) L.sLY[CAP - 1 - p] = posN[s] - p;  // P_p, stored from the end
+      }
+      __syncthreads();
+      int shiftA[K];
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        int q = s * kWave + lane;
+        int c = 0;
+        if (q < n) {
+          for (int i = 0; i < k; i++) c += (L.sLY[CAP - 1 - i] <= q) ? 1 : 0;
+        }
+        shiftA[s] = c;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        int q = s * kWave + lane;
+        if (q < n) {
+          int d = q + shiftA[s];
+          L.sX[d] = X[s];
+          L.sDX[d] = DX[s];
+          L.sDY[d] = DY[s];
+          L.sLY[d] = LY[s];
+          L.sM[d] = M[s];
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < K; s++) {
+        int p = s * kWave + lane;
+        if (p < k) {
+          int d = posN[s];
+          L.sX[d] = nX[s];
+          L.sDX[d] = nDX[s];
+          L.sDY[d] = nDY[s];
+          L.sLY[d] = nLY[s];
+          L.sM[d] = nM[s];
+        }
+      }
+      __syncthreads();
+      n += k;
+      pp += k;
+      load_from_scratch(n);
+      __syncthreads();
+      Fixed nu2 = pending_upper(pp);
+      if (nu2 > y && nu2 < nny) nny = nu2;
+    } else if (nu > y && nu < nny) {
+      nny = nu;
+    }
+  }
+  finish_band();
+  n_tiles = WaveSum(n_tiles);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) flags |= __shfl_xor(flags, o, kWave);
+  if (lane == 0) {
+    JobCov c{pb.l, pb.t, pb.r, pb.b, n_tiles, next_slot, max_active, flags};
+    P.cov[j] = c;
+  }
+}
+
+}  // namespace hipr
+
+#endif  // SKITY_HIP_WALK_HPP_

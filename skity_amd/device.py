@@ -1,0 +1,159 @@
+"""Python binding of the C-ABI in include/skity_hip.h (libskity_hip.so).
+
+The rasteriser runs entirely in the HIP library; this module only loads it,
+declares the entry points and wraps the opaque handles.  It raises if the
+library is missing instead of falling back to anything else.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = abi.load("libskity_hip.so")
+        c_ctx = ctypes.c_void_p
+        c_surf = ctypes.c_void_p
+        SP = ctypes.POINTER(abi.Scene)
+        L.hipr_context_create.argtypes = [ctypes.c_int32, ctypes.POINTER(c_ctx)]
+        L.hipr_context_create.restype = ctypes.c_int
+        L.hipr_context_destroy.argtypes = [c_ctx]
+        L.hipr_context_destroy.restype = None
+        L.hipr_surface_create.argtypes = [c_ctx, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.POINTER(c_surf)]
+        L.hipr_surface_create.restype = ctypes.c_int
+        L.hipr_surface_destroy.argtypes = [c_surf]
+        L.hipr_surface_destroy.restype = None
+        L.hipr_surface_clear.argtypes = [c_surf, ctypes.c_uint32]
+        L.hipr_surface_clear.restype = ctypes.c_int
+        L.hipr_surface_read_pixels.argtypes = [c_surf, ctypes.c_void_p, ctypes.c_size_t]
+        L.hipr_surface_read_pixels.restype = ctypes.c_int
+        L.hipr_surface_write_pixels.argtypes = [c_surf, ctypes.c_void_p, ctypes.c_size_t]
+        L.hipr_surface_write_pixels.restype = ctypes.c_int
+        L.hipr_surface_draw_scene.argtypes = [c_surf, SP, ctypes.POINTER(abi.Options)]
+        L.hipr_surface_draw_scene.restype = ctypes.c_int
+        L.hipr_surface_finish.argtypes = [c_surf]
+        L.hipr_surface_finish.restype = ctypes.c_int
+        L.hipr_surface_get_stats.argtypes = [c_surf, ctypes.POINTER(abi.Stats)]
+        L.hipr_surface_get_stats.restype = ctypes.c_int
+        L.hipr_debug_build_edges.argtypes = [c_ctx, SP, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int64]
+        L.hipr_debug_build_edges.restype = ctypes.c_int
+        L.hipr_stackblur.argtypes = [c_ctx, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_int32]
+        L.hipr_stackblur.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+class HiprError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what} failed with hipr_result {code}")
+        self.code = code
+
+
+def check(code, what):
+    if code != abi.SUCCESS:
+        raise HiprError(code, what)
+
+
+class Context:
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        check(lib().hipr_context_create(device, ctypes.byref(self._h)), "hipr_context_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def build_edges(self, scene_ptr, n_paths):
+        offs = np.zeros(n_paths + 1, np.int32)
+        rc = lib().hipr_debug_build_edges(self._h, scene_ptr, offs.ctypes.data, None, 0)
+        if rc not in (abi.SUCCESS, abi.ERROR_CAPACITY):
+            raise HiprError(rc, "hipr_debug_build_edges")
+        total = int(offs[-1])
+        edges = np.zeros((max(total, 1), 16), np.int32)
+        check(lib().hipr_debug_build_edges(self._h, scene_ptr, offs.ctypes.data,
+                                           edges.ctypes.data, total), "hipr_debug_build_edges")
+        return offs, edges[:total]
+
+    def stackblur(self, img, radius):
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape[:2]
+        out = np.zeros_like(img)
+        check(lib().hipr_stackblur(self._h, img.ctypes.data, out.ctypes.data, w, h, radius),
+              "hipr_stackblur")
+        return out
+
+    def close(self):
+        if self._h:
+            lib().hipr_context_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Surface:
+    def __init__(self, ctx, width, height):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self._h = ctypes.c_void_p()
+        check(lib().hipr_surface_create(ctx.handle, width, height, ctypes.byref(self._h)),
+              "hipr_surface_create")
+
+    def clear(self, rgba=0):
+        check(lib().hipr_surface_clear(self._h, rgba), "hipr_surface_clear")
+
+    def draw(self, scene_ptr, mode=0, edge_estimates=None, tile_estimates=None):
+        opt = abi.Options()
+        opt.mode = mode
+        keep = []
+        if edge_estimates is not None:
+            e = np.ascontiguousarray(edge_estimates, np.int32)
+            keep.append(e)
+            opt.edge_estimates = e.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        if tile_estimates is not None:
+            t = np.ascontiguousarray(tile_estimates, np.int32)
+            keep.append(t)
+            opt.tile_estimates = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        check(lib().hipr_surface_draw_scene(self._h, scene_ptr, ctypes.byref(opt)),
+              "hipr_surface_draw_scene")
+
+    def finish(self):
+        check(lib().hipr_surface_finish(self._h), "hipr_surface_finish")
+
+    def read(self):
+        img = np.zeros((self.height, self.width, 4), np.uint8)
+        check(lib().hipr_surface_read_pixels(self._h, img.ctypes.data, img.nbytes),
+              "hipr_surface_read_pixels")
+        return img
+
+    def write(self, img):
+        img = np.ascontiguousarray(img, np.uint8)
+        check(lib().hipr_surface_write_pixels(self._h, img.ctypes.data, img.nbytes),
+              "hipr_surface_write_pixels")
+
+    def stats(self):
+        s = abi.Stats()
+        check(lib().hipr_surface_get_stats(self._h, ctypes.byref(s)), "hipr_surface_get_stats")
+        return {k: getattr(s, k) for k, _ in abi.Stats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().hipr_surface_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

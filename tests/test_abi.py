@@ -1,0 +1,33 @@
+"""CPU checks of the C-ABI library: it loads and exports every entry point
+declared in include/skity_hip.h (no device calls)."""
+import ctypes
+import os
+import re
+
+from skity_amd import abi
+
+HEADER = os.path.join(abi.ROOT, "include", "skity_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"HIPR_API\s+[\w\s\*]+?\b(hipr_\w+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "hipr_surface_draw_scene" in syms
+    assert len(syms) >= 10
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(os.path.join(abi.HERE, "libskity_hip.so"))
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, f"missing exports: {missing}"
+
+
+def test_struct_sizes_match_header_layout():
+    # hipr_paint: 4 floats + 9 x 4-byte fields
+    assert ctypes.sizeof(abi.Paint) == 52
+    assert ctypes.sizeof(abi.Gradient) == 4 * (4 + 4 + 2 + 6 + 64 + 16)
+    assert ctypes.sizeof(abi.Stats) == 7 * 8 + 8
