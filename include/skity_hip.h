@@ -54,7 +54,8 @@ enum { HIPR_VERB_MOVE = 0, HIPR_VERB_LINE = 1, HIPR_VERB_QUAD = 2,
 /* Path::PathFillType (path.hpp:62-67). */
 enum { HIPR_FILL_WINDING = 0, HIPR_FILL_EVEN_ODD = 1 };
 /* Paint::Style / Cap / Join (include/skity/graphic/paint.hpp:46-105). */
-enum { HIPR_STYLE_FILL = 0, HIPR_STYLE_STROKE = 1, HIPR_STYLE_STROKE_AND_FILL = 2 };
+enum { HIPR_STYLE_FILL = 0, HIPR_STYLE_STROKE = 1, HIPR_STYLE_STROKE_AND_FILL = 2,
+       HIPR_STYLE_STROKE_THEN_FILL = 3 };
 enum { HIPR_CAP_BUTT = 0, HIPR_CAP_ROUND = 1, HIPR_CAP_SQUARE = 2 };
 enum { HIPR_JOIN_MITER = 0, HIPR_JOIN_ROUND = 1, HIPR_JOIN_BEVEL = 2 };
 /* Shader::GradientType (include/skity/effect/shader.hpp:40-47). */
@@ -71,11 +72,12 @@ enum { HIPR_BLEND_SRC_OVER = 3 };
 #define HIPR_MAX_GRADIENT_STOPS 16
 
 /* One Paint, as SWCanvas reads it (src/render/sw/sw_canvas.cc:357-411,727-795).
- * `color` is the Color4f the reference Paint stores (Paint::SetColor keeps
- * Color4fFromColor(c), src/graphic/paint.cc:80-84); fill and stroke colour are
- * the same value. */
+ * `color` and `stroke_color` are the Color4f fill and stroke colours the
+ * reference Paint stores (Paint::SetColor sets both to Color4fFromColor(c),
+ * src/graphic/paint.cc:48-84). */
 typedef struct hipr_paint {
-  float color[4];       /* r, g, b, a in [0,1] (unpremultiplied)            */
+  float color[4];       /* fill colour r, g, b, a in [0,1] (unpremultiplied) */
+  float stroke_color[4];/* stroke colour                                     */
   int32_t style;        /* HIPR_STYLE_*                                     */
   float stroke_width;   /* Paint::GetStrokeWidth                            */
   float miter_limit;    /* Paint::GetStrokeMiter (default 4)                */
@@ -180,6 +182,20 @@ HIPR_API hipr_result hipr_surface_draw_scene(hipr_surface surface,
 HIPR_API hipr_result hipr_surface_finish(hipr_surface surface);
 HIPR_API hipr_result hipr_surface_get_stats(hipr_surface surface,
                                             hipr_stats* out);
+
+/* A validated scene whose geometry, paints and draw plan are resident in
+ * device memory, so that it can be drawn repeatedly without host-to-device
+ * copies (the recorded DisplayList of include/skity/recorder/
+ * display_list.hpp:56-77 played onto a HIP surface).  The scene arrays may
+ * be released after hipr_scene_prepare returns. */
+typedef struct hipr_prepared_s* hipr_prepared;
+HIPR_API hipr_result hipr_scene_prepare(hipr_context ctx,
+                                        const hipr_scene* scene,
+                                        hipr_prepared* out);
+HIPR_API void hipr_prepared_destroy(hipr_prepared prepared);
+HIPR_API hipr_result hipr_surface_draw_prepared(hipr_surface surface,
+                                                hipr_prepared prepared,
+                                                const hipr_options* options);
 
 /* --- stage entry points used by the parity tests -------------------------
  * Stage (a): flatten + edge build (Stroke::QuadPath, src/geometry/

@@ -409,21 +409,25 @@ void StackBlur(const uint8_t* src, uint8_t* dst, int w, int h, int radius) {
 
 namespace {
 
-Brush MakeBrush(const hipr_scene* s, const hipr_paint& paint, const Aff& ctm) {
+// GenerateBrush (sw_canvas.cc:727-795): gradient shader, else the stroke or
+// fill colour.
+Brush MakeBrush(const hipr_scene* s, const hipr_paint& paint, const Aff& ctm,
+                bool stroke) {
   Brush br;
   if (paint.gradient >= 0 && s->gradients && paint.gradient < s->n_gradients) {
     br.kind = 1;
     PrepareGradient(s->gradients[paint.gradient], ctm, &br.grad);
   } else {
     br.kind = 0;
-    br.solid = ColorToPMColor(Color4fToColor(paint.color));
+    br.solid = ColorToPMColor(Color4fToColor(stroke ? paint.stroke_color : paint.color));
   }
   return br;
 }
 
 // One fill or stroke rasterisation plus brush (sw_canvas.cc:374-407).
-void DrawShape(const PathData& path, const hipr_paint& paint, const Brush& br,
+void DrawShape(const hipr_scene* s, const PathData& path, const hipr_paint& paint,
                bool stroke, const Aff& ctm, Target* t) {
+  Brush br = MakeBrush(s, paint, ctm, stroke);
   std::vector<Span> spans;
   float bounds[4];
   if (!stroke) {
@@ -440,15 +444,14 @@ void DrawShape(const PathData& path, const hipr_paint& paint, const Brush& br,
 // -filter_bounds.LT on a mask-filter layer).
 void DrawPathNoFilter(const hipr_scene* s, const PathData& path,
                       const hipr_paint& paint, const Aff& ctm, Target* t) {
-  Brush br = MakeBrush(s, paint, ctm);
   bool need_fill = paint.style != HIPR_STYLE_STROKE;
   bool need_stroke = paint.style != HIPR_STYLE_FILL;
-  if (paint.style == 3 /* kStrokeThenFill */) {
-    if (need_stroke) DrawShape(path, paint, br, true, ctm, t);
-    if (need_fill) DrawShape(path, paint, br, false, ctm, t);
+  if (paint.style == HIPR_STYLE_STROKE_THEN_FILL) {
+    if (need_stroke) DrawShape(s, path, paint, true, ctm, t);
+    if (need_fill) DrawShape(s, path, paint, false, ctm, t);
   } else {
-    if (need_fill) DrawShape(path, paint, br, false, ctm, t);
-    if (need_stroke) DrawShape(path, paint, br, true, ctm, t);
+    if (need_fill) DrawShape(s, path, paint, false, ctm, t);
+    if (need_stroke) DrawShape(s, path, paint, true, ctm, t);
   }
 }
 
@@ -522,7 +525,10 @@ void DrawPathBlurred(const hipr_scene* s, int p, Target* t) {
   Aff inv;
   Invert(local, &inv);
   br.img_ptu = Mul(Scale(1.f / W, 1.f / H), inv);
-  br.global_alpha = (uint8_t)(255 * paint.color[3]);
+  // Paint::GetAlphaF: the stroke colour's alpha for kStroke_Style, else the
+  // fill colour's (paint.cc:96-98).
+  float alpha_f = paint.style == HIPR_STYLE_STROKE ? paint.stroke_color[3] : paint.color[3];
+  br.global_alpha = (uint8_t)(255 * alpha_f);
   ApplySpans(spans, br, t);
 }
 

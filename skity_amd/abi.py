@@ -31,6 +31,7 @@ ERROR_CAPACITY = -8
 class Paint(ctypes.Structure):
     _fields_ = [
         ("color", ctypes.c_float * 4),
+        ("stroke_color", ctypes.c_float * 4),
         ("style", ctypes.c_int32),
         ("stroke_width", ctypes.c_float),
         ("miter_limit", ctypes.c_float),

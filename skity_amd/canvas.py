@@ -1,0 +1,250 @@
+"""Python mirror of skity's Path / Paint / Canvas::DrawPath recording API.
+
+A ``Canvas`` records DrawPath calls (include/skity/render/canvas.hpp:
+Canvas::DrawPath) into the hipr_scene layout of include/skity_hip.h and hands
+them to the HIP library in one call.  ``Path`` stores verbs and points the way
+skity's Path does (src/graphic/path.cc:494-560, MoveTo/LineTo/QuadTo/ConicTo/
+CubicTo with InjectMoveToIfNeed, :837-859 Close, :815-835 AddRect,
+:1001-1035 AddCircle/AddOval), so the arrays handed over are exactly the ones
+skity would iterate.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import abi
+
+FLOAT_ROOT2_OVER2 = 0.707106781
+
+
+class Path:
+    """Verb / point / conic-weight storage with skity's builder rules."""
+
+    def __init__(self):
+        self.verbs = []
+        self.points = []
+        self.weights = []
+        self.fill_type = abi.FILL_WINDING
+        self._last_move = -1  # index of the last move point; < 0 after Close
+
+    # -- builder (src/graphic/path.cc) --
+    def move_to(self, x, y):
+        if self.verbs and self.verbs[-1] == abi.VERB_MOVE:
+            self.points[-1] = (float(x), float(y))
+        else:
+            self._last_move = len(self.points)
+            self.verbs.append(abi.VERB_MOVE)
+            self.points.append((float(x), float(y)))
+        return self
+
+    def _inject_move(self):
+        if self._last_move < 0:
+            if not self.points:
+                x, y = 0.0, 0.0
+            else:
+                x, y = self.points[~self._last_move]
+            self.move_to(x, y)
+
+    def line_to(self, x, y):
+        self._inject_move()
+        self.verbs.append(abi.VERB_LINE)
+        self.points.append((float(x), float(y)))
+        return self
+
+    def quad_to(self, x1, y1, x2, y2):
+        self._inject_move()
+        self.verbs.append(abi.VERB_QUAD)
+        self.points += [(float(x1), float(y1)), (float(x2), float(y2))]
+        return self
+
+    def conic_to(self, x1, y1, x2, y2, w):
+        w = np.float32(w)
+        if not (w > 0):
+            return self.line_to(x2, y2)
+        if math.isinf(w):
+            self.line_to(x1, y1)
+            return self.line_to(x2, y2)
+        if w == 1:
+            return self.quad_to(x1, y1, x2, y2)
+        self._inject_move()
+        self.verbs.append(abi.VERB_CONIC)
+        self.weights.append(float(w))
+        self.points += [(float(x1), float(y1)), (float(x2), float(y2))]
+        return self
+
+    def cubic_to(self, x1, y1, x2, y2, x3, y3):
+        self._inject_move()
+        self.verbs.append(abi.VERB_CUBIC)
+        self.points += [(float(x1), float(y1)), (float(x2), float(y2)),
+                        (float(x3), float(y3))]
+        return self
+
+    def close(self):
+        if self.verbs and self.verbs[-1] != abi.VERB_CLOSE:
+            self.verbs.append(abi.VERB_CLOSE)
+        if self._last_move >= 0:
+            self._last_move = ~self._last_move
+        return self
+
+    def add_rect(self, l, t, r, b, ccw=False, start=0):
+        pts = [(l, t), (r, t), (r, b), (l, b)]
+        adv = 3 if ccw else 1
+        i = start % 4
+        self.move_to(*pts[i])
+        for _ in range(3):
+            i = (i + adv) % 4
+            self.line_to(*pts[i])
+        return self.close()
+
+    def add_oval(self, l, t, r, b, ccw=False, start=1):
+        l, t, r, b = (np.float32(v) for v in (l, t, r, b))
+        cx = np.float32(np.float32(0.5) * np.float32(r + l))  # Rect::CenterX, rect.cc:52
+        cy = np.float32(np.float32(0.5) * np.float32(t + b))
+        oval = [(cx, t), (r, cy), (cx, b), (l, cy)]
+        rect = [(l, t), (r, t), (r, b), (l, b)]
+        adv = 3 if ccw else 1
+        oi = start % 4
+        ri = (start + (0 if not ccw else 1)) % 4
+        self.move_to(*oval[oi])
+        for _ in range(4):
+            ri = (ri + adv) % 4
+            oi = (oi + adv) % 4
+            self.conic_to(*rect[ri], *oval[oi], FLOAT_ROOT2_OVER2)
+        return self.close()
+
+    def add_circle(self, x, y, radius, ccw=False):
+        if radius > 0:
+            self.add_oval(x - radius, y - radius, x + radius, y + radius, ccw)
+        return self
+
+
+class Paint:
+    """The Paint fields the DrawPath path reads (include/skity/graphic/paint.hpp)."""
+
+    def __init__(self, color=(0, 0, 0, 1)):
+        self.p = abi.Paint()
+        self.set_color(color)
+        self.p.style = abi.STYLE_FILL
+        self.p.stroke_width = 1.0
+        self.p.miter_limit = 4.0
+        self.p.cap = abi.CAP_BUTT
+        self.p.join = abi.JOIN_MITER
+        self.p.gradient = -1
+        self.p.blend = abi.BLEND_SRC_OVER
+        self.gradient = None
+
+    def set_color(self, rgba):
+        # Paint::SetColor sets both the fill and the stroke colour
+        for i in range(4):
+            self.p.color[i] = rgba[i]
+            self.p.stroke_color[i] = rgba[i]
+        return self
+
+    def set_fill_color(self, rgba):
+        for i in range(4):
+            self.p.color[i] = rgba[i]
+        return self
+
+    def set_stroke_color(self, rgba):
+        for i in range(4):
+            self.p.stroke_color[i] = rgba[i]
+        return self
+
+    def set_style(self, style):
+        self.p.style = style
+        return self
+
+    def set_stroke_width(self, w):
+        self.p.stroke_width = w
+        return self
+
+    def set_stroke_cap(self, cap):
+        self.p.cap = cap
+        return self
+
+    def set_stroke_join(self, join):
+        self.p.join = join
+        return self
+
+    def set_stroke_miter(self, m):
+        self.p.miter_limit = m
+        return self
+
+    def set_blur(self, radius):
+        """MaskFilter::MakeBlur(kNormal, radius)."""
+        self.p.blur_radius = radius
+        self.p.blur_style = abi.BLUR_NORMAL if radius > 0 else 0
+        return self
+
+    def set_gradient(self, g):
+        """A Shader::MakeLinear/Radial/Sweep gradient (abi.Gradient)."""
+        self.gradient = g
+        return self
+
+
+class Canvas:
+    """Records DrawPath calls; ``scene()`` packs them as a hipr_scene."""
+
+    def __init__(self, width, height):
+        self.width, self.height = width, height
+        self.draws = []
+
+    def draw_path(self, path, paint):
+        self.draws.append((path, paint))
+
+    def scene(self):
+        return PackedScene(self.draws, self.width, self.height)
+
+
+class PackedScene:
+    """Owns the flat arrays of a hipr_scene; ``.scene`` points at it."""
+
+    def __init__(self, draws, width, height):
+        self.width, self.height = width, height
+        n = len(draws)
+        vo = np.zeros(n + 1, np.int32)
+        po = np.zeros(n + 1, np.int32)
+        wo = np.zeros(n + 1, np.int32)
+        verbs, pts, ws, fills = [], [], [], []
+        grads = []
+        self.paints = (abi.Paint * max(n, 1))()
+        for i, (path, paint) in enumerate(draws):
+            verbs += path.verbs
+            pts += path.points
+            ws += path.weights
+            fills.append(path.fill_type)
+            vo[i + 1] = len(verbs)
+            po[i + 1] = len(pts)
+            wo[i + 1] = len(ws)
+            ctypes.memmove(ctypes.byref(self.paints[i]), ctypes.byref(paint.p),
+                           ctypes.sizeof(abi.Paint))
+            if paint.gradient is not None:
+                self.paints[i].gradient = len(grads)
+                grads.append(paint.gradient)
+            else:
+                self.paints[i].gradient = -1
+        self.verbs = np.array(verbs or [0], np.uint8)
+        self.points = np.array(pts or [(0.0, 0.0)], np.float32).reshape(-1)
+        self.weights = np.array(ws or [0.0], np.float32)
+        self.fills = np.array(fills or [0], np.uint8)
+        self.vo, self.po, self.wo = vo, po, wo
+        self.grads = (abi.Gradient * max(len(grads), 1))(*grads)
+        s = abi.Scene()
+        s.n_paths = n
+        s.verbs = self.verbs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        s.verb_offsets = vo.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        s.points = self.points.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        s.point_offsets = po.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        s.weights = self.weights.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        s.weight_offsets = wo.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        s.fill_types = self.fills.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        s.paints = ctypes.cast(self.paints, ctypes.POINTER(abi.Paint))
+        s.gradients = ctypes.cast(self.grads, ctypes.POINTER(abi.Gradient))
+        s.n_gradients = len(grads)
+        self._s = s
+        self.scene = ctypes.pointer(s)
+
+    @property
+    def n_paths(self):
+        return self._s.n_paths

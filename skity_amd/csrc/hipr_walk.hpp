@@ -120,6 +120,16 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   int flags = 0;
   int n_tiles = 0, n_planes = 0, max_active = 0;
   int nE = geo.edges;
+  int32_t* map0 = kMode == 1 ? P.map0 + P.map_base[j] : nullptr;
+  int32_t* map1 = kMode == 1 ? P.map1 + P.map_base[j] : nullptr;
+  if (kMode == 1) {
+    // every visible tile of the job starts empty
+    int64_t msz = (int64_t)tr.ntc * tr.ntr;
+    for (int64_t i = lane; i < msz; i += kWave) {
+      map0[i] = -1;
+      map1[i] = -1;
+    }
+  }
   if (pb.empty || tr.ntc == 0 || nE == 0 || W > P.lds_width) {
     if (!pb.empty && W > P.lds_width) flags |= 16;
     if (lane == 0) {
@@ -152,8 +162,6 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   const int64_t pbase = kMode == 1 ? P.plane_base[j] : 0;
   const int pcap = kMode == 1 ? P.plane_cap[j] : 0;
   uint8_t* planes = P.planes;
-  int32_t* map0 = kMode == 1 ? P.map0 + P.map_base[j] : nullptr;
-  int32_t* map1 = kMode == 1 ? P.map1 + P.map_base[j] : nullptr;
 
   const Fixed left_clip = (Fixed)((uint32_t)pb.l << 16);
   const Fixed right_clip = (Fixed)((uint32_t)pb.r << 16);

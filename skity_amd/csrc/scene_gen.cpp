@@ -131,6 +131,7 @@ hipr_paint SolidPaint(Pcg32& rng, int min_alpha) {
   p.color[1] = g / 255.f;
   p.color[2] = b / 255.f;
   p.color[3] = a / 255.f;
+  for (int c = 0; c < 4; c++) p.stroke_color[c] = p.color[c];
   p.style = HIPR_STYLE_FILL;
   p.stroke_width = 0.f;
   p.miter_limit = 4.f;

@@ -25,6 +25,7 @@
 #include "hipr_composite.hpp"
 #include "hipr_flatten.hpp"
 #include "hipr_sort.hpp"
+#include "hipr_stroke.hpp"
 #include "hipr_walk.hpp"
 
 using namespace hipr;
@@ -218,18 +219,18 @@ struct hipr_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // device scene
-  DevBuf verbs, pts, weights;
-  // pipeline buffers
-  DevBuf jobs, geo, cov, canvases, paints, grads;
-  DevBuf edge_base, edge_cap, edges, wedges, gkeys;
-  DevBuf plane_base, plane_cap, planes, map_base, map0, map1;
-  DevBuf walk_list, layers, blur_jobs, blur_lines, blur_tmp;
-  // pinned staging for readbacks
+  // per-draw device buffers (grow-only)
+  DevBuf canvases, layer_canvases, geo, cov, edge_base, edge_cap, edges, wedges, gkeys;
+  DevBuf plane_base, plane_cap, planes, map_base, map0, map1, sub_list;
+  DevBuf stroke_out, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w;
+  DevBuf layers, blur_rows, blur_cols, blur_rb, blur_cb, blur_tmp, blur_jobs, blur_lines;
+  // debug/utility uploads
+  DevBuf d_verbs, d_pts, d_w, d_jobs, run_jobs;
+  // pinned staging for read-backs
   JobGeo* h_geo = nullptr;
-  size_t h_geo_cap = 0;
   JobCov* h_cov = nullptr;
-  size_t h_cov_cap = 0;
+  StrokeOut* h_so = nullptr;
+  size_t h_cap = 0;
 };
 
 struct hipr_surface_s {
@@ -237,6 +238,27 @@ struct hipr_surface_s {
   int32_t width = 0, height = 0;
   uint8_t* pixels = nullptr;
   hipr_stats stats{};
+};
+
+// A scene validated, planned and resident in device memory.
+struct hipr_prepared_s {
+  hipr_context ctx = nullptr;
+  int32_t n_paths = 0;
+  std::vector<Job> jobs;
+  std::vector<Canvas> canvases;  // [0] is the target, then mask-filter layers
+  std::vector<PaintDev> paints;
+  std::vector<GradDev> grads;
+  std::vector<BlurJob> blurs;
+  std::vector<int32_t> layer_of_blur;
+  std::vector<int32_t> walk_jobs;    // job indices that are rasterised
+  std::vector<int32_t> stroke_jobs;  // job indices that need an outline
+  int64_t layer_bytes = 0;
+  DevBuf verbs, pts, weights, scene_paints, d_jobs, d_paints, d_grads, d_walk, d_stroke;
+  ~hipr_prepared_s() {
+    DevBuf* b[] = {&verbs, &pts, &weights, &scene_paints, &d_jobs, &d_paints, &d_grads,
+                   &d_walk, &d_stroke};
+    for (DevBuf* x : b) x->Free();
+  }
 };
 
 namespace {
@@ -250,15 +272,26 @@ hipr_result Upload(DevBuf* b, const T* src, size_t n, hipStream_t s) {
 }
 
 hipr_result HostStage(hipr_context ctx, size_t n) {
-  if (n > ctx->h_geo_cap) {
-    if (ctx->h_geo) hipHostFree(ctx->h_geo);
-    if (ctx->h_cov) hipHostFree(ctx->h_cov);
+  if (n > ctx->h_cap) {
+    if (ctx->h_geo) (void)hipHostFree(ctx->h_geo);
+    if (ctx->h_cov) (void)hipHostFree(ctx->h_cov);
+    if (ctx->h_so) (void)hipHostFree(ctx->h_so);
+    ctx->h_geo = nullptr;
+    ctx->h_cov = nullptr;
+    ctx->h_so = nullptr;
     size_t want = n + n / 2 + 64;
     HIP_TRY(hipHostMalloc((void**)&ctx->h_geo, want * sizeof(JobGeo)));
     HIP_TRY(hipHostMalloc((void**)&ctx->h_cov, want * sizeof(JobCov)));
-    ctx->h_geo_cap = ctx->h_cov_cap = want;
+    HIP_TRY(hipHostMalloc((void**)&ctx->h_so, want * sizeof(StrokeOut)));
+    ctx->h_cap = want;
   }
   return HIPR_SUCCESS;
+}
+
+int VerbPointCount(int verb) {
+  return verb == HIPR_VERB_MOVE || verb == HIPR_VERB_LINE ? 1
+         : verb == HIPR_VERB_QUAD || verb == HIPR_VERB_CONIC ? 2
+         : verb == HIPR_VERB_CUBIC ? 3 : 0;
 }
 
 hipr_result ValidateScene(const hipr_scene* s) {
@@ -268,27 +301,27 @@ hipr_result ValidateScene(const hipr_scene* s) {
   if (!s->verbs || !s->verb_offsets || !s->points || !s->point_offsets ||
       !s->weight_offsets || !s->paints)
     return HIPR_ERROR_INVALID_ARGUMENT;
+  if (s->verb_offsets[0] != 0 || s->point_offsets[0] != 0 || s->weight_offsets[0] != 0)
+    return HIPR_ERROR_INVALID_ARGUMENT;
   for (int p = 0; p < s->n_paths; p++) {
     int v0 = s->verb_offsets[p], v1 = s->verb_offsets[p + 1];
     if (v1 < v0) return HIPR_ERROR_INVALID_ARGUMENT;
     int need_pts = 0, need_w = 0;
-    bool open = false;  // a contour is open (a move was seen)
+    bool open = false;
     for (int v = v0; v < v1; v++) {
       int verb = s->verbs[v];
       if (verb < 0 || verb > HIPR_VERB_CLOSE) return HIPR_ERROR_INVALID_ARGUMENT;
       if (verb == HIPR_VERB_MOVE) {
+        if (v > v0 && s->verbs[v - 1] == HIPR_VERB_MOVE)
+          return HIPR_ERROR_INVALID_ARGUMENT;  // the Path builder merges moves
         open = true;
-      } else if (verb != HIPR_VERB_CLOSE && !open) {
-        return HIPR_ERROR_INVALID_ARGUMENT;  // segment without a move
       } else if (verb == HIPR_VERB_CLOSE) {
         open = false;
+      } else if (!open) {
+        return HIPR_ERROR_INVALID_ARGUMENT;  // a segment without a move
       }
-      need_pts += verb == HIPR_VERB_MOVE || verb == HIPR_VERB_LINE ? 1
-                  : verb == HIPR_VERB_QUAD || verb == HIPR_VERB_CONIC ? 2
-                  : verb == HIPR_VERB_CUBIC ? 3 : 0;
+      need_pts += VerbPointCount(verb);
       need_w += verb == HIPR_VERB_CONIC ? 1 : 0;
-      if (verb == HIPR_VERB_MOVE && v > v0 && s->verbs[v - 1] == HIPR_VERB_MOVE)
-        return HIPR_ERROR_INVALID_ARGUMENT;  // the Path builder merges moves
     }
     if (s->point_offsets[p + 1] - s->point_offsets[p] != need_pts)
       return HIPR_ERROR_INVALID_ARGUMENT;
@@ -296,13 +329,22 @@ hipr_result ValidateScene(const hipr_scene* s) {
       return HIPR_ERROR_INVALID_ARGUMENT;
     if (need_w && !s->weights) return HIPR_ERROR_INVALID_ARGUMENT;
     for (int i = s->point_offsets[p]; i < s->point_offsets[p + 1]; i++) {
+      // non-finite points are rejected (SURVEY: Rect::SetBoundsCheck cannot)
       if (!std::isfinite(s->points[2 * i]) || !std::isfinite(s->points[2 * i + 1]))
         return HIPR_ERROR_INVALID_ARGUMENT;
+    }
+    for (int i = s->weight_offsets[p]; i < s->weight_offsets[p + 1]; i++) {
+      float w = s->weights[i];
+      if (!std::isfinite(w) || !(w > 0)) return HIPR_ERROR_INVALID_ARGUMENT;
     }
     const hipr_paint& pt = s->paints[p];
     if (pt.blend != HIPR_BLEND_SRC_OVER) return HIPR_ERROR_NOT_SUPPORTED;
     if (pt.style < 0 || pt.style > 3) return HIPR_ERROR_INVALID_ARGUMENT;
-    if (pt.style != HIPR_STYLE_FILL) return HIPR_ERROR_NOT_SUPPORTED;
+    if (pt.style != HIPR_STYLE_FILL &&
+        (!std::isfinite(pt.stroke_width) || pt.stroke_width <= 0.f))
+      return HIPR_ERROR_NOT_SUPPORTED;  // hairlines take a different path
+    if (pt.cap < 0 || pt.cap > 2 || pt.join < 0 || pt.join > 2)
+      return HIPR_ERROR_INVALID_ARGUMENT;
     if (pt.gradient >= 0) {
       if (!s->gradients || pt.gradient >= s->n_gradients) return HIPR_ERROR_INVALID_ARGUMENT;
       const hipr_gradient& g = s->gradients[pt.gradient];
@@ -337,38 +379,8 @@ void SourceBounds(const hipr_scene* s, int p, float b[4]) {
   }
 }
 
-struct Plan {
-  std::vector<Job> jobs;
-  std::vector<Canvas> canvases;
-  std::vector<PaintDev> paints;
-  std::vector<GradDev> grads;
-  std::vector<BlurJob> blurs;
-  std::vector<int32_t> layer_of_blur;  // canvas index of each blur job
-  std::vector<int32_t> walk_jobs;      // non-image jobs
-  int64_t layer_bytes = 0;
-};
-
-void AddShapeJobs(const hipr_scene* s, int p, int canvas, float tx, float ty, int paint,
-                  std::vector<Job>* out) {
-  Job jb{};
-  jb.path = p;
-  jb.canvas = canvas;
-  jb.fill = s->fill_types ? s->fill_types[p] : HIPR_FILL_WINDING;
-  jb.geom = 0;
-  jb.verb_begin = s->verb_offsets[p];
-  jb.verb_end = s->verb_offsets[p + 1];
-  jb.pt_begin = s->point_offsets[p];
-  jb.w_begin = s->weight_offsets[p];
-  jb.tx = tx;
-  jb.ty = ty;
-  jb.image = -1;
-  jb.paint = paint;
-  jb.kind = kJobFill;  // strokes are rejected by ValidateScene
-  out->push_back(jb);
-}
-
 PaintDev SolidOrGradient(const hipr_scene* s, const hipr_paint& pt, const Aff& ctm,
-                         std::vector<GradDev>* grads) {
+                         bool stroke, std::vector<GradDev>* grads) {
   PaintDev pd{};
   pd.ga = 255;
   if (pt.gradient >= 0) {
@@ -377,24 +389,55 @@ PaintDev SolidOrGradient(const hipr_scene* s, const hipr_paint& pt, const Aff& c
     grads->push_back(MakeGradDev(s->gradients[pt.gradient], ctm));
   } else {
     pd.kind = kPaintSolid;
-    pd.color = PremulRGBAHost(pt.color);
+    pd.color = PremulRGBAHost(stroke ? pt.stroke_color : pt.color);
   }
   return pd;
 }
 
-void BuildPlan(const hipr_scene* s, hipr_surface surf, Plan* pl) {
-  // canvas 0 is the surface; layer canvases follow.
+// Fill and/or stroke jobs in paint order (DrawFillStrokeInPaintOrder,
+// src/render/paint_order.hpp:12-31).
+void AddShapeJobs(const hipr_scene* s, int p, int canvas, float tx, float ty, const Aff& ctm,
+                  hipr_prepared_s* pl, std::vector<Job>* out) {
+  const hipr_paint& pt = s->paints[p];
+  bool need_fill = pt.style != HIPR_STYLE_STROKE;
+  bool need_stroke = pt.style != HIPR_STYLE_FILL;
+  auto add = [&](bool stroke) {
+    Job jb{};
+    jb.path = p;
+    jb.canvas = canvas;
+    jb.kind = stroke ? kJobStroke : kJobFill;
+    jb.fill = stroke ? HIPR_FILL_WINDING
+                     : (s->fill_types ? s->fill_types[p] : HIPR_FILL_WINDING);
+    jb.geom = stroke ? 1 : 0;
+    jb.verb_begin = s->verb_offsets[p];
+    jb.verb_end = s->verb_offsets[p + 1];
+    jb.pt_begin = s->point_offsets[p];
+    jb.w_begin = s->weight_offsets[p];
+    jb.tx = tx;
+    jb.ty = ty;
+    jb.image = -1;
+    jb.paint = (int32_t)pl->paints.size();
+    pl->paints.push_back(SolidOrGradient(s, pt, ctm, stroke, &pl->grads));
+    out->push_back(jb);
+  };
+  if (pt.style == HIPR_STYLE_STROKE_THEN_FILL) {
+    if (need_stroke) add(true);
+    if (need_fill) add(false);
+  } else {
+    if (need_fill) add(false);
+    if (need_stroke) add(true);
+  }
+}
+
+void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
   std::vector<std::vector<Job>> canvas_jobs(1);
-  Canvas main{};
-  main.pixels = surf->pixels;
-  main.width = surf->width;
-  main.height = surf->height;
-  pl->canvases.push_back(main);
+  pl->canvases.push_back(Canvas{});
   for (int p = 0; p < s->n_paths; p++) {
     const hipr_paint& pt = s->paints[p];
     if (pt.blur_radius > 0) {
-      // HandleFilter (sw_canvas.cc:797-826): local bounds, stroke-expanded,
-      // +- radius, floored/ceiled (image_filter.cc:33-41).
+      // HandleFilter (sw_canvas.cc:797-826): local bounds, stroke-expanded
+      // (floor on all sides, sw_canvas.cc:135-144), +- radius, floored /
+      // ceiled (image_filter.cc:33-41).
       float b[4];
       SourceBounds(s, p, b);
       if (pt.style != HIPR_STYLE_FILL) {
@@ -413,79 +456,520 @@ void BuildPlan(const hipr_scene* s, hipr_surface surf, Plan* pl) {
       Canvas lc{};
       lc.width = W;
       lc.height = H;
-      lc.pixels = (uint8_t*)(intptr_t)pl->layer_bytes;  // offset, fixed later
-      pl->layer_bytes += (int64_t)W * H * 4;
+      lc.pixels = (uint8_t*)(intptr_t)pl->layer_bytes;  // offset into the layer pool
+      pl->layer_bytes += ((int64_t)W * H * 4 + 255) & ~(int64_t)255;
       pl->canvases.push_back(lc);
       canvas_jobs.emplace_back();
-      Aff ctm = Translate(-L, -T);
-      int pi = (int)pl->paints.size();
-      pl->paints.push_back(SolidOrGradient(s, pt, ctm, &pl->grads));
-      AddShapeJobs(s, p, layer, -L, -T, pi, &canvas_jobs[layer]);
-      int radius = std::min((int)std::round(r), 254);
+      AddShapeJobs(s, p, layer, -L, -T, Translate(-L, -T), pl, &canvas_jobs[layer]);
       BlurJob bj{};
       bj.width = W;
       bj.height = H;
-      bj.radius = radius;
+      bj.radius = std::min((int)std::round(r), 254);
       pl->blurs.push_back(bj);
       pl->layer_of_blur.push_back(layer);
-      // the composite draw: OnDrawImageRect with a decal nearest pixmap shader
+      // The composite draw: OnDrawImageRect with a decal nearest pixmap
+      // shader over the filter bounds (sw_canvas.cc:641-677); the paint's
+      // alpha is applied as the brush's global alpha (Paint::GetAlphaF).
       PaintDev ip{};
       ip.kind = kPaintImage;
       ip.layer = layer;
-      ip.ga = (int32_t)(uint8_t)(255 * pt.color[3]);
+      float alpha_f = pt.style == HIPR_STYLE_STROKE ? pt.stroke_color[3] : pt.color[3];
+      ip.ga = (int32_t)(uint8_t)(255 * alpha_f);
       ip.L = (int)L;
       ip.T = (int)T;
       ip.R = (int)R;
       ip.B = (int)B;
-      // Scale(1/W, 1/H) * inverse(Translate(L, T))
       Aff inv;
       Invert(Translate(L, T), &inv);
-      Aff m = Mul(Scale(1.f / W, 1.f / H), inv);
+      Aff m = Mul(Scale(1.f / W, 1.f / H), inv);  // Scale(1/w, 1/h) * inverse(local)
       ip.isx = m.sx;
       ip.itx = m.tx;
       ip.isy = m.sy;
       ip.ity = m.ty;
-      int ipi = (int)pl->paints.size();
-      pl->paints.push_back(ip);
       Job jb{};
       jb.path = p;
       jb.canvas = 0;
       jb.kind = kJobImage;
       jb.image = layer;
-      jb.paint = ipi;
+      jb.paint = (int32_t)pl->paints.size();
+      pl->paints.push_back(ip);
       canvas_jobs[0].push_back(jb);
     } else {
-      int pi = (int)pl->paints.size();
-      pl->paints.push_back(SolidOrGradient(s, pt, Aff(), &pl->grads));
-      AddShapeJobs(s, p, 0, 0.f, 0.f, pi, &canvas_jobs[0]);
+      AddShapeJobs(s, p, 0, 0.f, 0.f, Aff(), pl, &canvas_jobs[0]);
     }
   }
-  // layers first (they are composited and blurred before the canvas)
-  int total_tiles = 0;
+  // layers first (composited and blurred before the canvas), canvas last
   for (size_t c = 1; c <= canvas_jobs.size(); c++) {
-    size_t ci = c % canvas_jobs.size();  // 1..n-1 then 0
+    size_t ci = c % canvas_jobs.size();
     Canvas& cv = pl->canvases[ci];
     cv.job_begin = (int)pl->jobs.size();
     for (auto& jb : canvas_jobs[ci]) pl->jobs.push_back(jb);
     cv.job_end = (int)pl->jobs.size();
   }
-  for (auto& cv : pl->canvases) {
+  for (size_t i = 0; i < pl->jobs.size(); i++) {
+    if (pl->jobs[i].kind != kJobImage) pl->walk_jobs.push_back((int32_t)i);
+    if (pl->jobs[i].kind == kJobStroke) pl->stroke_jobs.push_back((int32_t)i);
+  }
+}
+
+void SetTiles(std::vector<Canvas>* cvs) {
+  int total = 0;
+  for (auto& cv : *cvs) {
     cv.tiles_x = (cv.width + kTile - 1) / kTile;
     cv.tiles_y = (cv.height + kTile - 1) / kTile;
-    cv.tile_begin = total_tiles;
-    total_tiles += cv.tiles_x * cv.tiles_y;
+    cv.tile_begin = total;
+    total += cv.tiles_x * cv.tiles_y;
   }
-  for (size_t i = 0; i < pl->jobs.size(); i++)
-    if (pl->jobs[i].kind != kJobImage) pl->walk_jobs.push_back((int32_t)i);
 }
 
 template <int K, int kMode>
-hipr_result LaunchWalk(const WalkParams& P, int lds_width, hipStream_t st) {
+hipr_result LaunchWalk(const WalkParams& P, hipStream_t st) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
-  size_t lds = (size_t)(2 * (lds_width + 1) + 2 * (lds_width / kTile + 2) + 5 * K * kWave) *
+  size_t lds = (size_t)(2 * (P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 5 * K * kWave) *
                sizeof(int32_t);
   hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds, st, P);
   HIP_TRY(hipGetLastError());
+  return HIPR_SUCCESS;
+}
+
+// ---- the device pipeline -----------------------------------------------------
+struct Frame {
+  hipr_context ctx;
+  hipr_prepared_s* pl;
+  hipStream_t st;
+  int nj;
+  Job* d_jobs = nullptr;  // the jobs as rasterised (stroke jobs point at outlines)
+  std::vector<Canvas> canvases;
+  std::vector<Job> jobs;  // stroke jobs get their outline ranges here
+  hipr_stats stats{};
+};
+
+hipr_result RunStrokes(Frame& f) {
+  hipr_context ctx = f.ctx;
+  hipr_prepared_s* pl = f.pl;
+  int ns = (int)pl->stroke_jobs.size();
+  if (ns == 0) return HIPR_SUCCESS;
+  hipStream_t st = f.st;
+  if (ctx->stroke_out.Ensure(sizeof(StrokeOut) * ns) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  GeomSrc gs{pl->verbs.as<uint8_t>(), pl->pts.as<float>(), pl->weights.as<float>()};
+  int blocks = (ns + 63) / 64;
+  hipLaunchKernelGGL((StrokeKernel<0>), dim3(blocks), dim3(64), 0, st, pl->d_jobs.as<Job>(),
+                     pl->d_stroke.as<int32_t>(), ns, gs, pl->scene_paints.as<hipr_paint>(),
+                     ctx->stroke_out.as<StrokeOut>(), (uint8_t*)nullptr, (float*)nullptr,
+                     (float*)nullptr, (uint8_t*)nullptr, (float*)nullptr, (float*)nullptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(ctx->h_so, ctx->stroke_out.p, sizeof(StrokeOut) * ns,
+                         hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int64_t tv = 0, tp = 0, tw = 0, ts = 0;
+  for (int i = 0; i < ns; i++) {
+    StrokeOut& so = ctx->h_so[i];
+    so.verb_begin = (int32_t)tv;
+    so.pt_begin = (int32_t)tp;
+    so.w_begin = (int32_t)tw;
+    so.scratch_begin = (int32_t)ts;
+    tv += so.nv;
+    tp += so.np;
+    tw += so.nw;
+    ts += std::max(so.inner_nv, std::max(so.inner_np, so.inner_nw));
+    Job& jb = f.jobs[pl->stroke_jobs[i]];
+    jb.verb_begin = so.verb_begin;
+    jb.verb_end = so.verb_begin + so.nv;
+    jb.pt_begin = so.pt_begin;
+    jb.w_begin = so.w_begin;
+  }
+  if (tv > INT32_MAX || tp > INT32_MAX) return HIPR_ERROR_CAPACITY;
+  if (ctx->o_verbs.Ensure(std::max<int64_t>(tv, 1)) != hipSuccess ||
+      ctx->o_pts.Ensure(8 * std::max<int64_t>(tp, 1)) != hipSuccess ||
+      ctx->o_w.Ensure(4 * std::max<int64_t>(tw, 1)) != hipSuccess ||
+      ctx->s_verbs.Ensure(std::max<int64_t>(ts, 1)) != hipSuccess ||
+      ctx->s_pts.Ensure(8 * std::max<int64_t>(ts, 1)) != hipSuccess ||
+      ctx->s_w.Ensure(4 * std::max<int64_t>(ts, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  HIP_TRY(hipMemcpyAsync(ctx->stroke_out.p, ctx->h_so, sizeof(StrokeOut) * ns,
+                         hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL((StrokeKernel<1>), dim3(blocks), dim3(64), 0, st, pl->d_jobs.as<Job>(),
+                     pl->d_stroke.as<int32_t>(), ns, gs, pl->scene_paints.as<hipr_paint>(),
+                     ctx->stroke_out.as<StrokeOut>(), ctx->o_verbs.as<uint8_t>(),
+                     ctx->o_pts.as<float>(), ctx->o_w.as<float>(), ctx->s_verbs.as<uint8_t>(),
+                     ctx->s_pts.as<float>(), ctx->s_w.as<float>());
+  HIP_TRY(hipGetLastError());
+  return HIPR_SUCCESS;
+}
+
+// Flatten + sort for the jobs in `list` (indices into f.jobs).
+hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int64_t>* ebase,
+                    std::vector<int32_t>* ecap) {
+  hipr_context ctx = f.ctx;
+  hipr_prepared_s* pl = f.pl;
+  hipStream_t st = f.st;
+  int nj = f.nj;
+  GeomSrc gs{pl->verbs.as<uint8_t>(), pl->pts.as<float>(), pl->weights.as<float>()};
+  GeomSrc go{ctx->o_verbs.as<uint8_t>(), ctx->o_pts.as<float>(), ctx->o_w.as<float>()};
+  ebase->assign(nj + 1, 0);
+  ecap->assign(nj, 0);
+  if (mode == 0) {
+    hipLaunchKernelGGL((FlattenKernel<0>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj,
+                       gs, go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
+                       (EdgeRec*)nullptr, ctx->geo.as<JobGeo>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * nj, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int j = 0; j < nj; j++) (*ecap)[j] = ctx->h_geo[j].edges;
+  } else {
+    for (int j = 0; j < nj; j++) (*ecap)[j] = f.jobs[j].kind == kJobImage ? 0 : edge_est[j];
+  }
+  for (int j = 0; j < nj; j++) (*ebase)[j + 1] = (*ebase)[j] + (*ecap)[j];
+  int64_t total = (*ebase)[nj];
+  if (ctx->edges.Ensure(sizeof(EdgeRec) * std::max<int64_t>(total, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  HIP_TRY(hipMemcpyAsync(ctx->edge_base.p, ebase->data(), sizeof(int64_t) * nj,
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ctx->edge_cap.p, ecap->data(), sizeof(int32_t) * nj,
+                         hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj, gs,
+                     go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
+                     ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>());
+  HIP_TRY(hipGetLastError());
+  if (mode == 1) {
+    // one read-back: exact counts, bounds and overflow flags
+    HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * nj, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int32_t> redo;
+    for (int j = 0; j < nj; j++)
+      if (ctx->h_geo[j].overflow) redo.push_back(j);
+    if (!redo.empty()) {
+      // exact fallback: the write pass already counted; append exact space
+      f.stats.fallbacks += (int64_t)redo.size();
+      for (int j : redo) {
+        (*ebase)[j] = total;
+        (*ecap)[j] = ctx->h_geo[j].edges;
+        total += (*ecap)[j];
+      }
+      if (ctx->edges.Ensure(sizeof(EdgeRec) * std::max<int64_t>(total, 1)) != hipSuccess)
+        return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+      // (Ensure may have reallocated: the redone jobs rewrite every edge
+      // they own; the others are rewritten too, cheaply, by re-running.)
+      HIP_TRY(hipMemcpyAsync(ctx->edge_base.p, ebase->data(), sizeof(int64_t) * nj,
+                             hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(ctx->edge_cap.p, ecap->data(), sizeof(int32_t) * nj,
+                             hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj,
+                         gs, go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
+                         ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>());
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * nj, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+  }
+  f.stats.edges = 0;
+  for (int j = 0; j < nj; j++) f.stats.edges += ctx->h_geo[j].edges;
+  if (ctx->wedges.Ensure(sizeof(EdgeRec) * std::max<int64_t>(total, 1)) != hipSuccess ||
+      ctx->gkeys.Ensure(sizeof(uint64_t) * 2 * std::max<int64_t>(total, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  return HIPR_SUCCESS;
+}
+
+hipr_result SortGather(Frame& f, const int32_t* d_list, int n) {
+  if (n == 0) return HIPR_SUCCESS;
+  hipr_context ctx = f.ctx;
+  hipLaunchKernelGGL(SortGatherKernel, dim3(n), dim3(kSortThreads), 0, f.st, d_list, n,
+                     ctx->geo.as<JobGeo>(), ctx->edge_base.as<int64_t>(),
+                     ctx->edges.as<EdgeRec>(), ctx->wedges.as<EdgeRec>(),
+                     ctx->gkeys.as<uint64_t>());
+  HIP_TRY(hipGetLastError());
+  return HIPR_SUCCESS;
+}
+
+hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_options* opt) {
+  hipr_context ctx = surf->ctx;
+  Frame f;
+  f.ctx = ctx;
+  f.pl = pl;
+  f.st = ctx->stream;
+  f.nj = (int)pl->jobs.size();
+  f.jobs = pl->jobs;
+  f.canvases = pl->canvases;
+  f.stats.paths = pl->n_paths;
+  f.stats.raster_paths = (int64_t)pl->walk_jobs.size();
+  hipStream_t st = f.st;
+  int nj = f.nj;
+  int mode = opt ? opt->mode : 0;
+  if (mode == 1 && (!opt->edge_estimates || !opt->tile_estimates)) return HIPR_ERROR_INVALID_ARGUMENT;
+  HIP_TRY(hipEventRecord(ctx->ev0, st));
+  if (nj == 0) {
+    HIP_TRY(hipEventRecord(ctx->ev1, st));
+    surf->stats = f.stats;
+    return HIPR_SUCCESS;
+  }
+  // canvases: the target and the mask-filter layers
+  f.canvases[0].pixels = surf->pixels;
+  f.canvases[0].width = surf->width;
+  f.canvases[0].height = surf->height;
+  if (pl->layer_bytes > 0) {
+    if (ctx->layers.Ensure(pl->layer_bytes) != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+    for (size_t c = 1; c < f.canvases.size(); c++)
+      f.canvases[c].pixels = ctx->layers.as<uint8_t>() + (intptr_t)pl->canvases[c].pixels;
+    HIP_TRY(hipMemsetAsync(ctx->layers.p, 0, pl->layer_bytes, st));
+  }
+  SetTiles(&f.canvases);
+  hipr_result r;
+  if ((r = Upload(&ctx->canvases, f.canvases.data(), f.canvases.size(), st)) != HIPR_SUCCESS) return r;
+  if ((r = HostStage(ctx, std::max<size_t>(nj, pl->stroke_jobs.size()))) != HIPR_SUCCESS) return r;
+  if (ctx->geo.Ensure(sizeof(JobGeo) * nj) != hipSuccess ||
+      ctx->cov.Ensure(sizeof(JobCov) * nj) != hipSuccess ||
+      ctx->edge_base.Ensure(sizeof(int64_t) * nj) != hipSuccess ||
+      ctx->edge_cap.Ensure(sizeof(int32_t) * nj) != hipSuccess ||
+      ctx->plane_base.Ensure(sizeof(int64_t) * nj) != hipSuccess ||
+      ctx->plane_cap.Ensure(sizeof(int32_t) * nj) != hipSuccess ||
+      ctx->map_base.Ensure(sizeof(int64_t) * nj) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  // ---- stage (a): stroke outlines, flatten, sort ----
+  if ((r = RunStrokes(f)) != HIPR_SUCCESS) return r;
+  // the prepared job table keeps the source ranges; stroke jobs are redirected
+  // to their outlines in a per-draw copy
+  f.d_jobs = pl->d_jobs.as<Job>();
+  if (!pl->stroke_jobs.empty()) {
+    if ((r = Upload(&ctx->run_jobs, f.jobs.data(), f.jobs.size(), st)) != HIPR_SUCCESS) return r;
+    f.d_jobs = ctx->run_jobs.as<Job>();
+  }
+  // per-job estimates are indexed by draw (path); spread them to jobs
+  std::vector<int32_t> eest, test;
+  if (mode == 1) {
+    eest.resize(nj);
+    test.resize(nj);
+    for (int j = 0; j < nj; j++) {
+      eest[j] = std::max(opt->edge_estimates[f.jobs[j].path], 1);
+      test[j] = std::max(opt->tile_estimates[f.jobs[j].path], 1);
+    }
+  }
+  std::vector<int64_t> ebase;
+  std::vector<int32_t> ecap;
+  if ((r = Flatten(f, mode, eest.data(), &ebase, &ecap)) != HIPR_SUCCESS) return r;
+  if (mode == 0) {
+    // (the count pass read the bounds back already)
+  }
+  int lds_width = 256;
+  std::vector<int64_t> mbase(nj + 1, 0);
+  for (int j = 0; j < nj; j++) {
+    int64_t msz = 0;
+    if (f.jobs[j].kind != kJobImage) {
+      PixBounds pb = JobPixBounds(ctx->h_geo[j]);
+      if (!pb.empty) lds_width = std::max(lds_width, pb.r - pb.l);
+      const Canvas& cv = f.canvases[f.jobs[j].canvas];
+      TileRange tr = JobTiles(pb, cv.width, cv.height);
+      msz = (int64_t)tr.ntc * tr.ntr;
+    }
+    mbase[j + 1] = mbase[j] + msz;
+  }
+  lds_width = (lds_width + 255) & ~255;
+  if (lds_width > 16384) return HIPR_ERROR_CAPACITY;  // 2 x 64 KiB row buffers
+  int64_t total_map = mbase[nj];
+  if (ctx->map0.Ensure(sizeof(int32_t) * std::max<int64_t>(total_map, 1)) != hipSuccess ||
+      ctx->map1.Ensure(sizeof(int32_t) * std::max<int64_t>(total_map, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  HIP_TRY(hipMemcpyAsync(ctx->map_base.p, mbase.data(), sizeof(int64_t) * nj,
+                         hipMemcpyHostToDevice, st));
+  int nwalk = (int)pl->walk_jobs.size();
+  if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk)) != HIPR_SUCCESS) return r;
+  WalkParams wp{};
+  wp.jobs = f.d_jobs;
+  wp.walk_list = pl->d_walk.as<int32_t>();
+  wp.n_walk = nwalk;
+  wp.geo = ctx->geo.as<JobGeo>();
+  wp.edge_base = ctx->edge_base.as<int64_t>();
+  wp.wedges = ctx->wedges.as<EdgeRec>();
+  wp.canvases = ctx->canvases.as<Canvas>();
+  wp.cov = ctx->cov.as<JobCov>();
+  wp.lds_width = lds_width;
+  wp.map_base = ctx->map_base.as<int64_t>();
+  // ---- stage (b): walk ----
+  std::vector<int64_t> pbase(nj + 1, 0);
+  std::vector<int32_t> pcap(nj, 0);
+  auto set_planes = [&]() -> hipr_result {
+    for (int j = 0; j < nj; j++) pbase[j + 1] = pbase[j] + pcap[j];
+    if (ctx->planes.Ensure((size_t)kTilePlaneBytes * std::max<int64_t>(pbase[nj], 1)) != hipSuccess)
+      return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+    HIP_TRY(hipMemcpyAsync(ctx->plane_base.p, pbase.data(), sizeof(int64_t) * nj,
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->plane_cap.p, pcap.data(), sizeof(int32_t) * nj,
+                           hipMemcpyHostToDevice, st));
+    return HIPR_SUCCESS;
+  };
+  auto check_limits = [&]() -> hipr_result {
+    for (int j = 0; j < nj; j++) {
+      if (f.jobs[j].kind == kJobImage) continue;
+      if (ctx->h_cov[j].flags & (2 | 16 | 32)) {
+        std::fprintf(stderr, "skity_hip: draw %d exceeds a device limit (flags %d)\n",
+                     f.jobs[j].path, ctx->h_cov[j].flags);
+        return HIPR_ERROR_CAPACITY;
+      }
+    }
+    return HIPR_SUCCESS;
+  };
+  if (mode == 0) {
+    if ((r = LaunchWalk<4, 0>(wp, st)) != HIPR_SUCCESS) return r;
+    HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if ((r = check_limits()) != HIPR_SUCCESS) return r;
+    for (int j = 0; j < nj; j++) pcap[j] = f.jobs[j].kind == kJobImage ? 0 : ctx->h_cov[j].planes;
+    if ((r = set_planes()) != HIPR_SUCCESS) return r;
+    if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk)) != HIPR_SUCCESS) return r;
+  } else {
+    for (int j = 0; j < nj; j++) pcap[j] = f.jobs[j].kind == kJobImage ? 0 : test[j];
+    if ((r = set_planes()) != HIPR_SUCCESS) return r;
+  }
+  wp.plane_base = ctx->plane_base.as<int64_t>();
+  wp.plane_cap = ctx->plane_cap.as<int32_t>();
+  wp.planes = ctx->planes.as<uint8_t>();
+  wp.map0 = ctx->map0.as<int32_t>();
+  wp.map1 = ctx->map1.as<int32_t>();
+  if ((r = LaunchWalk<4, 1>(wp, st)) != HIPR_SUCCESS) return r;
+  HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if ((r = check_limits()) != HIPR_SUCCESS) return r;
+  if (mode == 1) {
+    std::vector<int32_t> redo;
+    for (int j = 0; j < nj; j++)
+      if (f.jobs[j].kind != kJobImage && ctx->h_cov[j].planes > pcap[j]) redo.push_back(j);
+    if (!redo.empty()) {
+      // exact fallback for the under-estimated draws: their write pass
+      // counted every plane; give them exactly that and walk them again.
+      f.stats.fallbacks += (int64_t)redo.size();
+      for (int j : redo) pcap[j] = ctx->h_cov[j].planes;
+      if ((r = set_planes()) != HIPR_SUCCESS) return r;
+      if ((r = Upload(&ctx->sub_list, redo.data(), redo.size(), st)) != HIPR_SUCCESS) return r;
+      // every job's planes moved: re-walk all jobs whose base changed is
+      // equivalent to re-walking all; keep it simple and exact.
+      if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk)) != HIPR_SUCCESS) return r;
+      wp.planes = ctx->planes.as<uint8_t>();
+      if ((r = LaunchWalk<4, 1>(wp, st)) != HIPR_SUCCESS) return r;
+      HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+  }
+  int64_t max_active = 0;
+  for (int j = 0; j < nj; j++) {
+    if (f.jobs[j].kind == kJobImage) continue;
+    f.stats.tiles += ctx->h_cov[j].tiles;
+    f.stats.tile_planes += ctx->h_cov[j].planes;
+    max_active = std::max<int64_t>(max_active, ctx->h_cov[j].max_active);
+  }
+  f.stats.max_active = max_active;
+  // ---- composite: layers, blur, canvas ----
+  CompositeParams cp{};
+  cp.n_canvases = (int)f.canvases.size();
+  cp.jobs = f.d_jobs;
+  cp.cov = ctx->cov.as<JobCov>();
+  cp.map_base = ctx->map_base.as<int64_t>();
+  cp.map0 = ctx->map0.as<int32_t>();
+  cp.map1 = ctx->map1.as<int32_t>();
+  cp.plane_base = ctx->plane_base.as<int64_t>();
+  cp.planes = ctx->planes.as<uint8_t>();
+  cp.paints = pl->d_paints.as<PaintDev>();
+  cp.grads = pl->d_grads.as<GradDev>();
+  int main_tiles = f.canvases[0].tiles_x * f.canvases[0].tiles_y;
+  if (f.canvases.size() > 1) {
+    std::vector<Canvas> lc(f.canvases.begin() + 1, f.canvases.end());
+    int layer_tiles = 0;
+    for (auto& c : lc) {
+      c.tile_begin -= main_tiles;
+      layer_tiles += c.tiles_x * c.tiles_y;
+    }
+    if ((r = Upload(&ctx->layer_canvases, lc.data(), lc.size(), st)) != HIPR_SUCCESS) return r;
+    CompositeParams lp = cp;
+    lp.canvases = ctx->layer_canvases.as<Canvas>();
+    lp.n_canvases = (int)lc.size();
+    lp.total_tiles = layer_tiles;
+    hipLaunchKernelGGL(CompositeKernel, dim3(layer_tiles), dim3(64), 0, st, lp);
+    HIP_TRY(hipGetLastError());
+    // StackBlur each layer: rows into a temporary, columns back
+    std::vector<BlurJob> rows, cols;
+    std::vector<int64_t> rb{0}, cb{0};
+    int64_t tmp_bytes = 0;
+    for (auto& b : pl->blurs) tmp_bytes += (int64_t)b.width * b.height * 4;
+    if (ctx->blur_tmp.Ensure(std::max<int64_t>(tmp_bytes, 1)) != hipSuccess)
+      return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+    int64_t off = 0;
+    for (size_t b = 0; b < pl->blurs.size(); b++) {
+      BlurJob bj = pl->blurs[b];
+      uint8_t* layer_px = f.canvases[pl->layer_of_blur[b]].pixels;
+      uint8_t* tmp = ctx->blur_tmp.as<uint8_t>() + off;
+      off += (int64_t)bj.width * bj.height * 4;
+      if (bj.radius <= 1) continue;  // SWStackBlur copies for radius <= 1
+      BlurJob h = bj, v = bj;
+      h.src = layer_px;
+      h.dst = tmp;
+      v.src = tmp;
+      v.dst = layer_px;
+      rows.push_back(h);
+      cols.push_back(v);
+      rb.push_back(rb.back() + bj.height);
+      cb.push_back(cb.back() + bj.width);
+    }
+    if (!rows.empty()) {
+      if ((r = Upload(&ctx->blur_rows, rows.data(), rows.size(), st)) != HIPR_SUCCESS) return r;
+      if ((r = Upload(&ctx->blur_cols, cols.data(), cols.size(), st)) != HIPR_SUCCESS) return r;
+      if ((r = Upload(&ctx->blur_rb, rb.data(), rb.size(), st)) != HIPR_SUCCESS) return r;
+      if ((r = Upload(&ctx->blur_cb, cb.data(), cb.size(), st)) != HIPR_SUCCESS) return r;
+      int64_t nl = rb.back(), nc = cb.back();
+      hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st,
+                         ctx->blur_rows.as<BlurJob>(), ctx->blur_rb.as<int64_t>(),
+                         (int)rows.size(), nl, 0);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st,
+                         ctx->blur_cols.as<BlurJob>(), ctx->blur_cb.as<int64_t>(),
+                         (int)cols.size(), nc, 1);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  CompositeParams mp = cp;
+  mp.canvases = ctx->canvases.as<Canvas>();
+  mp.n_canvases = 1;
+  mp.total_tiles = main_tiles;
+  hipLaunchKernelGGL(CompositeKernel, dim3(main_tiles), dim3(64), 0, st, mp);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev1, st));
+  HIP_TRY(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  f.stats.ms_total = ms;
+  surf->stats = f.stats;
+  return HIPR_SUCCESS;
+}
+
+hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s** out) {
+  hipr_result vr = ValidateScene(scene);
+  if (vr != HIPR_SUCCESS) return vr;
+  hipr_prepared_s* pl = new (std::nothrow) hipr_prepared_s();
+  if (!pl) return HIPR_ERROR_OUT_OF_HOST_MEMORY;
+  pl->ctx = ctx;
+  pl->n_paths = scene->n_paths;
+  hipStream_t st = ctx->stream;
+  hipr_result r = HIPR_SUCCESS;
+  if (scene->n_paths > 0) {
+    BuildPlan(scene, pl);
+    int n = scene->n_paths;
+    int nv = scene->verb_offsets[n], np = scene->point_offsets[n], nw = scene->weight_offsets[n];
+    if ((r = Upload(&pl->verbs, scene->verbs, nv, st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->pts, scene->points, 2 * (size_t)np, st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->weights, scene->weights, nw, st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->scene_paints, scene->paints, n, st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->d_jobs, pl->jobs.data(), pl->jobs.size(), st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->d_paints, pl->paints.data(), pl->paints.size(), st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->d_grads, pl->grads.data(), pl->grads.size(), st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->d_walk, pl->walk_jobs.data(), pl->walk_jobs.size(), st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->d_stroke, pl->stroke_jobs.data(), pl->stroke_jobs.size(), st)) != HIPR_SUCCESS) {
+      delete pl;
+      return r;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+      delete pl;
+      return HIPR_ERROR_DEVICE;
+    }
+  }
+  *out = pl;
   return HIPR_SUCCESS;
 }
 
@@ -516,20 +1000,23 @@ HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out) {
 
 HIPR_API void hipr_context_destroy(hipr_context ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  DevBuf* bufs[] = {&ctx->verbs, &ctx->pts, &ctx->weights, &ctx->jobs, &ctx->geo,
-                    &ctx->cov, &ctx->canvases, &ctx->paints, &ctx->grads, &ctx->edge_base,
-                    &ctx->edge_cap, &ctx->edges, &ctx->wedges, &ctx->gkeys,
-                    &ctx->plane_base, &ctx->plane_cap, &ctx->planes, &ctx->map_base,
-                    &ctx->map0, &ctx->map1, &ctx->walk_list, &ctx->layers,
-                    &ctx->blur_jobs, &ctx->blur_lines, &ctx->blur_tmp};
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  DevBuf* bufs[] = {&ctx->canvases, &ctx->layer_canvases, &ctx->geo, &ctx->cov, &ctx->edge_base,
+                    &ctx->edge_cap, &ctx->edges, &ctx->wedges, &ctx->gkeys, &ctx->plane_base,
+                    &ctx->plane_cap, &ctx->planes, &ctx->map_base, &ctx->map0, &ctx->map1,
+                    &ctx->sub_list, &ctx->stroke_out, &ctx->o_verbs, &ctx->o_pts, &ctx->o_w,
+                    &ctx->s_verbs, &ctx->s_pts, &ctx->s_w, &ctx->layers, &ctx->blur_rows,
+                    &ctx->blur_cols, &ctx->blur_rb, &ctx->blur_cb, &ctx->blur_tmp,
+                    &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
+                    &ctx->d_jobs, &ctx->run_jobs};
   for (DevBuf* b : bufs) b->Free();
-  if (ctx->h_geo) hipHostFree(ctx->h_geo);
-  if (ctx->h_cov) hipHostFree(ctx->h_cov);
-  hipEventDestroy(ctx->ev0);
-  hipEventDestroy(ctx->ev1);
-  hipStreamDestroy(ctx->stream);
+  if (ctx->h_geo) (void)hipHostFree(ctx->h_geo);
+  if (ctx->h_cov) (void)hipHostFree(ctx->h_cov);
+  if (ctx->h_so) (void)hipHostFree(ctx->h_so);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
@@ -544,8 +1031,8 @@ HIPR_API hipr_result hipr_surface_create(hipr_context ctx, int32_t width, int32_
   s->ctx = ctx;
   s->width = width;
   s->height = height;
-  HIP_TRY(hipSetDevice(ctx->device));
-  if (hipMalloc(&s->pixels, (size_t)width * height * 4) != hipSuccess) {
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipMalloc(&s->pixels, (size_t)width * height * 4) != hipSuccess) {
     delete s;
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   }
@@ -557,8 +1044,8 @@ HIPR_API hipr_result hipr_surface_create(hipr_context ctx, int32_t width, int32_
 
 HIPR_API void hipr_surface_destroy(hipr_surface s) {
   if (!s) return;
-  hipStreamSynchronize(s->ctx->stream);
-  hipFree(s->pixels);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  (void)hipFree(s->pixels);
   delete s;
 }
 
@@ -582,8 +1069,7 @@ HIPR_API hipr_result hipr_surface_read_pixels(hipr_surface s, uint8_t* dst, size
   return HIPR_SUCCESS;
 }
 
-HIPR_API hipr_result hipr_surface_write_pixels(hipr_surface s, const uint8_t* src,
-                                               size_t bytes) {
+HIPR_API hipr_result hipr_surface_write_pixels(hipr_surface s, const uint8_t* src, size_t bytes) {
   if (!s) return HIPR_ERROR_INVALID_HANDLE;
   if (!src || bytes < (size_t)s->width * s->height * 4) return HIPR_ERROR_INVALID_ARGUMENT;
   HIP_TRY(hipMemcpyAsync(s->pixels, src, (size_t)s->width * s->height * 4,
@@ -605,264 +1091,39 @@ HIPR_API hipr_result hipr_surface_get_stats(hipr_surface s, hipr_stats* out) {
   return HIPR_SUCCESS;
 }
 
+HIPR_API hipr_result hipr_scene_prepare(hipr_context ctx, const hipr_scene* scene,
+                                        hipr_prepared* out) {
+  if (!out) return HIPR_ERROR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  HIP_TRY(hipSetDevice(ctx->device));
+  return Prepare(ctx, scene, out);
+}
+
+HIPR_API void hipr_prepared_destroy(hipr_prepared p) {
+  if (!p) return;
+  (void)hipStreamSynchronize(p->ctx->stream);
+  delete p;
+}
+
+HIPR_API hipr_result hipr_surface_draw_prepared(hipr_surface surf, hipr_prepared p,
+                                                const hipr_options* options) {
+  if (!surf || !p) return HIPR_ERROR_INVALID_HANDLE;
+  if (p->ctx != surf->ctx) return HIPR_ERROR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(surf->ctx->device));
+  return RunPipeline(surf, p, options);
+}
+
 HIPR_API hipr_result hipr_surface_draw_scene(hipr_surface surf, const hipr_scene* scene,
                                              const hipr_options* options) {
   if (!surf) return HIPR_ERROR_INVALID_HANDLE;
-  hipr_result vr = ValidateScene(scene);
-  if (vr != HIPR_SUCCESS) return vr;
-  hipr_context ctx = surf->ctx;
-  hipStream_t st = ctx->stream;
-  HIP_TRY(hipSetDevice(ctx->device));
-  hipr_stats stats{};
-  stats.paths = scene->n_paths;
-  if (scene->n_paths == 0) {
-    surf->stats = stats;
-    return HIPR_SUCCESS;
-  }
-  HIP_TRY(hipEventRecord(ctx->ev0, st));
-  Plan pl;
-  BuildPlan(scene, surf, &pl);
-  int nj = (int)pl.jobs.size();
-  int nwalk = (int)pl.walk_jobs.size();
-  stats.raster_paths = nwalk;
-  // ---- upload scene and plan ----
-  int nv = scene->verb_offsets[scene->n_paths];
-  int np = scene->point_offsets[scene->n_paths];
-  int nw = scene->weight_offsets[scene->n_paths];
-  hipr_result r;
-  if ((r = Upload(&ctx->verbs, scene->verbs, nv, st)) != HIPR_SUCCESS) return r;
-  if ((r = Upload(&ctx->pts, scene->points, 2 * (size_t)np, st)) != HIPR_SUCCESS) return r;
-  if ((r = Upload(&ctx->weights, scene->weights, nw, st)) != HIPR_SUCCESS) return r;
-  // layer pixel storage
-  if (pl.layer_bytes > 0) {
-    if (ctx->layers.Ensure(pl.layer_bytes) != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-    for (size_t c = 1; c < pl.canvases.size(); c++)
-      pl.canvases[c].pixels = ctx->layers.as<uint8_t>() + (intptr_t)pl.canvases[c].pixels;
-    HIP_TRY(hipMemsetAsync(ctx->layers.p, 0, pl.layer_bytes, st));
-  }
-  if ((r = Upload(&ctx->jobs, pl.jobs.data(), nj, st)) != HIPR_SUCCESS) return r;
-  if ((r = Upload(&ctx->canvases, pl.canvases.data(), pl.canvases.size(), st)) != HIPR_SUCCESS)
-    return r;
-  if ((r = Upload(&ctx->paints, pl.paints.data(), pl.paints.size(), st)) != HIPR_SUCCESS)
-    return r;
-  if ((r = Upload(&ctx->grads, pl.grads.data(), pl.grads.size(), st)) != HIPR_SUCCESS)
-    return r;
-  if ((r = Upload(&ctx->walk_list, pl.walk_jobs.data(), nwalk, st)) != HIPR_SUCCESS) return r;
-  if (ctx->geo.Ensure(sizeof(JobGeo) * std::max(nj, 1)) != hipSuccess ||
-      ctx->cov.Ensure(sizeof(JobCov) * std::max(nj, 1)) != hipSuccess ||
-      ctx->edge_base.Ensure(sizeof(int64_t) * std::max(nj, 1)) != hipSuccess ||
-      ctx->edge_cap.Ensure(sizeof(int32_t) * std::max(nj, 1)) != hipSuccess ||
-      ctx->plane_base.Ensure(sizeof(int64_t) * std::max(nj, 1)) != hipSuccess ||
-      ctx->plane_cap.Ensure(sizeof(int32_t) * std::max(nj, 1)) != hipSuccess ||
-      ctx->map_base.Ensure(sizeof(int64_t) * std::max(nj, 1)) != hipSuccess)
-    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-  if ((r = HostStage(ctx, nj)) != HIPR_SUCCESS) return r;
-  GeomSrc gs{ctx->verbs.as<uint8_t>(), ctx->pts.as<float>(), ctx->weights.as<float>()};
-  GeomSrc go = gs;  // stroke outlines: not produced yet
-
-  // ---- stage (a): flatten, count pass ----
-  if (nj > 0) {
-    hipLaunchKernelGGL((FlattenKernel<0>), dim3(nj), dim3(64), 0, st, ctx->jobs.as<Job>(), nj,
-                       gs, go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                       (EdgeRec*)nullptr, ctx->geo.as<JobGeo>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * nj, hipMemcpyDeviceToHost,
-                           st));
-    HIP_TRY(hipStreamSynchronize(st));
-  }
-  std::vector<int64_t> ebase(nj + 1, 0);
-  std::vector<int32_t> ecap(nj, 0);
-  int lds_width = 256;
-  for (int j = 0; j < nj; j++) {
-    ecap[j] = ctx->h_geo[j].edges;
-    ebase[j + 1] = ebase[j] + ecap[j];
-    if (pl.jobs[j].kind != kJobImage) {
-      PixBounds pb = JobPixBounds(ctx->h_geo[j]);
-      if (!pb.empty) lds_width = std::max(lds_width, pb.r - pb.l);
-    }
-  }
-  lds_width = (lds_width + 255) & ~255;
-  int64_t total_edges = ebase[nj];
-  stats.edges = total_edges;
-  if (ctx->edges.Ensure(sizeof(EdgeRec) * std::max<int64_t>(total_edges, 1)) != hipSuccess ||
-      ctx->wedges.Ensure(sizeof(EdgeRec) * std::max<int64_t>(total_edges, 1)) != hipSuccess ||
-      ctx->gkeys.Ensure(sizeof(uint64_t) * 2 * std::max<int64_t>(total_edges, 1)) != hipSuccess)
-    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-  HIP_TRY(hipMemcpyAsync(ctx->edge_base.p, ebase.data(), sizeof(int64_t) * nj,
-                         hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ctx->edge_cap.p, ecap.data(), sizeof(int32_t) * nj,
-                         hipMemcpyHostToDevice, st));
-  if (nj > 0) {
-    hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, ctx->jobs.as<Job>(), nj,
-                       gs, go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                       ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>());
-    HIP_TRY(hipGetLastError());
-  }
-  // ---- sort + gather ----
-  auto sort_gather = [&]() -> hipr_result {
-    if (nwalk == 0) return HIPR_SUCCESS;
-    hipLaunchKernelGGL(SortGatherKernel, dim3(nwalk), dim3(kSortThreads), 0, st,
-                       ctx->walk_list.as<int32_t>(), nwalk, ctx->geo.as<JobGeo>(),
-                       ctx->edge_base.as<int64_t>(), ctx->edges.as<EdgeRec>(),
-                       ctx->wedges.as<EdgeRec>(), ctx->gkeys.as<uint64_t>());
-    HIP_TRY(hipGetLastError());
-    return HIPR_SUCCESS;
-  };
-  if ((r = sort_gather()) != HIPR_SUCCESS) return r;
-  // ---- stage (b): walk, count pass ----
-  WalkParams wp{};
-  wp.jobs = ctx->jobs.as<Job>();
-  wp.walk_list = ctx->walk_list.as<int32_t>();
-  wp.n_walk = nwalk;
-  wp.geo = ctx->geo.as<JobGeo>();
-  wp.edge_base = ctx->edge_base.as<int64_t>();
-  wp.wedges = ctx->wedges.as<EdgeRec>();
-  wp.canvases = ctx->canvases.as<Canvas>();
-  wp.cov = ctx->cov.as<JobCov>();
-  wp.lds_width = lds_width;
-  if ((r = LaunchWalk<4, 0>(wp, lds_width, st)) != HIPR_SUCCESS) return r;
-  if (nj > 0) {
-    HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost,
-                           st));
-    HIP_TRY(hipStreamSynchronize(st));
-  }
-  std::vector<int64_t> pbase(nj + 1, 0), mbase(nj + 1, 0);
-  std::vector<int32_t> pcap(nj, 0);
-  int64_t max_active = 0;
-  for (int j = 0; j < nj; j++) {
-    const JobCov& c = ctx->h_cov[j];
-    bool walked = pl.jobs[j].kind != kJobImage;
-    if (walked && (c.flags & (2 | 16 | 32))) {
-      std::fprintf(stderr, "skity_hip: job %d exceeds a device limit (flags %d)\n", j,
-                   c.flags);
-      return HIPR_ERROR_CAPACITY;
-    }
-    pcap[j] = walked ? c.planes : 0;
-    pbase[j + 1] = pbase[j] + pcap[j];
-    int64_t msz = 0;
-    if (walked) {
-      const Canvas& cv = pl.canvases[pl.jobs[j].canvas];
-      PixBounds pb{c.left, c.top, c.right, c.bottom, !(c.left < c.right && c.top < c.bottom)};
-      TileRange tr = JobTiles(pb, cv.width, cv.height);
-      msz = (int64_t)tr.ntc * tr.ntr;
-    }
-    mbase[j + 1] = mbase[j] + msz;
-    stats.tiles += walked ? c.tiles : 0;
-    stats.tile_planes += pcap[j];
-    max_active = std::max<int64_t>(max_active, c.max_active);
-  }
-  stats.max_active = max_active;
-  int64_t total_planes = pbase[nj], total_map = mbase[nj];
-  if (ctx->planes.Ensure((size_t)kTilePlaneBytes * std::max<int64_t>(total_planes, 1)) !=
-          hipSuccess ||
-      ctx->map0.Ensure(sizeof(int32_t) * std::max<int64_t>(total_map, 1)) != hipSuccess ||
-      ctx->map1.Ensure(sizeof(int32_t) * std::max<int64_t>(total_map, 1)) != hipSuccess)
-    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-  HIP_TRY(hipMemsetAsync(ctx->map0.p, 0xFF, sizeof(int32_t) * std::max<int64_t>(total_map, 1), st));
-  HIP_TRY(hipMemsetAsync(ctx->map1.p, 0xFF, sizeof(int32_t) * std::max<int64_t>(total_map, 1), st));
-  HIP_TRY(hipMemcpyAsync(ctx->plane_base.p, pbase.data(), sizeof(int64_t) * nj,
-                         hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ctx->plane_cap.p, pcap.data(), sizeof(int32_t) * nj,
-                         hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ctx->map_base.p, mbase.data(), sizeof(int64_t) * nj,
-                         hipMemcpyHostToDevice, st));
-  // ---- stage (b): walk, write pass (on a fresh copy of the sorted edges) ----
-  if ((r = sort_gather()) != HIPR_SUCCESS) return r;
-  wp.plane_base = ctx->plane_base.as<int64_t>();
-  wp.plane_cap = ctx->plane_cap.as<int32_t>();
-  wp.planes = ctx->planes.as<uint8_t>();
-  wp.map_base = ctx->map_base.as<int64_t>();
-  wp.map0 = ctx->map0.as<int32_t>();
-  wp.map1 = ctx->map1.as<int32_t>();
-  if ((r = LaunchWalk<4, 1>(wp, lds_width, st)) != HIPR_SUCCESS) return r;
-  // ---- composite: layers, blur, canvas ----
-  CompositeParams cp{};
-  cp.canvases = ctx->canvases.as<Canvas>();
-  cp.n_canvases = (int)pl.canvases.size();
-  cp.jobs = ctx->jobs.as<Job>();
-  cp.cov = ctx->cov.as<JobCov>();
-  cp.map_base = ctx->map_base.as<int64_t>();
-  cp.map0 = ctx->map0.as<int32_t>();
-  cp.map1 = ctx->map1.as<int32_t>();
-  cp.plane_base = ctx->plane_base.as<int64_t>();
-  cp.planes = ctx->planes.as<uint8_t>();
-  cp.paints = ctx->paints.as<PaintDev>();
-  cp.grads = ctx->grads.as<GradDev>();
-  int main_tiles = pl.canvases[0].tiles_x * pl.canvases[0].tiles_y;
-  int total_tiles = 0;
-  for (auto& cv : pl.canvases) total_tiles += cv.tiles_x * cv.tiles_y;
-  if (pl.canvases.size() > 1) {
-    // layer tiles come after the main canvas's tiles in tile numbering;
-    // composite them first by launching over the layer range only.
-    CompositeParams lp = cp;
-    lp.canvases = ctx->canvases.as<Canvas>() + 1;
-    lp.n_canvases = cp.n_canvases - 1;
-    // re-base: the layer canvases' tile_begin starts at main_tiles
-    std::vector<Canvas> lc(pl.canvases.begin() + 1, pl.canvases.end());
-    for (auto& c : lc) c.tile_begin -= main_tiles;
-    DevBuf& tmpc = ctx->blur_jobs;  // reuse as a small scratch for layer canvases
-    (void)tmpc;
-    static thread_local DevBuf layer_canvases;
-    if ((r = Upload(&layer_canvases, lc.data(), lc.size(), st)) != HIPR_SUCCESS) return r;
-    lp.canvases = layer_canvases.as<Canvas>();
-    lp.total_tiles = total_tiles - main_tiles;
-    hipLaunchKernelGGL(CompositeKernel, dim3(lp.total_tiles), dim3(64), 0, st, lp);
-    HIP_TRY(hipGetLastError());
-    // blur each layer: rows into tmp, then columns back into the layer
-    std::vector<BlurJob> rows, cols;
-    std::vector<int64_t> rb{0}, cb{0};
-    int64_t tmp_bytes = 0;
-    for (size_t b = 0; b < pl.blurs.size(); b++) tmp_bytes += (int64_t)pl.blurs[b].width * pl.blurs[b].height * 4;
-    if (ctx->blur_tmp.Ensure(std::max<int64_t>(tmp_bytes, 1)) != hipSuccess)
-      return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-    int64_t off = 0;
-    std::vector<std::pair<uint8_t*, int64_t>> copies;
-    for (size_t b = 0; b < pl.blurs.size(); b++) {
-      BlurJob bj = pl.blurs[b];
-      uint8_t* layer_px = pl.canvases[pl.layer_of_blur[b]].pixels;
-      uint8_t* tmp = ctx->blur_tmp.as<uint8_t>() + off;
-      off += (int64_t)bj.width * bj.height * 4;
-      if (bj.radius <= 1) continue;  // SWStackBlur copies for radius <= 1
-      BlurJob h = bj, v = bj;
-      h.src = layer_px;
-      h.dst = tmp;
-      v.src = tmp;
-      v.dst = layer_px;
-      rows.push_back(h);
-      cols.push_back(v);
-      rb.push_back(rb.back() + bj.height);
-      cb.push_back(cb.back() + bj.width);
-    }
-    if (!rows.empty()) {
-      static thread_local DevBuf d_rows, d_cols, d_rb, d_cb;
-      if ((r = Upload(&d_rows, rows.data(), rows.size(), st)) != HIPR_SUCCESS) return r;
-      if ((r = Upload(&d_cols, cols.data(), cols.size(), st)) != HIPR_SUCCESS) return r;
-      if ((r = Upload(&d_rb, rb.data(), rb.size(), st)) != HIPR_SUCCESS) return r;
-      if ((r = Upload(&d_cb, cb.data(), cb.size(), st)) != HIPR_SUCCESS) return r;
-      int64_t nl = rb.back();
-      hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0,
-                         st, d_rows.as<BlurJob>(), d_rb.as<int64_t>(), (int)rows.size(), nl, 0);
-      HIP_TRY(hipGetLastError());
-      int64_t nc = cb.back();
-      hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0,
-                         st, d_cols.as<BlurJob>(), d_cb.as<int64_t>(), (int)cols.size(), nc, 1);
-      HIP_TRY(hipGetLastError());
-    }
-  }
-  {
-    CompositeParams mp = cp;
-    mp.n_canvases = 1;
-    mp.total_tiles = main_tiles;
-    hipLaunchKernelGGL(CompositeKernel, dim3(main_tiles), dim3(64), 0, st, mp);
-    HIP_TRY(hipGetLastError());
-  }
-  HIP_TRY(hipEventRecord(ctx->ev1, st));
-  HIP_TRY(hipEventSynchronize(ctx->ev1));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  stats.ms_total = ms;
-  surf->stats = stats;
-  return HIPR_SUCCESS;
+  hipr_prepared p = nullptr;
+  HIP_TRY(hipSetDevice(surf->ctx->device));
+  hipr_result r = Prepare(surf->ctx, scene, &p);
+  if (r != HIPR_SUCCESS) return r;
+  r = RunPipeline(surf, p, options);
+  hipr_prepared_destroy(p);
+  return r;
 }
 
 HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx, const hipr_scene* scene,
@@ -875,23 +1136,32 @@ HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx, const hipr_scene* 
   hipStream_t st = ctx->stream;
   HIP_TRY(hipSetDevice(ctx->device));
   int n = scene->n_paths;
-  std::vector<Job> jobs;
-  for (int p = 0; p < n; p++) AddShapeJobs(scene, p, 0, 0.f, 0.f, 0, &jobs);
+  std::vector<Job> jobs(n);
+  for (int p = 0; p < n; p++) {
+    Job jb{};
+    jb.path = p;
+    jb.kind = kJobFill;
+    jb.verb_begin = scene->verb_offsets[p];
+    jb.verb_end = scene->verb_offsets[p + 1];
+    jb.pt_begin = scene->point_offsets[p];
+    jb.w_begin = scene->weight_offsets[p];
+    jobs[p] = jb;
+  }
   int nv = n ? scene->verb_offsets[n] : 0, np = n ? scene->point_offsets[n] : 0;
   int nw = n ? scene->weight_offsets[n] : 0;
   hipr_result r;
-  if ((r = Upload(&ctx->verbs, scene->verbs, nv, st)) != HIPR_SUCCESS) return r;
-  if ((r = Upload(&ctx->pts, scene->points, 2 * (size_t)np, st)) != HIPR_SUCCESS) return r;
-  if ((r = Upload(&ctx->weights, scene->weights, nw, st)) != HIPR_SUCCESS) return r;
-  if ((r = Upload(&ctx->jobs, jobs.data(), jobs.size(), st)) != HIPR_SUCCESS) return r;
+  if ((r = Upload(&ctx->d_verbs, scene->verbs, nv, st)) != HIPR_SUCCESS) return r;
+  if ((r = Upload(&ctx->d_pts, scene->points, 2 * (size_t)np, st)) != HIPR_SUCCESS) return r;
+  if ((r = Upload(&ctx->d_w, scene->weights, nw, st)) != HIPR_SUCCESS) return r;
+  if ((r = Upload(&ctx->d_jobs, jobs.data(), jobs.size(), st)) != HIPR_SUCCESS) return r;
   if (ctx->geo.Ensure(sizeof(JobGeo) * std::max(n, 1)) != hipSuccess ||
       ctx->edge_base.Ensure(sizeof(int64_t) * std::max(n, 1)) != hipSuccess ||
       ctx->edge_cap.Ensure(sizeof(int32_t) * std::max(n, 1)) != hipSuccess)
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   if ((r = HostStage(ctx, n)) != HIPR_SUCCESS) return r;
-  GeomSrc gs{ctx->verbs.as<uint8_t>(), ctx->pts.as<float>(), ctx->weights.as<float>()};
+  GeomSrc gs{ctx->d_verbs.as<uint8_t>(), ctx->d_pts.as<float>(), ctx->d_w.as<float>()};
   if (n > 0) {
-    hipLaunchKernelGGL((FlattenKernel<0>), dim3(n), dim3(64), 0, st, ctx->jobs.as<Job>(), n, gs,
+    hipLaunchKernelGGL((FlattenKernel<0>), dim3(n), dim3(64), 0, st, ctx->d_jobs.as<Job>(), n, gs,
                        gs, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
                        (EdgeRec*)nullptr, ctx->geo.as<JobGeo>());
     HIP_TRY(hipGetLastError());
@@ -912,7 +1182,7 @@ HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx, const hipr_scene* 
   HIP_TRY(hipMemcpyAsync(ctx->edge_base.p, base.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ctx->edge_cap.p, cap.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
   if (n > 0) {
-    hipLaunchKernelGGL((FlattenKernel<1>), dim3(n), dim3(64), 0, st, ctx->jobs.as<Job>(), n, gs,
+    hipLaunchKernelGGL((FlattenKernel<1>), dim3(n), dim3(64), 0, st, ctx->d_jobs.as<Job>(), n, gs,
                        gs, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
                        ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>());
     HIP_TRY(hipGetLastError());
@@ -936,19 +1206,17 @@ HIPR_API hipr_result hipr_stackblur(hipr_context ctx, const uint8_t* src_host, u
   uint8_t* b = a + bytes;
   HIP_TRY(hipMemcpyAsync(a, src_host, bytes, hipMemcpyHostToDevice, st));
   if (radius > 1) {
-    BlurJob h{a, b, width, height, radius}, v{b, a, width, height, radius};
+    BlurJob jobs2[2] = {{a, b, width, height, radius}, {b, a, width, height, radius}};
+    int64_t lb[1] = {0};
     hipr_result r;
-    int64_t lb[2] = {0, 0};
-    if ((r = Upload(&ctx->blur_jobs, &h, 1, st)) != HIPR_SUCCESS) return r;
+    if ((r = Upload(&ctx->blur_jobs, jobs2, 2, st)) != HIPR_SUCCESS) return r;
     if ((r = Upload(&ctx->blur_lines, lb, 1, st)) != HIPR_SUCCESS) return r;
     hipLaunchKernelGGL(StackBlurLineKernel, dim3((height + 255) / 256), dim3(256), 0, st,
                        ctx->blur_jobs.as<BlurJob>(), ctx->blur_lines.as<int64_t>(), 1,
                        (int64_t)height, 0);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
-    if ((r = Upload(&ctx->blur_jobs, &v, 1, st)) != HIPR_SUCCESS) return r;
     hipLaunchKernelGGL(StackBlurLineKernel, dim3((width + 255) / 256), dim3(256), 0, st,
-                       ctx->blur_jobs.as<BlurJob>(), ctx->blur_lines.as<int64_t>(), 1,
+                       ctx->blur_jobs.as<BlurJob>() + 1, ctx->blur_lines.as<int64_t>(), 1,
                        (int64_t)width, 1);
     HIP_TRY(hipGetLastError());
   }

@@ -41,6 +41,13 @@ def lib():
         L.hipr_surface_finish.restype = ctypes.c_int
         L.hipr_surface_get_stats.argtypes = [c_surf, ctypes.POINTER(abi.Stats)]
         L.hipr_surface_get_stats.restype = ctypes.c_int
+        L.hipr_scene_prepare.argtypes = [c_ctx, SP, ctypes.POINTER(ctypes.c_void_p)]
+        L.hipr_scene_prepare.restype = ctypes.c_int
+        L.hipr_prepared_destroy.argtypes = [ctypes.c_void_p]
+        L.hipr_prepared_destroy.restype = None
+        L.hipr_surface_draw_prepared.argtypes = [c_surf, ctypes.c_void_p,
+                                                 ctypes.POINTER(abi.Options)]
+        L.hipr_surface_draw_prepared.restype = ctypes.c_int
         L.hipr_debug_build_edges.argtypes = [c_ctx, SP, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_int64]
         L.hipr_debug_build_edges.restype = ctypes.c_int
@@ -102,6 +109,46 @@ class Context:
             pass
 
 
+def _options(mode, edge_estimates, tile_estimates):
+    opt = abi.Options()
+    opt.mode = mode
+    keep = []
+    if edge_estimates is not None:
+        e = np.ascontiguousarray(edge_estimates, np.int32)
+        keep.append(e)
+        opt.edge_estimates = e.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    if tile_estimates is not None:
+        t = np.ascontiguousarray(tile_estimates, np.int32)
+        keep.append(t)
+        opt.tile_estimates = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    return opt, keep
+
+
+class Prepared:
+    """A scene validated, planned and uploaded once (hipr_scene_prepare)."""
+
+    def __init__(self, ctx, scene_ptr):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        check(lib().hipr_scene_prepare(ctx.handle, scene_ptr, ctypes.byref(self._h)),
+              "hipr_scene_prepare")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().hipr_prepared_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Surface:
     def __init__(self, ctx, width, height):
         self.ctx = ctx
@@ -113,20 +160,16 @@ class Surface:
     def clear(self, rgba=0):
         check(lib().hipr_surface_clear(self._h, rgba), "hipr_surface_clear")
 
-    def draw(self, scene_ptr, mode=0, edge_estimates=None, tile_estimates=None):
-        opt = abi.Options()
-        opt.mode = mode
-        keep = []
-        if edge_estimates is not None:
-            e = np.ascontiguousarray(edge_estimates, np.int32)
-            keep.append(e)
-            opt.edge_estimates = e.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-        if tile_estimates is not None:
-            t = np.ascontiguousarray(tile_estimates, np.int32)
-            keep.append(t)
-            opt.tile_estimates = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-        check(lib().hipr_surface_draw_scene(self._h, scene_ptr, ctypes.byref(opt)),
-              "hipr_surface_draw_scene")
+    def draw(self, scene, mode=0, edge_estimates=None, tile_estimates=None):
+        """Draws a scene (a Scene pointer or a Prepared scene) onto the surface."""
+        opt, keep = _options(mode, edge_estimates, tile_estimates)
+        if isinstance(scene, Prepared):
+            check(lib().hipr_surface_draw_prepared(self._h, scene.handle, ctypes.byref(opt)),
+                  "hipr_surface_draw_prepared")
+        else:
+            check(lib().hipr_surface_draw_scene(self._h, scene, ctypes.byref(opt)),
+                  "hipr_surface_draw_scene")
+        del keep
 
     def finish(self):
         check(lib().hipr_surface_finish(self._h), "hipr_surface_finish")
