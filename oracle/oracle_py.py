@@ -103,3 +103,14 @@ def blur_layer(scene_ptr, p):
     out = np.zeros((h, w, 4), np.uint8)
     lib().orc_blur_layer(scene_ptr, p, out.ctypes.data, n, b.ctypes.data)
     return out, b
+
+
+def quad_path(scene_ptr, p):
+    """Stroke::QuadPath of path p: (verbs uint8[n], points float32[m, 2])."""
+    npts = ctypes.c_int64()
+    nv = lib().orc_quad_path(scene_ptr, p, None, 0, None, 0, ctypes.byref(npts))
+    verbs = np.zeros(max(nv, 1), np.uint8)
+    pts = np.zeros((max(npts.value, 1), 2), np.float32)
+    lib().orc_quad_path(scene_ptr, p, verbs.ctypes.data, nv, pts.ctypes.data, npts.value,
+                        ctypes.byref(npts))
+    return verbs[:nv], pts[:npts.value]
