@@ -139,6 +139,8 @@ typedef struct hipr_stats {
   int64_t fallbacks;      /* paths re-run by the exact two-pass method      */
   int64_t max_active;     /* largest active edge list seen                  */
   double ms_total;        /* device time of the whole draw                  */
+  int64_t blends;         /* pixel blend operations on the target (F)       */
+  int64_t pixels;         /* distinct target pixels touched (U)             */
 } hipr_stats;
 
 /* Execution options.  mode 0 = plain exact two-pass (count, scan, write);
@@ -182,6 +184,27 @@ HIPR_API hipr_result hipr_surface_draw_scene(hipr_surface surface,
 HIPR_API hipr_result hipr_surface_finish(hipr_surface surface);
 HIPR_API hipr_result hipr_surface_get_stats(hipr_surface surface,
                                             hipr_stats* out);
+
+/* Device time of each pipeline stage of the last draw, measured with HIP
+ * events recorded on the surface's stream around the stage's kernels
+ * (ms summed over the stage's launches; launches counts them).  No
+ * reference counterpart: the software backend has no stages to time. */
+typedef enum hipr_stage {
+  HIPR_STAGE_STROKE = 0,        /* outline count + write                  */
+  HIPR_STAGE_FLATTEN_COUNT = 1, /* edge count pass                        */
+  HIPR_STAGE_FLATTEN = 2,       /* edge write pass                        */
+  HIPR_STAGE_SORT = 3,          /* per-path edge sort + gather            */
+  HIPR_STAGE_WALK_COUNT = 4,    /* coverage count pass                    */
+  HIPR_STAGE_WALK = 5,          /* coverage write pass (tile planes)      */
+  HIPR_STAGE_LAYERS = 6,        /* mask-filter layers composite           */
+  HIPR_STAGE_BLUR = 7,          /* StackBlur of the layers                */
+  HIPR_STAGE_COMPOSITE = 8,     /* tile composite onto the target         */
+  HIPR_STAGE_COUNT = 9
+} hipr_stage;
+HIPR_API hipr_result hipr_surface_get_stage_times(hipr_surface surface,
+                                                  double* ms,
+                                                  int32_t* launches,
+                                                  int32_t n_stages);
 
 /* A validated scene whose geometry, paints and draw plan are resident in
  * device memory, so that it can be drawn repeatedly without host-to-device

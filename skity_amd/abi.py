@@ -21,6 +21,10 @@ TILE_CLAMP, TILE_REPEAT, TILE_MIRROR, TILE_DECAL = range(4)
 BLUR_NORMAL = 1
 BLEND_SRC_OVER = 3
 
+# hipr_stage, in enum order
+STAGES = ("stroke", "flatten_count", "flatten", "sort", "walk_count", "walk", "layers",
+          "blur", "composite")
+
 SUCCESS = 0
 ERROR_INVALID_HANDLE = -1
 ERROR_INVALID_ARGUMENT = -2
@@ -84,6 +88,8 @@ class Stats(ctypes.Structure):
         ("fallbacks", ctypes.c_int64),
         ("max_active", ctypes.c_int64),
         ("ms_total", ctypes.c_double),
+        ("blends", ctypes.c_int64),
+        ("pixels", ctypes.c_int64),
     ]
 
 

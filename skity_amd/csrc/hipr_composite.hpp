@@ -158,6 +158,8 @@ struct CompositeParams {
   const uint8_t* planes;
   const PaintDev* paints;
   const GradDev* grads;
+  // optional: [0] += pixel blends performed, [1] += target pixels touched
+  unsigned long long* counters;
 };
 
 __device__ __forceinline__ uint32_t ApplyCover(uint32_t dst, uint32_t color, uint32_t cover) {
@@ -192,6 +194,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
     int x = px0 + q;
     pix[q] = (row_ok && x < cv.width) ? rowp[x] : 0;
   }
+  int blends = 0;
+  uint32_t touched = 0;  // bit q: pixel q blended at least once
   for (int jb = cv.job_begin; jb < cv.job_end; jb += kWave) {
     int j = jb + lane;
     int s0 = -1, s1 = -1;
@@ -254,6 +258,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
           }
           uint32_t alpha = 255u & ga;
           pix[q] = ApplyCover(pix[q], color, alpha);
+          blends++;
+          touched |= 1u << q;
         }
         continue;
       }
@@ -269,6 +275,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
           if (c0) pix[q] = ApplyCover(pix[q], pd.color, c0);
           if (c1) pix[q] = ApplyCover(pix[q], pd.color, c1);
+          blends += (c0 != 0) + (c1 != 0);
+          touched |= (uint32_t)((c0 | c1) != 0) << q;
         }
       } else {
         const GradDev& g = P.grads[pd.grad];
@@ -279,6 +287,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
             uint32_t color = GradientColor(g, px0 + q, py);
             if (c0) pix[q] = ApplyCover(pix[q], color, c0);
             if (c1) pix[q] = ApplyCover(pix[q], color, c1);
+            blends += (c0 != 0) + (c1 != 0);
+            touched |= 1u << q;
           }
         }
       }
@@ -288,6 +298,14 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
   for (int q = 0; q < 4; q++) {
     int x = px0 + q;
     if (row_ok && x < cv.width) rowp[x] = pix[q];
+  }
+  if (P.counters) {
+    int nb = WaveSum(blends);
+    int nt = WaveSum(__popc(touched));
+    if (lane == 0 && (nb | nt)) {
+      atomicAdd(&P.counters[0], (unsigned long long)nb);
+      atomicAdd(&P.counters[1], (unsigned long long)nt);
+    }
   }
 }
 

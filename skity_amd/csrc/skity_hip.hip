@@ -225,7 +225,15 @@ struct hipr_context_s {
   DevBuf stroke_out, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w;
   DevBuf layers, blur_rows, blur_cols, blur_rb, blur_cb, blur_tmp, blur_jobs, blur_lines;
   // debug/utility uploads
-  DevBuf d_verbs, d_pts, d_w, d_jobs, run_jobs;
+  DevBuf d_verbs, d_pts, d_w, d_jobs, run_jobs, counters;
+  // stage timing: events recorded on `stream` around each kernel launch
+  struct StageMark {
+    int32_t stage;
+    hipEvent_t begin, end;
+  };
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<StageMark> marks;
   // pinned staging for read-backs
   JobGeo* h_geo = nullptr;
   JobCov* h_cov = nullptr;
@@ -238,6 +246,8 @@ struct hipr_surface_s {
   int32_t width = 0, height = 0;
   uint8_t* pixels = nullptr;
   hipr_stats stats{};
+  double stage_ms[HIPR_STAGE_COUNT] = {};
+  int32_t stage_launches[HIPR_STAGE_COUNT] = {};
 };
 
 // A scene validated, planned and resident in device memory.
@@ -270,6 +280,33 @@ hipr_result Upload(DevBuf* b, const T* src, size_t n, hipStream_t s) {
   if (n) HIP_TRY(hipMemcpyAsync(b->p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
   return HIPR_SUCCESS;
 }
+
+hipEvent_t NextEvent(hipr_context ctx) {
+  if (ctx->ev_used == ctx->ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->ev_pool.push_back(e);
+  }
+  return ctx->ev_pool[ctx->ev_used++];
+}
+hipEvent_t StageBegin(hipr_context ctx) {
+  hipEvent_t e = NextEvent(ctx);
+  if (e) (void)hipEventRecord(e, ctx->stream);
+  return e;
+}
+void StageEnd(hipr_context ctx, int stage, hipEvent_t begin) {
+  hipEvent_t e = NextEvent(ctx);
+  if (!e || !begin) return;
+  (void)hipEventRecord(e, ctx->stream);
+  ctx->marks.push_back({stage, begin, e});
+}
+// Runs a kernel launch statement between two stage events.
+#define STAGE_LAUNCH(ctx, stage, ...)          \
+  do {                                         \
+    hipEvent_t stage_begin__ = StageBegin(ctx); \
+    __VA_ARGS__;                               \
+    StageEnd(ctx, stage, stage_begin__);       \
+  } while (0)
 
 hipr_result HostStage(hipr_context ctx, size_t n) {
   if (n > ctx->h_cap) {
@@ -523,11 +560,12 @@ void SetTiles(std::vector<Canvas>* cvs) {
 }
 
 template <int K, int kMode>
-hipr_result LaunchWalk(const WalkParams& P, hipStream_t st) {
+hipr_result LaunchWalk(hipr_context ctx, const WalkParams& P, hipStream_t st) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
   size_t lds = (size_t)(2 * (P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 5 * K * kWave) *
                sizeof(int32_t);
-  hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds, st, P);
+  STAGE_LAUNCH(ctx, kMode == 0 ? HIPR_STAGE_WALK_COUNT : HIPR_STAGE_WALK,
+               hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds, st, P));
   HIP_TRY(hipGetLastError());
   return HIPR_SUCCESS;
 }
@@ -554,10 +592,11 @@ hipr_result RunStrokes(Frame& f) {
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   GeomSrc gs{pl->verbs.as<uint8_t>(), pl->pts.as<float>(), pl->weights.as<float>()};
   int blocks = (ns + 63) / 64;
-  hipLaunchKernelGGL((StrokeKernel<0>), dim3(blocks), dim3(64), 0, st, pl->d_jobs.as<Job>(),
+  STAGE_LAUNCH(ctx, HIPR_STAGE_STROKE,
+    hipLaunchKernelGGL((StrokeKernel<0>), dim3(blocks), dim3(64), 0, st, pl->d_jobs.as<Job>(),
                      pl->d_stroke.as<int32_t>(), ns, gs, pl->scene_paints.as<hipr_paint>(),
                      ctx->stroke_out.as<StrokeOut>(), (uint8_t*)nullptr, (float*)nullptr,
-                     (float*)nullptr, (uint8_t*)nullptr, (float*)nullptr, (float*)nullptr);
+                     (float*)nullptr, (uint8_t*)nullptr, (float*)nullptr, (float*)nullptr));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(ctx->h_so, ctx->stroke_out.p, sizeof(StrokeOut) * ns,
                          hipMemcpyDeviceToHost, st));
@@ -589,11 +628,12 @@ hipr_result RunStrokes(Frame& f) {
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   HIP_TRY(hipMemcpyAsync(ctx->stroke_out.p, ctx->h_so, sizeof(StrokeOut) * ns,
                          hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL((StrokeKernel<1>), dim3(blocks), dim3(64), 0, st, pl->d_jobs.as<Job>(),
+  STAGE_LAUNCH(ctx, HIPR_STAGE_STROKE,
+    hipLaunchKernelGGL((StrokeKernel<1>), dim3(blocks), dim3(64), 0, st, pl->d_jobs.as<Job>(),
                      pl->d_stroke.as<int32_t>(), ns, gs, pl->scene_paints.as<hipr_paint>(),
                      ctx->stroke_out.as<StrokeOut>(), ctx->o_verbs.as<uint8_t>(),
                      ctx->o_pts.as<float>(), ctx->o_w.as<float>(), ctx->s_verbs.as<uint8_t>(),
-                     ctx->s_pts.as<float>(), ctx->s_w.as<float>());
+                     ctx->s_pts.as<float>(), ctx->s_w.as<float>()));
   HIP_TRY(hipGetLastError());
   return HIPR_SUCCESS;
 }
@@ -610,9 +650,10 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
   ebase->assign(nj + 1, 0);
   ecap->assign(nj, 0);
   if (mode == 0) {
+    STAGE_LAUNCH(ctx, HIPR_STAGE_FLATTEN_COUNT,
     hipLaunchKernelGGL((FlattenKernel<0>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj,
                        gs, go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                       (EdgeRec*)nullptr, ctx->geo.as<JobGeo>());
+                       (EdgeRec*)nullptr, ctx->geo.as<JobGeo>()));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * nj, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -628,9 +669,10 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
                          hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ctx->edge_cap.p, ecap->data(), sizeof(int32_t) * nj,
                          hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj, gs,
+  STAGE_LAUNCH(ctx, HIPR_STAGE_FLATTEN,
+    hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj, gs,
                      go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                     ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>());
+                     ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>()));
   HIP_TRY(hipGetLastError());
   if (mode == 1) {
     // one read-back: exact counts, bounds and overflow flags
@@ -655,9 +697,10 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
                              hipMemcpyHostToDevice, st));
       HIP_TRY(hipMemcpyAsync(ctx->edge_cap.p, ecap->data(), sizeof(int32_t) * nj,
                              hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj,
+      STAGE_LAUNCH(ctx, HIPR_STAGE_FLATTEN,
+    hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj,
                          gs, go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                         ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>());
+                         ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>()));
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * nj, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
@@ -674,10 +717,11 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
 hipr_result SortGather(Frame& f, const int32_t* d_list, int n) {
   if (n == 0) return HIPR_SUCCESS;
   hipr_context ctx = f.ctx;
-  hipLaunchKernelGGL(SortGatherKernel, dim3(n), dim3(kSortThreads), 0, f.st, d_list, n,
+  STAGE_LAUNCH(ctx, HIPR_STAGE_SORT,
+    hipLaunchKernelGGL(SortGatherKernel, dim3(n), dim3(kSortThreads), 0, f.st, d_list, n,
                      ctx->geo.as<JobGeo>(), ctx->edge_base.as<int64_t>(),
                      ctx->edges.as<EdgeRec>(), ctx->wedges.as<EdgeRec>(),
-                     ctx->gkeys.as<uint64_t>());
+                     ctx->gkeys.as<uint64_t>()));
   HIP_TRY(hipGetLastError());
   return HIPR_SUCCESS;
 }
@@ -697,6 +741,12 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   int nj = f.nj;
   int mode = opt ? opt->mode : 0;
   if (mode == 1 && (!opt->edge_estimates || !opt->tile_estimates)) return HIPR_ERROR_INVALID_ARGUMENT;
+  ctx->ev_used = 0;
+  ctx->marks.clear();
+  for (int k = 0; k < HIPR_STAGE_COUNT; k++) {
+    surf->stage_ms[k] = 0;
+    surf->stage_launches[k] = 0;
+  }
   HIP_TRY(hipEventRecord(ctx->ev0, st));
   if (nj == 0) {
     HIP_TRY(hipEventRecord(ctx->ev1, st));
@@ -809,7 +859,7 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     return HIPR_SUCCESS;
   };
   if (mode == 0) {
-    if ((r = LaunchWalk<4, 0>(wp, st)) != HIPR_SUCCESS) return r;
+    if ((r = LaunchWalk<4, 0>(ctx, wp, st)) != HIPR_SUCCESS) return r;
     HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if ((r = check_limits()) != HIPR_SUCCESS) return r;
@@ -825,7 +875,7 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   wp.planes = ctx->planes.as<uint8_t>();
   wp.map0 = ctx->map0.as<int32_t>();
   wp.map1 = ctx->map1.as<int32_t>();
-  if ((r = LaunchWalk<4, 1>(wp, st)) != HIPR_SUCCESS) return r;
+  if ((r = LaunchWalk<4, 1>(ctx, wp, st)) != HIPR_SUCCESS) return r;
   HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   if ((r = check_limits()) != HIPR_SUCCESS) return r;
@@ -844,7 +894,7 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
       // equivalent to re-walking all; keep it simple and exact.
       if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk)) != HIPR_SUCCESS) return r;
       wp.planes = ctx->planes.as<uint8_t>();
-      if ((r = LaunchWalk<4, 1>(wp, st)) != HIPR_SUCCESS) return r;
+      if ((r = LaunchWalk<4, 1>(ctx, wp, st)) != HIPR_SUCCESS) return r;
       HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
@@ -882,7 +932,8 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     lp.canvases = ctx->layer_canvases.as<Canvas>();
     lp.n_canvases = (int)lc.size();
     lp.total_tiles = layer_tiles;
-    hipLaunchKernelGGL(CompositeKernel, dim3(layer_tiles), dim3(64), 0, st, lp);
+    STAGE_LAUNCH(ctx, HIPR_STAGE_LAYERS,
+    hipLaunchKernelGGL(CompositeKernel, dim3(layer_tiles), dim3(64), 0, st, lp));
     HIP_TRY(hipGetLastError());
     // StackBlur each layer: rows into a temporary, columns back
     std::vector<BlurJob> rows, cols;
@@ -914,13 +965,15 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
       if ((r = Upload(&ctx->blur_rb, rb.data(), rb.size(), st)) != HIPR_SUCCESS) return r;
       if ((r = Upload(&ctx->blur_cb, cb.data(), cb.size(), st)) != HIPR_SUCCESS) return r;
       int64_t nl = rb.back(), nc = cb.back();
-      hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st,
+      STAGE_LAUNCH(ctx, HIPR_STAGE_BLUR,
+    hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st,
                          ctx->blur_rows.as<BlurJob>(), ctx->blur_rb.as<int64_t>(),
-                         (int)rows.size(), nl, 0);
+                         (int)rows.size(), nl, 0));
       HIP_TRY(hipGetLastError());
-      hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st,
+      STAGE_LAUNCH(ctx, HIPR_STAGE_BLUR,
+    hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st,
                          ctx->blur_cols.as<BlurJob>(), ctx->blur_cb.as<int64_t>(),
-                         (int)cols.size(), nc, 1);
+                         (int)cols.size(), nc, 1));
       HIP_TRY(hipGetLastError());
     }
   }
@@ -928,13 +981,29 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   mp.canvases = ctx->canvases.as<Canvas>();
   mp.n_canvases = 1;
   mp.total_tiles = main_tiles;
-  hipLaunchKernelGGL(CompositeKernel, dim3(main_tiles), dim3(64), 0, st, mp);
+  if (ctx->counters.Ensure(2 * sizeof(unsigned long long)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, 2 * sizeof(unsigned long long), st));
+  mp.counters = ctx->counters.as<unsigned long long>();
+  STAGE_LAUNCH(ctx, HIPR_STAGE_COMPOSITE,
+    hipLaunchKernelGGL(CompositeKernel, dim3(main_tiles), dim3(64), 0, st, mp));
   HIP_TRY(hipGetLastError());
+  unsigned long long counts[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(counts, ctx->counters.p, sizeof(counts), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipEventRecord(ctx->ev1, st));
   HIP_TRY(hipEventSynchronize(ctx->ev1));
+  f.stats.blends = (int64_t)counts[0];
+  f.stats.pixels = (int64_t)counts[1];
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   f.stats.ms_total = ms;
+  for (const auto& m : ctx->marks) {
+    float sm = 0;
+    if (hipEventElapsedTime(&sm, m.begin, m.end) == hipSuccess) {
+      surf->stage_ms[m.stage] += sm;
+      surf->stage_launches[m.stage]++;
+    }
+  }
   surf->stats = f.stats;
   return HIPR_SUCCESS;
 }
@@ -1009,11 +1078,12 @@ HIPR_API void hipr_context_destroy(hipr_context ctx) {
                     &ctx->s_verbs, &ctx->s_pts, &ctx->s_w, &ctx->layers, &ctx->blur_rows,
                     &ctx->blur_cols, &ctx->blur_rb, &ctx->blur_cb, &ctx->blur_tmp,
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
-                    &ctx->d_jobs, &ctx->run_jobs};
+                    &ctx->d_jobs, &ctx->run_jobs, &ctx->counters};
   for (DevBuf* b : bufs) b->Free();
   if (ctx->h_geo) (void)hipHostFree(ctx->h_geo);
   if (ctx->h_cov) (void)hipHostFree(ctx->h_cov);
   if (ctx->h_so) (void)hipHostFree(ctx->h_so);
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipStreamDestroy(ctx->stream);
@@ -1088,6 +1158,18 @@ HIPR_API hipr_result hipr_surface_get_stats(hipr_surface s, hipr_stats* out) {
   if (!s) return HIPR_ERROR_INVALID_HANDLE;
   if (!out) return HIPR_ERROR_INVALID_ARGUMENT;
   *out = s->stats;
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_surface_get_stage_times(hipr_surface s, double* ms,
+                                                  int32_t* launches, int32_t n_stages) {
+  if (!s) return HIPR_ERROR_INVALID_HANDLE;
+  if (n_stages < 0 || (n_stages > 0 && !ms && !launches)) return HIPR_ERROR_INVALID_ARGUMENT;
+  for (int k = 0; k < n_stages; k++) {
+    bool in = k < HIPR_STAGE_COUNT;
+    if (ms) ms[k] = in ? s->stage_ms[k] : 0.0;
+    if (launches) launches[k] = in ? s->stage_launches[k] : 0;
+  }
   return HIPR_SUCCESS;
 }
 

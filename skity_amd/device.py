@@ -41,6 +41,9 @@ def lib():
         L.hipr_surface_finish.restype = ctypes.c_int
         L.hipr_surface_get_stats.argtypes = [c_surf, ctypes.POINTER(abi.Stats)]
         L.hipr_surface_get_stats.restype = ctypes.c_int
+        L.hipr_surface_get_stage_times.argtypes = [c_surf, ctypes.c_void_p, ctypes.c_void_p,
+                                                   ctypes.c_int32]
+        L.hipr_surface_get_stage_times.restype = ctypes.c_int
         L.hipr_scene_prepare.argtypes = [c_ctx, SP, ctypes.POINTER(ctypes.c_void_p)]
         L.hipr_scene_prepare.restype = ctypes.c_int
         L.hipr_prepared_destroy.argtypes = [ctypes.c_void_p]
@@ -189,6 +192,15 @@ class Surface:
         s = abi.Stats()
         check(lib().hipr_surface_get_stats(self._h, ctypes.byref(s)), "hipr_surface_get_stats")
         return {k: getattr(s, k) for k, _ in abi.Stats._fields_}
+
+    def stage_times(self):
+        """{stage: (ms, launches)} of the last draw (HIP events on the stream)."""
+        n = len(abi.STAGES)
+        ms = np.zeros(n, np.float64)
+        cnt = np.zeros(n, np.int32)
+        check(lib().hipr_surface_get_stage_times(self._h, ms.ctypes.data, cnt.ctypes.data, n),
+              "hipr_surface_get_stage_times")
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(abi.STAGES)}
 
     def close(self):
         if self._h:
