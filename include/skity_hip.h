@@ -128,6 +128,7 @@ typedef struct hipr_scene {
 
 typedef struct hipr_context_s* hipr_context;
 typedef struct hipr_surface_s* hipr_surface;
+typedef struct hipr_prepared_s* hipr_prepared; /* see hipr_scene_prepare */
 
 /* Per-draw statistics of the last hipr_surface_draw_scene call. */
 typedef struct hipr_stats {
@@ -146,7 +147,9 @@ typedef struct hipr_stats {
 /* Execution options.  mode 0 = plain exact two-pass (count, scan, write);
  * mode 1 = pre-sized single pass from the caller-supplied per-path estimates
  * (estimates != NULL), falling back to the exact two-pass method for every
- * path whose estimate was too small. */
+ * path whose estimate was too small; mode 2 = pre-sized from the context's
+ * learned pre-sizer (hipr_context_set_presizer), with the same fallback. */
+enum { HIPR_MODE_EXACT = 0, HIPR_MODE_ESTIMATES = 1, HIPR_MODE_LEARNED = 2 };
 typedef struct hipr_options {
   int32_t mode;
   const int32_t* edge_estimates;  /* n_paths or NULL */
@@ -185,6 +188,50 @@ HIPR_API hipr_result hipr_surface_finish(hipr_surface surface);
 HIPR_API hipr_result hipr_surface_get_stats(hipr_surface surface,
                                             hipr_stats* out);
 
+/* --- learned pre-sizer ---------------------------------------------------
+ * No reference counterpart: the software backend sizes its edge and span
+ * vectors by growing them.  A 16-64-64-2 MLP (FP16 weights, FP32
+ * accumulation on MFMA) predicts per raster job log1p(edges) and
+ * log1p(tile planes) from 16 path features; its forward and backward passes
+ * and its Adam training loop run on the device.  Parameters are
+ * HIPR_PRESIZER_PARAMS floats: W1[16][64] b1[64] W2[64][64] b2[64] W3[64][2]
+ * b3[2] (trained), then feature mean[16] and scale[16], target mean[2] and
+ * scale[2] (fixed normalisation: x' = (x - mean) * scale,
+ * y' = (y - mean) / scale). */
+#define HIPR_PRESIZER_FEATURES 16
+#define HIPR_PRESIZER_TRAINED 5378
+#define HIPR_PRESIZER_PARAMS 5414
+/* Raster jobs of a prepared scene (fills, strokes and layer composites). */
+HIPR_API hipr_result hipr_prepared_job_count(hipr_prepared prepared, int32_t* n_jobs);
+/* Per-job features (n_jobs x 16 floats, zero rows for layer composites) for
+ * a draw onto a width x height target. */
+HIPR_API hipr_result hipr_prepared_features(hipr_prepared prepared, int32_t width,
+                                            int32_t height, float* features,
+                                            int32_t n_jobs);
+/* Exact per-job counts of the last draw on the surface (-1 for layer
+ * composites): the labels of the pre-sizer. */
+HIPR_API hipr_result hipr_surface_get_job_counts(hipr_surface surface, int32_t* edges,
+                                                 int32_t* planes, int32_t n_jobs);
+/* Predictions (n x 2, log1p units) for host-side features. */
+HIPR_API hipr_result hipr_presizer_predict(hipr_context ctx, const float* params,
+                                           const float* features, int64_t n, float* out);
+/* Gradient of the mean squared error (normalised target space) over n
+ * samples (HIPR_PRESIZER_TRAINED floats) and the mean loss. */
+HIPR_API hipr_result hipr_presizer_gradients(hipr_context ctx, const float* params,
+                                             const float* features, const float* targets,
+                                             int64_t n, float* grads, float* loss);
+/* Adam training on the device: `steps` mini-batches of `batch` samples drawn
+ * in a seeded shuffled order; params is updated in place; loss_curve (steps
+ * floats, may be NULL) receives each batch's mean loss. */
+HIPR_API hipr_result hipr_presizer_train(hipr_context ctx, float* params,
+                                         const float* features, const float* targets,
+                                         int64_t n, int32_t steps, int32_t batch, float lr,
+                                         uint64_t seed, float* loss_curve);
+/* Installs the parameters used by HIPR_MODE_LEARNED.  Capacities are
+ * ceil(expm1(prediction + margin)) per job and count. */
+HIPR_API hipr_result hipr_context_set_presizer(hipr_context ctx, const float* params,
+                                               float margin_edges, float margin_planes);
+
 /* Device time of each pipeline stage of the last draw, measured with HIP
  * events recorded on the surface's stream around the stage's kernels
  * (ms summed over the stage's launches; launches counts them).  No
@@ -199,7 +246,8 @@ typedef enum hipr_stage {
   HIPR_STAGE_LAYERS = 6,        /* mask-filter layers composite           */
   HIPR_STAGE_BLUR = 7,          /* StackBlur of the layers                */
   HIPR_STAGE_COMPOSITE = 8,     /* tile composite onto the target         */
-  HIPR_STAGE_COUNT = 9
+  HIPR_STAGE_PRESIZE = 9,       /* features + MLP inference               */
+  HIPR_STAGE_COUNT = 10
 } hipr_stage;
 HIPR_API hipr_result hipr_surface_get_stage_times(hipr_surface surface,
                                                   double* ms,
@@ -211,7 +259,6 @@ HIPR_API hipr_result hipr_surface_get_stage_times(hipr_surface surface,
  * copies (the recorded DisplayList of include/skity/recorder/
  * display_list.hpp:56-77 played onto a HIP surface).  The scene arrays may
  * be released after hipr_scene_prepare returns. */
-typedef struct hipr_prepared_s* hipr_prepared;
 HIPR_API hipr_result hipr_scene_prepare(hipr_context ctx,
                                         const hipr_scene* scene,
                                         hipr_prepared* out);

@@ -23,7 +23,9 @@ BLEND_SRC_OVER = 3
 
 # hipr_stage, in enum order
 STAGES = ("stroke", "flatten_count", "flatten", "sort", "walk_count", "walk", "layers",
-          "blur", "composite")
+          "blur", "composite", "presize")
+MODE_EXACT, MODE_ESTIMATES, MODE_LEARNED = 0, 1, 2
+PRESIZER_FEATURES, PRESIZER_TRAINED, PRESIZER_PARAMS = 16, 5378, 5414
 
 SUCCESS = 0
 ERROR_INVALID_HANDLE = -1

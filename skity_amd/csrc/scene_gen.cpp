@@ -305,6 +305,11 @@ void sg_canvas_size(sg_scene s, int32_t* w, int32_t* h) {
   *h = s->height;
 }
 
+void sg_struct_sizes(int32_t* paint, int32_t* gradient) {
+  *paint = (int32_t)sizeof(hipr_paint);
+  *gradient = (int32_t)sizeof(hipr_gradient);
+}
+
 sg_scene sg_slice(sg_scene s, int32_t begin, int32_t end) {
   sg_scene_s* r = new sg_scene_s();
   r->width = s->width;

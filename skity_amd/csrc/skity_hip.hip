@@ -18,12 +18,15 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <numeric>
+#include <random>
 #include <vector>
 
 #include "../../include/skity_hip.h"
 #include "hipr_blur.hpp"
 #include "hipr_composite.hpp"
 #include "hipr_flatten.hpp"
+#include "hipr_mlp.hpp"
 #include "hipr_sort.hpp"
 #include "hipr_stroke.hpp"
 #include "hipr_walk.hpp"
@@ -226,6 +229,13 @@ struct hipr_context_s {
   DevBuf layers, blur_rows, blur_cols, blur_rb, blur_cb, blur_tmp, blur_jobs, blur_lines;
   // debug/utility uploads
   DevBuf d_verbs, d_pts, d_w, d_jobs, run_jobs, counters;
+  // learned pre-sizer
+  bool has_presizer = false;
+  bool mlp_dev_current = false;  // mlp_prm/mlp_ops hold presizer_host
+  std::vector<float> presizer_host;
+  float margin_edges = 0.f, margin_planes = 0.f;
+  DevBuf mlp_prm, mlp_ops, mlp_feats, mlp_preds, mlp_targets, mlp_grads, mlp_loss, mlp_m,
+      mlp_v, mlp_gsum, mlp_curve;
   // stage timing: events recorded on `stream` around each kernel launch
   struct StageMark {
     int32_t stage;
@@ -246,6 +256,7 @@ struct hipr_surface_s {
   int32_t width = 0, height = 0;
   uint8_t* pixels = nullptr;
   hipr_stats stats{};
+  std::vector<int32_t> job_edges, job_planes;  // exact counts of the last draw
   double stage_ms[HIPR_STAGE_COUNT] = {};
   int32_t stage_launches[HIPR_STAGE_COUNT] = {};
 };
@@ -726,6 +737,71 @@ hipr_result SortGather(Frame& f, const int32_t* d_list, int n) {
   return HIPR_SUCCESS;
 }
 
+// Features of every job of a prepared scene into ctx->mlp_feats (device).
+hipr_result LaunchFeatures(hipr_context ctx, hipr_prepared_s* pl, const Canvas* d_canvases) {
+  int nj = (int)pl->jobs.size();
+  if (ctx->mlp_feats.Ensure(sizeof(float) * kMlpIn * std::max(nj, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  if (nj == 0) return HIPR_SUCCESS;
+  FeatureParams fp{};
+  fp.jobs = pl->d_jobs.as<Job>();
+  fp.n_jobs = nj;
+  fp.scene = GeomSrc{pl->verbs.as<uint8_t>(), pl->pts.as<float>(), pl->weights.as<float>()};
+  fp.paints = pl->scene_paints.as<hipr_paint>();
+  fp.canvases = d_canvases;
+  fp.out = ctx->mlp_feats.as<float>();
+  hipLaunchKernelGGL(FeaturesKernel, dim3(nj), dim3(64), 0, ctx->stream, fp);
+  HIP_TRY(hipGetLastError());
+  return HIPR_SUCCESS;
+}
+
+hipr_result UploadPresizer(hipr_context ctx, const float* params) {
+  hipr_result r;
+  if ((r = Upload(&ctx->mlp_prm, params, kMlpTotal, ctx->stream)) != HIPR_SUCCESS) return r;
+  if (ctx->mlp_ops.Ensure(sizeof(_Float16) * kOpBlocks * 256) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  hipLaunchKernelGGL(MlpPackKernel, dim3(kOpBlocks), dim3(256), 0, ctx->stream,
+                     ctx->mlp_prm.as<float>(), ctx->mlp_ops.as<_Float16>());
+  HIP_TRY(hipGetLastError());
+  return HIPR_SUCCESS;
+}
+
+// Learned capacities for every job: features -> MLP -> ceil(expm1(p + m)).
+hipr_result RunPresize(Frame& f, std::vector<int32_t>* eest, std::vector<int32_t>* test) {
+  hipr_context ctx = f.ctx;
+  int nj = f.nj;
+  hipr_result r;
+  if (ctx->mlp_preds.Ensure(sizeof(float) * kMlpOut * std::max(nj, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  if (!ctx->mlp_dev_current) {
+    if ((r = UploadPresizer(ctx, ctx->presizer_host.data())) != HIPR_SUCCESS) return r;
+    ctx->mlp_dev_current = true;
+  }
+  hipEvent_t b = StageBegin(ctx);
+  if ((r = LaunchFeatures(ctx, f.pl, ctx->canvases.as<Canvas>())) != HIPR_SUCCESS) return r;
+  int blocks = std::min((nj + 15) / 16, 8192);
+  hipLaunchKernelGGL(MlpInferKernel, dim3(blocks), dim3(64), 0, f.st, ctx->mlp_ops.as<_Float16>(),
+                     ctx->mlp_prm.as<float>(), ctx->mlp_feats.as<float>(), (int64_t)nj,
+                     ctx->mlp_preds.as<float>());
+  HIP_TRY(hipGetLastError());
+  StageEnd(ctx, HIPR_STAGE_PRESIZE, b);
+  std::vector<float> pred((size_t)kMlpOut * nj);
+  HIP_TRY(hipMemcpyAsync(pred.data(), ctx->mlp_preds.p, sizeof(float) * pred.size(),
+                         hipMemcpyDeviceToHost, f.st));
+  HIP_TRY(hipStreamSynchronize(f.st));
+  eest->resize(nj);
+  test->resize(nj);
+  auto cap = [](float p, float m) {
+    double v = std::ceil(std::expm1((double)p + m));
+    return (int32_t)std::min(std::max(v, 1.0), 1.0e9);
+  };
+  for (int j = 0; j < nj; j++) {
+    (*eest)[j] = cap(pred[2 * j], ctx->margin_edges);
+    (*test)[j] = cap(pred[2 * j + 1], ctx->margin_planes);
+  }
+  return HIPR_SUCCESS;
+}
+
 hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_options* opt) {
   hipr_context ctx = surf->ctx;
   Frame f;
@@ -739,8 +815,11 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   f.stats.raster_paths = (int64_t)pl->walk_jobs.size();
   hipStream_t st = f.st;
   int nj = f.nj;
-  int mode = opt ? opt->mode : 0;
-  if (mode == 1 && (!opt->edge_estimates || !opt->tile_estimates)) return HIPR_ERROR_INVALID_ARGUMENT;
+  int mode = opt ? opt->mode : HIPR_MODE_EXACT;
+  if (mode < HIPR_MODE_EXACT || mode > HIPR_MODE_LEARNED) return HIPR_ERROR_INVALID_ARGUMENT;
+  if (mode == HIPR_MODE_ESTIMATES && (!opt->edge_estimates || !opt->tile_estimates))
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  if (mode == HIPR_MODE_LEARNED && !ctx->has_presizer) return HIPR_ERROR_INVALID_ARGUMENT;
   ctx->ev_used = 0;
   ctx->marks.clear();
   for (int k = 0; k < HIPR_STAGE_COUNT; k++) {
@@ -786,7 +865,10 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   }
   // per-job estimates are indexed by draw (path); spread them to jobs
   std::vector<int32_t> eest, test;
-  if (mode == 1) {
+  if (mode == HIPR_MODE_LEARNED) {
+    if ((r = RunPresize(f, &eest, &test)) != HIPR_SUCCESS) return r;
+    mode = 1;  // from here on exactly the estimates path
+  } else if (mode == 1) {
     eest.resize(nj);
     test.resize(nj);
     for (int j = 0; j < nj; j++) {
@@ -907,6 +989,13 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     max_active = std::max<int64_t>(max_active, ctx->h_cov[j].max_active);
   }
   f.stats.max_active = max_active;
+  surf->job_edges.assign(nj, -1);
+  surf->job_planes.assign(nj, -1);
+  for (int j = 0; j < nj; j++) {
+    if (f.jobs[j].kind == kJobImage) continue;
+    surf->job_edges[j] = ctx->h_geo[j].edges;
+    surf->job_planes[j] = ctx->h_cov[j].planes;
+  }
   // ---- composite: layers, blur, canvas ----
   CompositeParams cp{};
   cp.n_canvases = (int)f.canvases.size();
@@ -1042,6 +1131,26 @@ hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s**
   return HIPR_SUCCESS;
 }
 
+constexpr int kMlpSamplesPerWave = 256;
+
+// One forward/backward pass over n device-resident samples into
+// ctx->mlp_grads / mlp_loss; returns the number of waves used.
+hipr_result LaunchGrad(hipr_context ctx, const float* d_feats, const float* d_targets, int64_t n,
+                       int* waves_out) {
+  int waves = (int)((n + kMlpSamplesPerWave - 1) / kMlpSamplesPerWave);
+  waves = std::max(waves, 1);
+  if (ctx->mlp_grads.Ensure(sizeof(float) * kMlpTrained * waves) != hipSuccess ||
+      ctx->mlp_loss.Ensure(sizeof(float) * waves) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  hipLaunchKernelGGL(MlpGradKernel, dim3(waves), dim3(64), 0, ctx->stream,
+                     ctx->mlp_ops.as<_Float16>(), ctx->mlp_prm.as<float>(), d_feats, d_targets, n,
+                     kMlpSamplesPerWave, 1.0f / (float)std::max<int64_t>(n, 1),
+                     ctx->mlp_grads.as<float>(), ctx->mlp_loss.as<float>());
+  HIP_TRY(hipGetLastError());
+  *waves_out = waves;
+  return HIPR_SUCCESS;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1080,6 +1189,10 @@ HIPR_API void hipr_context_destroy(hipr_context ctx) {
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
                     &ctx->d_jobs, &ctx->run_jobs, &ctx->counters};
   for (DevBuf* b : bufs) b->Free();
+  DevBuf* mlp[] = {&ctx->mlp_prm, &ctx->mlp_ops, &ctx->mlp_feats, &ctx->mlp_preds,
+                   &ctx->mlp_targets, &ctx->mlp_grads, &ctx->mlp_loss, &ctx->mlp_m,
+                   &ctx->mlp_v, &ctx->mlp_gsum, &ctx->mlp_curve};
+  for (DevBuf* b : mlp) b->Free();
   if (ctx->h_geo) (void)hipHostFree(ctx->h_geo);
   if (ctx->h_cov) (void)hipHostFree(ctx->h_cov);
   if (ctx->h_so) (void)hipHostFree(ctx->h_so);
@@ -1170,6 +1283,187 @@ HIPR_API hipr_result hipr_surface_get_stage_times(hipr_surface s, double* ms,
     if (ms) ms[k] = in ? s->stage_ms[k] : 0.0;
     if (launches) launches[k] = in ? s->stage_launches[k] : 0;
   }
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_prepared_job_count(hipr_prepared p, int32_t* n_jobs) {
+  if (!p) return HIPR_ERROR_INVALID_HANDLE;
+  if (!n_jobs) return HIPR_ERROR_INVALID_ARGUMENT;
+  *n_jobs = (int32_t)p->jobs.size();
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_prepared_features(hipr_prepared p, int32_t width, int32_t height,
+                                            float* features, int32_t n_jobs) {
+  if (!p) return HIPR_ERROR_INVALID_HANDLE;
+  if (!features || n_jobs != (int32_t)p->jobs.size() || width <= 0 || height <= 0)
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  hipr_context ctx = p->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (n_jobs == 0) return HIPR_SUCCESS;
+  std::vector<Canvas> cvs = p->canvases;
+  cvs[0].width = width;
+  cvs[0].height = height;
+  SetTiles(&cvs);
+  hipr_result r;
+  if ((r = Upload(&ctx->canvases, cvs.data(), cvs.size(), ctx->stream)) != HIPR_SUCCESS) return r;
+  if ((r = LaunchFeatures(ctx, p, ctx->canvases.as<Canvas>())) != HIPR_SUCCESS) return r;
+  HIP_TRY(hipMemcpyAsync(features, ctx->mlp_feats.p, sizeof(float) * kMlpIn * n_jobs,
+                         hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_surface_get_job_counts(hipr_surface s, int32_t* edges,
+                                                 int32_t* planes, int32_t n_jobs) {
+  if (!s) return HIPR_ERROR_INVALID_HANDLE;
+  if (n_jobs != (int32_t)s->job_edges.size()) return HIPR_ERROR_INVALID_ARGUMENT;
+  for (int j = 0; j < n_jobs; j++) {
+    if (edges) edges[j] = s->job_edges[j];
+    if (planes) planes[j] = s->job_planes[j];
+  }
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_presizer_predict(hipr_context ctx, const float* params,
+                                           const float* features, int64_t n, float* out) {
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  if (!params || n < 0 || (n > 0 && (!features || !out))) return HIPR_ERROR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (n == 0) return HIPR_SUCCESS;
+  hipr_result r;
+  if ((r = UploadPresizer(ctx, params)) != HIPR_SUCCESS) return r;
+  if ((r = Upload(&ctx->mlp_feats, features, (size_t)n * kMlpIn, ctx->stream)) != HIPR_SUCCESS)
+    return r;
+  if (ctx->mlp_preds.Ensure(sizeof(float) * kMlpOut * n) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  int blocks = (int)std::min<int64_t>((n + 15) / 16, 8192);
+  hipLaunchKernelGGL(MlpInferKernel, dim3(blocks), dim3(64), 0, ctx->stream,
+                     ctx->mlp_ops.as<_Float16>(), ctx->mlp_prm.as<float>(),
+                     ctx->mlp_feats.as<float>(), n, ctx->mlp_preds.as<float>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->mlp_preds.p, sizeof(float) * kMlpOut * n,
+                         hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->mlp_dev_current = false;  // the installed pre-sizer is re-uploaded on use
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_presizer_gradients(hipr_context ctx, const float* params,
+                                             const float* features, const float* targets,
+                                             int64_t n, float* grads, float* loss) {
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  if (!params || !features || !targets || !grads || n <= 0) return HIPR_ERROR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipr_result r;
+  if ((r = UploadPresizer(ctx, params)) != HIPR_SUCCESS) return r;
+  if ((r = Upload(&ctx->mlp_feats, features, (size_t)n * kMlpIn, ctx->stream)) != HIPR_SUCCESS ||
+      (r = Upload(&ctx->mlp_targets, targets, (size_t)n * kMlpOut, ctx->stream)) != HIPR_SUCCESS)
+    return r;
+  int waves = 0;
+  if ((r = LaunchGrad(ctx, ctx->mlp_feats.as<float>(), ctx->mlp_targets.as<float>(), n, &waves)) !=
+      HIPR_SUCCESS)
+    return r;
+  if (ctx->mlp_gsum.Ensure(sizeof(float) * (kMlpTrained + 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  float* gsum = ctx->mlp_gsum.as<float>();
+  hipLaunchKernelGGL(MlpAdamKernel, dim3((kMlpTrained + 255) / 256), dim3(256), 0, ctx->stream,
+                     ctx->mlp_prm.as<float>(), (float*)nullptr, (float*)nullptr,
+                     ctx->mlp_grads.as<float>(), waves, gsum, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f,
+                     ctx->mlp_loss.as<float>(), 1.0f / (float)n, gsum + kMlpTrained);
+  HIP_TRY(hipGetLastError());
+  std::vector<float> h(kMlpTrained + 1);
+  HIP_TRY(hipMemcpyAsync(h.data(), gsum, sizeof(float) * h.size(), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  std::memcpy(grads, h.data(), sizeof(float) * kMlpTrained);
+  if (loss) *loss = h[kMlpTrained];
+  ctx->mlp_dev_current = false;
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_presizer_train(hipr_context ctx, float* params, const float* features,
+                                         const float* targets, int64_t n, int32_t steps,
+                                         int32_t batch, float lr, uint64_t seed,
+                                         float* loss_curve) {
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  if (!params || !features || !targets || n <= 0 || steps < 0 || batch <= 0 || !(lr > 0))
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  hipr_result r;
+  if ((r = UploadPresizer(ctx, params)) != HIPR_SUCCESS) return r;
+  if (ctx->mlp_m.Ensure(sizeof(float) * kMlpTrained) != hipSuccess ||
+      ctx->mlp_v.Ensure(sizeof(float) * kMlpTrained) != hipSuccess ||
+      ctx->mlp_curve.Ensure(sizeof(float) * std::max(steps, 1)) != hipSuccess ||
+      ctx->mlp_feats.Ensure(sizeof(float) * kMlpIn * n) != hipSuccess ||
+      ctx->mlp_targets.Ensure(sizeof(float) * kMlpOut * n) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  HIP_TRY(hipMemsetAsync(ctx->mlp_m.p, 0, sizeof(float) * kMlpTrained, st));
+  HIP_TRY(hipMemsetAsync(ctx->mlp_v.p, 0, sizeof(float) * kMlpTrained, st));
+  int64_t bsz = std::min<int64_t>(batch, n);
+  int64_t n_batches = std::max<int64_t>(n / bsz, 1);
+  std::vector<int64_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::mt19937_64 rng(seed);
+  std::vector<float> hf((size_t)n * kMlpIn), ht((size_t)n * kMlpOut);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int32_t t = 0; t < steps; t++) {
+    int64_t b = t % n_batches;
+    if (b == 0) {  // new epoch: shuffle and upload
+      std::shuffle(perm.begin(), perm.end(), rng);
+      for (int64_t i = 0; i < n; i++) {
+        std::memcpy(&hf[i * kMlpIn], features + perm[i] * kMlpIn, sizeof(float) * kMlpIn);
+        std::memcpy(&ht[i * kMlpOut], targets + perm[i] * kMlpOut, sizeof(float) * kMlpOut);
+      }
+      HIP_TRY(hipMemcpyAsync(ctx->mlp_feats.p, hf.data(), sizeof(float) * hf.size(),
+                             hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(ctx->mlp_targets.p, ht.data(), sizeof(float) * ht.size(),
+                             hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    int waves = 0;
+    if ((r = LaunchGrad(ctx, ctx->mlp_feats.as<float>() + b * bsz * kMlpIn,
+                        ctx->mlp_targets.as<float>() + b * bsz * kMlpOut, bsz, &waves)) !=
+        HIPR_SUCCESS)
+      return r;
+    float corr1 = 1.f - std::pow(b1, (float)(t + 1)), corr2 = 1.f - std::pow(b2, (float)(t + 1));
+    hipLaunchKernelGGL(MlpAdamKernel, dim3((kMlpTrained + 255) / 256), dim3(256), 0, st,
+                       ctx->mlp_prm.as<float>(), ctx->mlp_m.as<float>(), ctx->mlp_v.as<float>(),
+                       ctx->mlp_grads.as<float>(), waves, (float*)nullptr, lr, b1, b2, eps, corr1,
+                       corr2, ctx->mlp_loss.as<float>(), 1.0f / (float)bsz,
+                       ctx->mlp_curve.as<float>() + t);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(MlpPackKernel, dim3(kOpBlocks), dim3(256), 0, st, ctx->mlp_prm.as<float>(),
+                       ctx->mlp_ops.as<_Float16>());
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemcpyAsync(params, ctx->mlp_prm.p, sizeof(float) * kMlpTrained,
+                         hipMemcpyDeviceToHost, st));
+  if (loss_curve && steps > 0)
+    HIP_TRY(hipMemcpyAsync(loss_curve, ctx->mlp_curve.p, sizeof(float) * steps,
+                           hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  ctx->mlp_dev_current = false;
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_context_set_presizer(hipr_context ctx, const float* params,
+                                               float margin_edges, float margin_planes) {
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  if (!params || !std::isfinite(margin_edges) || !std::isfinite(margin_planes))
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  for (int i = 0; i < kMlpTotal; i++)
+    if (!std::isfinite(params[i])) return HIPR_ERROR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipr_result r;
+  if ((r = UploadPresizer(ctx, params)) != HIPR_SUCCESS) return r;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->presizer_host.assign(params, params + kMlpTotal);
+  ctx->mlp_dev_current = true;
+  ctx->has_presizer = true;
+  ctx->margin_edges = margin_edges;
+  ctx->margin_planes = margin_planes;
   return HIPR_SUCCESS;
 }
 
