@@ -43,6 +43,14 @@ ORC_API int64_t orc_quad_path(const hipr_scene* scene, int32_t p,
                               uint8_t* verbs, int64_t verb_cap, float* pts,
                               int64_t pt_cap, int64_t* n_pts);
 
+/* Stroke outline of path p (Stroke::StrokePath of its quad path with the
+ * draw's paint): verbs, points, conic weights; counts = {verbs, points,
+ * weights}.  Returns the verb count. */
+ORC_API int64_t orc_stroke_outline(const hipr_scene* scene, int32_t p,
+                                   uint8_t* verbs, int64_t verb_cap, float* pts,
+                                   int64_t pt_cap, float* weights, int64_t w_cap,
+                                   int64_t* counts);
+
 /* StackBlur of a premultiplied RGBA8 image (radius = round(r), clamp 254). */
 ORC_API void orc_stackblur(const uint8_t* src, uint8_t* dst, int32_t w,
                            int32_t h, int32_t radius);

@@ -16,6 +16,7 @@ uint32_t GradientColorAt(const hipr_gradient& g, int x, int y);
 void BlurredLayer(const hipr_scene* s, int p, std::vector<uint8_t>* blurred,
                   int fb[4]);
 int CubicQuadCountPublic(const Pt p[4]);
+PathData StrokeOutline(const PathData& quad_path, const hipr_paint& paint);
 }  // namespace orc
 
 using namespace orc;
@@ -103,6 +104,24 @@ int64_t orc_quad_path(const hipr_scene* scene, int32_t p, uint8_t* verbs,
   }
   if (n_pts) *n_pts = (int64_t)q.pts.size();
   return (int64_t)q.verbs.size();
+}
+
+int64_t orc_stroke_outline(const hipr_scene* scene, int32_t p, uint8_t* verbs,
+                           int64_t verb_cap, float* pts, int64_t pt_cap, float* weights,
+                           int64_t w_cap, int64_t* counts) {
+  PathData o = StrokeOutline(QuadPath(ScenePath(scene, p)), scene->paints[p]);
+  for (int64_t i = 0; i < (int64_t)o.verbs.size() && i < verb_cap; i++) verbs[i] = o.verbs[i];
+  for (int64_t i = 0; i < (int64_t)o.pts.size() && i < pt_cap; i++) {
+    pts[2 * i] = o.pts[i].x;
+    pts[2 * i + 1] = o.pts[i].y;
+  }
+  for (int64_t i = 0; i < (int64_t)o.weights.size() && i < w_cap; i++) weights[i] = o.weights[i];
+  if (counts) {
+    counts[0] = (int64_t)o.verbs.size();
+    counts[1] = (int64_t)o.pts.size();
+    counts[2] = (int64_t)o.weights.size();
+  }
+  return (int64_t)o.verbs.size();
 }
 
 void orc_stackblur(const uint8_t* src, uint8_t* dst, int32_t w, int32_t h,

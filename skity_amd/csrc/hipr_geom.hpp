@@ -246,7 +246,7 @@ HD bool SetQuad(EdgeRec* e, const P2 pts[3]) {
 
 // ---- Conic -> 2 quads ------------------------------------------------------
 HD bool Between(float a, float b, float c) { return (a - b) * (c - b) <= 0; }
-HD bool Finite(float v) { return !(isinf(v) || isnan(v)); }
+HD bool Finite(float v) { return !(__builtin_isinf(v) || __builtin_isnan(v)); }
 
 HD void ConicToQuads(const P2 src[3], float w, P2 out[5]) {
   float scale = 1.0f / (w + 1.0f);
@@ -348,7 +348,7 @@ HD int ValidUnitDivide(float number, float denom, float* ratio) {
   }
   if (denom == 0 || number == 0 || number >= denom) return 0;
   float r = number / denom;
-  if (isnan(r)) return 0;
+  if (__builtin_isnan(r)) return 0;
   if (r == 0) return 0;
   *ratio = r;
   return 1;

@@ -16,9 +16,15 @@
 
 #include "hipr_flatten.hpp"
 
+// The stroker is host-and-device code: the library runs it on the device;
+// a host build of the same functions checks it against the oracle on the CPU.
+#define HIPR_SD __host__ __device__
+
 namespace hipr {
 
 namespace stroke {
+
+HIPR_SD inline int Max(int a, int b) { return a > b ? a : b; }
 
 constexpr float kNearlyZero = 1.f / 4096;
 constexpr float kRoot2Over2 = 0.707106781f;
@@ -28,19 +34,19 @@ constexpr int kRecursiveLimits = 11 * 3;
 struct V {
   float x, y;
 };
-__device__ __forceinline__ V Add(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
-__device__ __forceinline__ V Sub(V a, V b) { return V{a.x - b.x, a.y - b.y}; }
-__device__ __forceinline__ V Mul(V a, float s) { return V{a.x * s, a.y * s}; }
-__device__ __forceinline__ bool Eq(V a, V b) { return a.x == b.x && a.y == b.y; }
-__device__ __forceinline__ float Dot(V a, V b) { return a.x * b.x + a.y * b.y; }
-__device__ __forceinline__ float Cross(V a, V b) { return a.x * b.y - a.y * b.x; }
-__device__ __forceinline__ bool NearlyZero(float x, float tol = kNearlyZero) {
+HIPR_SD inline V Add(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
+HIPR_SD inline V Sub(V a, V b) { return V{a.x - b.x, a.y - b.y}; }
+HIPR_SD inline V Mul(V a, float s) { return V{a.x * s, a.y * s}; }
+HIPR_SD inline bool Eq(V a, V b) { return a.x == b.x && a.y == b.y; }
+HIPR_SD inline float Dot(V a, V b) { return a.x * b.x + a.y * b.y; }
+HIPR_SD inline float Cross(V a, V b) { return a.x * b.y - a.y * b.x; }
+HIPR_SD inline bool NearlyZero(float x, float tol = kNearlyZero) {
   return fabsf(x) <= tol;
 }
-__device__ __forceinline__ bool IsFin(float v) { return !(isinf(v) || isnan(v)); }
+HIPR_SD inline bool IsFin(float v) { return !(__builtin_isinf(v) || __builtin_isnan(v)); }
 
 // PointSetLength<false> (src/geometry/point_priv.hpp)
-__device__ bool SetLength(V* pt, float x, float y, float length) {
+HIPR_SD bool SetLength(V* pt, float x, float y, float length) {
   double xx = x, yy = y;
   double dmag = sqrt(xx * xx + yy * yy);
   double dscale = (double)length / dmag;
@@ -55,30 +61,30 @@ __device__ bool SetLength(V* pt, float x, float y, float length) {
   pt->y = y;
   return true;
 }
-__device__ __forceinline__ V RotCCW(V s) { return V{s.y, -s.x}; }
-__device__ __forceinline__ V RotCW(V s) { return V{-s.y, s.x}; }
-__device__ __forceinline__ float DistSqd(V a, V b) {
+HIPR_SD inline V RotCCW(V s) { return V{s.y, -s.x}; }
+HIPR_SD inline V RotCW(V s) { return V{-s.y, s.x}; }
+HIPR_SD inline float DistSqd(V a, V b) {
   float dx = a.x - b.x, dy = a.y - b.y;
   return dx * dx + dy * dy;
 }
-__device__ __forceinline__ bool CanNorm(float dx, float dy) {
+HIPR_SD inline bool CanNorm(float dx, float dy) {
   return IsFin(dx) && IsFin(dy) && (dx != 0 || dy != 0);
 }
 
-__device__ V QuadEval(const V q[3], float t) {
+HIPR_SD V QuadEval(const V q[3], float t) {
   V C = q[0];
   V B = Add(Sub(q[1], C), Sub(q[1], C));
   V A = Add(Sub(q[2], Add(q[1], q[1])), C);
   return V{(A.x * t + B.x) * t + C.x, (A.y * t + B.y) * t + C.y};
 }
-__device__ V QuadTangent(const V q[3], float t) {
+HIPR_SD V QuadTangent(const V q[3], float t) {
   if ((t == 0 && Eq(q[0], q[1])) || (t == 1 && Eq(q[1], q[2]))) return Sub(q[2], q[0]);
   V B = Sub(q[1], q[0]);
   V A = Sub(Sub(q[2], q[1]), B);
   V T{A.x * t + B.x, A.y * t + B.y};
   return Add(T, T);
 }
-__device__ float PtToLine(V pt, V a, V b) {
+HIPR_SD float PtToLine(V pt, V a, V b) {
   V dxy = Sub(b, a), ab0 = Sub(pt, a);
   float number = Dot(dxy, ab0), denom = Dot(dxy, dxy);
   float t = number / denom;
@@ -88,14 +94,14 @@ __device__ float PtToLine(V pt, V a, V b) {
   }
   return DistSqd(pt, a);
 }
-__device__ int FindUnitQuadRoots(float A, float B, float C, float roots[2]) {
+HIPR_SD int FindUnitQuadRoots(float A, float B, float C, float roots[2]) {
   if (A == 0) return ValidUnitDivide(-C, B, roots);
   int n = 0;
   double dr = (double)B * B - 4 * (double)A * C;
   if (dr < 0) return 0;
   dr = sqrt(dr);
   float R = (float)dr;
-  if (isinf(R)) return 0;
+  if (__builtin_isinf(R)) return 0;
   float Q = (B < 0) ? -(B - R) / 2 : -(B + R) / 2;
   n += ValidUnitDivide(Q, A, &roots[n]);
   n += ValidUnitDivide(C, Q, &roots[n]);
@@ -123,7 +129,7 @@ struct PathW {
   V move_pt;          // last MoveTo point (for an injected move)
   bool move_open;     // a contour is open (Close not yet seen)
   bool any_move;
-  __device__ void Init(uint8_t* v, float* p, float* ww, int cv, int cp, int cw) {
+  HIPR_SD void Init(uint8_t* v, float* p, float* ww, int cv, int cp, int cw) {
     verbs = v;
     pts = p;
     w = ww;
@@ -132,7 +138,7 @@ struct PathW {
     cap_w = cw;
     Reset();
   }
-  __device__ void Reset() {
+  HIPR_SD void Reset() {
     nv = np = nw = 0;
     last_verb = -1;
     last_pt = V{0, 0};
@@ -140,12 +146,12 @@ struct PathW {
     move_open = false;
     any_move = false;
   }
-  __device__ void PutVerb(int v) {
+  HIPR_SD void PutVerb(int v) {
     if (verbs && nv < cap_v) verbs[nv] = (uint8_t)v;
     nv++;
     last_verb = v;
   }
-  __device__ void PutPt(V p) {
+  HIPR_SD void PutPt(V p) {
     if (pts && np < cap_p) {
       pts[2 * np] = p.x;
       pts[2 * np + 1] = p.y;
@@ -153,11 +159,11 @@ struct PathW {
     np++;
     last_pt = p;
   }
-  __device__ void PutW(float x) {
+  HIPR_SD void PutW(float x) {
     if (w && nw < cap_w) w[nw] = x;
     nw++;
   }
-  __device__ void MoveTo(V p) {
+  HIPR_SD void MoveTo(V p) {
     if (last_verb == HIPR_VERB_MOVE) {
       if (pts && np - 1 < cap_p) {
         pts[2 * (np - 1)] = p.x;
@@ -172,24 +178,24 @@ struct PathW {
     move_open = true;
     any_move = true;
   }
-  __device__ void Inject() {
+  HIPR_SD void Inject() {
     if (!move_open) MoveTo(any_move ? move_pt : V{0, 0});
   }
-  __device__ void LineTo(V p) {
+  HIPR_SD void LineTo(V p) {
     Inject();
     PutVerb(HIPR_VERB_LINE);
     PutPt(p);
   }
-  __device__ void QuadTo(V c, V p) {
+  HIPR_SD void QuadTo(V c, V p) {
     Inject();
     PutVerb(HIPR_VERB_QUAD);
     PutPt(c);
     PutPt(p);
   }
-  __device__ void ConicTo(V c, V p, float wt) {
+  HIPR_SD void ConicTo(V c, V p, float wt) {
     if (!(wt > 0)) {
       LineTo(p);
-    } else if (isinf(wt)) {
+    } else if (__builtin_isinf(wt)) {
       LineTo(c);
       LineTo(p);
     } else if (wt == 1.f) {
@@ -202,7 +208,7 @@ struct PathW {
       PutW(wt);
     }
   }
-  __device__ void Close() {
+  HIPR_SD void Close() {
     if (nv > 0 && last_verb != HIPR_VERB_CLOSE) PutVerb(HIPR_VERB_CLOSE);
     move_open = false;
   }
@@ -211,12 +217,12 @@ struct PathW {
 struct M3 {
   float a, b, tx, c, d, ty;
 };
-__device__ bool NearIdentity(const M3& m) {
+HIPR_SD bool NearIdentity(const M3& m) {
   const float e = 1.1920929e-07f;
   return fabsf(m.a - 1) <= e && fabsf(m.d - 1) <= e && fabsf(m.b) <= e && fabsf(m.c) <= e &&
          fabsf(m.tx) <= e && fabsf(m.ty) <= e;
 }
-__device__ M3 MMul(const M3& p, const M3& q) {
+HIPR_SD M3 MMul(const M3& p, const M3& q) {
   if (NearIdentity(p)) return q;
   if (NearIdentity(q)) return p;
   M3 r;
@@ -228,7 +234,7 @@ __device__ M3 MMul(const M3& p, const M3& q) {
   r.ty = (p.c * q.tx + p.d * q.ty) + p.ty;
   return r;
 }
-__device__ __forceinline__ V MApply(const M3& m, V p) {
+HIPR_SD inline V MApply(const M3& m, V p) {
   return V{(m.a * p.x + m.b * p.y) + m.tx, (m.c * p.x + m.d * p.y) + m.ty};
 }
 
@@ -237,20 +243,20 @@ struct QC {
   V tan_start, tan_end;
   float start_t, mid_t, end_t;
   bool start_set, end_set;
-  __device__ bool Init(float s, float e) {
+  HIPR_SD bool Init(float s, float e) {
     start_t = s;
     mid_t = (s + e) * 0.5f;
     end_t = e;
     start_set = end_set = false;
     return start_t < mid_t && mid_t < end_t;
   }
-  __device__ void InitWithStart(const QC& p) {
+  HIPR_SD void InitWithStart(const QC& p) {
     Init(p.start_t, p.mid_t);
     quad[0] = p.quad[0];
     tan_start = p.tan_start;
     start_set = true;
   }
-  __device__ void InitWithEnd(const QC& p) {
+  HIPR_SD void InitWithEnd(const QC& p) {
     Init(p.mid_t, p.end_t);
     quad[2] = p.quad[2];
     tan_end = p.tan_end;
@@ -265,13 +271,14 @@ struct Stroker {
   PathW* inner;
   V first_normal, prev_normal, first_unit, prev_unit, first_pt, prev_pt, first_outer_pt;
   int seg_count;
+  int max_inner_v, max_inner_p, max_inner_w;  // largest inner path reversed
   bool join_completed;
   int depth;
   int stroke_type;
   QC stack[kRecursiveLimits + 3];
   int state[kRecursiveLimits + 3];
 
-  __device__ void Setup(float r, float ml, int cp, int jn, PathW* o, PathW* in) {
+  HIPR_SD void Setup(float r, float ml, int cp, int jn, PathW* o, PathW* in) {
     radius = r;
     miter_limit = ml;
     cap = cp;
@@ -281,6 +288,7 @@ struct Stroker {
     outer = o;
     inner = in;
     seg_count = -1;
+    max_inner_v = max_inner_p = max_inner_w = 0;
     join_completed = false;
     depth = 0;
     stroke_type = 1;
@@ -289,15 +297,15 @@ struct Stroker {
   }
 
   // ---- joins (stroke.cc:200-330) ----
-  __device__ void InnerJoin(PathW* in, V pivot, V after) {
+  HIPR_SD void InnerJoin(PathW* in, V pivot, V after) {
     in->LineTo(pivot);
     in->LineTo(V{pivot.x - after.x, pivot.y - after.y});
   }
-  __device__ int AngleType(float dot) {  // 0 180, 1 sharp, 2 shallow, 3 line
+  HIPR_SD int AngleType(float dot) {  // 0 180, 1 sharp, 2 shallow, 3 line
     if (dot >= 0) return NearlyZero(1.f - dot) ? 3 : 2;
     return NearlyZero(1.f + dot) ? 0 : 1;
   }
-  __device__ int BuildUnitArc(V start, V stop, bool ccw, const M3& user, V pts[5][3],
+  HIPR_SD int BuildUnitArc(V start, V stop, bool ccw, const M3& user, V pts[5][3],
                               float ws[5]) {
     float x = Dot(start, stop);
     float y = Cross(start, stop);
@@ -346,7 +354,7 @@ struct Stroker {
       for (int jj = 0; jj < 3; jj++) pts[i][jj] = MApply(m, pts[i][jj]);
     return count;
   }
-  __device__ void Join(V before_un, V pivot, V after_un) {
+  HIPR_SD void Join(V before_un, V pivot, V after_un) {
     PathW* o = outer;
     PathW* in = inner;
     if (join == HIPR_JOIN_BEVEL) {
@@ -423,7 +431,7 @@ struct Stroker {
     InnerJoin(in, pivot, after);
   }
   // ---- caps (stroke.cc:163-194) ----
-  __device__ void Cap(V pivot, V normal, V stop) {
+  HIPR_SD void Cap(V pivot, V normal, V stop) {
     if (cap == HIPR_CAP_BUTT) {
       outer->LineTo(stop);
     } else if (cap == HIPR_CAP_SQUARE) {
@@ -438,14 +446,14 @@ struct Stroker {
       outer->ConicTo(Sub(pc, normal), stop, kRoot2Over2);
     }
   }
-  __device__ bool NormalUnit(V before, V after, V* normal, V* unit) {
+  HIPR_SD bool NormalUnit(V before, V after, V* normal, V* unit) {
     if (!SetLength(unit, (after.x - before.x) * 1.f, (after.y - before.y) * 1.f, 1.0f))
       return false;
     *unit = RotCCW(*unit);
     *normal = Mul(*unit, radius);
     return true;
   }
-  __device__ bool PreJoinTo(V curr, V* normal, V* unit) {
+  HIPR_SD bool PreJoinTo(V curr, V* normal, V* unit) {
     float px = prev_pt.x, py = prev_pt.y;
     if (!NormalUnit(prev_pt, curr, normal, unit)) {
       if (cap == HIPR_CAP_BUTT) return false;
@@ -463,7 +471,7 @@ struct Stroker {
     }
     return true;
   }
-  __device__ void PostJoinTo(V curr, V normal, V unit) {
+  HIPR_SD void PostJoinTo(V curr, V normal, V unit) {
     join_completed = true;
     prev_pt = curr;
     prev_unit = unit;
@@ -471,7 +479,11 @@ struct Stroker {
     seg_count += 1;
   }
   // Path::ReversePathTo of the inner contour onto the outer path.
-  __device__ void ReverseInner() {
+  HIPR_SD void ReverseInner() {
+    // the inner path is largest here (after the closing join or the end cap)
+    max_inner_v = Max(max_inner_v, inner->nv);
+    max_inner_p = Max(max_inner_p, inner->np);
+    max_inner_w = Max(max_inner_w, inner->nw);
     if (inner->nv == 0) return;
     if (!outer->verbs) {
       // count mode: the reversal appends every non-move verb and point
@@ -498,7 +510,7 @@ struct Stroker {
       }
     }
   }
-  __device__ void FinishContour(bool close) {
+  HIPR_SD void FinishContour(bool close) {
     if (seg_count > 0) {
       if (close) {
         Join(prev_unit, prev_pt, first_unit);
@@ -518,13 +530,13 @@ struct Stroker {
     inner->Reset();
     seg_count = -1;
   }
-  __device__ void MoveTo(V pt) {
+  HIPR_SD void MoveTo(V pt) {
     if (seg_count > 0) FinishContour(false);
     seg_count = 0;
     first_pt = prev_pt = pt;
     join_completed = false;
   }
-  __device__ void LineTo(V curr) {
+  HIPR_SD void LineTo(V curr) {
     bool teeny = NearlyZero(prev_pt.x - curr.x, kNearlyZero * inv_res) &&
                  NearlyZero(prev_pt.y - curr.y, kNearlyZero * inv_res);
     if (teeny && cap == HIPR_CAP_BUTT) return;
@@ -536,7 +548,7 @@ struct Stroker {
     PostJoinTo(curr, normal, unit);
   }
   // ---- quad stroking (stroke.cc:486-712) ----
-  __device__ void SetRayPts(V tpt, V* dxy, V* on, V* tangent) {
+  HIPR_SD void SetRayPts(V tpt, V* dxy, V* on, V* tangent) {
     if (!SetLength(dxy, dxy->x, dxy->y, radius)) {
       dxy->x = radius;
       dxy->y = 0;
@@ -549,14 +561,14 @@ struct Stroker {
       tangent->y = on->y + dxy->y;
     }
   }
-  __device__ void QuadPerpRay(const V q[3], float t, V* tpt, V* on, V* tangent) {
+  HIPR_SD void QuadPerpRay(const V q[3], float t, V* tpt, V* on, V* tangent) {
     float tc = t < 0 ? 0 : (t > 1.f ? 1.f : t);
     *tpt = QuadEval(q, tc);
     V dxy = QuadTangent(q, tc);
     if (dxy.x == 0 && dxy.y == 0) dxy = Sub(q[2], q[0]);
     SetRayPts(*tpt, &dxy, on, tangent);
   }
-  __device__ int IntersectRay(QC* qp) {  // 0 split, 1 degenerate, 2 quad
+  HIPR_SD int IntersectRay(QC* qp) {  // 0 split, 1 degenerate, 2 quad
     V start = qp->quad[0], end = qp->quad[2];
     V aLen = Sub(qp->tan_start, start), bLen = Sub(qp->tan_end, end);
     float denom = Cross(aLen, bLen);
@@ -578,7 +590,7 @@ struct Stroker {
     }
     return 1;
   }
-  __device__ bool SharpAngle(const V q[3]) {
+  HIPR_SD bool SharpAngle(const V q[3]) {
     V smaller = Sub(q[1], q[0]), larger = Sub(q[1], q[2]);
     float sl = Dot(smaller, smaller), ll = Dot(larger, larger);
     if (sl > ll) {
@@ -590,7 +602,7 @@ struct Stroker {
     if (!SetLength(&smaller, smaller.x, smaller.y, ll)) return false;
     return Dot(smaller, larger) > 0;
   }
-  __device__ int CloseEnough(const V s[3], const V ray[2], QC* qp) {
+  HIPR_SD int CloseEnough(const V s[3], const V ray[2], QC* qp) {
     V mid = QuadEval(s, 0.5f);
     if (DistSqd(ray[0], mid) <= inv_res * inv_res) return SharpAngle(qp->quad) ? 0 : 2;
     float xmin = fminf(fminf(s[0].x, s[1].x), s[2].x);
@@ -615,7 +627,7 @@ struct Stroker {
     if (DistSqd(ray[0], qpt) <= error * error) return SharpAngle(qp->quad) ? 0 : 2;
     return 0;
   }
-  __device__ int Compare(const V q[3], QC* qp) {
+  HIPR_SD int Compare(const V q[3], QC* qp) {
     if (!qp->start_set) {
       V tpt;
       QuadPerpRay(q, qp->start_t, &tpt, &qp->quad[0], &qp->tan_start);
@@ -633,7 +645,7 @@ struct Stroker {
     return CloseEnough(qp->quad, ray, qp);
   }
   // PathStroker::QuadStroke with its recursion as an explicit stack.
-  __device__ void QuadStroke(const V q[3]) {
+  HIPR_SD void QuadStroke(const V q[3]) {
     int sp = 0;
     stack[0].Init(0, 1);
     state[0] = 0;
@@ -667,7 +679,7 @@ struct Stroker {
       }
     }
   }
-  __device__ int CheckQuadLinear(const V q[3], V* reduction) {
+  HIPR_SD int CheckQuadLinear(const V q[3], V* reduction) {
     V p1c = Sub(q[1], q[0]);
     bool d1 = !CanNorm(p1c.x, p1c.y);
     V cp2 = Sub(q[2], q[1]);
@@ -713,7 +725,7 @@ struct Stroker {
     *reduction = QuadEval(q, t);
     return 3;
   }
-  __device__ void QuadTo(V c, V p2) {
+  HIPR_SD void QuadTo(V c, V p2) {
     V quad[3] = {prev_pt, c, p2};
     V reduction;
     int rt = CheckQuadLinear(quad, &reduction);
@@ -756,20 +768,15 @@ struct StrokeOut {
   int32_t inner_nv, inner_np, inner_nw;    // largest inner contour
 };
 
-template <int kMode>
-__global__ void __launch_bounds__(64)
-    StrokeKernel(const Job* __restrict__ jobs, const int32_t* __restrict__ list, int n,
-                 GeomSrc scene, const hipr_paint* __restrict__ paints, StrokeOut* outs,
-                 uint8_t* o_verbs, float* o_pts, float* o_w, uint8_t* s_verbs, float* s_pts,
-                 float* s_w) {
-  int li = blockIdx.x * blockDim.x + threadIdx.x;
-  if (li >= n) return;
+// Outline of one stroke job.  mode 0 counts (fills so's sizes); mode 1
+// writes into the regions `so` describes.
+HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc scene,
+                              StrokeOut* so_io, int mode, uint8_t* o_verbs, float* o_pts,
+                              float* o_w, uint8_t* s_verbs, float* s_pts, float* s_w) {
   using namespace stroke;
-  const Job job = jobs[list[li]];
-  const hipr_paint paint = paints[job.path];
-  StrokeOut so = outs[li];
+  StrokeOut so = *so_io;
   PathW outer, inner;
-  if (kMode == 1) {
+  if (mode == 1) {
     outer.Init(o_verbs + so.verb_begin, o_pts + 2 * (int64_t)so.pt_begin, o_w + so.w_begin,
                so.nv, so.np, so.nw);
     inner.Init(s_verbs + so.scratch_begin, s_pts + 2 * (int64_t)so.scratch_begin,
@@ -778,17 +785,10 @@ __global__ void __launch_bounds__(64)
     outer.Init(nullptr, nullptr, nullptr, 0, 0, 0);
     inner.Init(nullptr, nullptr, nullptr, 0, 0, 0);
   }
-  Stroker* st = nullptr;
   Stroker stk;
-  st = &stk;
+  Stroker* st = &stk;
   st->Setup(paint.stroke_width * 0.5f, paint.miter_limit, paint.cap, paint.join, &outer,
             &inner);
-  int max_inner_v = 0, max_inner_p = 0, max_inner_w = 0;
-  auto track_inner = [&]() {
-    max_inner_v = max(max_inner_v, inner.nv);
-    max_inner_p = max(max_inner_p, inner.np);
-    max_inner_w = max(max_inner_w, inner.nw);
-  };
   // Iterate the source path as its quad path (Stroke::QuadPath) and feed the
   // quad path through Path::Iter(quad, false) into the stroker.
   const uint8_t* verbs = scene.verbs + job.verb_begin;
@@ -814,7 +814,6 @@ __global__ void __launch_bounds__(64)
       src_move = src_last = P(pi);
       pi++;
       q_move = q_last = src_move;
-      track_inner();
       st->MoveTo(q_move);
     } else if (v == HIPR_VERB_LINE) {
       V e = P(pi);
@@ -859,22 +858,34 @@ __global__ void __launch_bounds__(64)
         src_last = src_move;
       }
       if (!Eq(q_last, q_move)) q_line(q_move);
-      track_inner();
       st->FinishContour(true);
       src_last = src_move;
     }
   }
-  track_inner();
   st->FinishContour(false);
-  if (kMode == 0) {
+  if (mode == 0) {
     so.nv = outer.nv;
     so.np = outer.np;
     so.nw = outer.nw;
-    so.inner_nv = max(max_inner_v, 1);
-    so.inner_np = max(max_inner_p, 1);
-    so.inner_nw = max(max_inner_w, 1);
-    outs[li] = so;
+    so.inner_nv = Max(st->max_inner_v, 1);
+    so.inner_np = Max(st->max_inner_p, 1);
+    so.inner_nw = Max(st->max_inner_w, 1);
+    *so_io = so;
   }
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(64)
+    StrokeKernel(const Job* __restrict__ jobs, const int32_t* __restrict__ list, int n,
+                 GeomSrc scene, const hipr_paint* __restrict__ paints, StrokeOut* outs,
+                 uint8_t* o_verbs, float* o_pts, float* o_w, uint8_t* s_verbs, float* s_pts,
+                 float* s_w) {
+  int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= n) return;
+  const Job job = jobs[list[li]];
+  StrokeOut so = outs[li];
+  StrokeOne(job, paints[job.path], scene, &so, kMode, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w);
+  if (kMode == 0) outs[li] = so;
 }
 
 }  // namespace hipr

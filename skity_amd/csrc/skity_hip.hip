@@ -59,6 +59,24 @@ struct DevBuf {
     if (e == hipSuccess) cap = want;
     return e;
   }
+  // Grows like Ensure but keeps the first `keep` bytes of the old contents.
+  hipError_t EnsureKeep(size_t bytes, size_t keep, hipStream_t s) {
+    if (bytes <= cap && p) return hipSuccess;
+    size_t want = std::max<size_t>(bytes, 256);
+    want = want + want / 4;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, want);
+    if (e != hipSuccess) return e;
+    if (p) {
+      keep = std::min(keep, cap);
+      if (keep) e = hipMemcpyAsync(q, p, keep, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      hipFree(p);
+    }
+    p = q;
+    cap = want;
+    return e;
+  }
   template <class T>
   T* as() const {
     return (T*)p;
@@ -224,7 +242,7 @@ struct hipr_context_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // per-draw device buffers (grow-only)
   DevBuf canvases, layer_canvases, geo, cov, edge_base, edge_cap, edges, wedges, gkeys;
-  DevBuf plane_base, plane_cap, planes, map_base, map0, map1, sub_list;
+  DevBuf plane_base, plane_cap, planes, map_base, map0, map1, sub_list, big_list;
   DevBuf stroke_out, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w;
   DevBuf layers, blur_rows, blur_cols, blur_rb, blur_cb, blur_tmp, blur_jobs, blur_lines;
   // debug/utility uploads
@@ -917,11 +935,15 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   wp.lds_width = lds_width;
   wp.map_base = ctx->map_base.as<int64_t>();
   // ---- stage (b): walk ----
-  std::vector<int64_t> pbase(nj + 1, 0);
+  // The walk keeps the active edge list in registers, 4 slots per lane; a
+  // job whose list outgrows 256 edges (flag 2) is walked again by the
+  // 16-slot instantiation (1024 edges).
+  std::vector<int64_t> pbase(nj, 0);
   std::vector<int32_t> pcap(nj, 0);
-  auto set_planes = [&]() -> hipr_result {
-    for (int j = 0; j < nj; j++) pbase[j + 1] = pbase[j] + pcap[j];
-    if (ctx->planes.Ensure((size_t)kTilePlaneBytes * std::max<int64_t>(pbase[nj], 1)) != hipSuccess)
+  int64_t plane_total = 0;
+  auto upload_planes = [&]() -> hipr_result {
+    if (ctx->planes.EnsureKeep((size_t)kTilePlaneBytes * std::max<int64_t>(plane_total, 1),
+                               ctx->planes.cap, st) != hipSuccess)
       return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
     HIP_TRY(hipMemcpyAsync(ctx->plane_base.p, pbase.data(), sizeof(int64_t) * nj,
                            hipMemcpyHostToDevice, st));
@@ -929,10 +951,20 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
                            hipMemcpyHostToDevice, st));
     return HIPR_SUCCESS;
   };
-  auto check_limits = [&]() -> hipr_result {
-    for (int j = 0; j < nj; j++) {
-      if (f.jobs[j].kind == kJobImage) continue;
-      if (ctx->h_cov[j].flags & (2 | 16 | 32)) {
+  auto assign_planes = [&](const std::vector<int32_t>& js) {
+    for (int32_t j : js) {
+      pbase[j] = plane_total;
+      plane_total += pcap[j];
+    }
+  };
+  auto read_cov = [&]() -> hipr_result {
+    HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return HIPR_SUCCESS;
+  };
+  auto limit_error = [&](const std::vector<int32_t>& js, int mask) -> hipr_result {
+    for (int32_t j : js) {
+      if (ctx->h_cov[j].flags & mask) {
         std::fprintf(stderr, "skity_hip: draw %d exceeds a device limit (flags %d)\n",
                      f.jobs[j].path, ctx->h_cov[j].flags);
         return HIPR_ERROR_CAPACITY;
@@ -940,45 +972,91 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     }
     return HIPR_SUCCESS;
   };
+  auto walk = [&](int kmode, const std::vector<int32_t>& js, bool wide, DevBuf* dl) -> hipr_result {
+    if (js.empty()) return HIPR_SUCCESS;
+    WalkParams w = wp;
+    if (dl) {
+      hipr_result rr = Upload(dl, js.data(), js.size(), st);
+      if (rr != HIPR_SUCCESS) return rr;
+      w.walk_list = dl->as<int32_t>();
+    }
+    w.n_walk = (int32_t)js.size();
+    if (kmode == 0) return wide ? LaunchWalk<16, 0>(ctx, w, st) : LaunchWalk<4, 0>(ctx, w, st);
+    return wide ? LaunchWalk<16, 1>(ctx, w, st) : LaunchWalk<4, 1>(ctx, w, st);
+  };
+  auto sort_list = [&](const std::vector<int32_t>& js, DevBuf* dl) -> hipr_result {
+    if (js.empty()) return HIPR_SUCCESS;
+    hipr_result rr = Upload(dl, js.data(), js.size(), st);
+    if (rr != HIPR_SUCCESS) return rr;
+    return SortGather(f, dl->as<int32_t>(), (int)js.size());
+  };
+  const std::vector<int32_t>& all = pl->walk_jobs;
+  std::vector<int32_t> small, big;
+  auto split_wide = [&]() {
+    small.clear();
+    big.clear();
+    for (int32_t j : all) (ctx->h_cov[j].flags & 2 ? big : small).push_back(j);
+  };
   if (mode == 0) {
-    if ((r = LaunchWalk<4, 0>(ctx, wp, st)) != HIPR_SUCCESS) return r;
-    HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if ((r = check_limits()) != HIPR_SUCCESS) return r;
-    for (int j = 0; j < nj; j++) pcap[j] = f.jobs[j].kind == kJobImage ? 0 : ctx->h_cov[j].planes;
-    if ((r = set_planes()) != HIPR_SUCCESS) return r;
+    if ((r = walk(0, all, false, nullptr)) != HIPR_SUCCESS) return r;
+    if ((r = read_cov()) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(all, 16 | 32)) != HIPR_SUCCESS) return r;
+    split_wide();
+    if (!big.empty()) {
+      if ((r = sort_list(big, &ctx->big_list)) != HIPR_SUCCESS) return r;
+      if ((r = walk(0, big, true, &ctx->big_list)) != HIPR_SUCCESS) return r;
+      if ((r = read_cov()) != HIPR_SUCCESS) return r;
+      if ((r = limit_error(big, 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+    }
+    for (int32_t j : all) pcap[j] = ctx->h_cov[j].planes;
+    assign_planes(all);
+    if ((r = upload_planes()) != HIPR_SUCCESS) return r;
     if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk)) != HIPR_SUCCESS) return r;
   } else {
-    for (int j = 0; j < nj; j++) pcap[j] = f.jobs[j].kind == kJobImage ? 0 : test[j];
-    if ((r = set_planes()) != HIPR_SUCCESS) return r;
+    for (int32_t j : all) pcap[j] = test[j];
+    assign_planes(all);
+    if ((r = upload_planes()) != HIPR_SUCCESS) return r;
   }
   wp.plane_base = ctx->plane_base.as<int64_t>();
   wp.plane_cap = ctx->plane_cap.as<int32_t>();
-  wp.planes = ctx->planes.as<uint8_t>();
   wp.map0 = ctx->map0.as<int32_t>();
   wp.map1 = ctx->map1.as<int32_t>();
-  if ((r = LaunchWalk<4, 1>(ctx, wp, st)) != HIPR_SUCCESS) return r;
-  HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if ((r = check_limits()) != HIPR_SUCCESS) return r;
-  if (mode == 1) {
-    std::vector<int32_t> redo;
-    for (int j = 0; j < nj; j++)
-      if (f.jobs[j].kind != kJobImage && ctx->h_cov[j].planes > pcap[j]) redo.push_back(j);
-    if (!redo.empty()) {
-      // exact fallback for the under-estimated draws: their write pass
-      // counted every plane; give them exactly that and walk them again.
-      f.stats.fallbacks += (int64_t)redo.size();
-      for (int j : redo) pcap[j] = ctx->h_cov[j].planes;
-      if ((r = set_planes()) != HIPR_SUCCESS) return r;
-      if ((r = Upload(&ctx->sub_list, redo.data(), redo.size(), st)) != HIPR_SUCCESS) return r;
-      // every job's planes moved: re-walk all jobs whose base changed is
-      // equivalent to re-walking all; keep it simple and exact.
-      if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk)) != HIPR_SUCCESS) return r;
+  wp.planes = ctx->planes.as<uint8_t>();
+  if (mode == 0) {
+    if ((r = walk(1, small, false, &ctx->sub_list)) != HIPR_SUCCESS) return r;
+    if ((r = walk(1, big, true, &ctx->big_list)) != HIPR_SUCCESS) return r;
+    if ((r = read_cov()) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(all, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+  } else {
+    if ((r = walk(1, all, false, nullptr)) != HIPR_SUCCESS) return r;
+    if ((r = read_cov()) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(all, 16 | 32)) != HIPR_SUCCESS) return r;
+    // exact fallback: under-estimated jobs (flag 1; the write pass counted
+    // their planes exactly) get exactly that many planes appended; jobs too
+    // wide for 4 slots are counted by the 16-slot walk first.
+    std::vector<int32_t> redo_small;
+    split_wide();
+    for (int32_t j : small)
+      if (ctx->h_cov[j].planes > pcap[j]) redo_small.push_back(j);
+    if (!big.empty()) {
+      if ((r = sort_list(big, &ctx->big_list)) != HIPR_SUCCESS) return r;
+      if ((r = walk(0, big, true, &ctx->big_list)) != HIPR_SUCCESS) return r;
+      if ((r = read_cov()) != HIPR_SUCCESS) return r;
+      if ((r = limit_error(big, 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+    }
+    if (!redo_small.empty() || !big.empty()) {
+      f.stats.fallbacks += (int64_t)(redo_small.size() + big.size());
+      std::vector<int32_t> redo = redo_small;
+      redo.insert(redo.end(), big.begin(), big.end());
+      for (int32_t j : redo) pcap[j] = ctx->h_cov[j].planes;
+      assign_planes(redo);
+      if ((r = upload_planes()) != HIPR_SUCCESS) return r;
       wp.planes = ctx->planes.as<uint8_t>();
-      if ((r = LaunchWalk<4, 1>(ctx, wp, st)) != HIPR_SUCCESS) return r;
-      HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
+      if ((r = sort_list(redo, &ctx->sub_list)) != HIPR_SUCCESS) return r;
+      if ((r = walk(1, redo_small, false, &ctx->sub_list)) != HIPR_SUCCESS) return r;
+      if ((r = walk(1, big, true, &ctx->big_list)) != HIPR_SUCCESS) return r;
+      if ((r = read_cov()) != HIPR_SUCCESS) return r;
+      if ((r = limit_error(redo, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
     }
   }
   int64_t max_active = 0;
@@ -1183,7 +1261,7 @@ HIPR_API void hipr_context_destroy(hipr_context ctx) {
   DevBuf* bufs[] = {&ctx->canvases, &ctx->layer_canvases, &ctx->geo, &ctx->cov, &ctx->edge_base,
                     &ctx->edge_cap, &ctx->edges, &ctx->wedges, &ctx->gkeys, &ctx->plane_base,
                     &ctx->plane_cap, &ctx->planes, &ctx->map_base, &ctx->map0, &ctx->map1,
-                    &ctx->sub_list, &ctx->stroke_out, &ctx->o_verbs, &ctx->o_pts, &ctx->o_w,
+                    &ctx->sub_list, &ctx->big_list, &ctx->stroke_out, &ctx->o_verbs, &ctx->o_pts, &ctx->o_w,
                     &ctx->s_verbs, &ctx->s_pts, &ctx->s_w, &ctx->layers, &ctx->blur_rows,
                     &ctx->blur_cols, &ctx->blur_rb, &ctx->blur_cb, &ctx->blur_tmp,
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,

@@ -1,0 +1,35 @@
+// Host build of the device stroker (StrokeOne) for CPU comparison with the oracle.
+#include <cstdint>
+#include <vector>
+#include "../../include/skity_hip.h"
+#include "../../skity_amd/csrc/hipr_stroke.hpp"
+using namespace hipr;
+extern "C" __attribute__((visibility("default")))
+int64_t hs_stroke_outline(const hipr_scene* s, int32_t p, uint8_t* verbs, int64_t vcap,
+                          float* pts, int64_t pcap, float* ws, int64_t wcap, int64_t* counts) {
+  Job job{};
+  job.path = p;
+  job.kind = kJobStroke;
+  job.verb_begin = s->verb_offsets[p];
+  job.verb_end = s->verb_offsets[p + 1];
+  job.pt_begin = s->point_offsets[p];
+  job.w_begin = s->weight_offsets[p];
+  GeomSrc g{s->verbs, s->points, s->weights};
+  StrokeOut so{};
+  StrokeOne(job, s->paints[p], g, &so, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  int scratch = so.inner_nv > so.inner_np ? so.inner_nv : so.inner_np;
+  if (so.inner_nw > scratch) scratch = so.inner_nw;
+  std::vector<uint8_t> ov(so.nv + 1), sv(scratch + 1);
+  std::vector<float> op(2 * so.np + 2), ow(so.nw + 1), sp(2 * scratch + 2), sw(scratch + 1);
+  StrokeOut w = so;
+  w.verb_begin = w.pt_begin = w.w_begin = w.scratch_begin = 0;
+  StrokeOne(job, s->paints[p], g, &w, 1, ov.data(), op.data(), ow.data(), sv.data(), sp.data(),
+            sw.data());
+  for (int i = 0; i < so.nv && i < vcap; i++) verbs[i] = ov[i];
+  for (int i = 0; i < 2 * so.np && i < 2 * pcap; i++) pts[i] = op[i];
+  for (int i = 0; i < so.nw && i < wcap; i++) ws[i] = ow[i];
+  counts[0] = so.nv;
+  counts[1] = so.np;
+  counts[2] = so.nw;
+  return so.nv;
+}

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_py as O
-from skity_amd import device
+from skity_amd import abi, device
 
 from helpers import diff_report, fill_only, no_blur, workload
 
@@ -93,3 +93,26 @@ def test_gradients_with_blur(ctx):
     mx, cnt = diff_report(got, want)
     print(f"gradients+blur: max diff {mx}, differing pixels {cnt}", flush=True)
     assert mx <= 1
+
+
+@pytest.mark.parametrize("fill", [abi.FILL_WINDING, abi.FILL_EVEN_ODD])
+def test_wide_active_edge_list(ctx, fill):
+    """A comb whose rows cross 600 edges: more than the 4-slot (256-edge)
+    register list holds, so the 16-slot walk takes the job."""
+    from skity_amd.canvas import Canvas, Paint, Path
+    c = Canvas(1220, 240)
+    p = Path().move_to(0.5, 220.25)
+    for i in range(300):
+        x = 2.3 + i * 4
+        p.line_to(x, 10.6 + (i % 7))
+        p.line_to(x + 2.1, 220.25 - (i % 5))
+    p.close()
+    p.fill_type = fill
+    c.draw_path(p, Paint((0.2, 0.5, 0.9, 0.9)))
+    sc = c.scene()
+    got, st = render_gpu(ctx, sc)
+    want = O.render(sc.scene, sc.width, sc.height)
+    mx, cnt = diff_report(got, want)
+    print(f"comb: max active {st['max_active']}, max diff {mx}, differing pixels {cnt}", flush=True)
+    assert st["max_active"] > 256
+    assert mx <= 1 and cnt == 0
