@@ -231,10 +231,10 @@ __global__ void __launch_bounds__(64)
                              true, false);
       if (last) sink.Line(P2{qend.x + job.tx, qend.y + job.ty}, mv, true);
     }
-    edge_carry += __shfl(incl, kWave - 1, kWave);
-    pt_carry += __shfl(pi_incl, kWave - 1, kWave);
-    w_carry += __shfl(wi_incl, kWave - 1, kWave);
-    move_carry = __shfl(move_pt, kWave - 1, kWave);
+    edge_carry += __builtin_amdgcn_readlane(incl, kWave - 1);
+    pt_carry += __builtin_amdgcn_readlane(pi_incl, kWave - 1);
+    w_carry += __builtin_amdgcn_readlane(wi_incl, kWave - 1);
+    move_carry = __builtin_amdgcn_readlane(move_pt, kWave - 1);
   }
   float l = WaveMinF(sink.minx), t = WaveMinF(sink.miny);
   float r = WaveMaxF(sink.maxx), b = WaveMaxF(sink.maxy);

@@ -251,8 +251,8 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       }
       int iA = WaveInclusiveSum(sA), iD = WaveInclusiveSum(sD);
       int eA = iA - sA + carryA, eD = iD - sD + carryD;
-      carryA += __shfl(iA, kWave - 1, kWave);
-      carryD += __shfl(iD, kWave - 1, kWave);
+      carryA += __builtin_amdgcn_readlane(iA, kWave - 1);
+      carryD += __builtin_amdgcn_readlane(iD, kWave - 1);
       uint32_t c0 = 0, c1 = 0;
       bool stacked = false;
 #pragma unroll
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       int req = (need0 ? 1 : 0) + (need1 ? 1 : 0);
       int rincl = WaveInclusiveSum(req);
       int first_slot = next_slot + rincl - req;
-      next_slot += __shfl(rincl, kWave - 1, kWave);
+      next_slot += __builtin_amdgcn_readlane(rincl, kWave - 1);
       if (need0) {
         L.slot0[tcol] = first_slot;
         n_tiles++;
@@ -546,20 +546,20 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         dmax = Max32(dmax, hi);
       }
       // carries
-      carry_w = __shfl(incl, kWave - 1, kWave);
-      carry_in = __shfl(in, kWave - 1, kWave);
-      int lastl = __shfl(lsel, kWave - 1, kWave);
+      carry_w = __builtin_amdgcn_readlane(incl, kWave - 1);
+      carry_in = __builtin_amdgcn_readlane(in, kWave - 1);
+      int lastl = __builtin_amdgcn_readlane(lsel, kWave - 1);
       if (lastl >= 0) {
         carry_lidx = s * kWave + lastl;
-        carry_lxo = __shfl(xo, lastl, kWave);
-        carry_lxn = __shfl(xn, lastl, kWave);
-        carry_ldy = __shfl(DY[s], lastl, kWave);
+        carry_lxo = __builtin_amdgcn_readlane(xo, lastl);
+        carry_lxn = __builtin_amdgcn_readlane(xn, lastl);
+        carry_ldy = __builtin_amdgcn_readlane(DY[s], lastl);
       }
-      int lastr = __shfl(rsel_incl, kWave - 1, kWave);
+      int lastr = __builtin_amdgcn_readlane(rsel_incl, kWave - 1);
       if (lastr >= 0) {
         carry_ridx = s * kWave + lastr;
-        carry_rxo = __shfl(xo, lastr, kWave);
-        carry_rxn = __shfl(xn, lastr, kWave);
+        carry_rxo = __builtin_amdgcn_readlane(xo, lastr);
+        carry_rxn = __builtin_amdgcn_readlane(xn, lastr);
       }
     }
     // open interval at the end of the list (right edge culled away)
@@ -642,12 +642,12 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         }
         if (sv && hp && X[s] < px) sorted = false;
         (void)pdx;
-        int lastp = __shfl(psel_incl, kWave - 1, kWave);
+        int lastp = __builtin_amdgcn_readlane(psel_incl, kWave - 1);
         if (lastp >= 0) {
-          prevX_carry = __shfl(X[s], lastp, kWave);
+          prevX_carry = __builtin_amdgcn_readlane(X[s], lastp);
           has_prev_carry = true;
         }
-        cnt_carry += __shfl(inc, kWave - 1, kWave);
+        cnt_carry += __builtin_amdgcn_readlane(inc, kWave - 1);
       }
       m = cnt_carry;
       bool all_sorted = __ballot(!sorted) == 0;
@@ -674,58 +674,204 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         flag_isect = __ballot(f) != 0;
         load_from_scratch(m);
       } else {
-        // stable rank by x and predecessor among earlier survivors
-        int rk[K];
+        // Stable re-sort by x: the survivors are reloaded in list order and
+        // sorted by (x, list position) with odd-even transposition (a step
+        // only swaps edges that crossed, so few passes are needed); a
+        // pathological order falls back to ranking by counting.
+        load_from_scratch(m);
+        int O[K];
+#pragma unroll
+        for (int s = 0; s < K; s++) O[s] = s * kWave + lane;
+        const int S2 = (m + kWave - 1) / kWave;
+        // compare-exchange of element p (this lane, slot s) with partner data
+        auto gt_key = [](Fixed xa, int oa, Fixed xb, int ob) {
+          return xa > xb || (xa == xb && oa > ob);
+        };
+        bool done = false;
+        for (int pass = 0; pass < 24 && !done; pass++) {
+          bool swapped = false;
+          // even phase: pairs (2t, 2t+1) are lanes (2t, 2t+1) of one slot
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            if (s >= S2) break;
+            int p = s * kWave + lane;
+            int q = p ^ 1;
+            Fixed ox = __shfl_xor(X[s], 1, kWave);
+            int oo = __shfl_xor(O[s], 1, kWave);
+            bool both = p < m && q < m;
+            bool left = (lane & 1) == 0;
+            bool take = both && (left ? gt_key(X[s], O[s], ox, oo) : gt_key(ox, oo, X[s], O[s]));
+            Fixed odx = __shfl_xor(DX[s], 1, kWave), ody = __shfl_xor(DY[s], 1, kWave);
+            Fixed oly = __shfl_xor(LY[s], 1, kWave);
+            int om = __shfl_xor(M[s], 1, kWave);
+            if (take) {
+              X[s] = ox;
+              O[s] = oo;
+              DX[s] = odx;
+              DY[s] = ody;
+              LY[s] = oly;
+              M[s] = om;
+              swapped = true;
+            }
+          }
+          // odd phase: pairs (2t+1, 2t+2); lane 63 of slot s pairs with
+          // lane 0 of slot s+1
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            if (s >= S2) break;
+            int p = s * kWave + lane;
+            bool left = (lane & 1) == 1;  // odd position = left of its pair
+            int q = left ? p + 1 : p - 1;
+            Fixed dnx = __shfl_down(X[s], 1, kWave), upx = __shfl_up(X[s], 1, kWave);
+            int dno = __shfl_down(O[s], 1, kWave), upo = __shfl_up(O[s], 1, kWave);
+            Fixed dndx = __shfl_down(DX[s], 1, kWave), updx = __shfl_up(DX[s], 1, kWave);
+            Fixed dndy = __shfl_down(DY[s], 1, kWave), updy = __shfl_up(DY[s], 1, kWave);
+            Fixed dnly = __shfl_down(LY[s], 1, kWave), uply = __shfl_up(LY[s], 1, kWave);
+            int dnm = __shfl_down(M[s], 1, kWave), upm = __shfl_up(M[s], 1, kWave);
+            // cross-slot partners (v_readlane reads the lane whatever EXEC is)
+            {
+              const int t = s + 1 < K ? s + 1 : s;
+              const int u = s > 0 ? s - 1 : s;
+              Fixed cx = __builtin_amdgcn_readlane(X[t], 0);
+              int co = __builtin_amdgcn_readlane(O[t], 0);
+              Fixed cdx = __builtin_amdgcn_readlane(DX[t], 0);
+              Fixed cdy = __builtin_amdgcn_readlane(DY[t], 0);
+              Fixed cly = __builtin_amdgcn_readlane(LY[t], 0);
+              int cm = __builtin_amdgcn_readlane(M[t], 0);
+              Fixed ux = __builtin_amdgcn_readlane(X[u], kWave - 1);
+              int uo = __builtin_amdgcn_readlane(O[u], kWave - 1);
+              Fixed udx = __builtin_amdgcn_readlane(DX[u], kWave - 1);
+              Fixed udy = __builtin_amdgcn_readlane(DY[u], kWave - 1);
+              Fixed uly = __builtin_amdgcn_readlane(LY[u], kWave - 1);
+              int um = __builtin_amdgcn_readlane(M[u], kWave - 1);
+              if (lane == kWave - 1) {
+                dnx = cx;
+                dno = co;
+                dndx = cdx;
+                dndy = cdy;
+                dnly = cly;
+                dnm = cm;
+              }
+              if (lane == 0) {
+                upx = ux;
+                upo = uo;
+                updx = udx;
+                updy = udy;
+                uply = uly;
+                upm = um;
+              }
+            }
+            bool both = p < m && q >= 0 && q < m;
+            bool take;
+            if (left) {
+              take = both && gt_key(X[s], O[s], dnx, dno);
+            } else {
+              take = both && gt_key(upx, upo, X[s], O[s]);
+            }
+            if (take) {
+              if (left) {
+                X[s] = dnx;
+                O[s] = dno;
+                DX[s] = dndx;
+                DY[s] = dndy;
+                LY[s] = dnly;
+                M[s] = dnm;
+              } else {
+                X[s] = upx;
+                O[s] = upo;
+                DX[s] = updx;
+                DY[s] = updy;
+                LY[s] = uply;
+                M[s] = upm;
+              }
+              swapped = true;
+            }
+          }
+          done = __ballot(swapped) == 0;
+        }
+        __syncthreads();
+        if (!done) {
+          // rank by counting on (x, list position)
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            int c = s * kWave + lane;
+            if (c < m) {
+              L.sX[c] = X[s];
+              L.sDX[c] = DX[s];
+              L.sDY[c] = DY[s];
+              L.sLY[c] = LY[s];
+              L.sM[c] = M[s];
+            }
+          }
+          int* sO = L.sX + 5 * CAP;  // list positions (spare scratch)
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            int c = s * kWave + lane;
+            if (c < m) sO[c] = O[s];
+          }
+          __syncthreads();
+          int rk[K];
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            int c = s * kWave + lane;
+            rk[s] = 0;
+            if (c < m) {
+              Fixed xi = L.sX[c];
+              int oi = sO[c];
+              int r = 0;
+              for (int q = 0; q < m; q++) {
+                Fixed xq = L.sX[q];
+                r += (xq < xi || (xq == xi && sO[q] < oi)) ? 1 : 0;
+              }
+              rk[s] = r;
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            int c = s * kWave + lane;
+            if (c < m) {
+              L.sX[rk[s]] = X[s];
+              L.sDX[rk[s]] = DX[s];
+              L.sDY[rk[s]] = DY[s];
+              L.sLY[rk[s]] = LY[s];
+              L.sM[rk[s]] = M[s];
+              sO[rk[s]] = O[s];
+            }
+          }
+          __syncthreads();
+          load_from_scratch(m);
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            int c = s * kWave + lane;
+            if (c < m) O[s] = sO[c];
+          }
+          __syncthreads();
+        }
+        // check_intersection with the predecessor at insertion time: the
+        // nearest earlier element of the sorted list whose list position is
+        // smaller (edges that moved in front of it were processed later).
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          int c = s * kWave + lane;
+          if (c < m) {
+            L.sX[c] = X[s];
+            L.sDX[c] = DX[s];
+            L.sLY[c] = O[s];
+          }
+        }
+        __syncthreads();
         bool f = false;
 #pragma unroll
         for (int s = 0; s < K; s++) {
           int c = s * kWave + lane;
-          rk[s] = 0;
           if (c < m) {
-            Fixed xi = L.sX[c];
-            int r = 0;
-            int pred = -1;
-            Fixed pxv = kMinS32;
-            for (int q = 0; q < m; q++) {
-              Fixed xq = L.sX[q];
-              r += (xq < xi || (xq == xi && q < c)) ? 1 : 0;
-              if (q < c && xq <= xi && xq >= pxv) {
-                pxv = xq;
-                pred = q;
-              }
-            }
-            rk[s] = r;
-            if (pred >= 0 && L.sX[pred] + L.sDX[pred] > xi + L.sDX[c]) f = true;
+            int q = c - 1;
+            while (q >= 0 && L.sLY[q] > O[s]) q--;
+            if (q >= 0 && L.sX[q] + L.sDX[q] > X[s] + DX[s]) f = true;
           }
         }
         flag_isect = __ballot(f) != 0;
-        Fixed tX[K], tDX[K], tDY[K], tLY[K];
-        int tM[K];
-#pragma unroll
-        for (int s = 0; s < K; s++) {
-          int c = s * kWave + lane;
-          if (c < m) {
-            tX[s] = L.sX[c];
-            tDX[s] = L.sDX[c];
-            tDY[s] = L.sDY[c];
-            tLY[s] = L.sLY[c];
-            tM[s] = L.sM[c];
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < K; s++) {
-          int c = s * kWave + lane;
-          if (c < m) {
-            L.sX[rk[s]] = tX[s];
-            L.sDX[rk[s]] = tDX[s];
-            L.sDY[rk[s]] = tDY[s];
-            L.sLY[rk[s]] = tLY[s];
-            L.sM[rk[s]] = tM[s];
-          }
-        }
-        __syncthreads();
-        load_from_scratch(m);
       }
       __syncthreads();
     }

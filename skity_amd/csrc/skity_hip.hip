@@ -591,7 +591,7 @@ void SetTiles(std::vector<Canvas>* cvs) {
 template <int K, int kMode>
 hipr_result LaunchWalk(hipr_context ctx, const WalkParams& P, hipStream_t st) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
-  size_t lds = (size_t)(2 * (P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 5 * K * kWave) *
+  size_t lds = (size_t)(2 * (P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 6 * K * kWave) *
                sizeof(int32_t);
   STAGE_LAUNCH(ctx, kMode == 0 ? HIPR_STAGE_WALK_COUNT : HIPR_STAGE_WALK,
                hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds, st, P));
@@ -1045,7 +1045,9 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
       if ((r = limit_error(big, 2 | 16 | 32)) != HIPR_SUCCESS) return r;
     }
     if (!redo_small.empty() || !big.empty()) {
-      f.stats.fallbacks += (int64_t)(redo_small.size() + big.size());
+      // only under-estimates count as fallbacks; wide jobs need the 16-slot
+      // walk whatever the estimate
+      f.stats.fallbacks += (int64_t)redo_small.size();
       std::vector<int32_t> redo = redo_small;
       redo.insert(redo.end(), big.begin(), big.end());
       for (int32_t j : redo) pcap[j] = ctx->h_cov[j].planes;

@@ -113,7 +113,8 @@ def test_prepared_scene_redraw(ctx):
 def test_presized_mode_matches_exact(ctx):
     """Mode 1 (capacities from estimates, exact fallback on overflow) draws the
     same pixels as the exact two-pass mode, whether the estimates are short
-    (every draw falls back) or ample (none does)."""
+    (every draw falls back) or ample (none does: a 2048^2 canvas has 16384
+    tiles and a tile holds at most two planes)."""
     sc = workload(2, 32, seed=25)
     n = sc.n_paths
     ref, _ = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=0)
@@ -122,7 +123,7 @@ def test_presized_mode_matches_exact(ctx):
                              tile_estimates=np.ones(n, np.int32))
     ample, st_a = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=1,
                              edge_estimates=np.full(n, 20000, np.int32),
-                             tile_estimates=np.full(n, 16384, np.int32))
+                             tile_estimates=np.full(n, 2 * 16384, np.int32))
     print(f"fallbacks: short {st_s['fallbacks']}, ample {st_a['fallbacks']}", flush=True)
     assert np.array_equal(short, ref)
     assert np.array_equal(ample, ref)
