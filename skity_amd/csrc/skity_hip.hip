@@ -620,9 +620,11 @@ hipr_result RunStrokes(Frame& f) {
   if (ctx->stroke_out.Ensure(sizeof(StrokeOut) * ns) != hipSuccess)
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   GeomSrc gs{pl->verbs.as<uint8_t>(), pl->pts.as<float>(), pl->weights.as<float>()};
-  int blocks = (ns + 63) / 64;
+  // One stroke job per wavefront: the stroker is sequential, data-dependent
+  // code, and 64 different paths in one wave would serialise their branches.
+  int blocks = ns;
   STAGE_LAUNCH(ctx, HIPR_STAGE_STROKE,
-    hipLaunchKernelGGL((StrokeKernel<0>), dim3(blocks), dim3(64), 0, st, pl->d_jobs.as<Job>(),
+    hipLaunchKernelGGL((StrokeKernel<0>), dim3(blocks), dim3(1), 0, st, pl->d_jobs.as<Job>(),
                      pl->d_stroke.as<int32_t>(), ns, gs, pl->scene_paints.as<hipr_paint>(),
                      ctx->stroke_out.as<StrokeOut>(), (uint8_t*)nullptr, (float*)nullptr,
                      (float*)nullptr, (uint8_t*)nullptr, (float*)nullptr, (float*)nullptr));
@@ -658,7 +660,7 @@ hipr_result RunStrokes(Frame& f) {
   HIP_TRY(hipMemcpyAsync(ctx->stroke_out.p, ctx->h_so, sizeof(StrokeOut) * ns,
                          hipMemcpyHostToDevice, st));
   STAGE_LAUNCH(ctx, HIPR_STAGE_STROKE,
-    hipLaunchKernelGGL((StrokeKernel<1>), dim3(blocks), dim3(64), 0, st, pl->d_jobs.as<Job>(),
+    hipLaunchKernelGGL((StrokeKernel<1>), dim3(blocks), dim3(1), 0, st, pl->d_jobs.as<Job>(),
                      pl->d_stroke.as<int32_t>(), ns, gs, pl->scene_paints.as<hipr_paint>(),
                      ctx->stroke_out.as<StrokeOut>(), ctx->o_verbs.as<uint8_t>(),
                      ctx->o_pts.as<float>(), ctx->o_w.as<float>(), ctx->s_verbs.as<uint8_t>(),
