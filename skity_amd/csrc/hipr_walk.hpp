@@ -716,6 +716,24 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           }
           // odd phase: pairs (2t+1, 2t+2); lane 63 of slot s pairs with
           // lane 0 of slot s+1
+          // the slot-boundary elements, read before any of them moves
+          Fixed b0x[K], b0dx[K], b0dy[K], b0ly[K], b63x[K], b63dx[K], b63dy[K], b63ly[K];
+          int b0o[K], b0m[K], b63o[K], b63m[K];
+#pragma unroll
+          for (int s = 0; s < K; s++) {
+            b0x[s] = __builtin_amdgcn_readlane(X[s], 0);
+            b0o[s] = __builtin_amdgcn_readlane(O[s], 0);
+            b0dx[s] = __builtin_amdgcn_readlane(DX[s], 0);
+            b0dy[s] = __builtin_amdgcn_readlane(DY[s], 0);
+            b0ly[s] = __builtin_amdgcn_readlane(LY[s], 0);
+            b0m[s] = __builtin_amdgcn_readlane(M[s], 0);
+            b63x[s] = __builtin_amdgcn_readlane(X[s], kWave - 1);
+            b63o[s] = __builtin_amdgcn_readlane(O[s], kWave - 1);
+            b63dx[s] = __builtin_amdgcn_readlane(DX[s], kWave - 1);
+            b63dy[s] = __builtin_amdgcn_readlane(DY[s], kWave - 1);
+            b63ly[s] = __builtin_amdgcn_readlane(LY[s], kWave - 1);
+            b63m[s] = __builtin_amdgcn_readlane(M[s], kWave - 1);
+          }
 #pragma unroll
           for (int s = 0; s < K; s++) {
             if (s >= S2) break;
@@ -728,37 +746,25 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
             Fixed dndy = __shfl_down(DY[s], 1, kWave), updy = __shfl_up(DY[s], 1, kWave);
             Fixed dnly = __shfl_down(LY[s], 1, kWave), uply = __shfl_up(LY[s], 1, kWave);
             int dnm = __shfl_down(M[s], 1, kWave), upm = __shfl_up(M[s], 1, kWave);
-            // cross-slot partners (v_readlane reads the lane whatever EXEC is)
+            // cross-slot partners
             {
               const int t = s + 1 < K ? s + 1 : s;
               const int u = s > 0 ? s - 1 : s;
-              Fixed cx = __builtin_amdgcn_readlane(X[t], 0);
-              int co = __builtin_amdgcn_readlane(O[t], 0);
-              Fixed cdx = __builtin_amdgcn_readlane(DX[t], 0);
-              Fixed cdy = __builtin_amdgcn_readlane(DY[t], 0);
-              Fixed cly = __builtin_amdgcn_readlane(LY[t], 0);
-              int cm = __builtin_amdgcn_readlane(M[t], 0);
-              Fixed ux = __builtin_amdgcn_readlane(X[u], kWave - 1);
-              int uo = __builtin_amdgcn_readlane(O[u], kWave - 1);
-              Fixed udx = __builtin_amdgcn_readlane(DX[u], kWave - 1);
-              Fixed udy = __builtin_amdgcn_readlane(DY[u], kWave - 1);
-              Fixed uly = __builtin_amdgcn_readlane(LY[u], kWave - 1);
-              int um = __builtin_amdgcn_readlane(M[u], kWave - 1);
               if (lane == kWave - 1) {
-                dnx = cx;
-                dno = co;
-                dndx = cdx;
-                dndy = cdy;
-                dnly = cly;
-                dnm = cm;
+                dnx = b0x[t];
+                dno = b0o[t];
+                dndx = b0dx[t];
+                dndy = b0dy[t];
+                dnly = b0ly[t];
+                dnm = b0m[t];
               }
               if (lane == 0) {
-                upx = ux;
-                upo = uo;
-                updx = udx;
-                updy = udy;
-                uply = uly;
-                upm = um;
+                upx = b63x[u];
+                upo = b63o[u];
+                updx = b63dx[u];
+                updy = b63dy[u];
+                uply = b63ly[u];
+                upm = b63m[u];
               }
             }
             bool both = p < m && q >= 0 && q < m;
