@@ -2,6 +2,8 @@
 
   skity_amd/libskity_hip.so    gfx950 kernels + C ABI (hipcc)
   skity_amd/libskity_scene.so  seeded scene generator (g++)
+  skity_amd/libskity_hip_canvas.so  C++ Path/Paint/Canvas front end (g++)
+  tests/cpp/hip_canvas_test    C++ known-answer tests of the front end (g++)
   oracle/liboracle.so          CPU oracle, test infrastructure (make)
 """
 import os
@@ -43,6 +45,19 @@ def build_scene():
     os.replace(out + ".tmp", out)
 
 
+def build_host():
+    src = os.path.join(HERE, "host", "hip_canvas.cc")
+    out = os.path.join(HERE, "libskity_hip_canvas.so")
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", out + ".tmp", src,
+         "-L" + HERE, "-lskity_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    test_src = os.path.join(ROOT, "tests", "cpp", "hip_canvas_test.cc")
+    test_out = os.path.join(ROOT, "tests", "cpp", "hip_canvas_test")
+    run(["g++", "-O2", "-std=c++17", "-o", test_out + ".tmp", test_src, "-L" + HERE,
+         "-lskity_hip_canvas", "-lskity_hip", "-Wl,-rpath,$ORIGIN/../../skity_amd"])
+    os.replace(test_out + ".tmp", test_out)
+
+
 def build_oracle():
     run(["make", "-s", "-B", "liboracle.so"], cwd=os.path.join(ROOT, "oracle"))
 
@@ -51,10 +66,11 @@ def build_all():
     build_oracle()
     build_scene()
     build_hip()
+    build_host()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "hip": build_hip, "scene": build_scene,
-         "oracle": build_oracle}[w]()
+         "oracle": build_oracle, "host": build_host}[w]()

@@ -1,0 +1,342 @@
+// hip_canvas.cc -- C++ host front end over the C ABI (include/skity_hip.h).
+#include "hip_canvas.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace skity_hip {
+
+namespace {
+constexpr float kRoot2Over2 = 0.707106781f;  // FloatRoot2Over2
+uint8_t V(Path::Verb v) { return static_cast<uint8_t>(v); }
+}  // namespace
+
+Color4f Color4fFromColor(Color c) {
+  return Color4f{ColorGetR(c) / 255.f, ColorGetG(c) / 255.f, ColorGetB(c) / 255.f,
+                 ColorGetA(c) / 255.f};
+}
+
+// ---- Path (src/graphic/path.cc:494-560, 815-859, 1001-1035) ----------------
+Path& Path::MoveTo(float x, float y) {
+  if (!verbs_.empty() && verbs_.back() == V(Verb::kMove)) {
+    points_.back() = Point{x, y};
+  } else {
+    last_move_to_index_ = (int64_t)points_.size();
+    verbs_.push_back(V(Verb::kMove));
+    points_.push_back(Point{x, y});
+  }
+  return *this;
+}
+
+void Path::InjectMoveToIfNeeded() {
+  if (last_move_to_index_ < 0) {
+    Point p{0, 0};
+    if (!points_.empty()) p = points_[(size_t)~last_move_to_index_];
+    MoveTo(p.x, p.y);
+  }
+}
+
+Path& Path::LineTo(float x, float y) {
+  InjectMoveToIfNeeded();
+  verbs_.push_back(V(Verb::kLine));
+  points_.push_back(Point{x, y});
+  return *this;
+}
+
+Path& Path::QuadTo(float x1, float y1, float x2, float y2) {
+  InjectMoveToIfNeeded();
+  verbs_.push_back(V(Verb::kQuad));
+  points_.push_back(Point{x1, y1});
+  points_.push_back(Point{x2, y2});
+  return *this;
+}
+
+Path& Path::ConicTo(float x1, float y1, float x2, float y2, float w) {
+  if (!(w > 0)) return LineTo(x2, y2);
+  if (std::isinf(w)) {
+    LineTo(x1, y1);
+    return LineTo(x2, y2);
+  }
+  if (w == 1.f) return QuadTo(x1, y1, x2, y2);
+  InjectMoveToIfNeeded();
+  verbs_.push_back(V(Verb::kConic));
+  weights_.push_back(w);
+  points_.push_back(Point{x1, y1});
+  points_.push_back(Point{x2, y2});
+  return *this;
+}
+
+Path& Path::CubicTo(float x1, float y1, float x2, float y2, float x3, float y3) {
+  InjectMoveToIfNeeded();
+  verbs_.push_back(V(Verb::kCubic));
+  points_.push_back(Point{x1, y1});
+  points_.push_back(Point{x2, y2});
+  points_.push_back(Point{x3, y3});
+  return *this;
+}
+
+Path& Path::Close() {
+  if (!verbs_.empty() && verbs_.back() != V(Verb::kClose)) verbs_.push_back(V(Verb::kClose));
+  if (last_move_to_index_ >= 0) last_move_to_index_ = ~last_move_to_index_;
+  return *this;
+}
+
+Path& Path::AddRect(const Rect& r, Direction dir, uint32_t start) {
+  const Point pts[4] = {{r.left, r.top}, {r.right, r.top}, {r.right, r.bottom},
+                        {r.left, r.bottom}};
+  const uint32_t adv = dir == Direction::kCW ? 1 : 3;
+  uint32_t i = start % 4;
+  MoveTo(pts[i].x, pts[i].y);
+  for (int k = 0; k < 3; k++) {
+    i = (i + adv) % 4;
+    LineTo(pts[i].x, pts[i].y);
+  }
+  return Close();
+}
+
+Path& Path::AddOval(const Rect& r, Direction dir, uint32_t start) {
+  const float cx = r.CenterX(), cy = r.CenterY();
+  const Point oval[4] = {{cx, r.top}, {r.right, cy}, {cx, r.bottom}, {r.left, cy}};
+  const Point rect[4] = {{r.left, r.top}, {r.right, r.top}, {r.right, r.bottom},
+                         {r.left, r.bottom}};
+  const uint32_t adv = dir == Direction::kCW ? 1 : 3;
+  uint32_t oi = start % 4, ri = (start + (dir == Direction::kCW ? 0 : 1)) % 4;
+  MoveTo(oval[oi].x, oval[oi].y);
+  for (int k = 0; k < 4; k++) {
+    ri = (ri + adv) % 4;
+    oi = (oi + adv) % 4;
+    ConicTo(rect[ri].x, rect[ri].y, oval[oi].x, oval[oi].y, kRoot2Over2);
+  }
+  return Close();
+}
+
+Path& Path::AddCircle(float x, float y, float radius, Direction dir) {
+  if (radius > 0) AddOval(Rect::MakeLTRB(x - radius, y - radius, x + radius, y + radius), dir);
+  return *this;
+}
+
+// ---- Shader (src/effect/shader.cc) --------------------------------------------
+namespace {
+std::shared_ptr<Shader> MakeGradient(int type, const Color4f colors[], const float pos[],
+                                     int count, TileMode mode, hipr_gradient* g) {
+  if (count < 2 || count > HIPR_MAX_GRADIENT_STOPS) return nullptr;
+  g->type = type;
+  g->tile_mode = static_cast<int32_t>(mode);
+  g->color_count = count;
+  g->has_offsets = pos ? 1 : 0;
+  for (int i = 0; i < count; i++) {
+    g->colors[i][0] = colors[i].r;
+    g->colors[i][1] = colors[i].g;
+    g->colors[i][2] = colors[i].b;
+    g->colors[i][3] = colors[i].a;
+    g->offsets[i] = pos ? pos[i] : 0.f;
+  }
+  const float id[6] = {1, 0, 0, 0, 1, 0};
+  std::memcpy(g->local_matrix, id, sizeof(id));
+  return nullptr;
+}
+}  // namespace
+
+std::shared_ptr<Shader> Shader::MakeLinear(const Point pts[2], const Color4f colors[],
+                                           const float pos[], int count, TileMode mode) {
+  if (count < 2 || count > HIPR_MAX_GRADIENT_STOPS) return nullptr;
+  auto s = std::make_shared<Shader>();
+  MakeGradient(HIPR_GRADIENT_LINEAR, colors, pos, count, mode, &s->info_);
+  s->info_.points[0] = pts[0].x;
+  s->info_.points[1] = pts[0].y;
+  s->info_.points[2] = pts[1].x;
+  s->info_.points[3] = pts[1].y;
+  return s;
+}
+
+std::shared_ptr<Shader> Shader::MakeRadial(const Point& c, float radius, const Color4f colors[],
+                                           const float pos[], int count, TileMode mode) {
+  if (count < 2 || count > HIPR_MAX_GRADIENT_STOPS) return nullptr;
+  auto s = std::make_shared<Shader>();
+  MakeGradient(HIPR_GRADIENT_RADIAL, colors, pos, count, mode, &s->info_);
+  s->info_.points[0] = c.x;
+  s->info_.points[1] = c.y;
+  s->info_.radius[0] = radius;
+  return s;
+}
+
+std::shared_ptr<Shader> Shader::MakeSweep(float cx, float cy, float start_angle, float end_angle,
+                                          const Color4f colors[], const float pos[], int count,
+                                          TileMode mode) {
+  if (count < 2 || count > HIPR_MAX_GRADIENT_STOPS) return nullptr;
+  if (start_angle >= end_angle) return nullptr;  // shader.cc:95-97
+  auto s = std::make_shared<Shader>();
+  MakeGradient(HIPR_GRADIENT_SWEEP, colors, pos, count, mode, &s->info_);
+  s->info_.points[0] = cx;
+  s->info_.points[1] = cy;
+  float t0 = start_angle / 360, t1 = end_angle / 360;
+  s->info_.radius[0] = -t0;             // bias
+  s->info_.radius[1] = 1 / (t1 - t0);   // scale
+  return s;
+}
+
+void Shader::SetLocalMatrix(const float m[6]) { std::memcpy(info_.local_matrix, m, 6 * sizeof(float)); }
+
+std::shared_ptr<MaskFilter> MaskFilter::MakeBlur(BlurStyle style, float radius) {
+  auto m = std::make_shared<MaskFilter>();
+  m->style_ = style;
+  m->radius_ = radius;
+  return m;
+}
+
+// ---- Paint ------------------------------------------------------------------
+void Paint::SetColor(Color c) { fill_ = stroke_ = Color4fFromColor(c); }
+
+void Paint::SetAlphaF(float a) {
+  a = std::min(std::max(a, 0.f), 1.f);
+  fill_.a = a;
+  stroke_.a = a;
+}
+
+hipr_paint Paint::Pack(int32_t gradient_index) const {
+  hipr_paint p{};
+  p.color[0] = fill_.r;
+  p.color[1] = fill_.g;
+  p.color[2] = fill_.b;
+  p.color[3] = fill_.a;
+  p.stroke_color[0] = stroke_.r;
+  p.stroke_color[1] = stroke_.g;
+  p.stroke_color[2] = stroke_.b;
+  p.stroke_color[3] = stroke_.a;
+  p.style = style_;
+  p.stroke_width = width_;
+  p.miter_limit = miter_;
+  p.cap = cap_;
+  p.join = join_;
+  p.gradient = gradient_index;
+  p.blur_radius = mask_ ? mask_->radius() : 0.f;
+  p.blur_style = mask_ ? static_cast<int32_t>(mask_->style()) : 0;
+  p.blend = HIPR_BLEND_SRC_OVER;
+  return p;
+}
+
+// ---- Canvas -----------------------------------------------------------------
+void HIPCanvas::DrawPath(const Path& path, const Paint& paint) {
+  draws_.push_back(Draw{path, paint});
+  surface_->readback_valid_ = false;
+}
+
+void HIPCanvas::DrawRect(const Rect& rect, const Paint& paint) {
+  Path p;
+  p.AddRect(rect);
+  DrawPath(p, paint);
+}
+
+void HIPCanvas::DrawOval(const Rect& oval, const Paint& paint) {
+  // OnDrawOval -> RRect::SetOval -> Path::AddRRect(kCW, start 6) -> AddOval
+  // with start 6 / 2 (path.cc:881-896)
+  Path p;
+  p.AddOval(oval, Path::Direction::kCW, 3);
+  DrawPath(p, paint);
+}
+
+void HIPCanvas::DrawCircle(float cx, float cy, float radius, const Paint& paint) {
+  if (radius < 0) radius = 0;
+  DrawOval(Rect::MakeLTRB(cx - radius, cy - radius, cx + radius, cy + radius), paint);
+}
+
+void HIPCanvas::DrawLine(float x0, float y0, float x1, float y1, const Paint& paint) {
+  Path p;
+  p.MoveTo(x0, y0);
+  p.LineTo(x1, y1);
+  DrawPath(p, paint);
+}
+
+int HIPCanvas::Flush() {
+  if (draws_.empty()) return HIPR_SUCCESS;
+  const int32_t n = (int32_t)draws_.size();
+  std::vector<uint8_t> verbs;
+  std::vector<float> pts, ws;
+  std::vector<int32_t> vo(n + 1, 0), po(n + 1, 0), wo(n + 1, 0);
+  std::vector<uint8_t> fills(n);
+  std::vector<hipr_paint> paints(n);
+  std::vector<hipr_gradient> grads;
+  for (int32_t i = 0; i < n; i++) {
+    const Path& p = draws_[i].path;
+    verbs.insert(verbs.end(), p.verbs().begin(), p.verbs().end());
+    for (const Point& q : p.points()) {
+      pts.push_back(q.x);
+      pts.push_back(q.y);
+    }
+    ws.insert(ws.end(), p.weights().begin(), p.weights().end());
+    vo[i + 1] = (int32_t)verbs.size();
+    po[i + 1] = (int32_t)(pts.size() / 2);
+    wo[i + 1] = (int32_t)ws.size();
+    fills[i] = p.GetFillType() == Path::PathFillType::kEvenOdd ? HIPR_FILL_EVEN_ODD
+                                                               : HIPR_FILL_WINDING;
+    int32_t gi = -1;
+    if (draws_[i].paint.shader()) {
+      gi = (int32_t)grads.size();
+      grads.push_back(draws_[i].paint.shader()->info());
+    }
+    paints[i] = draws_[i].paint.Pack(gi);
+  }
+  hipr_scene scene{};
+  scene.n_paths = n;
+  scene.verbs = verbs.data();
+  scene.verb_offsets = vo.data();
+  scene.points = pts.data();
+  scene.point_offsets = po.data();
+  scene.weights = ws.data();
+  scene.weight_offsets = wo.data();
+  scene.fill_types = fills.data();
+  scene.paints = paints.data();
+  scene.gradients = grads.empty() ? nullptr : grads.data();
+  scene.n_gradients = (int32_t)grads.size();
+  hipr_options opt{};
+  opt.mode = HIPR_MODE_EXACT;
+  int r = hipr_surface_draw_scene(surface_->surface_, &scene, &opt);
+  draws_.clear();
+  surface_->last_result_ = r;
+  return r;
+}
+
+// ---- Surface ----------------------------------------------------------------
+std::unique_ptr<HIPSurface> HIPSurface::Make(hipr_context ctx, int width, int height) {
+  std::unique_ptr<HIPSurface> s(new HIPSurface());
+  if (hipr_surface_create(ctx, width, height, &s->surface_) != HIPR_SUCCESS) return nullptr;
+  s->width_ = width;
+  s->height_ = height;
+  s->canvas_.reset(new HIPCanvas(s.get()));
+  return s;
+}
+
+HIPSurface::~HIPSurface() { hipr_surface_destroy(surface_); }
+
+HIPCanvas* HIPSurface::LockCanvas(bool clear) {
+  if (clear) {
+    canvas_->draws_.clear();
+    last_result_ = hipr_surface_clear(surface_, 0);
+    readback_valid_ = false;
+  }
+  return canvas_.get();
+}
+
+int HIPSurface::Flush() {
+  int r = canvas_->Flush();
+  readback_valid_ = false;
+  return r;
+}
+
+int HIPSurface::ReadPixels(void* dst, size_t bytes) {
+  return hipr_surface_read_pixels(surface_, static_cast<uint8_t*>(dst), bytes);
+}
+
+Color HIPSurface::GetPixel(int x, int y) {
+  if (x < 0 || y < 0 || x >= width_ || y >= height_) return Color_TRANSPARENT;
+  if (!readback_valid_) {
+    readback_.resize((size_t)width_ * height_ * 4);
+    if (ReadPixels(readback_.data(), readback_.size()) != HIPR_SUCCESS) return Color_TRANSPARENT;
+    readback_valid_ = true;
+  }
+  const uint8_t* p = &readback_[((size_t)y * width_ + x) * 4];
+  return ColorSetARGB(p[3], p[0], p[1], p[2]);
+}
+
+}  // namespace skity_hip

@@ -1,0 +1,227 @@
+// hip_canvas.hpp -- C++ host front end of the MI355X DrawPath backend.
+//
+// The same shapes as skity's public API for the DrawPath path
+// (include/skity/graphic/path.hpp, paint.hpp, include/skity/effect/
+// shader.hpp, mask_filter.hpp, include/skity/render/canvas.hpp), so code
+// written against skity's Canvas::DrawPath reads the same here:
+//
+//   auto surface = skity_hip::HIPSurface::Make(ctx, 48, 48);
+//   skity_hip::HIPCanvas* canvas = surface->LockCanvas();
+//   skity_hip::Paint paint;
+//   paint.SetStyle(skity_hip::Paint::kStrokeThenFill_Style);
+//   canvas->DrawPath(path, paint);
+//   surface->Flush();                     // one hipr_surface_draw_scene call
+//   skity_hip::Color c = surface->GetPixel(12, 20);
+//
+// A HIPCanvas records the DrawPath calls in order and Flush() packs them into
+// one hipr_scene (include/skity_hip.h) for the device.  Only the calls of the
+// DrawPath path exist; the library refuses what it cannot draw exactly
+// (hipr_result != 0) and never falls back to a CPU rasteriser.
+#ifndef SKITY_HIP_HOST_HIP_CANVAS_HPP_
+#define SKITY_HIP_HOST_HIP_CANVAS_HPP_
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "../../include/skity_hip.h"
+
+namespace skity_hip {
+
+// include/skity/graphic/color.hpp: 0xAARRGGBB
+using Color = uint32_t;
+constexpr Color Color_TRANSPARENT = 0x00000000;
+constexpr Color Color_BLACK = 0xFF000000;
+constexpr Color Color_WHITE = 0xFFFFFFFF;
+constexpr Color Color_RED = 0xFFFF0000;
+constexpr Color Color_GREEN = 0xFF00FF00;
+constexpr Color Color_BLUE = 0xFF0000FF;
+inline constexpr Color ColorSetARGB(uint32_t a, uint32_t r, uint32_t g, uint32_t b) {
+  return (a << 24) | (r << 16) | (g << 8) | b;
+}
+inline constexpr uint32_t ColorGetA(Color c) { return (c >> 24) & 0xFF; }
+inline constexpr uint32_t ColorGetR(Color c) { return (c >> 16) & 0xFF; }
+inline constexpr uint32_t ColorGetG(Color c) { return (c >> 8) & 0xFF; }
+inline constexpr uint32_t ColorGetB(Color c) { return c & 0xFF; }
+
+struct Color4f {
+  float r, g, b, a;
+};
+// Color4fFromColor (src/graphic/color.cc): channel / 255
+Color4f Color4fFromColor(Color c);
+
+struct Point {
+  float x, y;
+};
+
+struct Rect {
+  float left, top, right, bottom;
+  static Rect MakeLTRB(float l, float t, float r, float b) { return Rect{l, t, r, b}; }
+  static Rect MakeXYWH(float x, float y, float w, float h) { return Rect{x, y, x + w, y + h}; }
+  static Rect MakeWH(float w, float h) { return Rect{0, 0, w, h}; }
+  float CenterX() const { return 0.5f * (right + left); }
+  float CenterY() const { return 0.5f * (top + bottom); }
+};
+
+// Path storage and builder rules of src/graphic/path.cc.
+class Path {
+ public:
+  enum class Verb : uint8_t { kMove, kLine, kQuad, kConic, kCubic, kClose };
+  enum class PathFillType { kWinding, kEvenOdd };
+  enum class Direction { kCW, kCCW };
+
+  Path& MoveTo(float x, float y);
+  Path& LineTo(float x, float y);
+  Path& QuadTo(float x1, float y1, float x2, float y2);
+  Path& ConicTo(float x1, float y1, float x2, float y2, float weight);
+  Path& CubicTo(float x1, float y1, float x2, float y2, float x3, float y3);
+  Path& Close();
+  Path& AddRect(const Rect& rect, Direction dir = Direction::kCW, uint32_t start = 0);
+  Path& AddOval(const Rect& oval, Direction dir = Direction::kCW, uint32_t start = 1);
+  Path& AddCircle(float x, float y, float radius, Direction dir = Direction::kCW);
+  void SetFillType(PathFillType t) { fill_type_ = t; }
+  PathFillType GetFillType() const { return fill_type_; }
+  bool IsEmpty() const { return verbs_.empty(); }
+
+  const std::vector<uint8_t>& verbs() const { return verbs_; }
+  const std::vector<Point>& points() const { return points_; }
+  const std::vector<float>& weights() const { return weights_; }
+
+ private:
+  void InjectMoveToIfNeeded();
+  std::vector<uint8_t> verbs_;
+  std::vector<Point> points_;
+  std::vector<float> weights_;
+  PathFillType fill_type_ = PathFillType::kWinding;
+  int64_t last_move_to_index_ = -1;  // ~index after Close
+};
+
+enum class TileMode { kClamp, kRepeat, kMirror, kDecal };
+
+// Gradient shaders (include/skity/effect/shader.hpp:88-146), premultiply
+// flag 0 (interpolation of unpremultiplied colours).
+class Shader {
+ public:
+  static std::shared_ptr<Shader> MakeLinear(const Point pts[2], const Color4f colors[],
+                                            const float pos[], int count,
+                                            TileMode tile_mode = TileMode::kClamp);
+  static std::shared_ptr<Shader> MakeRadial(const Point& center, float radius,
+                                            const Color4f colors[], const float pos[],
+                                            int count, TileMode tile_mode = TileMode::kClamp);
+  static std::shared_ptr<Shader> MakeSweep(float cx, float cy, float start_angle,
+                                           float end_angle, const Color4f colors[],
+                                           const float pos[], int count,
+                                           TileMode tile_mode = TileMode::kClamp);
+  // {scale_x, skew_x, trans_x, skew_y, scale_y, trans_y}
+  void SetLocalMatrix(const float m[6]);
+  const hipr_gradient& info() const { return info_; }
+
+ private:
+  hipr_gradient info_{};
+};
+
+enum class BlurStyle { kNormal = HIPR_BLUR_NORMAL };
+
+class MaskFilter {
+ public:
+  static std::shared_ptr<MaskFilter> MakeBlur(BlurStyle style, float radius);
+  float radius() const { return radius_; }
+  BlurStyle style() const { return style_; }
+
+ private:
+  BlurStyle style_ = BlurStyle::kNormal;
+  float radius_ = 0;
+};
+
+class Paint {
+ public:
+  enum Style { kFill_Style, kStroke_Style, kStrokeAndFill_Style, kStrokeThenFill_Style };
+  enum Cap { kButt_Cap, kRound_Cap, kSquare_Cap };
+  enum Join { kMiter_Join, kRound_Join, kBevel_Join };
+
+  void SetStyle(Style s) { style_ = s; }
+  Style GetStyle() const { return style_; }
+  void SetStrokeWidth(float w) { width_ = w; }
+  void SetStrokeMiter(float m) { miter_ = m; }
+  void SetStrokeCap(Cap c) { cap_ = c; }
+  void SetStrokeJoin(Join j) { join_ = j; }
+  void SetColor(Color c);  // fill and stroke colour (paint.cc:80-84)
+  void SetFillColor(Color c) { fill_ = Color4fFromColor(c); }
+  void SetStrokeColor(Color c) { stroke_ = Color4fFromColor(c); }
+  void SetFillColor(float r, float g, float b, float a) { fill_ = Color4f{r, g, b, a}; }
+  void SetStrokeColor(float r, float g, float b, float a) { stroke_ = Color4f{r, g, b, a}; }
+  void SetAlphaF(float a);  // both colours (paint.cc:100-104)
+  void SetAntiAlias(bool) {}  // the software path always anti-aliases
+  void SetShader(std::shared_ptr<Shader> s) { shader_ = std::move(s); }
+  void SetMaskFilter(std::shared_ptr<MaskFilter> m) { mask_ = std::move(m); }
+
+  hipr_paint Pack(int32_t gradient_index) const;
+  const std::shared_ptr<Shader>& shader() const { return shader_; }
+
+ private:
+  Style style_ = kFill_Style;
+  float width_ = 1.f, miter_ = 4.f;
+  Cap cap_ = kButt_Cap;
+  Join join_ = kMiter_Join;
+  Color4f fill_{0, 0, 0, 1}, stroke_{0, 0, 0, 1};
+  std::shared_ptr<Shader> shader_;
+  std::shared_ptr<MaskFilter> mask_;
+};
+
+class HIPSurface;
+
+// Records DrawPath calls (Canvas::DrawPath, include/skity/render/canvas.hpp)
+// and the shape helpers that the reference dispatches to OnDrawPath.
+class HIPCanvas {
+ public:
+  void DrawPath(const Path& path, const Paint& paint);
+  void DrawRect(const Rect& rect, const Paint& paint);
+  void DrawOval(const Rect& oval, const Paint& paint);
+  void DrawCircle(float cx, float cy, float radius, const Paint& paint);
+  void DrawLine(float x0, float y0, float x1, float y1, const Paint& paint);
+  // Draws everything recorded since the last flush; returns the hipr_result.
+  int Flush();
+  size_t PendingDraws() const { return draws_.size(); }
+
+ private:
+  friend class HIPSurface;
+  explicit HIPCanvas(HIPSurface* s) : surface_(s) {}
+  struct Draw {
+    Path path;
+    Paint paint;
+  };
+  HIPSurface* surface_;
+  std::vector<Draw> draws_;
+};
+
+// A device RGBA8 premultiplied target (the GPUSurface pattern of
+// include/skity/gpu/gpu_surface.hpp:95-113).
+class HIPSurface {
+ public:
+  static std::unique_ptr<HIPSurface> Make(hipr_context ctx, int width, int height);
+  ~HIPSurface();
+  HIPCanvas* LockCanvas(bool clear = true);
+  int Flush();
+  int Width() const { return width_; }
+  int Height() const { return height_; }
+  // RGBA8 premultiplied, row-major, width*4 bytes per row.
+  int ReadPixels(void* dst, size_t bytes);
+  // skity Color (ARGB) of one pixel, read back from the device.
+  Color GetPixel(int x, int y);
+  hipr_surface handle() const { return surface_; }
+  int last_result() const { return last_result_; }
+
+ private:
+  HIPSurface() = default;
+  hipr_surface surface_ = nullptr;
+  int width_ = 0, height_ = 0;
+  std::unique_ptr<HIPCanvas> canvas_;
+  std::vector<uint8_t> readback_;
+  bool readback_valid_ = false;
+  int last_result_ = 0;
+  friend class HIPCanvas;
+};
+
+}  // namespace skity_hip
+
+#endif  // SKITY_HIP_HOST_HIP_CANVAS_HPP_

@@ -1,0 +1,156 @@
+// Known-answer tests of the reference's SWCanvas / Path suites, written
+// against the C++ front end (skity_amd/host/hip_canvas.hpp) and drawn by the
+// HIP library.  Exit status 0 when every check passes.
+#include <cstdio>
+#include <cstring>
+
+#include "../../skity_amd/host/hip_canvas.hpp"
+
+using namespace skity_hip;
+
+static int g_failed = 0;
+#define EXPECT_EQ(a, b)                                                              \
+  do {                                                                               \
+    auto va = (a);                                                                   \
+    auto vb = (b);                                                                   \
+    if (!(va == vb)) {                                                               \
+      std::printf("  FAILED %s:%d: %s == %s (0x%08x vs 0x%08x)\n", __FILE__, __LINE__, #a, \
+                  #b, (unsigned)va, (unsigned)vb);                                   \
+      g_failed++;                                                                    \
+    }                                                                                \
+  } while (0)
+#define EXPECT_TRUE(c)                                                               \
+  do {                                                                               \
+    if (!(c)) {                                                                      \
+      std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #c);                   \
+      g_failed++;                                                                    \
+    }                                                                                \
+  } while (0)
+
+// test/ut/render/sw_canvas_test.cc:33-48 (the draw inside the save layer)
+static void RedRect(hipr_context ctx) {
+  auto surface = HIPSurface::Make(ctx, 16, 16);
+  HIPCanvas* canvas = surface->LockCanvas();
+  Paint paint;
+  paint.SetColor(Color_RED);
+  canvas->DrawRect(Rect::MakeXYWH(4.f, 4.f, 4.f, 4.f), paint);
+  EXPECT_EQ(surface->Flush(), HIPR_SUCCESS);
+  EXPECT_EQ(surface->GetPixel(5, 5), Color_RED);
+  EXPECT_EQ(surface->GetPixel(0, 0), Color_TRANSPARENT);
+}
+
+// test/ut/render/sw_canvas_test.cc:67-85
+static void StrokeThenFillDrawsFillAfterStroke(hipr_context ctx) {
+  auto surface = HIPSurface::Make(ctx, 48, 48);
+  HIPCanvas* canvas = surface->LockCanvas();
+  Paint paint;
+  paint.SetStyle(Paint::kStrokeThenFill_Style);
+  paint.SetStrokeWidth(10.f);
+  paint.SetStrokeColor(Color_RED);
+  paint.SetFillColor(Color_WHITE);
+  paint.SetAntiAlias(false);
+  Path path;
+  path.AddRect(Rect::MakeXYWH(10.f, 10.f, 24.f, 24.f));
+  canvas->DrawPath(path, paint);
+  EXPECT_EQ(surface->Flush(), HIPR_SUCCESS);
+  EXPECT_EQ(surface->GetPixel(12, 20), Color_WHITE);
+  EXPECT_EQ(surface->GetPixel(7, 20), Color_RED);
+}
+
+// test/ut/graphic/path_test.cc:1309-1319: the width-2 stroke of the quad
+// contains (120, 30); drawn, the pixels on both sides of y = 30 just left of
+// x = 120 are covered and the pixel right of the butt end is not.
+static void StrokePathContainsEnd(hipr_context ctx) {
+  auto surface = HIPSurface::Make(ctx, 130, 40);
+  HIPCanvas* canvas = surface->LockCanvas();
+  Path src;
+  src.MoveTo(30, 30);
+  src.QuadTo(40, 30, 120, 30);
+  Paint paint;
+  paint.SetStyle(Paint::kStroke_Style);
+  paint.SetStrokeWidth(2);
+  canvas->DrawPath(src, paint);
+  EXPECT_EQ(surface->Flush(), HIPR_SUCCESS);
+  EXPECT_EQ(surface->GetPixel(119, 29), Color_BLACK);
+  EXPECT_EQ(surface->GetPixel(119, 30), Color_BLACK);
+  EXPECT_EQ(surface->GetPixel(120, 30), Color_TRANSPARENT);
+}
+
+// Canvas::DrawCircle / DrawLine dispatch to DrawPath (canvas.cc:398-437).
+static void ShapesDispatchToDrawPath(hipr_context ctx) {
+  auto a = HIPSurface::Make(ctx, 64, 64);
+  auto b = HIPSurface::Make(ctx, 64, 64);
+  Paint paint;
+  paint.SetColor(ColorSetARGB(200, 30, 160, 90));
+  a->LockCanvas()->DrawCircle(31.3f, 30.6f, 17.2f, paint);
+  Path circle;
+  circle.AddOval(Rect::MakeLTRB(31.3f - 17.2f, 30.6f - 17.2f, 31.3f + 17.2f, 30.6f + 17.2f),
+                 Path::Direction::kCW, 3);
+  b->LockCanvas()->DrawPath(circle, paint);
+  Paint line;
+  line.SetStyle(Paint::kStroke_Style);
+  line.SetStrokeWidth(3.5f);
+  line.SetStrokeCap(Paint::kRound_Cap);
+  line.SetColor(Color_BLUE);
+  a->LockCanvas(false)->DrawLine(4, 60, 60, 4, line);
+  Path lp;
+  lp.MoveTo(4, 60);
+  lp.LineTo(60, 4);
+  b->LockCanvas(false)->DrawPath(lp, line);
+  EXPECT_EQ(a->Flush(), HIPR_SUCCESS);
+  EXPECT_EQ(b->Flush(), HIPR_SUCCESS);
+  int diff = 0;
+  for (int y = 0; y < 64; y++)
+    for (int x = 0; x < 64; x++) diff += a->GetPixel(x, y) != b->GetPixel(x, y);
+  EXPECT_EQ(diff, 0);
+  // premultiplied centre: 200/255 alpha over transparent
+  EXPECT_EQ(ColorGetA(a->GetPixel(31, 30)), 200u);
+}
+
+// Shader::MakeLinear end colours under clamp tiling.
+static void LinearGradientEnds(hipr_context ctx) {
+  auto s = HIPSurface::Make(ctx, 100, 8);
+  Point pts[2] = {{10, 0}, {90, 0}};
+  Color4f colors[2] = {{1, 0, 0, 1}, {0, 0, 1, 1}};
+  Paint paint;
+  paint.SetShader(Shader::MakeLinear(pts, colors, nullptr, 2));
+  s->LockCanvas()->DrawRect(Rect::MakeWH(100, 8), paint);
+  EXPECT_EQ(s->Flush(), HIPR_SUCCESS);
+  EXPECT_EQ(s->GetPixel(2, 4), Color_RED);
+  EXPECT_EQ(s->GetPixel(97, 4), Color_BLUE);
+}
+
+// Unsupported input is refused, not approximated.
+static void RefusesHairline(hipr_context ctx) {
+  auto s = HIPSurface::Make(ctx, 16, 16);
+  Paint paint;
+  paint.SetStyle(Paint::kStroke_Style);
+  paint.SetStrokeWidth(0);
+  s->LockCanvas()->DrawLine(1, 1, 10, 10, paint);
+  EXPECT_EQ(s->Flush(), HIPR_ERROR_NOT_SUPPORTED);
+}
+
+int main() {
+  hipr_context ctx = nullptr;
+  if (hipr_context_create(0, &ctx) != HIPR_SUCCESS) {
+    std::printf("no device\n");
+    return 2;
+  }
+  struct {
+    const char* name;
+    void (*fn)(hipr_context);
+  } tests[] = {{"SWCanvas.RedRect", RedRect},
+               {"SWCanvas.StrokeThenFillDrawsFillAfterStroke", StrokeThenFillDrawsFillAfterStroke},
+               {"Path.StrokePathContainsEnd", StrokePathContainsEnd},
+               {"Canvas.ShapesDispatchToDrawPath", ShapesDispatchToDrawPath},
+               {"Shader.LinearGradientEnds", LinearGradientEnds},
+               {"HIPCanvas.RefusesHairline", RefusesHairline}};
+  for (auto& t : tests) {
+    int before = g_failed;
+    t.fn(ctx);
+    std::printf("[%s] %s\n", g_failed == before ? "  OK  " : "FAILED", t.name);
+  }
+  hipr_context_destroy(ctx);
+  std::printf("%d check(s) failed\n", g_failed);
+  return g_failed ? 1 : 0;
+}
