@@ -281,6 +281,14 @@ HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx,
 /* StackBlur on a device-resident premultiplied RGBA8 image
  * (SWStackBlur::Blur, src/render/sw/sw_stack_blur.cc:18-284): radius is
  * round(MaskFilter radius), clamped to 254; radius <= 1 copies. */
+/* Self-check of the wavefront primitives the kernels use (DPP scans and
+ * shifts against their shuffle forms): for every wave of 64 inputs writes
+ * 6 x 64 outputs (DPP sum scan, shuffle sum scan, DPP max scan, shuffle max
+ * scan, shift up with fill -7, shift down with fill -9).  No reference
+ * counterpart. */
+HIPR_API hipr_result hipr_debug_wave_ops(hipr_context ctx, const int32_t* in, int32_t* out,
+                                         int32_t n_waves);
+
 HIPR_API hipr_result hipr_stackblur(hipr_context ctx, const uint8_t* src_host,
                                     uint8_t* dst_host, int32_t width,
                                     int32_t height, int32_t radius);

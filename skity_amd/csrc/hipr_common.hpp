@@ -118,6 +118,92 @@ __device__ __forceinline__ int WaveSum(int v) {
   return v;
 }
 
+// ---- DPP forms (GFX9 data-parallel primitives; every lane must be active) --
+// Inclusive scans: row_shr 1, 2, 4, 8 inside each row of 16 lanes, then
+// row_bcast 15 (into rows 1, 3) and row_bcast 31 (into rows 2, 3).  A lane
+// whose source is outside the row keeps the identity passed as `old`.
+constexpr int kDppRowShr = 0x110, kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138,
+              kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+__device__ __forceinline__ int DppScanSum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr | 1, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr | 2, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr | 4, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr | 8, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppBcast15, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppBcast31, 0xc, 0xf, false);
+  return v;
+}
+__device__ __forceinline__ int DppMax2(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int DppScanMax(int v) {
+  const int lo = (int)0x80000000;
+  v = DppMax2(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr | 1, 0xf, 0xf, false));
+  v = DppMax2(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr | 2, 0xf, 0xf, false));
+  v = DppMax2(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr | 4, 0xf, 0xf, false));
+  v = DppMax2(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr | 8, 0xf, 0xf, false));
+  v = DppMax2(v, __builtin_amdgcn_update_dpp(lo, v, kDppBcast15, 0xa, 0xf, false));
+  v = DppMax2(v, __builtin_amdgcn_update_dpp(lo, v, kDppBcast31, 0xc, 0xf, false));
+  return v;
+}
+// Lane i receives lane i-1 (lane 0 receives `fill`).
+__device__ __forceinline__ int DppShiftUp(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, kDppWaveShr1, 0xf, 0xf, false);
+}
+// Lane i receives lane i+1 (lane 63 receives `fill`).
+__device__ __forceinline__ int DppShiftDown(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, kDppWaveShl1, 0xf, 0xf, false);
+}
+
+// The forms the kernels call: DPP unless HIPR_USE_DPP is 0.
+#ifndef HIPR_USE_DPP
+#define HIPR_USE_DPP 1
+#endif
+__device__ __forceinline__ int ScanSum(int v) {
+#if HIPR_USE_DPP
+  return DppScanSum(v);
+#else
+  return WaveInclusiveSum(v);
+#endif
+}
+__device__ __forceinline__ int ScanMax(int v) {
+#if HIPR_USE_DPP
+  return DppScanMax(v);
+#else
+  return WaveInclusiveMax(v);
+#endif
+}
+__device__ __forceinline__ int ShiftUp(int v, int fill) {
+#if HIPR_USE_DPP
+  return DppShiftUp(v, fill);
+#else
+  int t = __shfl_up(v, 1, kWave);
+  return Lane() == 0 ? fill : t;
+#endif
+}
+__device__ __forceinline__ int ShiftDown(int v, int fill) {
+#if HIPR_USE_DPP
+  return DppShiftDown(v, fill);
+#else
+  int t = __shfl_down(v, 1, kWave);
+  return Lane() == kWave - 1 ? fill : t;
+#endif
+}
+
+// Checks the DPP forms against the shuffle forms: per wave of 64 inputs,
+// out[0..5][64] = DppScanSum, WaveInclusiveSum, DppScanMax, WaveInclusiveMax,
+// DppShiftUp(fill -7), DppShiftDown(fill -9).
+__global__ void __launch_bounds__(64) WaveOpsCheckKernel(const int* in, int* out) {
+  const int lane = Lane();
+  const int v = in[blockIdx.x * kWave + lane];
+  int* o = out + blockIdx.x * 6 * kWave;
+  o[0 * kWave + lane] = DppScanSum(v);
+  o[1 * kWave + lane] = WaveInclusiveSum(v);
+  o[2 * kWave + lane] = DppScanMax(v);
+  o[3 * kWave + lane] = WaveInclusiveMax(v);
+  o[4 * kWave + lane] = DppShiftUp(v, -7);
+  o[5 * kWave + lane] = DppShiftDown(v, -9);
+}
+
 }  // namespace hipr
 
 #endif  // SKITY_HIP_COMMON_HPP_

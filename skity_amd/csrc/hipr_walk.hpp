@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         vA[q] = sA;
         vD[q] = sD;
       }
-      int iA = WaveInclusiveSum(sA), iD = WaveInclusiveSum(sD);
+      int iA = ScanSum(sA), iD = ScanSum(sD);
       int eA = iA - sA + carryA, eD = iD - sD + carryD;
       carryA += __builtin_amdgcn_readlane(iA, kWave - 1);
       carryD += __builtin_amdgcn_readlane(iD, kWave - 1);
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       bool need0 = in_cols && leader && chunk0 && cur0 < 0;
       bool need1 = in_cols && leader && chunk1 && cur1 < 0;
       int req = (need0 ? 1 : 0) + (need1 ? 1 : 0);
-      int rincl = WaveInclusiveSum(req);
+      int rincl = ScanSum(req);
       int first_slot = next_slot + rincl - req;
       next_slot += __builtin_amdgcn_readlane(rincl, kWave - 1);
       if (need0) {
@@ -484,17 +484,16 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       int p = s * kWave + lane;
       bool valid = p < n;
       int wv = valid ? MetaWinding(M[s]) : 0;
-      int incl = WaveInclusiveSum(wv) + carry_w;
+      int incl = ScanSum(wv) + carry_w;
       int in = (valid && (incl & winding_mask) != 0) ? 1 : 0;
-      int prev_in = __shfl_up(in, 1, kWave);
-      if (lane == 0) prev_in = carry_in;
+      int prev_in = ShiftUp(in, carry_in);
       bool is_left = valid && in && !prev_in;
       bool is_right = valid && !in && prev_in;
       Fixed xo = X[s];
       Fixed xn = valid ? xo + (DX[s] >> y_shift) : 0;
       XN[s] = xn;
       // nearest left edge at or before p
-      int lsel = WaveInclusiveMax(is_left ? lane : -1);
+      int lsel = ScanMax(is_left ? lane : -1);
       Fixed lxo = __shfl(xo, lsel < 0 ? 0 : lsel, kWave);
       Fixed lxn = __shfl(xn, lsel < 0 ? 0 : lsel, kWave);
       Fixed ldy = __shfl(DY[s], lsel < 0 ? 0 : lsel, kWave);
@@ -504,9 +503,8 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         ldy = carry_ldy;
       }
       // previous right edge strictly before p
-      int rsel_incl = WaveInclusiveMax(is_right ? lane : -1);
-      int rsel = __shfl_up(rsel_incl, 1, kWave);
-      if (lane == 0) rsel = -1;
+      int rsel_incl = ScanMax(is_right ? lane : -1);
+      int rsel = ShiftUp(rsel_incl, -1);
       Fixed rxo = __shfl(xo, rsel < 0 ? 0 : rsel, kWave);
       Fixed rxn = __shfl(xn, rsel < 0 ? 0 : rsel, kWave);
       bool has_prev_right = rsel >= 0 || carry_ridx >= 0;
@@ -515,7 +513,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         rxn = carry_rxn;
       }
       // next element (old x, dx); lane 63's successor is lane 0 of slot s+1
-      Fixed nxo = __shfl_down(xo, 1, kWave), ndx = __shfl_down(DX[s], 1, kWave);
+      Fixed nxo = ShiftDown(xo, xo), ndx = ShiftDown(DX[s], DX[s]);
       int next_p = p + 1;
       if (s + 1 < K) {
         Fixed n0x = __shfl(X[s + 1 < K ? s + 1 : s], 0, kWave);
@@ -627,12 +625,11 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         int p = s * kWave + lane;
         bool sv = p < n && LY[s] > next_y;
         SV[s] = sv;
-        int inc = WaveInclusiveSum(sv ? 1 : 0);
+        int inc = ScanSum(sv ? 1 : 0);
         C[s] = cnt_carry + inc - (sv ? 1 : 0);
         // previous survivor's x
-        int psel_incl = WaveInclusiveMax(sv ? lane : -1);
-        int psel = __shfl_up(psel_incl, 1, kWave);
-        if (lane == 0) psel = -1;
+        int psel_incl = ScanMax(sv ? lane : -1);
+        int psel = ShiftUp(psel_incl, -1);
         Fixed px = __shfl(X[s], psel < 0 ? 0 : psel, kWave);
         Fixed pdx = __shfl(DX[s], psel < 0 ? 0 : psel, kWave);
         bool hp = psel >= 0;
@@ -740,12 +737,12 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
             int p = s * kWave + lane;
             bool left = (lane & 1) == 1;  // odd position = left of its pair
             int q = left ? p + 1 : p - 1;
-            Fixed dnx = __shfl_down(X[s], 1, kWave), upx = __shfl_up(X[s], 1, kWave);
-            int dno = __shfl_down(O[s], 1, kWave), upo = __shfl_up(O[s], 1, kWave);
-            Fixed dndx = __shfl_down(DX[s], 1, kWave), updx = __shfl_up(DX[s], 1, kWave);
-            Fixed dndy = __shfl_down(DY[s], 1, kWave), updy = __shfl_up(DY[s], 1, kWave);
-            Fixed dnly = __shfl_down(LY[s], 1, kWave), uply = __shfl_up(LY[s], 1, kWave);
-            int dnm = __shfl_down(M[s], 1, kWave), upm = __shfl_up(M[s], 1, kWave);
+            Fixed dnx = ShiftDown(X[s], 0), upx = ShiftUp(X[s], 0);
+            int dno = ShiftDown(O[s], 0), upo = ShiftUp(O[s], 0);
+            Fixed dndx = ShiftDown(DX[s], 0), updx = ShiftUp(DX[s], 0);
+            Fixed dndy = ShiftDown(DY[s], 0), updy = ShiftUp(DY[s], 0);
+            Fixed dnly = ShiftDown(LY[s], 0), uply = ShiftUp(LY[s], 0);
+            int dnm = ShiftDown(M[s], 0), upm = ShiftUp(M[s], 0);
             // cross-slot partners
             {
               const int t = s + 1 < K ? s + 1 : s;

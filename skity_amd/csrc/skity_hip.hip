@@ -1651,6 +1651,26 @@ HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx, const hipr_scene* 
   return HIPR_SUCCESS;
 }
 
+HIPR_API hipr_result hipr_debug_wave_ops(hipr_context ctx, const int32_t* in, int32_t* out,
+                                         int32_t n_waves) {
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  if (!in || !out || n_waves <= 0) return HIPR_ERROR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipr_result r;
+  if ((r = Upload(&ctx->d_verbs, (const uint8_t*)in, sizeof(int32_t) * 64 * n_waves,
+                  ctx->stream)) != HIPR_SUCCESS)
+    return r;
+  if (ctx->d_pts.Ensure(sizeof(int32_t) * 6 * 64 * n_waves) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  hipLaunchKernelGGL(WaveOpsCheckKernel, dim3(n_waves), dim3(64), 0, ctx->stream,
+                     ctx->d_verbs.as<int>(), ctx->d_pts.as<int>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, ctx->d_pts.p, sizeof(int32_t) * 6 * 64 * n_waves,
+                         hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return HIPR_SUCCESS;
+}
+
 HIPR_API hipr_result hipr_stackblur(hipr_context ctx, const uint8_t* src_host, uint8_t* dst_host,
                                     int32_t width, int32_t height, int32_t radius) {
   if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
