@@ -68,6 +68,14 @@ struct Canvas {
 // ---- wave helpers ------------------------------------------------------------
 __device__ __forceinline__ int Lane() { return threadIdx.x & (kWave - 1); }
 
+// Orders the LDS accesses of a one-wavefront workgroup.  The lanes of a wave
+// issue their LDS instructions in one stream, so a compiler fence is enough;
+// unlike __syncthreads() it does not wait for outstanding global stores.
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ int WaveInclusiveSum(int v) {
   int lane = Lane();
 #pragma unroll

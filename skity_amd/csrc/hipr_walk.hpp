@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     L.slot0[i] = -1;
     L.slot1[i] = -1;
   }
-  __syncthreads();
+  WaveSync();
 
   EdgeRec* E = P.wedges + P.edge_base[j];
   const int64_t pbase = kMode == 1 ? P.plane_base[j] : 0;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       L.slot0[c] = -1;
       L.slot1[c] = -1;
     }
-    __syncthreads();
+    WaveSync();
   };
 
   // Resolve pixel row r (prefix-sum the deltas) and emit its tile rows.
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     dmin = 0x7FFFFFFF;
     dmax = -0x7FFFFFFF;
     if (lo > hi) return;  // nothing drawn in this row
-    __syncthreads();
+    WaveSync();
     bool visible = r >= cy0 && r < cy1;
     int rb = r >> 4;
     if (visible && rb != band) {
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         n_tiles++;
       }
       if (need1) L.slot1[tcol] = first_slot + (need0 ? 1 : 0);
-      __syncthreads();
+      WaveSync();
       cur0 = in_cols ? L.slot0[tcol] : -1;
       cur1 = in_cols ? L.slot1[tcol] : -1;
       if (kMode == 1) {
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       L.dA[W] = 0;
       L.dD[W] = 0;
     }
-    __syncthreads();
+    WaveSync();
   };
 
   // Insert `k` new edges already in scratch (sX.. at [CAP-k.. ) no: at 0..k-1
@@ -348,8 +348,14 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
 
   // ---- pending edges ---------------------------------------------------------
   int pp = 0;  // next pending edge (sorted position)
+  int pu_idx = -1;
+  Fixed pu_val = kMaxS32;  // upper_y of edge pu_idx, read once
   auto pending_upper = [&](int idx) -> Fixed {
-    return idx < nE ? E[idx].upper_y : kMaxS32;
+    if (idx != pu_idx) {
+      pu_idx = idx;
+      pu_val = idx < nE ? E[idx].upper_y : kMaxS32;
+    }
+    return pu_val;
   };
 
   // Gathers the new edges with upper_y <= y into scratch slots
@@ -390,7 +396,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       flags |= 2;
       k = 0;
     }
-    __syncthreads();
+    WaveSync();
     // sort the k new edges by (x, dx, index): rank by counting
     // (the sorted-by-upper_y input keeps index order within equal keys)
     int rk[K];
@@ -422,7 +428,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         tM[s] = L.sM[p];
       }
     }
-    __syncthreads();
+    WaveSync();
 #pragma unroll
     for (int s = 0; s < K; s++) {
       int p = s * kWave + lane;
@@ -434,7 +440,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         L.sM[rk[s]] = tM[s];
       }
     }
-    __syncthreads();
+    WaveSync();
     load_from_scratch(k);
     n = k;
     pp = k;
@@ -447,7 +453,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     nny = WaveMin(mn);
     Fixed nu = pending_upper(pp);
     if (nu > y && nu < nny) nny = nu;
-    __syncthreads();
+    WaveSync();
   }
 
   const Fixed stop_fixed = IntToFixed(pb.b);
@@ -576,7 +582,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         dmax = Max32(dmax, W);
       }
     }
-    __syncthreads();
+    WaveSync();
 
     // ---- phase 4: GoY and quad updates -----------------------------------------
 #pragma unroll
@@ -661,7 +667,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           L.sM[c] = M[s];
         }
       }
-      __syncthreads();
+      WaveSync();
       if (all_sorted) {
         // predecessor = previous survivor
         bool f = false;
@@ -792,7 +798,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           }
           done = __ballot(swapped) == 0;
         }
-        __syncthreads();
+        WaveSync();
         if (!done) {
           // rank by counting on (x, list position)
 #pragma unroll
@@ -812,7 +818,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
             int c = s * kWave + lane;
             if (c < m) sO[c] = O[s];
           }
-          __syncthreads();
+          WaveSync();
           int rk[K];
 #pragma unroll
           for (int s = 0; s < K; s++) {
@@ -829,7 +835,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
               rk[s] = r;
             }
           }
-          __syncthreads();
+          WaveSync();
 #pragma unroll
           for (int s = 0; s < K; s++) {
             int c = s * kWave + lane;
@@ -842,14 +848,14 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
               sO[rk[s]] = O[s];
             }
           }
-          __syncthreads();
+          WaveSync();
           load_from_scratch(m);
 #pragma unroll
           for (int s = 0; s < K; s++) {
             int c = s * kWave + lane;
             if (c < m) O[s] = sO[c];
           }
-          __syncthreads();
+          WaveSync();
         }
         // check_intersection with the predecessor at insertion time: the
         // nearest earlier element of the sorted list whose list position is
@@ -863,7 +869,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
             L.sLY[c] = O[s];
           }
         }
-        __syncthreads();
+        WaveSync();
         bool f = false;
 #pragma unroll
         for (int s = 0; s < K; s++) {
@@ -876,7 +882,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         }
         flag_isect = __ballot(f) != 0;
       }
-      __syncthreads();
+      WaveSync();
     }
     n = m;
     {
@@ -903,7 +909,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         flags |= 2;
         break;
       }
-      __syncthreads();
+      WaveSync();
       // sort the new edges by (x, dx, sorted position)
       int rk[K];
       Fixed tX[K], tDX[K], tDY[K], tLY[K];
@@ -927,7 +933,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           tM[s] = L.sM[p];
         }
       }
-      __syncthreads();
+      WaveSync();
 #pragma unroll
       for (int s = 0; s < K; s++) {
         int p = s * kWave + lane;
@@ -939,7 +945,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           L.sM[rk[s]] = tM[s];
         }
       }
-      __syncthreads();
+      WaveSync();
       // insertion positions: P1 = #{a : x_a <= x_N1};
       // Pi = max(P(i-1), #{a : x_a < x_Ni}).
       // The new edges are at scratch[0..k); positions go to sLY? keep in regs
@@ -1017,13 +1023,13 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         }
       }
       // P_i broadcast through LDS (reuse sLY region after reading)
-      __syncthreads();
+      WaveSync();
 #pragma unroll
       for (int s = 0; s < K; s++) {
         int p = s * kWave + lane;
         if (p < k) L.sLY[CAP - 1 - p] = posN[s] - p;  // P_p, stored from the end
       }
-      __syncthreads();
+      WaveSync();
       int shiftA[K];
 #pragma unroll
       for (int s = 0; s < K; s++) {
@@ -1034,7 +1040,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         }
         shiftA[s] = c;
       }
-      __syncthreads();
+      WaveSync();
 #pragma unroll
       for (int s = 0; s < K; s++) {
         int q = s * kWave + lane;
@@ -1059,11 +1065,11 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           L.sM[d] = nM[s];
         }
       }
-      __syncthreads();
+      WaveSync();
       n += k;
       pp += k;
       load_from_scratch(n);
-      __syncthreads();
+      WaveSync();
       Fixed nu2 = pending_upper(pp);
       if (nu2 > y && nu2 < nny) nny = nu2;
     } else if (nu > y && nu < nny) {

@@ -103,8 +103,8 @@ static void ShapesDispatchToDrawPath(hipr_context ctx) {
   for (int y = 0; y < 64; y++)
     for (int x = 0; x < 64; x++) diff += a->GetPixel(x, y) != b->GetPixel(x, y);
   EXPECT_EQ(diff, 0);
-  // premultiplied centre: 200/255 alpha over transparent
-  EXPECT_EQ(ColorGetA(a->GetPixel(31, 30)), 200u);
+  // inside the circle, away from the line: 200/255 alpha over transparent
+  EXPECT_EQ(ColorGetA(a->GetPixel(31, 20)), 200u);
 }
 
 // Shader::MakeLinear end colours under clamp tiling.
