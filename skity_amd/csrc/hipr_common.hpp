@@ -55,6 +55,8 @@ struct JobCov {
   int32_t max_active;
   int32_t flags;        // bit0 plane overflow, bit1 active overflow,
                         // bit2 stacked direct spans, bit3 open interval
+  int32_t steps;        // walk sub-steps taken
+  int32_t time_10ns;    // wall time of the job's walk, in 10 ns units
 };
 
 struct Canvas {

@@ -110,6 +110,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   const int lane = Lane();
   const int wi = blockIdx.x;
   if (wi >= P.n_walk) return;
+  const uint64_t t_start = wall_clock64();
   const int j = P.walk_list[wi];
   const Job job = P.jobs[j];
   const JobGeo geo = P.geo[j];
@@ -133,7 +134,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   if (pb.empty || tr.ntc == 0 || nE == 0 || W > P.lds_width) {
     if (!pb.empty && W > P.lds_width) flags |= 16;
     if (lane == 0) {
-      JobCov c{pb.l, pb.t, pb.r, pb.b, 0, 0, 0, flags};
+      JobCov c{pb.l, pb.t, pb.r, pb.b, 0, 0, 0, flags, 0, 0};
       P.cov[j] = c;
     }
     return;
@@ -1081,7 +1082,9 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) flags |= __shfl_xor(flags, o, kWave);
   if (lane == 0) {
-    JobCov c{pb.l, pb.t, pb.r, pb.b, n_tiles, next_slot, max_active, flags};
+    uint64_t dt = wall_clock64() - t_start;
+    JobCov c{pb.l, pb.t, pb.r, pb.b, n_tiles, next_slot, max_active, flags, guard,
+             (int32_t)(dt < 0x7FFFFFFF ? dt : 0x7FFFFFFF)};
     P.cov[j] = c;
   }
 }
