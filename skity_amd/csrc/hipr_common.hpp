@@ -57,6 +57,8 @@ struct JobCov {
                         // bit2 stacked direct spans, bit3 open interval
   int32_t steps;        // walk sub-steps taken
   int32_t time_10ns;    // wall time of the job's walk, in 10 ns units
+  int32_t phase[5];     // of which: roles+trapezoids, quad updates, re-sort,
+                        // row flushes, edge insertion (10 ns units)
 };
 
 struct Canvas {

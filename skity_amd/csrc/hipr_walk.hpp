@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   if (pb.empty || tr.ntc == 0 || nE == 0 || W > P.lds_width) {
     if (!pb.empty && W > P.lds_width) flags |= 16;
     if (lane == 0) {
-      JobCov c{pb.l, pb.t, pb.r, pb.b, 0, 0, 0, flags, 0, 0};
+      JobCov c{pb.l, pb.t, pb.r, pb.b, 0, 0, 0, flags, 0, 0, {0, 0, 0, 0, 0}};
       P.cov[j] = c;
     }
     return;
@@ -459,6 +459,8 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
 
   const Fixed stop_fixed = IntToFixed(pb.b);
   int guard = 0;
+  uint64_t ph[5] = {0, 0, 0, 0, 0};  // phase times (10 ns units)
+  uint64_t tp_loop = wall_clock64();
   for (;;) {
     if (++guard > (1 << 26)) {  // cannot happen: y strictly increases
       flags |= 32;
@@ -477,6 +479,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     const int row = y >> 16;
     const int S = (n + kWave - 1) / kWave;
 
+    uint64_t tp0 = wall_clock64();
     // ---- phases 1-3: roles, trapezoids --------------------------------------
     int carry_w = 0;
     int carry_in = 0;
@@ -585,6 +588,8 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     }
     WaveSync();
 
+    uint64_t tp1 = wall_clock64();
+    ph[0] += tp1 - tp0;
     // ---- phase 4: GoY and quad updates -----------------------------------------
 #pragma unroll
     for (int s = 0; s < K; s++) {
@@ -616,6 +621,8 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       }
     }
 
+    uint64_t tp2 = wall_clock64();
+    ph[1] += tp2 - tp1;
     // ---- phases 5-6: survivors, stable x order, intersection checks ----------
     int m = 0;
     bool flag_isect = false;
@@ -897,9 +904,13 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       if (flag_isect) nny = next_y + (kFixed1 >> kAcc);
     }
 
+    uint64_t tp3 = wall_clock64();
+    ph[2] += tp3 - tp2;
     // ---- row completion ------------------------------------------------------
     y = next_y;
     if ((y & 0xFFFF) == 0) flush_row(row);
+    uint64_t tp4 = wall_clock64();
+    ph[3] += tp4 - tp3;
     if (y >= stop_fixed) break;
 
     // ---- phase 7: insert new edges (insert_new_edges) --------------------------
@@ -1082,9 +1093,13 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) flags |= __shfl_xor(flags, o, kWave);
   if (lane == 0) {
-    uint64_t dt = wall_clock64() - t_start;
+    uint64_t t_end = wall_clock64();
+    uint64_t dt = t_end - t_start;
+    ph[4] = (t_end - tp_loop) - (ph[0] + ph[1] + ph[2] + ph[3]);
     JobCov c{pb.l, pb.t, pb.r, pb.b, n_tiles, next_slot, max_active, flags, guard,
              (int32_t)(dt < 0x7FFFFFFF ? dt : 0x7FFFFFFF)};
+#pragma unroll
+    for (int k = 0; k < 5; k++) c.phase[k] = (int32_t)(ph[k] < 0x7FFFFFFF ? ph[k] : 0x7FFFFFFF);
     P.cov[j] = c;
   }
 }

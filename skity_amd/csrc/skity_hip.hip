@@ -276,6 +276,7 @@ struct hipr_surface_s {
   hipr_stats stats{};
   std::vector<int32_t> job_edges, job_planes;  // exact counts of the last draw
   std::vector<int32_t> job_steps, job_time;    // walk profile of the last draw
+  std::vector<int32_t> job_phase;              // 5 per job
   double stage_ms[HIPR_STAGE_COUNT] = {};
   int32_t stage_launches[HIPR_STAGE_COUNT] = {};
 };
@@ -1076,12 +1077,14 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   surf->job_planes.assign(nj, -1);
   surf->job_steps.assign(nj, 0);
   surf->job_time.assign(nj, 0);
+  surf->job_phase.assign((size_t)nj * 5, 0);
   for (int j = 0; j < nj; j++) {
     if (f.jobs[j].kind == kJobImage) continue;
     surf->job_edges[j] = ctx->h_geo[j].edges;
     surf->job_planes[j] = ctx->h_cov[j].planes;
     surf->job_steps[j] = ctx->h_cov[j].steps;
     surf->job_time[j] = ctx->h_cov[j].time_10ns;
+    for (int k = 0; k < 5; k++) surf->job_phase[(size_t)j * 5 + k] = ctx->h_cov[j].phase[k];
   }
   // ---- composite: layers, blur, canvas ----
   CompositeParams cp{};
@@ -1413,12 +1416,15 @@ HIPR_API hipr_result hipr_surface_get_job_counts(hipr_surface s, int32_t* edges,
 }
 
 HIPR_API hipr_result hipr_surface_get_job_profile(hipr_surface s, int32_t* steps,
-                                                  int32_t* time_10ns, int32_t n_jobs) {
+                                                  int32_t* time_10ns, int32_t* phases,
+                                                  int32_t n_jobs) {
   if (!s) return HIPR_ERROR_INVALID_HANDLE;
   if (n_jobs != (int32_t)s->job_steps.size()) return HIPR_ERROR_INVALID_ARGUMENT;
   for (int j = 0; j < n_jobs; j++) {
     if (steps) steps[j] = s->job_steps[j];
     if (time_10ns) time_10ns[j] = s->job_time[j];
+    if (phases)
+      for (int k = 0; k < 5; k++) phases[(size_t)j * 5 + k] = s->job_phase[(size_t)j * 5 + k];
   }
   return HIPR_SUCCESS;
 }
