@@ -2,13 +2,15 @@
 //
 // blit_trapezoid_row / blit_aaa_trapezoid_row and their alpha helpers
 // (src/render/sw/sw_raster.cc:138-544), evaluated per pixel in closed form.
-// Coverage goes into two LDS delta rows (prefix-summed when the pixel row is
-// complete):
-//   dD: spans the reference sends straight to RealSpanBuilder (full-height
-//       steps whose edges are not too close); each adds (1 << 16) + cover, so
-//       the prefix sum carries the number of such spans in its high half;
-//   dA: spans the reference accumulates in SpanBuilder; saturating addition
-//       of non-negative alphas equals min(255, sum), so plain sums suffice.
+// Coverage goes into one LDS delta row, prefix-summed when the pixel row is
+// complete.  Each 32-bit entry carries two fields whose prefix sums are
+// non-negative and below 2^16, so one integer prefix sum S decodes as
+// A = S & 0xFFFF, D = S >> 16:
+//   D: spans the reference sends straight to RealSpanBuilder (full-height
+//      steps whose edges are not too close); each adds (1 << 12) + cover, so
+//      D carries the number of such spans above bit 12;
+//   A: spans the reference accumulates in SpanBuilder; saturating addition
+//      of non-negative alphas equals min(255, sum), so plain sums suffice.
 // Spans with cover 0 blend nothing and are skipped.
 #ifndef SKITY_HIP_BLIT_HPP_
 #define SKITY_HIP_BLIT_HPP_
@@ -19,21 +21,23 @@ namespace hipr {
 
 typedef uint8_t Alpha;
 
+constexpr int kDirectCount = 1 << 12;  // one direct span, in the D field
+
 struct RowSink {
-  int32_t* dA;
-  int32_t* dD;
+  int32_t* d;
   int32_t left;   // pixel x of delta index 0
   __device__ __forceinline__ void Direct(int x, int len, int v) {
     if (v == 0 || len <= 0) return;
     int i = x - left;
-    atomicAdd(&dD[i], (1 << 16) + v);
-    atomicAdd(&dD[i + len], -((1 << 16) + v));
+    int inc = (kDirectCount + v) << 16;
+    atomicAdd(&d[i], inc);
+    atomicAdd(&d[i + len], -inc);
   }
   __device__ __forceinline__ void Acc(int x, int len, int v) {
     if (v == 0 || len <= 0) return;
     int i = x - left;
-    atomicAdd(&dA[i], v);
-    atomicAdd(&dA[i + len], -v);
+    atomicAdd(&d[i], v);
+    atomicAdd(&d[i + len], -v);
   }
   __device__ __forceinline__ void Put(bool direct, int x, int len, int v) {
     if (direct) {

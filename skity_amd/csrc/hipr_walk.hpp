@@ -92,8 +92,7 @@ __device__ __forceinline__ int MakeMeta(int winding, bool quad, int idx) {
 }
 
 struct WalkLds {
-  int32_t* dA;
-  int32_t* dD;
+  int32_t* d;      // packed row deltas (RowSink)
   int32_t* slot0;  // per tile column of the current band
   int32_t* slot1;
   int32_t* sX;     // permutation scratch (CAP entries each)
@@ -102,6 +101,49 @@ struct WalkLds {
   int32_t* sLY;
   int32_t* sM;
 };
+
+// Bitonic sort (ascending) of N = 64 * 2^j keys held as element
+// p = slot * 64 + lane; keys at p >= N are not touched (N <= 64 * K).
+template <int K, int SS>
+__device__ __forceinline__ void SlotExchange(int64_t (&key)[K], int size, int lane) {
+#pragma unroll
+  for (int s = 0; s < K; s++) {
+    const int t = s ^ SS;
+    if (t > s && t < K) {
+      int p = s * kWave + lane;
+      bool asc = (p & size) == 0;
+      int64_t a = key[s], b = key[t];
+      bool sw = asc ? (a > b) : (a < b);
+      key[s] = sw ? b : a;
+      key[t] = sw ? a : b;
+    }
+  }
+}
+template <int K>
+__device__ void BitonicSortSlots(int64_t (&key)[K], int N, int lane) {
+  for (int size = 2; size <= N; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= kWave) {
+        int ss = stride / kWave;
+        if (ss == 1) SlotExchange<K, 1>(key, size, lane);
+        else if (ss == 2) SlotExchange<K, 2>(key, size, lane);
+        else if (ss == 4) SlotExchange<K, 4>(key, size, lane);
+        else if (ss == 8) SlotExchange<K, 8>(key, size, lane);
+      } else {
+        bool lower = (lane & stride) == 0;
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          if (s * kWave >= N) break;
+          int64_t o = __shfl_xor(key[s], stride, kWave);
+          int p = s * kWave + lane;
+          bool asc = (p & size) == 0;
+          int64_t mn = key[s] < o ? key[s] : o, mx = key[s] < o ? o : key[s];
+          key[s] = (lower == asc) ? mn : mx;
+        }
+      }
+    }
+  }
+}
 
 template <int K, int kMode>
 __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
@@ -140,9 +182,8 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     return;
   }
   WalkLds L;
-  L.dA = lds;
-  L.dD = L.dA + (P.lds_width + 1);
-  L.slot0 = L.dD + (P.lds_width + 1);
+  L.d = lds;
+  L.slot0 = L.d + (P.lds_width + 1);
   L.slot1 = L.slot0 + (P.lds_width / kTile + 2);
   L.sX = L.slot1 + (P.lds_width / kTile + 2);
   L.sDX = L.sX + CAP;
@@ -150,8 +191,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   L.sLY = L.sDY + CAP;
   L.sM = L.sLY + CAP;
   for (int i = lane; i <= W; i += kWave) {
-    L.dA[i] = 0;
-    L.dD[i] = 0;
+    L.d[i] = 0;
   }
   for (int i = lane; i < tr.ntc; i += kWave) {
     L.slot0[i] = -1;
@@ -169,7 +209,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   const int winding_mask = job.fill == HIPR_FILL_EVEN_ODD ? 1 : -1;
   const int cx0 = pb.l > 0 ? pb.l : 0, cx1 = pb.r < cv.width ? pb.r : cv.width;
   const int cy0 = pb.t > 0 ? pb.t : 0, cy1 = pb.b < cv.height ? pb.b : cv.height;
-  RowSink sink{L.dA, L.dD, pb.l};
+  RowSink sink{L.d, pb.l};
 
   // Active list registers.
   Fixed X[K], DX[K], DY[K], LY[K];
@@ -229,37 +269,33 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     // canvas-aligned 16-pixel chunks; lane owns 4 pixels
     int base = (pb.l + lo) & ~15;
     int end = pb.l + hi;  // exclusive pixel bound of the dirty range
-    int carryA = 0, carryD = 0;
+    int carry = 0;
     // deltas left of `base` are zero: base <= pb.l + lo
     for (int x0 = base; x0 <= end; x0 += 4 * kWave) {
       int px = x0 + 4 * lane;
-      int32_t vA[4], vD[4];
-      int sA = 0, sD = 0;
+      int32_t vS[4];
+      int sS = 0;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         int i = px + q - pb.l;
-        int a = 0, d = 0;
+        int dv = 0;
         if (i >= 0 && i <= W) {
-          a = L.dA[i];
-          d = L.dD[i];
-          L.dA[i] = 0;
-          L.dD[i] = 0;
+          dv = L.d[i];
+          L.d[i] = 0;
         }
-        sA += a;
-        sD += d;
-        vA[q] = sA;
-        vD[q] = sD;
+        sS += dv;
+        vS[q] = sS;
       }
-      int iA = ScanSum(sA), iD = ScanSum(sD);
-      int eA = iA - sA + carryA, eD = iD - sD + carryD;
-      carryA += __builtin_amdgcn_readlane(iA, kWave - 1);
-      carryD += __builtin_amdgcn_readlane(iD, kWave - 1);
+      int iS = ScanSum(sS);
+      int eS = iS - sS + carry;
+      carry += __builtin_amdgcn_readlane(iS, kWave - 1);
       uint32_t c0 = 0, c1 = 0;
       bool stacked = false;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        int a = vA[q] + eA, d = vD[q] + eD;
-        int cnt = d >> 16, dv = d & 0xFFFF;
+        int S = vS[q] + eS;
+        int a = S & 0xFFFF, d = S >> 16;
+        int cnt = d >> 12, dv = d & 0xFFF;
         int av = a > 255 ? 255 : a;
         int first = cnt ? dv : av;
         int second = (cnt && av) ? av : 0;
@@ -337,10 +373,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       }
     }
     // clear the end-of-row delta entry
-    if (lane == 0) {
-      L.dA[W] = 0;
-      L.dD[W] = 0;
-    }
+    if (lane == 0) L.d[W] = 0;
     WaveSync();
   };
 
@@ -699,7 +732,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           return xa > xb || (xa == xb && oa > ob);
         };
         bool done = false;
-        for (int pass = 0; pass < 24 && !done; pass++) {
+        for (int pass = 0; pass < 8 && !done; pass++) {
           bool swapped = false;
           // even phase: pairs (2t, 2t+1) are lanes (2t, 2t+1) of one slot
 #pragma unroll
@@ -808,60 +841,41 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
         }
         WaveSync();
         if (!done) {
-          // rank by counting on (x, list position)
+          // bitonic sort of the keys (x, list position); the other fields
+          // are parked in scratch by list position and gathered back
 #pragma unroll
           for (int s = 0; s < K; s++) {
             int c = s * kWave + lane;
             if (c < m) {
-              L.sX[c] = X[s];
-              L.sDX[c] = DX[s];
-              L.sDY[c] = DY[s];
-              L.sLY[c] = LY[s];
-              L.sM[c] = M[s];
+              L.sDX[O[s]] = DX[s];
+              L.sDY[O[s]] = DY[s];
+              L.sLY[O[s]] = LY[s];
+              L.sM[O[s]] = M[s];
             }
           }
-          int* sO = L.sX + 5 * CAP;  // list positions (spare scratch)
+          int64_t key[K];
 #pragma unroll
           for (int s = 0; s < K; s++) {
             int c = s * kWave + lane;
-            if (c < m) sO[c] = O[s];
+            // signed x in the high word, list position (>= 0) in the low word
+            key[s] = c < m ? (int64_t)(((uint64_t)(uint32_t)X[s] << 32) | (uint32_t)O[s])
+                           : INT64_MAX;
           }
-          WaveSync();
-          int rk[K];
-#pragma unroll
-          for (int s = 0; s < K; s++) {
-            int c = s * kWave + lane;
-            rk[s] = 0;
-            if (c < m) {
-              Fixed xi = L.sX[c];
-              int oi = sO[c];
-              int r = 0;
-              for (int q = 0; q < m; q++) {
-                Fixed xq = L.sX[q];
-                r += (xq < xi || (xq == xi && sO[q] < oi)) ? 1 : 0;
-              }
-              rk[s] = r;
-            }
-          }
+          int N = kWave;
+          while (N < m) N <<= 1;
+          BitonicSortSlots<K>(key, N, lane);
           WaveSync();
 #pragma unroll
           for (int s = 0; s < K; s++) {
             int c = s * kWave + lane;
             if (c < m) {
-              L.sX[rk[s]] = X[s];
-              L.sDX[rk[s]] = DX[s];
-              L.sDY[rk[s]] = DY[s];
-              L.sLY[rk[s]] = LY[s];
-              L.sM[rk[s]] = M[s];
-              sO[rk[s]] = O[s];
+              X[s] = (Fixed)(key[s] >> 32);
+              O[s] = (int)(uint32_t)((uint64_t)key[s] & 0xFFFFFFFFull);
+              DX[s] = L.sDX[O[s]];
+              DY[s] = L.sDY[O[s]];
+              LY[s] = L.sLY[O[s]];
+              M[s] = L.sM[O[s]];
             }
-          }
-          WaveSync();
-          load_from_scratch(m);
-#pragma unroll
-          for (int s = 0; s < K; s++) {
-            int c = s * kWave + lane;
-            if (c < m) O[s] = sO[c];
           }
           WaveSync();
         }

@@ -593,7 +593,7 @@ void SetTiles(std::vector<Canvas>* cvs) {
 template <int K, int kMode>
 hipr_result LaunchWalk(hipr_context ctx, const WalkParams& P, hipStream_t st) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
-  size_t lds = (size_t)(2 * (P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 6 * K * kWave) *
+  size_t lds = (size_t)((P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 6 * K * kWave) *
                sizeof(int32_t);
   STAGE_LAUNCH(ctx, kMode == 0 ? HIPR_STAGE_WALK_COUNT : HIPR_STAGE_WALK,
                hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds, st, P));
