@@ -214,11 +214,12 @@ HIPR_API hipr_result hipr_surface_get_job_counts(hipr_surface surface, int32_t* 
                                                  int32_t* planes, int32_t n_jobs);
 /* Walk profile of the last draw per job: sub-steps taken, the walk's wall
  * time and (phases, 5 per job) its split into roles + trapezoids, quad
- * updates, re-sort, row flushes and edge insertion, in 10 ns units (last
- * write pass).  Any pointer may be NULL.  No reference counterpart. */
+ * updates, re-sort, row flushes and edge insertion, in 10 ns units, and the
+ * largest active edge list (last write pass).  Any pointer may be NULL.
+ * No reference counterpart. */
 HIPR_API hipr_result hipr_surface_get_job_profile(hipr_surface surface, int32_t* steps,
                                                   int32_t* time_10ns, int32_t* phases,
-                                                  int32_t n_jobs);
+                                                  int32_t* max_active, int32_t n_jobs);
 /* Predictions (n x 2, log1p units) for host-side features. */
 HIPR_API hipr_result hipr_presizer_predict(hipr_context ctx, const float* params,
                                            const float* features, int64_t n, float* out);

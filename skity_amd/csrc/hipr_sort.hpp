@@ -22,17 +22,25 @@ __device__ __forceinline__ uint64_t EdgeKey(const EdgeRec& e, int idx) {
 }
 
 // keys: per-job scratch of next_pow2(n) entries (used when n exceeds LDS).
+// active_bound (may be null) receives an upper bound of the job's active
+// edge list: the largest number of edges whose row range [upper_y, lower_y)
+// meets one bin of rows (a bin is one pixel row when the job spans at most
+// 16384 rows).  The walk keeps exactly the edges with upper_y <= y < lower_y
+// active, so this bounds every sub-step's list.
 __global__ void __launch_bounds__(kSortThreads)
     SortGatherKernel(const int32_t* __restrict__ job_list, int n_list,
                      const JobGeo* __restrict__ geo, const int64_t* __restrict__ edge_base,
                      const EdgeRec* __restrict__ edges, EdgeRec* __restrict__ wedges,
-                     uint64_t* __restrict__ gkeys) {
+                     uint64_t* __restrict__ gkeys, int32_t* __restrict__ active_bound) {
   __shared__ uint64_t skeys[kSortLdsKeys];
   int li = blockIdx.x;
   if (li >= n_list) return;
   int j = job_list[li];
   int n = geo[j].edges;
-  if (n <= 0) return;
+  if (n <= 0) {
+    if (active_bound && threadIdx.x == 0) active_bound[j] = 0;
+    return;
+  }
   int64_t base = edge_base[j];
   const EdgeRec* src = edges + base;
   EdgeRec* dst = wedges + base;
@@ -64,6 +72,57 @@ __global__ void __launch_bounds__(kSortThreads)
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     int idx = (int)(uint32_t)(keys[i] & 0xFFFFFFFFu);
     dst[i] = src[idx];
+  }
+  if (!active_bound) return;
+  // ---- row-overlap bound (LDS reused as a histogram of row bins) ----
+  __syncthreads();
+  constexpr int kBins = kSortLdsKeys * 2;
+  int32_t* hist = (int32_t*)skeys;
+  const JobGeo g = geo[j];
+  const int top = (int)floorf(g.t), bottom = (int)ceilf(g.b);
+  const int H = bottom > top ? bottom - top : 1;
+  const int bin = (H + kBins - 3) / (kBins - 2);  // rows per bin, >= 1
+  const int nb = H / bin + 2;                     // <= kBins
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const EdgeRec& e = src[i];
+    int r0 = (e.upper_y >> 16) - top, r1 = ((e.lower_y - 1) >> 16) - top;
+    r0 = r0 < 0 ? 0 : (r0 >= H ? H - 1 : r0);
+    r1 = r1 < r0 ? r0 : (r1 >= H ? H - 1 : r1);
+    atomicAdd(&hist[r0 / bin], 1);
+    atomicAdd(&hist[r1 / bin + 1], -1);
+  }
+  __syncthreads();
+  // chunked scan: each thread owns a contiguous chunk of bins
+  __shared__ int32_t part[kSortThreads];
+  const int chunk = (nb + kSortThreads - 1) / kSortThreads;
+  const int c0 = threadIdx.x * chunk, c1 = c0 + chunk < nb ? c0 + chunk : nb;
+  int sum = 0;
+  for (int i = c0; i < c1; i++) sum += hist[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int t = 0; t < kSortThreads; t++) {
+      int v = part[t];
+      part[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = part[threadIdx.x], mx = 0;
+  for (int i = c0; i < c1; i++) {
+    run += hist[i];
+    mx = run > mx ? run : mx;
+  }
+  __syncthreads();
+  part[threadIdx.x] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = 0;
+    for (int t = 0; t < kSortThreads; t++) m = part[t] > m ? part[t] : m;
+    active_bound[j] = m;
   }
 }
 

@@ -239,10 +239,13 @@ GradDev MakeGradDev(const hipr_gradient& g, const Aff& ctm) {
 struct hipr_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // wide walk jobs run here, beside the main stream
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // per-draw device buffers (grow-only)
   DevBuf canvases, layer_canvases, geo, cov, edge_base, edge_cap, edges, wedges, gkeys;
-  DevBuf plane_base, plane_cap, planes, map_base, map0, map1, sub_list, big_list;
+  DevBuf plane_base, plane_cap, planes, map_base, map0, map1, sub_list, big_list, bound,
+      gather_list;
   DevBuf stroke_out, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w;
   DevBuf layers, blur_rows, blur_cols, blur_rb, blur_cb, blur_tmp, blur_jobs, blur_lines;
   // debug/utility uploads
@@ -277,6 +280,7 @@ struct hipr_surface_s {
   std::vector<int32_t> job_edges, job_planes;  // exact counts of the last draw
   std::vector<int32_t> job_steps, job_time;    // walk profile of the last draw
   std::vector<int32_t> job_phase;              // 5 per job
+  std::vector<int32_t> job_active;             // largest active edge list
   double stage_ms[HIPR_STAGE_COUNT] = {};
   int32_t stage_launches[HIPR_STAGE_COUNT] = {};
 };
@@ -320,24 +324,25 @@ hipEvent_t NextEvent(hipr_context ctx) {
   }
   return ctx->ev_pool[ctx->ev_used++];
 }
-hipEvent_t StageBegin(hipr_context ctx) {
+hipEvent_t StageBegin(hipr_context ctx, hipStream_t st = nullptr) {
   hipEvent_t e = NextEvent(ctx);
-  if (e) (void)hipEventRecord(e, ctx->stream);
+  if (e) (void)hipEventRecord(e, st ? st : ctx->stream);
   return e;
 }
-void StageEnd(hipr_context ctx, int stage, hipEvent_t begin) {
+void StageEnd(hipr_context ctx, int stage, hipEvent_t begin, hipStream_t st = nullptr) {
   hipEvent_t e = NextEvent(ctx);
   if (!e || !begin) return;
-  (void)hipEventRecord(e, ctx->stream);
+  (void)hipEventRecord(e, st ? st : ctx->stream);
   ctx->marks.push_back({stage, begin, e});
 }
-// Runs a kernel launch statement between two stage events.
-#define STAGE_LAUNCH(ctx, stage, ...)          \
-  do {                                         \
-    hipEvent_t stage_begin__ = StageBegin(ctx); \
-    __VA_ARGS__;                               \
-    StageEnd(ctx, stage, stage_begin__);       \
+// Runs a kernel launch statement between two stage events on `st`.
+#define STAGE_LAUNCH_ON(ctx, st, stage, ...)        \
+  do {                                              \
+    hipEvent_t stage_begin__ = StageBegin(ctx, st); \
+    __VA_ARGS__;                                    \
+    StageEnd(ctx, stage, stage_begin__, st);        \
   } while (0)
+#define STAGE_LAUNCH(ctx, stage, ...) STAGE_LAUNCH_ON(ctx, ctx->stream, stage, __VA_ARGS__)
 
 hipr_result HostStage(hipr_context ctx, size_t n) {
   if (n > ctx->h_cap) {
@@ -595,8 +600,9 @@ hipr_result LaunchWalk(hipr_context ctx, const WalkParams& P, hipStream_t st) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
   size_t lds = (size_t)((P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 6 * K * kWave) *
                sizeof(int32_t);
-  STAGE_LAUNCH(ctx, kMode == 0 ? HIPR_STAGE_WALK_COUNT : HIPR_STAGE_WALK,
-               hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds, st, P));
+  STAGE_LAUNCH_ON(ctx, st, kMode == 0 ? HIPR_STAGE_WALK_COUNT : HIPR_STAGE_WALK,
+                  hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds,
+                                     st, P));
   HIP_TRY(hipGetLastError());
   return HIPR_SUCCESS;
 }
@@ -747,14 +753,14 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
   return HIPR_SUCCESS;
 }
 
-hipr_result SortGather(Frame& f, const int32_t* d_list, int n) {
+hipr_result SortGather(Frame& f, const int32_t* d_list, int n, int32_t* d_bound = nullptr) {
   if (n == 0) return HIPR_SUCCESS;
   hipr_context ctx = f.ctx;
   STAGE_LAUNCH(ctx, HIPR_STAGE_SORT,
     hipLaunchKernelGGL(SortGatherKernel, dim3(n), dim3(kSortThreads), 0, f.st, d_list, n,
                      ctx->geo.as<JobGeo>(), ctx->edge_base.as<int64_t>(),
                      ctx->edges.as<EdgeRec>(), ctx->wedges.as<EdgeRec>(),
-                     ctx->gkeys.as<uint64_t>()));
+                     ctx->gkeys.as<uint64_t>(), d_bound));
   HIP_TRY(hipGetLastError());
   return HIPR_SUCCESS;
 }
@@ -926,7 +932,11 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   HIP_TRY(hipMemcpyAsync(ctx->map_base.p, mbase.data(), sizeof(int64_t) * nj,
                          hipMemcpyHostToDevice, st));
   int nwalk = (int)pl->walk_jobs.size();
-  if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk)) != HIPR_SUCCESS) return r;
+  if (ctx->bound.Ensure(sizeof(int32_t) * std::max(nj, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk, ctx->bound.as<int32_t>())) !=
+      HIPR_SUCCESS)
+    return r;
   WalkParams wp{};
   wp.jobs = f.d_jobs;
   wp.walk_list = pl->d_walk.as<int32_t>();
@@ -939,9 +949,24 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   wp.lds_width = lds_width;
   wp.map_base = ctx->map_base.as<int64_t>();
   // ---- stage (b): walk ----
-  // The walk keeps the active edge list in registers, 4 slots per lane; a
-  // job whose list outgrows 256 edges (flag 2) is walked again by the
-  // 16-slot instantiation (1024 edges).
+  // The walk keeps the active edge list in registers.  The sort bounded each
+  // job's list by its largest row overlap: jobs within 256 edges take the
+  // 4-slot walk on the main stream, the others the 16-slot walk (1024
+  // edges) on the side stream, at the same time.
+  std::vector<int32_t> bound(nj, 0);
+  HIP_TRY(hipMemcpyAsync(bound.data(), ctx->bound.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost,
+                         st));
+  HIP_TRY(hipStreamSynchronize(st));
+  const std::vector<int32_t>& all = pl->walk_jobs;
+  std::vector<int32_t> small, wide;
+  for (int32_t j : all) {
+    if (bound[j] > 16 * kWave) {
+      std::fprintf(stderr, "skity_hip: draw %d keeps up to %d edges active (limit %d)\n",
+                   f.jobs[j].path, bound[j], 16 * kWave);
+      return HIPR_ERROR_CAPACITY;
+    }
+    (bound[j] > 4 * kWave ? wide : small).push_back(j);
+  }
   std::vector<int64_t> pbase(nj, 0);
   std::vector<int32_t> pcap(nj, 0);
   int64_t plane_total = 0;
@@ -976,17 +1001,35 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     }
     return HIPR_SUCCESS;
   };
-  auto walk = [&](int kmode, const std::vector<int32_t>& js, bool wide, DevBuf* dl) -> hipr_result {
-    if (js.empty()) return HIPR_SUCCESS;
-    WalkParams w = wp;
-    if (dl) {
-      hipr_result rr = Upload(dl, js.data(), js.size(), st);
-      if (rr != HIPR_SUCCESS) return rr;
-      w.walk_list = dl->as<int32_t>();
+  // One walk pass: the 4-slot jobs on the main stream, the 16-slot jobs on
+  // the side stream; the main stream waits for both.
+  auto walk_pass = [&](int kmode, const std::vector<int32_t>& sm,
+                       const std::vector<int32_t>& wd) -> hipr_result {
+    hipr_result rr;
+    WalkParams ws = wp, ww = wp;
+    if (!sm.empty()) {
+      if ((rr = Upload(&ctx->sub_list, sm.data(), sm.size(), st)) != HIPR_SUCCESS) return rr;
+      ws.walk_list = ctx->sub_list.as<int32_t>();
+      ws.n_walk = (int32_t)sm.size();
     }
-    w.n_walk = (int32_t)js.size();
-    if (kmode == 0) return wide ? LaunchWalk<16, 0>(ctx, w, st) : LaunchWalk<4, 0>(ctx, w, st);
-    return wide ? LaunchWalk<16, 1>(ctx, w, st) : LaunchWalk<4, 1>(ctx, w, st);
+    if (!wd.empty()) {
+      if ((rr = Upload(&ctx->big_list, wd.data(), wd.size(), st)) != HIPR_SUCCESS) return rr;
+      ww.walk_list = ctx->big_list.as<int32_t>();
+      ww.n_walk = (int32_t)wd.size();
+      HIP_TRY(hipEventRecord(ctx->ev_fork, st));
+      HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+      rr = kmode == 0 ? LaunchWalk<16, 0>(ctx, ww, ctx->side) : LaunchWalk<16, 1>(ctx, ww, ctx->side);
+      if (rr != HIPR_SUCCESS) return rr;
+    }
+    if (!sm.empty()) {
+      rr = kmode == 0 ? LaunchWalk<4, 0>(ctx, ws, st) : LaunchWalk<4, 1>(ctx, ws, st);
+      if (rr != HIPR_SUCCESS) return rr;
+    }
+    if (!wd.empty()) {
+      HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
+      HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    }
+    return HIPR_SUCCESS;
   };
   auto sort_list = [&](const std::vector<int32_t>& js, DevBuf* dl) -> hipr_result {
     if (js.empty()) return HIPR_SUCCESS;
@@ -994,24 +1037,10 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     if (rr != HIPR_SUCCESS) return rr;
     return SortGather(f, dl->as<int32_t>(), (int)js.size());
   };
-  const std::vector<int32_t>& all = pl->walk_jobs;
-  std::vector<int32_t> small, big;
-  auto split_wide = [&]() {
-    small.clear();
-    big.clear();
-    for (int32_t j : all) (ctx->h_cov[j].flags & 2 ? big : small).push_back(j);
-  };
   if (mode == 0) {
-    if ((r = walk(0, all, false, nullptr)) != HIPR_SUCCESS) return r;
+    if ((r = walk_pass(0, small, wide)) != HIPR_SUCCESS) return r;
     if ((r = read_cov()) != HIPR_SUCCESS) return r;
-    if ((r = limit_error(all, 16 | 32)) != HIPR_SUCCESS) return r;
-    split_wide();
-    if (!big.empty()) {
-      if ((r = sort_list(big, &ctx->big_list)) != HIPR_SUCCESS) return r;
-      if ((r = walk(0, big, true, &ctx->big_list)) != HIPR_SUCCESS) return r;
-      if ((r = read_cov()) != HIPR_SUCCESS) return r;
-      if ((r = limit_error(big, 2 | 16 | 32)) != HIPR_SUCCESS) return r;
-    }
+    if ((r = limit_error(all, 2 | 16 | 32)) != HIPR_SUCCESS) return r;
     for (int32_t j : all) pcap[j] = ctx->h_cov[j].planes;
     assign_planes(all);
     if ((r = upload_planes()) != HIPR_SUCCESS) return r;
@@ -1026,41 +1055,28 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   wp.map0 = ctx->map0.as<int32_t>();
   wp.map1 = ctx->map1.as<int32_t>();
   wp.planes = ctx->planes.as<uint8_t>();
-  if (mode == 0) {
-    if ((r = walk(1, small, false, &ctx->sub_list)) != HIPR_SUCCESS) return r;
-    if ((r = walk(1, big, true, &ctx->big_list)) != HIPR_SUCCESS) return r;
-    if ((r = read_cov()) != HIPR_SUCCESS) return r;
-    if ((r = limit_error(all, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
-  } else {
-    if ((r = walk(1, all, false, nullptr)) != HIPR_SUCCESS) return r;
-    if ((r = read_cov()) != HIPR_SUCCESS) return r;
-    if ((r = limit_error(all, 16 | 32)) != HIPR_SUCCESS) return r;
-    // exact fallback: under-estimated jobs (flag 1; the write pass counted
-    // their planes exactly) get exactly that many planes appended; jobs too
-    // wide for 4 slots are counted by the 16-slot walk first.
-    std::vector<int32_t> redo_small;
-    split_wide();
+  if ((r = walk_pass(1, small, wide)) != HIPR_SUCCESS) return r;
+  if ((r = read_cov()) != HIPR_SUCCESS) return r;
+  if ((r = limit_error(all, (mode == 0 ? 1 : 0) | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+  if (mode == 1) {
+    // exact fallback: the write pass counted the planes of every
+    // under-estimated job; they get exactly that many, appended, and are
+    // walked again
+    std::vector<int32_t> redo_small, redo_wide;
     for (int32_t j : small)
       if (ctx->h_cov[j].planes > pcap[j]) redo_small.push_back(j);
-    if (!big.empty()) {
-      if ((r = sort_list(big, &ctx->big_list)) != HIPR_SUCCESS) return r;
-      if ((r = walk(0, big, true, &ctx->big_list)) != HIPR_SUCCESS) return r;
-      if ((r = read_cov()) != HIPR_SUCCESS) return r;
-      if ((r = limit_error(big, 2 | 16 | 32)) != HIPR_SUCCESS) return r;
-    }
-    if (!redo_small.empty() || !big.empty()) {
-      // only under-estimates count as fallbacks; wide jobs need the 16-slot
-      // walk whatever the estimate
-      f.stats.fallbacks += (int64_t)redo_small.size();
+    for (int32_t j : wide)
+      if (ctx->h_cov[j].planes > pcap[j]) redo_wide.push_back(j);
+    if (!redo_small.empty() || !redo_wide.empty()) {
+      f.stats.fallbacks += (int64_t)(redo_small.size() + redo_wide.size());
       std::vector<int32_t> redo = redo_small;
-      redo.insert(redo.end(), big.begin(), big.end());
+      redo.insert(redo.end(), redo_wide.begin(), redo_wide.end());
       for (int32_t j : redo) pcap[j] = ctx->h_cov[j].planes;
       assign_planes(redo);
       if ((r = upload_planes()) != HIPR_SUCCESS) return r;
       wp.planes = ctx->planes.as<uint8_t>();
-      if ((r = sort_list(redo, &ctx->sub_list)) != HIPR_SUCCESS) return r;
-      if ((r = walk(1, redo_small, false, &ctx->sub_list)) != HIPR_SUCCESS) return r;
-      if ((r = walk(1, big, true, &ctx->big_list)) != HIPR_SUCCESS) return r;
+      if ((r = sort_list(redo, &ctx->gather_list)) != HIPR_SUCCESS) return r;
+      if ((r = walk_pass(1, redo_small, redo_wide)) != HIPR_SUCCESS) return r;
       if ((r = read_cov()) != HIPR_SUCCESS) return r;
       if ((r = limit_error(redo, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
     }
@@ -1078,12 +1094,14 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   surf->job_steps.assign(nj, 0);
   surf->job_time.assign(nj, 0);
   surf->job_phase.assign((size_t)nj * 5, 0);
+  surf->job_active.assign(nj, 0);
   for (int j = 0; j < nj; j++) {
     if (f.jobs[j].kind == kJobImage) continue;
     surf->job_edges[j] = ctx->h_geo[j].edges;
     surf->job_planes[j] = ctx->h_cov[j].planes;
     surf->job_steps[j] = ctx->h_cov[j].steps;
     surf->job_time[j] = ctx->h_cov[j].time_10ns;
+    surf->job_active[j] = ctx->h_cov[j].max_active;
     for (int k = 0; k < 5; k++) surf->job_phase[(size_t)j * 5 + k] = ctx->h_cov[j].phase[k];
   }
   // ---- composite: layers, blur, canvas ----
@@ -1258,6 +1276,9 @@ HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out) {
   if (!ctx) return HIPR_ERROR_OUT_OF_HOST_MEMORY;
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+      hipStreamCreate(&ctx->side) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     delete ctx;
     return HIPR_ERROR_INITIALIZATION_FAILED;
@@ -1273,7 +1294,7 @@ HIPR_API void hipr_context_destroy(hipr_context ctx) {
   DevBuf* bufs[] = {&ctx->canvases, &ctx->layer_canvases, &ctx->geo, &ctx->cov, &ctx->edge_base,
                     &ctx->edge_cap, &ctx->edges, &ctx->wedges, &ctx->gkeys, &ctx->plane_base,
                     &ctx->plane_cap, &ctx->planes, &ctx->map_base, &ctx->map0, &ctx->map1,
-                    &ctx->sub_list, &ctx->big_list, &ctx->stroke_out, &ctx->o_verbs, &ctx->o_pts, &ctx->o_w,
+                    &ctx->sub_list, &ctx->big_list, &ctx->bound, &ctx->gather_list, &ctx->stroke_out, &ctx->o_verbs, &ctx->o_pts, &ctx->o_w,
                     &ctx->s_verbs, &ctx->s_pts, &ctx->s_w, &ctx->layers, &ctx->blur_rows,
                     &ctx->blur_cols, &ctx->blur_rb, &ctx->blur_cb, &ctx->blur_tmp,
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
@@ -1287,8 +1308,12 @@ HIPR_API void hipr_context_destroy(hipr_context ctx) {
   if (ctx->h_cov) (void)hipHostFree(ctx->h_cov);
   if (ctx->h_so) (void)hipHostFree(ctx->h_so);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  (void)hipStreamSynchronize(ctx->side);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
+  (void)hipEventDestroy(ctx->ev_fork);
+  (void)hipEventDestroy(ctx->ev_join);
+  (void)hipStreamDestroy(ctx->side);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1417,12 +1442,13 @@ HIPR_API hipr_result hipr_surface_get_job_counts(hipr_surface s, int32_t* edges,
 
 HIPR_API hipr_result hipr_surface_get_job_profile(hipr_surface s, int32_t* steps,
                                                   int32_t* time_10ns, int32_t* phases,
-                                                  int32_t n_jobs) {
+                                                  int32_t* max_active, int32_t n_jobs) {
   if (!s) return HIPR_ERROR_INVALID_HANDLE;
   if (n_jobs != (int32_t)s->job_steps.size()) return HIPR_ERROR_INVALID_ARGUMENT;
   for (int j = 0; j < n_jobs; j++) {
     if (steps) steps[j] = s->job_steps[j];
     if (time_10ns) time_10ns[j] = s->job_time[j];
+    if (max_active) max_active[j] = s->job_active[j];
     if (phases)
       for (int k = 0; k < 5; k++) phases[(size_t)j * 5 + k] = s->job_phase[(size_t)j * 5 + k];
   }
