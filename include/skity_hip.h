@@ -161,6 +161,9 @@ typedef struct hipr_options {
  * 146-156) and the GPUSurface/LockCanvas pattern
  * (include/skity/gpu/gpu_surface.hpp:95-113). */
 HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out);
+/* Destroy calls may come in any order: a context's device resources are
+ * released once its handle and every surface and prepared scene made from
+ * it have been destroyed. */
 HIPR_API void hipr_context_destroy(hipr_context ctx);
 /* A premultiplied RGBA8 target (bytes R,G,B,A), cleared to transparent.
  * width or height 0 -> HIPR_ERROR_INVALID_ARGUMENT, mirroring

@@ -23,8 +23,8 @@ __device__ __forceinline__ uint64_t EdgeKey(const EdgeRec& e, int idx) {
 
 // keys: per-job scratch of next_pow2(n) entries (used when n exceeds LDS).
 // active_bound (may be null) receives an upper bound of the job's active
-// edge list: the largest number of edges whose row range [upper_y, lower_y)
-// meets one bin of rows (a bin is one pixel row when the job spans at most
+// edge list: the largest number of edges whose row range (upper_y to lower_y,
+// or to the last step's y for a quad edge) meets one bin of rows (a bin is one pixel row when the job spans at most
 // 16384 rows).  The walk keeps exactly the edges with upper_y <= y < lower_y
 // active, so this bounds every sub-step's list.
 __global__ void __launch_bounds__(kSortThreads)
@@ -87,7 +87,9 @@ __global__ void __launch_bounds__(kSortThreads)
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const EdgeRec& e = src[i];
-    int r0 = (e.upper_y >> 16) - top, r1 = ((e.lower_y - 1) >> 16) - top;
+    // a quad edge stays in the list until its last step (q_last_y)
+    Fixed last = EdgeIsQuad(e.packed) && e.q_last_y > e.lower_y ? e.q_last_y : e.lower_y;
+    int r0 = (e.upper_y >> 16) - top, r1 = ((last - 1) >> 16) - top;
     r0 = r0 < 0 ? 0 : (r0 >= H ? H - 1 : r0);
     r1 = r1 < r0 ? r0 : (r1 >= H ? H - 1 : r1);
     atomicAdd(&hist[r0 / bin], 1);

@@ -237,6 +237,7 @@ GradDev MakeGradDev(const hipr_gradient& g, const Aff& ctm) {
 }  // namespace
 
 struct hipr_context_s {
+  int refs = 1;  // the handle itself + live surfaces and prepared scenes
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // wide walk jobs run here, beside the main stream
@@ -1037,10 +1038,27 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     if (rr != HIPR_SUCCESS) return rr;
     return SortGather(f, dl->as<int32_t>(), (int)js.size());
   };
+  // A 4-slot job whose list still outgrew 256 edges (flag 2) moves to the
+  // 16-slot walk, which counts it; the bound makes this a safety net.
+  auto move_overflowed = [&]() -> hipr_result {
+    std::vector<int32_t> moved, keep;
+    for (int32_t j : small) (ctx->h_cov[j].flags & 2 ? moved : keep).push_back(j);
+    if (moved.empty()) return HIPR_SUCCESS;
+    small.swap(keep);
+    hipr_result rr;
+    if ((rr = sort_list(moved, &ctx->gather_list)) != HIPR_SUCCESS) return rr;
+    if ((rr = walk_pass(0, {}, moved)) != HIPR_SUCCESS) return rr;
+    if ((rr = read_cov()) != HIPR_SUCCESS) return rr;
+    if ((rr = limit_error(moved, 2 | 16 | 32)) != HIPR_SUCCESS) return rr;
+    wide.insert(wide.end(), moved.begin(), moved.end());
+    return HIPR_SUCCESS;
+  };
   if (mode == 0) {
     if ((r = walk_pass(0, small, wide)) != HIPR_SUCCESS) return r;
     if ((r = read_cov()) != HIPR_SUCCESS) return r;
-    if ((r = limit_error(all, 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(wide, 2)) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(all, 16 | 32)) != HIPR_SUCCESS) return r;
+    if ((r = move_overflowed()) != HIPR_SUCCESS) return r;
     for (int32_t j : all) pcap[j] = ctx->h_cov[j].planes;
     assign_planes(all);
     if ((r = upload_planes()) != HIPR_SUCCESS) return r;
@@ -1057,7 +1075,17 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   wp.planes = ctx->planes.as<uint8_t>();
   if ((r = walk_pass(1, small, wide)) != HIPR_SUCCESS) return r;
   if ((r = read_cov()) != HIPR_SUCCESS) return r;
-  if ((r = limit_error(all, (mode == 0 ? 1 : 0) | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+  if (mode == 0) {
+    if ((r = limit_error(all, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+  } else {
+    if ((r = limit_error(wide, 2)) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(all, 16 | 32)) != HIPR_SUCCESS) return r;
+    // moved jobs were counted exactly by the 16-slot walk; they are redone
+    // below as under-estimates if their planes exceed the estimate
+    std::vector<int32_t> before = wide;
+    if ((r = move_overflowed()) != HIPR_SUCCESS) return r;
+    for (size_t i = before.size(); i < wide.size(); i++) pcap[wide[i]] = -1;
+  }
   if (mode == 1) {
     // exact fallback: the write pass counted the planes of every
     // under-estimated job; they get exactly that many, appended, and are
@@ -1235,6 +1263,7 @@ hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s**
       return HIPR_ERROR_DEVICE;
     }
   }
+  ctx->refs++;  // released by hipr_prepared_destroy
   *out = pl;
   return HIPR_SUCCESS;
 }
@@ -1287,8 +1316,17 @@ HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out) {
   return HIPR_SUCCESS;
 }
 
-HIPR_API void hipr_context_destroy(hipr_context ctx) {
-  if (!ctx) return;
+static void FreeContext(hipr_context ctx);
+
+// A context is released when its handle and every surface and prepared scene
+// made from it are destroyed, in any order.
+static void ReleaseContext(hipr_context ctx) {
+  if (ctx && --ctx->refs == 0) FreeContext(ctx);
+}
+
+HIPR_API void hipr_context_destroy(hipr_context ctx) { ReleaseContext(ctx); }
+
+static void FreeContext(hipr_context ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   DevBuf* bufs[] = {&ctx->canvases, &ctx->layer_canvases, &ctx->geo, &ctx->cov, &ctx->edge_base,
@@ -1334,17 +1372,25 @@ HIPR_API hipr_result hipr_surface_create(hipr_context ctx, int32_t width, int32_
     delete s;
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   }
-  HIP_TRY(hipMemsetAsync(s->pixels, 0, (size_t)width * height * 4, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (hipMemsetAsync(s->pixels, 0, (size_t)width * height * 4, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    (void)hipFree(s->pixels);
+    delete s;
+    return HIPR_ERROR_DEVICE;
+  }
+  ctx->refs++;  // released by hipr_surface_destroy
   *out = s;
   return HIPR_SUCCESS;
 }
 
 HIPR_API void hipr_surface_destroy(hipr_surface s) {
   if (!s) return;
-  (void)hipStreamSynchronize(s->ctx->stream);
+  hipr_context ctx = s->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(s->pixels);
   delete s;
+  ReleaseContext(ctx);
 }
 
 HIPR_API hipr_result hipr_surface_clear(hipr_surface s, uint32_t rgba) {
@@ -1608,8 +1654,11 @@ HIPR_API hipr_result hipr_scene_prepare(hipr_context ctx, const hipr_scene* scen
 
 HIPR_API void hipr_prepared_destroy(hipr_prepared p) {
   if (!p) return;
-  (void)hipStreamSynchronize(p->ctx->stream);
+  hipr_context ctx = p->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete p;
+  ReleaseContext(ctx);
 }
 
 HIPR_API hipr_result hipr_surface_draw_prepared(hipr_surface surf, hipr_prepared p,
