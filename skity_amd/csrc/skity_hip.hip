@@ -952,8 +952,8 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   // ---- stage (b): walk ----
   // The walk keeps the active edge list in registers.  The sort bounded each
   // job's list by its largest row overlap: jobs within 256 edges take the
-  // 4-slot walk on the main stream, the others the 16-slot walk (1024
-  // edges) on the side stream, at the same time.
+  // 4-slot walk on the main stream, the others the 8-slot (512 edges) or
+  // 16-slot (1024 edges) walk on the side stream, at the same time.
   std::vector<int32_t> bound(nj, 0);
   HIP_TRY(hipMemcpyAsync(bound.data(), ctx->bound.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost,
                          st));
@@ -1014,13 +1014,30 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
       ws.n_walk = (int32_t)sm.size();
     }
     if (!wd.empty()) {
-      if ((rr = Upload(&ctx->big_list, wd.data(), wd.size(), st)) != HIPR_SUCCESS) return rr;
-      ww.walk_list = ctx->big_list.as<int32_t>();
-      ww.n_walk = (int32_t)wd.size();
+      // 8 slots for lists within 512 edges, 16 above; both on the side stream
+      std::vector<int32_t> w8, w16;
+      for (int32_t j : wd) (bound[j] <= 8 * kWave ? w8 : w16).push_back(j);
+      std::vector<int32_t> ordered = w8;
+      ordered.insert(ordered.end(), w16.begin(), w16.end());
+      if ((rr = Upload(&ctx->big_list, ordered.data(), ordered.size(), st)) != HIPR_SUCCESS)
+        return rr;
       HIP_TRY(hipEventRecord(ctx->ev_fork, st));
       HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-      rr = kmode == 0 ? LaunchWalk<16, 0>(ctx, ww, ctx->side) : LaunchWalk<16, 1>(ctx, ww, ctx->side);
-      if (rr != HIPR_SUCCESS) return rr;
+      if (!w8.empty()) {
+        WalkParams w = ww;
+        w.walk_list = ctx->big_list.as<int32_t>();
+        w.n_walk = (int32_t)w8.size();
+        rr = kmode == 0 ? LaunchWalk<8, 0>(ctx, w, ctx->side) : LaunchWalk<8, 1>(ctx, w, ctx->side);
+        if (rr != HIPR_SUCCESS) return rr;
+      }
+      if (!w16.empty()) {
+        WalkParams w = ww;
+        w.walk_list = ctx->big_list.as<int32_t>() + w8.size();
+        w.n_walk = (int32_t)w16.size();
+        rr = kmode == 0 ? LaunchWalk<16, 0>(ctx, w, ctx->side)
+                        : LaunchWalk<16, 1>(ctx, w, ctx->side);
+        if (rr != HIPR_SUCCESS) return rr;
+      }
     }
     if (!sm.empty()) {
       rr = kmode == 0 ? LaunchWalk<4, 0>(ctx, ws, st) : LaunchWalk<4, 1>(ctx, ws, st);
