@@ -48,6 +48,26 @@ def geometry_bytes(sc):
     return 8 * s.point_offsets[n] + s.verb_offsets[n] + 4 * s.weight_offsets[n] + 32 * n
 
 
+PMC_SUMMARY = os.path.join(HERE, "profiles", "pmc_curves_strokes_4096.json")
+STAGE_KERNEL = {"walk": "hipr::WalkKernel<4, 1>", "walk_count": "hipr::WalkKernel<4, 0>",
+                "composite": "hipr::CompositeKernel", "flatten": "hipr::FlattenKernel<1>",
+                "sort": "hipr::SortGatherKernel"}
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of the stage's main kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (counter
+    values scaled by the copy-kernel calibration), or None."""
+    if not os.path.exists(PMC_SUMMARY) or stage not in STAGE_KERNEL:
+        return None
+    with open(PMC_SUMMARY) as fh:
+        js = json.load(fh)
+    for d in js["dispatches"]:
+        if d["kernel"] == STAGE_KERNEL[stage]:
+            return int((d["fetch_mib"] + d["write_mib"]) * 1024 * 1024)
+    return None
+
+
 def cpu_baseline(sc, budget_s):
     """The oracle (a single-threaded CPU restatement of skity's software
     backend) on a bounded prefix of the same scene."""
@@ -151,7 +171,7 @@ def main():
         achieved = alg / (avg_launch_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "alg_bytes_per_launch": int(alg),
+                "traffic": measured_traffic(dom), "alg_bytes_per_launch": int(alg),
                 "avg_launch_ms": round(avg_launch_ms, 4)}
     ref = out_forms.get("exact", out_forms[best])
     out = {
