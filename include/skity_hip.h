@@ -150,10 +150,14 @@ typedef struct hipr_stats {
  * path whose estimate was too small; mode 2 = pre-sized from the context's
  * learned pre-sizer (hipr_context_set_presizer), with the same fallback. */
 enum { HIPR_MODE_EXACT = 0, HIPR_MODE_ESTIMATES = 1, HIPR_MODE_LEARNED = 2 };
+/* StackBlur implementations; both are exact (identical output). */
+enum { HIPR_BLUR_IMPL_MFMA = 0,  /* banded-matrix products on MFMA (fp16, fp32 sums) */
+       HIPR_BLUR_IMPL_LINES = 1  /* one thread per line, integer running sums    */ };
 typedef struct hipr_options {
   int32_t mode;
   const int32_t* edge_estimates;  /* n_paths or NULL */
   const int32_t* tile_estimates;  /* n_paths or NULL */
+  int32_t blur_impl;              /* HIPR_BLUR_IMPL_*; 0 = MFMA */
 } hipr_options;
 
 /* --- context / surface ---------------------------------------------------
@@ -300,6 +304,10 @@ HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx,
 HIPR_API hipr_result hipr_debug_wave_ops(hipr_context ctx, const int32_t* in, int32_t* out,
                                          int32_t n_waves);
 
+/* StackBlur with a chosen implementation (HIPR_BLUR_IMPL_*). */
+HIPR_API hipr_result hipr_stackblur_ex(hipr_context ctx, const uint8_t* src_host,
+                                       uint8_t* dst_host, int32_t width, int32_t height,
+                                       int32_t radius, int32_t impl);
 HIPR_API hipr_result hipr_stackblur(hipr_context ctx, const uint8_t* src_host,
                                     uint8_t* dst_host, int32_t width,
                                     int32_t height, int32_t radius);

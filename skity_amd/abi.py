@@ -95,11 +95,15 @@ class Stats(ctypes.Structure):
     ]
 
 
+BLUR_IMPL_MFMA, BLUR_IMPL_LINES = 0, 1
+
+
 class Options(ctypes.Structure):
     _fields_ = [
         ("mode", ctypes.c_int32),
         ("edge_estimates", ctypes.POINTER(ctypes.c_int32)),
         ("tile_estimates", ctypes.POINTER(ctypes.c_int32)),
+        ("blur_impl", ctypes.c_int32),
     ]
 
 

@@ -24,6 +24,7 @@
 
 #include "../../include/skity_hip.h"
 #include "hipr_blur.hpp"
+#include "hipr_blur_mfma.hpp"
 #include "hipr_composite.hpp"
 #include "hipr_flatten.hpp"
 #include "hipr_mlp.hpp"
@@ -344,6 +345,13 @@ void StageEnd(hipr_context ctx, int stage, hipEvent_t begin, hipStream_t st = nu
     StageEnd(ctx, stage, stage_begin__, st);        \
   } while (0)
 #define STAGE_LAUNCH(ctx, stage, ...) STAGE_LAUNCH_ON(ctx, ctx->stream, stage, __VA_ARGS__)
+
+// Both StackBlur passes of every job with the MFMA banded-product kernels:
+// rows src -> tmp, then columns tmp -> dst (jobs hold src/dst of the row
+// pass; `tmp` per job is in `tmps`).
+hipr_result LaunchBlurMfma(hipr_context ctx, const std::vector<BlurJob>& rows,
+                           const std::vector<BlurJob>& cols, DevBuf* d_rows, DevBuf* d_cols,
+                           hipStream_t st, bool timed);
 
 hipr_result HostStage(hipr_context ctx, size_t n) {
   if (n > ctx->h_cap) {
@@ -845,6 +853,9 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   hipStream_t st = f.st;
   int nj = f.nj;
   int mode = opt ? opt->mode : HIPR_MODE_EXACT;
+  const int blur_impl = opt ? opt->blur_impl : HIPR_BLUR_IMPL_MFMA;
+  if (blur_impl != HIPR_BLUR_IMPL_MFMA && blur_impl != HIPR_BLUR_IMPL_LINES)
+    return HIPR_ERROR_INVALID_ARGUMENT;
   if (mode < HIPR_MODE_EXACT || mode > HIPR_MODE_LEARNED) return HIPR_ERROR_INVALID_ARGUMENT;
   if (mode == HIPR_MODE_ESTIMATES && (!opt->edge_estimates || !opt->tile_estimates))
     return HIPR_ERROR_INVALID_ARGUMENT;
@@ -1201,7 +1212,11 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
       rb.push_back(rb.back() + bj.height);
       cb.push_back(cb.back() + bj.width);
     }
-    if (!rows.empty()) {
+    if (!rows.empty() && blur_impl == HIPR_BLUR_IMPL_MFMA) {
+      if ((r = LaunchBlurMfma(ctx, rows, cols, &ctx->blur_rows, &ctx->blur_cols, st, true)) !=
+          HIPR_SUCCESS)
+        return r;
+    } else if (!rows.empty()) {
       if ((r = Upload(&ctx->blur_rows, rows.data(), rows.size(), st)) != HIPR_SUCCESS) return r;
       if ((r = Upload(&ctx->blur_cols, cols.data(), cols.size(), st)) != HIPR_SUCCESS) return r;
       if ((r = Upload(&ctx->blur_rb, rb.data(), rb.size(), st)) != HIPR_SUCCESS) return r;
@@ -1247,6 +1262,45 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     }
   }
   surf->stats = f.stats;
+  return HIPR_SUCCESS;
+}
+
+hipr_result LaunchBlurMfma(hipr_context ctx, const std::vector<BlurJob>& rows,
+                           const std::vector<BlurJob>& cols, DevBuf* d_rows, DevBuf* d_cols,
+                           hipStream_t st, bool timed) {
+  auto tile_jobs = [](const std::vector<BlurJob>& js, int pass, int64_t* total) {
+    std::vector<BlurTileJob> out;
+    int64_t t = 0;
+    for (const BlurJob& b : js) {
+      BlurTileJob tj{};
+      tj.b = b;
+      tj.tile_begin = t;
+      int n_lines = pass == 0 ? b.height : b.width, len = pass == 0 ? b.width : b.height;
+      tj.line_tiles = (n_lines + kBlurLines - 1) / kBlurLines;
+      tj.out_tiles = (len + kBlurOut - 1) / kBlurOut;
+      t += (int64_t)tj.line_tiles * tj.out_tiles;
+      out.push_back(tj);
+    }
+    *total = t;
+    return out;
+  };
+  int64_t tr = 0, tc = 0;
+  std::vector<BlurTileJob> jr = tile_jobs(rows, 0, &tr), jc = tile_jobs(cols, 1, &tc);
+  hipr_result r;
+  if ((r = Upload(d_rows, jr.data(), jr.size(), st)) != HIPR_SUCCESS) return r;
+  if ((r = Upload(d_cols, jc.data(), jc.size(), st)) != HIPR_SUCCESS) return r;
+  unsigned gr = (unsigned)std::min<int64_t>(std::max<int64_t>(tr, 1), 1 << 16);
+  unsigned gc = (unsigned)std::min<int64_t>(std::max<int64_t>(tc, 1), 1 << 16);
+  hipEvent_t b0 = timed ? StageBegin(ctx, st) : nullptr;
+  hipLaunchKernelGGL(StackBlurMfmaKernel<0>, dim3(gr), dim3(64), 0, st,
+                     d_rows->as<BlurTileJob>(), (int)jr.size(), tr);
+  HIP_TRY(hipGetLastError());
+  if (timed) StageEnd(ctx, HIPR_STAGE_BLUR, b0, st);
+  hipEvent_t b1 = timed ? StageBegin(ctx, st) : nullptr;
+  hipLaunchKernelGGL(StackBlurMfmaKernel<1>, dim3(gc), dim3(64), 0, st,
+                     d_cols->as<BlurTileJob>(), (int)jc.size(), tc);
+  HIP_TRY(hipGetLastError());
+  if (timed) StageEnd(ctx, HIPR_STAGE_BLUR, b1, st);
   return HIPR_SUCCESS;
 }
 
@@ -1782,6 +1836,33 @@ HIPR_API hipr_result hipr_debug_wave_ops(hipr_context ctx, const int32_t* in, in
   HIP_TRY(hipMemcpyAsync(out, ctx->d_pts.p, sizeof(int32_t) * 6 * 64 * n_waves,
                          hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_stackblur_ex(hipr_context ctx, const uint8_t* src_host,
+                                       uint8_t* dst_host, int32_t width, int32_t height,
+                                       int32_t radius, int32_t impl) {
+  if (impl == HIPR_BLUR_IMPL_LINES) return hipr_stackblur(ctx, src_host, dst_host, width, height, radius);
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  if (impl != HIPR_BLUR_IMPL_MFMA || !src_host || !dst_host || width <= 0 || height <= 0)
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  hipStream_t st = ctx->stream;
+  HIP_TRY(hipSetDevice(ctx->device));
+  size_t bytes = (size_t)width * height * 4;
+  radius = std::min(radius, 254);
+  if (ctx->blur_tmp.Ensure(bytes * 2) != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  uint8_t* a = ctx->blur_tmp.as<uint8_t>();
+  uint8_t* b = a + bytes;
+  HIP_TRY(hipMemcpyAsync(a, src_host, bytes, hipMemcpyHostToDevice, st));
+  if (radius > 1) {
+    std::vector<BlurJob> rows{{a, b, width, height, radius}}, cols{{b, a, width, height, radius}};
+    hipr_result r;
+    if ((r = LaunchBlurMfma(ctx, rows, cols, &ctx->blur_jobs, &ctx->blur_lines, st, false)) !=
+        HIPR_SUCCESS)
+      return r;
+  }
+  HIP_TRY(hipMemcpyAsync(dst_host, a, bytes, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
   return HIPR_SUCCESS;
 }
 

@@ -57,6 +57,9 @@ def lib():
         L.hipr_stackblur.argtypes = [c_ctx, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                      ctypes.c_int32, ctypes.c_int32]
         L.hipr_stackblur.restype = ctypes.c_int
+        L.hipr_stackblur_ex.argtypes = [c_ctx, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+        L.hipr_stackblur_ex.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -92,12 +95,12 @@ class Context:
                                            edges.ctypes.data, total), "hipr_debug_build_edges")
         return offs, edges[:total]
 
-    def stackblur(self, img, radius):
+    def stackblur(self, img, radius, impl=abi.BLUR_IMPL_LINES):
         img = np.ascontiguousarray(img, np.uint8)
         h, w = img.shape[:2]
         out = np.zeros_like(img)
-        check(lib().hipr_stackblur(self._h, img.ctypes.data, out.ctypes.data, w, h, radius),
-              "hipr_stackblur")
+        check(lib().hipr_stackblur_ex(self._h, img.ctypes.data, out.ctypes.data, w, h, radius,
+                                      impl), "hipr_stackblur_ex")
         return out
 
     def close(self):
@@ -112,9 +115,10 @@ class Context:
             pass
 
 
-def _options(mode, edge_estimates, tile_estimates):
+def _options(mode, edge_estimates, tile_estimates, blur_impl=abi.BLUR_IMPL_MFMA):
     opt = abi.Options()
     opt.mode = mode
+    opt.blur_impl = blur_impl
     keep = []
     if edge_estimates is not None:
         e = np.ascontiguousarray(edge_estimates, np.int32)
@@ -163,9 +167,10 @@ class Surface:
     def clear(self, rgba=0):
         check(lib().hipr_surface_clear(self._h, rgba), "hipr_surface_clear")
 
-    def draw(self, scene, mode=0, edge_estimates=None, tile_estimates=None):
+    def draw(self, scene, mode=0, edge_estimates=None, tile_estimates=None,
+             blur_impl=abi.BLUR_IMPL_MFMA):
         """Draws a scene (a Scene pointer or a Prepared scene) onto the surface."""
-        opt, keep = _options(mode, edge_estimates, tile_estimates)
+        opt, keep = _options(mode, edge_estimates, tile_estimates, blur_impl)
         if isinstance(scene, Prepared):
             check(lib().hipr_surface_draw_prepared(self._h, scene.handle, ctypes.byref(opt)),
                   "hipr_surface_draw_prepared")

@@ -77,6 +77,42 @@ def test_stackblur_bit_exact(ctx):
         assert np.array_equal(got, want), f"{h}x{w} r={r}"
 
 
+@pytest.mark.parametrize("h,w,r", [(37, 53, 2), (64, 80, 7), (120, 33, 25), (50, 300, 55),
+                                   (1, 1, 3), (3, 700, 16), (700, 5, 17), (130, 129, 128),
+                                   (300, 260, 254), (64, 64, 254)])
+def test_stackblur_mfma_bit_exact(ctx, h, w, r):
+    """The MFMA banded-product StackBlur equals the oracle byte for byte:
+    fp16 holds every pixel and weight exactly and every fp32 partial sum is
+    an integer below 2^24 (255 (r+1)^2 at r = 254), so no rounding occurs."""
+    rng = np.random.default_rng(1000 + r)
+    img = rng.integers(0, 256, size=(h, w, 4), dtype=np.uint8)
+    img[..., 3] = np.maximum(img[..., 3], 250) if r % 2 else img[..., 3]
+    want = O.stackblur(img, r)
+    got = ctx.stackblur(img, r, impl=abi.BLUR_IMPL_MFMA)
+    bad = int(np.count_nonzero(np.any(got != want, axis=-1)))
+    assert bad == 0, f"{h}x{w} r={r}: {bad} pixels differ"
+
+
+def test_stackblur_mfma_saturated(ctx):
+    """All-255 input at the largest radius: the largest sums the kernel sees."""
+    img = np.full((90, 600, 4), 255, np.uint8)
+    want = O.stackblur(img, 254)
+    got = ctx.stackblur(img, 254, impl=abi.BLUR_IMPL_MFMA)
+    assert np.array_equal(got, want)
+
+
+def test_blur_impls_render_identically(ctx):
+    sc = workload(3, 12)
+    surfs = []
+    for impl in (abi.BLUR_IMPL_MFMA, abi.BLUR_IMPL_LINES):
+        s = device.Surface(ctx, sc.width, sc.height)
+        s.clear(0)
+        s.draw(sc.scene, blur_impl=impl)
+        surfs.append(s.read())
+        s.close()
+    assert np.array_equal(surfs[0], surfs[1])
+
+
 def test_gradients_no_blur(ctx):
     sc = no_blur(workload(3, 24))
     got, st = render_gpu(ctx, sc)
