@@ -87,6 +87,12 @@ def cpu_baseline(sc, budget_s):
                       f"{sc.width}x{sc.height} target, {t_used:.1f} s, 1 thread"}
 
 
+def device_info(torch):
+    p = torch.cuda.get_device_properties(0)
+    return {"name": p.name, "compute_units": p.multi_processor_count,
+            "arch": getattr(p, "gcnArchName", "")}
+
+
 def run_form(surf, prep, mode, steps, warmup, torch):
     """Times `steps` frames of one form; returns wall seconds, per-stage
     totals (HIP events) and the last frame's statistics."""
@@ -191,6 +197,7 @@ def main():
                   "device_ms": round(st["ms_total"], 3)},
         "forms": out_forms,
         "roofline": roof,
+        "device": device_info(torch),
     }
     if args.cpu_seconds > 0:
         cb = cpu_baseline(sc, args.cpu_seconds)

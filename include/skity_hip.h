@@ -158,6 +158,11 @@ typedef struct hipr_options {
   const int32_t* edge_estimates;  /* n_paths or NULL */
   const int32_t* tile_estimates;  /* n_paths or NULL */
   int32_t blur_impl;              /* HIPR_BLUR_IMPL_*; 0 = MFMA */
+  int32_t band_rows;              /* rows per walk band: 0 = default (128),
+                                     < 0 = one band per path */
+  int32_t band_min_edges;         /* estimate modes: band only the paths whose
+                                     active-edge bound reaches this; 0 = default
+                                     (no banding), < 0 = every path */
 } hipr_options;
 
 /* --- context / surface ---------------------------------------------------
@@ -262,7 +267,8 @@ typedef enum hipr_stage {
   HIPR_STAGE_BLUR = 7,          /* StackBlur of the layers                */
   HIPR_STAGE_COMPOSITE = 8,     /* tile composite onto the target         */
   HIPR_STAGE_PRESIZE = 9,       /* features + MLP inference               */
-  HIPR_STAGE_COUNT = 10
+  HIPR_STAGE_WALK_SCHEDULE = 10, /* step schedule pass (band snapshots)    */
+  HIPR_STAGE_COUNT = 11
 } hipr_stage;
 HIPR_API hipr_result hipr_surface_get_stage_times(hipr_surface surface,
                                                   double* ms,

@@ -23,7 +23,7 @@ BLEND_SRC_OVER = 3
 
 # hipr_stage, in enum order
 STAGES = ("stroke", "flatten_count", "flatten", "sort", "walk_count", "walk", "layers",
-          "blur", "composite", "presize")
+          "blur", "composite", "presize", "walk_schedule")
 MODE_EXACT, MODE_ESTIMATES, MODE_LEARNED = 0, 1, 2
 PRESIZER_FEATURES, PRESIZER_TRAINED, PRESIZER_PARAMS = 16, 5378, 5414
 
@@ -104,6 +104,8 @@ class Options(ctypes.Structure):
         ("edge_estimates", ctypes.POINTER(ctypes.c_int32)),
         ("tile_estimates", ctypes.POINTER(ctypes.c_int32)),
         ("blur_impl", ctypes.c_int32),
+        ("band_rows", ctypes.c_int32),
+        ("band_min_edges", ctypes.c_int32),
     ]
 
 

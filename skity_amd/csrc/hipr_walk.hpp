@@ -31,13 +31,41 @@ namespace hipr {
 
 constexpr int kTilePlaneBytes = kTile * kTile;
 
+// Walk modes: count the tile planes, write them, or only follow the step
+// schedule (no coverage) to record the row-band snapshots.
+enum WalkMode : int { kWalkCount = 0, kWalkWrite = 1, kWalkSchedule = 2 };
+
+// Row bands.  A path taller than `band_rows` is walked by several wavefronts
+// at once, each over a range of rows.  A sequential pass (count or schedule)
+// records the complete walk state at the start of every band after the
+// first: the first loop iteration that begins on a pixel row that is a
+// multiple of 16 (a tile row boundary) at or below the band's nominal start
+// row.  Restarted from that state, the walk of a band is the same computation
+// as the sequential walk over those rows, and every tile row belongs to one
+// band.
+constexpr int kMaxBands = 16;  // work items are job << 4 | band
+struct SnapHeader {
+  int32_t y, nny, pp, n;  // y = kMaxS32: the band has no rows
+  int32_t pad[12];
+};
+static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
+// header, then X, DX, DY, LY, M [cap] and the quad edge state [cap]
+__host__ __device__ inline int64_t SnapStride(int cap) {
+  return (int64_t)sizeof(SnapHeader) + (int64_t)cap * (5 * 4 + (int64_t)sizeof(EdgeRec));
+}
+
 struct WalkParams {
   const Job* jobs;
-  const int32_t* walk_list;
+  const int32_t* walk_list;  // work items: job << 4 | band
   int32_t n_walk;
   const JobGeo* geo;
   const int64_t* edge_base;
-  EdgeRec* wedges;          // per job, sorted by (upper_y, index); mutated
+  const EdgeRec* wedges;    // per job, sorted by (upper_y, index); read only
+  EdgeRec* qstate;          // quad edge state: nE records per job and band
+  const int64_t* q_base;    // per job: first qstate record of band 0
+  uint8_t* snaps;           // band snapshots
+  const int64_t* snap_base; // per job: byte offset of band 1's snapshot
+  const int32_t* band_count;  // per job: row bands (1 = one sequential walk)
   const Canvas* canvases;
   const int64_t* plane_base;
   const int32_t* plane_cap;
@@ -81,6 +109,18 @@ __host__ __device__ inline TileRange JobTiles(const PixBounds& pb, int cw, int c
   tr.ntc = ((x1 - 1) >> 4) - tr.tc0 + 1;
   tr.ntr = ((y1 - 1) >> 4) - tr.tr0 + 1;
   return tr;
+}
+
+// Number of row bands of a job and the nominal first row of band b >= 1,
+// rounded up to a tile row.
+__host__ __device__ inline int BandCount(const PixBounds& pb, int band_rows) {
+  if (pb.empty || band_rows <= 0) return 1;
+  int nb = (pb.b - pb.t) / band_rows;
+  return nb < 1 ? 1 : (nb > kMaxBands ? kMaxBands : nb);
+}
+__host__ __device__ inline int BandStartRow(const PixBounds& pb, int nb, int b) {
+  int v = pb.t + (int)((int64_t)(pb.b - pb.t) * b / nb);
+  return (v + kTile - 1) & ~(kTile - 1);
 }
 
 // meta word of an active edge: bit 0 winding < 0, bit 1 quad, bits 2.. index
@@ -145,15 +185,24 @@ __device__ void BitonicSortSlots(int64_t (&key)[K], int N, int lane) {
   }
 }
 
+// Write pass (kWalkWrite): one wave per (job, band) item; the job's map
+// starts at -1 (filled before the launch), its JobCov at zero, and planes are
+// taken from the job's counter (JobCov::planes) with atomics, so the bands of
+// a job run concurrently.  Count and schedule passes: one wave per job, band
+// 0, walking every row and recording the band snapshots.
 template <int K, int kMode>
 __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   constexpr int CAP = K * kWave;
+  constexpr bool kWrite = kMode == kWalkWrite;
+  constexpr bool kBlit = kMode != kWalkSchedule;
   extern __shared__ int32_t lds[];
   const int lane = Lane();
   const int wi = blockIdx.x;
   if (wi >= P.n_walk) return;
   const uint64_t t_start = wall_clock64();
-  const int j = P.walk_list[wi];
+  const int item = P.walk_list[wi];
+  const int j = item >> 4;
+  const int rband = kWrite ? (item & (kMaxBands - 1)) : 0;
   const Job job = P.jobs[j];
   const JobGeo geo = P.geo[j];
   const Canvas cv = P.canvases[job.canvas];
@@ -161,26 +210,25 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   TileRange tr = JobTiles(pb, cv.width, cv.height);
   int W = pb.r - pb.l;
   int flags = 0;
-  int n_tiles = 0, n_planes = 0, max_active = 0;
+  int n_tiles = 0, max_active = 0;
   int nE = geo.edges;
-  int32_t* map0 = kMode == 1 ? P.map0 + P.map_base[j] : nullptr;
-  int32_t* map1 = kMode == 1 ? P.map1 + P.map_base[j] : nullptr;
-  if (kMode == 1) {
-    // every visible tile of the job starts empty
-    int64_t msz = (int64_t)tr.ntc * tr.ntr;
-    for (int64_t i = lane; i < msz; i += kWave) {
-      map0[i] = -1;
-      map1[i] = -1;
-    }
-  }
+  int32_t* map0 = kWrite ? P.map0 + P.map_base[j] : nullptr;
+  int32_t* map1 = kWrite ? P.map1 + P.map_base[j] : nullptr;
   if (pb.empty || tr.ntc == 0 || nE == 0 || W > P.lds_width) {
     if (!pb.empty && W > P.lds_width) flags |= 16;
-    if (lane == 0) {
+    if (lane == 0 && rband == 0) {
       JobCov c{pb.l, pb.t, pb.r, pb.b, 0, 0, 0, flags, 0, 0, {0, 0, 0, 0, 0}};
       P.cov[j] = c;
     }
     return;
   }
+  const int nb = P.band_count[j];
+  uint8_t* sbase = nb > 1 ? P.snaps + P.snap_base[j] : nullptr;
+  auto snap_at = [&](int b) -> SnapHeader* {
+    return (SnapHeader*)(sbase + (int64_t)(b - 1) * SnapStride(CAP));
+  };
+  // the band ends where the next band's snapshot was taken
+  const Fixed stop_y = (kWrite && rband + 1 < nb) ? snap_at(rband + 1)->y : kMaxS32;
   WalkLds L;
   L.d = lds;
   L.slot0 = L.d + (P.lds_width + 1);
@@ -199,9 +247,10 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   }
   WaveSync();
 
-  EdgeRec* E = P.wedges + P.edge_base[j];
-  const int64_t pbase = kMode == 1 ? P.plane_base[j] : 0;
-  const int pcap = kMode == 1 ? P.plane_cap[j] : 0;
+  const EdgeRec* E = P.wedges + P.edge_base[j];
+  EdgeRec* Q = P.qstate + P.q_base[j] + (int64_t)rband * nE;
+  const int64_t pbase = kWrite ? P.plane_base[j] : 0;
+  const int pcap = kWrite ? P.plane_cap[j] : 0;
   uint8_t* planes = P.planes;
 
   const Fixed left_clip = (Fixed)((uint32_t)pb.l << 16);
@@ -243,7 +292,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       int s0 = L.slot0[c], s1 = L.slot1[c];
       // Only planes inside this job's allocation are published; an overflow
       // is reported through flags and the job is redone by the exact path.
-      if (kMode == 1 && s0 >= 0 && s0 < pcap) {
+      if (kWrite && s0 >= 0 && s0 < pcap) {
         map0[(int64_t)trow * tr.ntc + c] = s0;
         map1[(int64_t)trow * tr.ntc + c] = s1 < pcap ? s1 : -1;
       }
@@ -325,8 +374,16 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       bool need1 = in_cols && leader && chunk1 && cur1 < 0;
       int req = (need0 ? 1 : 0) + (need1 ? 1 : 0);
       int rincl = ScanSum(req);
-      int first_slot = next_slot + rincl - req;
-      next_slot += __builtin_amdgcn_readlane(rincl, kWave - 1);
+      int rtot = __builtin_amdgcn_readlane(rincl, kWave - 1);
+      int slot_base = next_slot;
+      if (kWrite && rtot > 0) {
+        // the bands of a job share its plane range
+        int got = 0;
+        if (lane == 0) got = atomicAdd(&P.cov[j].planes, rtot);
+        slot_base = __builtin_amdgcn_readlane(got, 0);
+      }
+      int first_slot = slot_base + rincl - req;
+      next_slot += rtot;
       if (need0) {
         L.slot0[tcol] = first_slot;
         n_tiles++;
@@ -335,7 +392,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       WaveSync();
       cur0 = in_cols ? L.slot0[tcol] : -1;
       cur1 = in_cols ? L.slot1[tcol] : -1;
-      if (kMode == 1) {
+      if (kWrite) {
         int sub = lane & 3;
         int rowk = r & 15;
         bool new0 = __shfl(need0 ? 1 : 0, g * 4, kWave) != 0;
@@ -407,6 +464,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       if (k + cnt > CAP) return false;
       if (ready) {
         EdgeRec e = E[idx];
+        if (EdgeIsQuad(e.packed)) Q[idx] = e;  // the quad's state from here on
         int p = k + lane;
         L.sX[p] = e.x;
         L.sDX[p] = e.dx;
@@ -424,7 +482,29 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   // ---- initial activation (sw_raster.cc:554-563) -------------------------------
   Fixed y = Max32(E[0].upper_y, IntToFixed(pb.t));
   Fixed nny = kMaxS32;
-  {
+  if (rband > 0) {
+    // resume from the band's snapshot
+    const SnapHeader* h = snap_at(rband);
+    y = h->y;
+    if (y == kMaxS32) return;  // the walk ended before this band
+    nny = h->nny;
+    pp = h->pp;
+    n = h->n;
+    const int32_t* a = (const int32_t*)(h + 1);
+    const EdgeRec* qs = (const EdgeRec*)(a + 5 * CAP);
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+      int p = s * kWave + lane;
+      if (p < n) {
+        X[s] = a[p];
+        DX[s] = a[CAP + p];
+        DY[s] = a[2 * CAP + p];
+        LY[s] = a[3 * CAP + p];
+        M[s] = a[4 * CAP + p];
+        if (MetaQuad(M[s])) Q[MetaIndex(M[s])] = qs[p];
+      }
+    }
+  } else {
     int k;
     if (!gather_new(y, &k)) {
       flags |= 2;
@@ -494,7 +574,36 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   int guard = 0;
   uint64_t ph[5] = {0, 0, 0, 0, 0};  // phase times (10 ns units)
   uint64_t tp_loop = wall_clock64();
+  int next_b = 1;  // next band snapshot to record (count / schedule passes)
   for (;;) {
+    if (y >= stop_y) break;  // the next band starts here
+    if (!kWrite && next_b < nb && (y & 0xFFFF) == 0 && ((y >> 16) & (kTile - 1)) == 0) {
+      const int yrow = y >> 16;
+      while (next_b < nb && yrow >= BandStartRow(pb, nb, next_b)) {
+        SnapHeader* h = snap_at(next_b);
+        if (lane == 0) {
+          h->y = y;
+          h->nny = nny;
+          h->pp = pp;
+          h->n = n;
+        }
+        int32_t* a = (int32_t*)(h + 1);
+        EdgeRec* qs = (EdgeRec*)(a + 5 * CAP);
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          int p = s * kWave + lane;
+          if (p < n) {
+            a[p] = X[s];
+            a[CAP + p] = DX[s];
+            a[2 * CAP + p] = DY[s];
+            a[3 * CAP + p] = LY[s];
+            a[4 * CAP + p] = M[s];
+            if (MetaQuad(M[s])) qs[p] = Q[MetaIndex(M[s])];
+          }
+        }
+        next_b++;
+      }
+    }
     if (++guard > (1 << 26)) {  // cannot happen: y strictly increases
       flags |= 32;
       break;
@@ -513,14 +622,19 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     const int S = (n + kWave - 1) / kWave;
 
     uint64_t tp0 = wall_clock64();
+    Fixed XN[K];
+    if (!kBlit) {
+#pragma unroll
+      for (int s = 0; s < K; s++) XN[s] = X[s] + (DX[s] >> y_shift);
+    }
     // ---- phases 1-3: roles, trapezoids --------------------------------------
+    if (kBlit) {
     int carry_w = 0;
     int carry_in = 0;
     int carry_lidx = -1;
     Fixed carry_lxo = 0, carry_lxn = 0, carry_ldy = 0;
     int carry_ridx = -1;
     Fixed carry_rxo = 0, carry_rxn = 0;
-    Fixed XN[K];
 #pragma unroll
     for (int s = 0; s < K; s++) {
       if (s >= S) break;
@@ -620,6 +734,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       }
     }
     WaveSync();
+    }  // kBlit
 
     uint64_t tp1 = wall_clock64();
     ph[0] += tp1 - tp0;
@@ -635,7 +750,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           // KeepContinuous taking the edge's current x and y
           // (sw_raster.cc:628-638, sw_edge.cc:294-297).
           int idx = MetaIndex(M[s]);
-          EdgeRec e = E[idx];
+          EdgeRec e = Q[idx];
           Fixed sy = next_y;
           Fixed cx = X[s];
           while (LY[s] <= next_y && EdgeCurveCount(e.packed) > 0) {
@@ -649,7 +764,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
           }
           X[s] = cx;
           M[s] = MakeMeta(EdgeWinding(e.packed), true, idx);
-          E[idx] = e;
+          Q[idx] = e;
         }
       }
     }
@@ -922,7 +1037,7 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     ph[2] += tp3 - tp2;
     // ---- row completion ------------------------------------------------------
     y = next_y;
-    if ((y & 0xFFFF) == 0) flush_row(row);
+    if (kBlit && (y & 0xFFFF) == 0) flush_row(row);
     uint64_t tp4 = wall_clock64();
     ph[3] += tp4 - tp3;
     if (y >= stop_fixed) break;
@@ -1103,6 +1218,11 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     }
   }
   finish_band();
+  if (!kWrite) {
+    // bands the walk never reached are empty
+    for (; next_b < nb; next_b++)
+      if (lane == 0) snap_at(next_b)->y = kMaxS32;
+  }
   n_tiles = WaveSum(n_tiles);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) flags |= __shfl_xor(flags, o, kWave);
@@ -1110,12 +1230,42 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
     uint64_t t_end = wall_clock64();
     uint64_t dt = t_end - t_start;
     ph[4] = (t_end - tp_loop) - (ph[0] + ph[1] + ph[2] + ph[3]);
-    JobCov c{pb.l, pb.t, pb.r, pb.b, n_tiles, next_slot, max_active, flags, guard,
-             (int32_t)(dt < 0x7FFFFFFF ? dt : 0x7FFFFFFF)};
+    int32_t dt32 = (int32_t)(dt < 0x7FFFFFFF ? dt : 0x7FFFFFFF);
+    if (kWrite) {
+      JobCov* c = &P.cov[j];
+      c->left = pb.l;
+      c->top = pb.t;
+      c->right = pb.r;
+      c->bottom = pb.b;
+      atomicAdd(&c->tiles, n_tiles);
+      atomicMax(&c->max_active, max_active);
+      atomicOr(&c->flags, flags);
+      atomicAdd(&c->steps, guard);
+      atomicMax(&c->time_10ns, dt32);
 #pragma unroll
-    for (int k = 0; k < 5; k++) c.phase[k] = (int32_t)(ph[k] < 0x7FFFFFFF ? ph[k] : 0x7FFFFFFF);
-    P.cov[j] = c;
+      for (int k = 0; k < 5; k++)
+        atomicAdd(&c->phase[k], (int32_t)(ph[k] < 0x7FFFFFFF ? ph[k] : 0x7FFFFFFF));
+    } else {
+      JobCov c{pb.l, pb.t, pb.r, pb.b, n_tiles, next_slot, max_active, flags, guard, dt32};
+#pragma unroll
+      for (int k = 0; k < 5; k++) c.phase[k] = (int32_t)(ph[k] < 0x7FFFFFFF ? ph[k] : 0x7FFFFFFF);
+      P.cov[j] = c;
+    }
   }
+}
+
+// Grid-stride fill of n int32 words.
+__global__ void FillKernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Zeroes the coverage records of the listed jobs (before a write pass redoes
+// them).
+__global__ void ResetCovKernel(const int32_t* __restrict__ list, int n, JobCov* __restrict__ cov) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cov[list[i]] = JobCov{};
 }
 
 }  // namespace hipr

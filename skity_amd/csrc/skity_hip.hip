@@ -248,6 +248,7 @@ struct hipr_context_s {
   DevBuf canvases, layer_canvases, geo, cov, edge_base, edge_cap, edges, wedges, gkeys;
   DevBuf plane_base, plane_cap, planes, map_base, map0, map1, sub_list, big_list, bound,
       gather_list;
+  DevBuf wcov, qstate, q_base, snaps, snap_base, band_count;  // write-pass coverage, row bands
   DevBuf stroke_out, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w;
   DevBuf layers, blur_rows, blur_cols, blur_rb, blur_cb, blur_tmp, blur_jobs, blur_lines;
   // debug/utility uploads
@@ -270,6 +271,7 @@ struct hipr_context_s {
   // pinned staging for read-backs
   JobGeo* h_geo = nullptr;
   JobCov* h_cov = nullptr;
+  JobCov* h_wcov = nullptr;
   StrokeOut* h_so = nullptr;
   size_t h_cap = 0;
 };
@@ -357,13 +359,16 @@ hipr_result HostStage(hipr_context ctx, size_t n) {
   if (n > ctx->h_cap) {
     if (ctx->h_geo) (void)hipHostFree(ctx->h_geo);
     if (ctx->h_cov) (void)hipHostFree(ctx->h_cov);
+    if (ctx->h_wcov) (void)hipHostFree(ctx->h_wcov);
     if (ctx->h_so) (void)hipHostFree(ctx->h_so);
     ctx->h_geo = nullptr;
     ctx->h_cov = nullptr;
+    ctx->h_wcov = nullptr;
     ctx->h_so = nullptr;
     size_t want = n + n / 2 + 64;
     HIP_TRY(hipHostMalloc((void**)&ctx->h_geo, want * sizeof(JobGeo)));
     HIP_TRY(hipHostMalloc((void**)&ctx->h_cov, want * sizeof(JobCov)));
+    HIP_TRY(hipHostMalloc((void**)&ctx->h_wcov, want * sizeof(JobCov)));
     HIP_TRY(hipHostMalloc((void**)&ctx->h_so, want * sizeof(StrokeOut)));
     ctx->h_cap = want;
   }
@@ -604,12 +609,19 @@ void SetTiles(std::vector<Canvas>* cvs) {
   }
 }
 
+// Nominal rows per walk band (the write pass runs the bands of a path
+// concurrently).
+constexpr int kBandRows = 128;
+
 template <int K, int kMode>
 hipr_result LaunchWalk(hipr_context ctx, const WalkParams& P, hipStream_t st) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
   size_t lds = (size_t)((P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 6 * K * kWave) *
                sizeof(int32_t);
-  STAGE_LAUNCH_ON(ctx, st, kMode == 0 ? HIPR_STAGE_WALK_COUNT : HIPR_STAGE_WALK,
+  STAGE_LAUNCH_ON(ctx, st,
+                  kMode == kWalkCount   ? HIPR_STAGE_WALK_COUNT
+                  : kMode == kWalkWrite ? HIPR_STAGE_WALK
+                                        : HIPR_STAGE_WALK_SCHEDULE,
                   hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds,
                                      st, P));
   HIP_TRY(hipGetLastError());
@@ -971,14 +983,58 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   HIP_TRY(hipStreamSynchronize(st));
   const std::vector<int32_t>& all = pl->walk_jobs;
   std::vector<int32_t> small, wide;
+  std::vector<int8_t> tier(nj, 4);  // slots of each job's walk
   for (int32_t j : all) {
     if (bound[j] > 16 * kWave) {
       std::fprintf(stderr, "skity_hip: draw %d keeps up to %d edges active (limit %d)\n",
                    f.jobs[j].path, bound[j], 16 * kWave);
       return HIPR_ERROR_CAPACITY;
     }
-    (bound[j] > 4 * kWave ? wide : small).push_back(j);
+    tier[j] = bound[j] <= 4 * kWave ? 4 : bound[j] <= 8 * kWave ? 8 : 16;
+    (tier[j] == 4 ? small : wide).push_back(j);
   }
+  // Row bands: each job's rows are split into bands walked concurrently by
+  // the write pass; the count or schedule pass records where they start.
+  const int band_rows = opt && opt->band_rows != 0 ? std::max(opt->band_rows, 0) : kBandRows;
+  // In the estimate modes banding costs a schedule pass, more than the write
+  // pass gains on a full chip, so it is off unless asked for.
+  const int band_min =
+      mode == 0 ? 0
+      : (opt && opt->band_min_edges != 0) ? std::max(opt->band_min_edges, 0)
+                                          : INT32_MAX;
+  std::vector<int32_t> nbands(nj, 1);
+  std::vector<int64_t> qb(nj, 0), sb(nj, 0);
+  int64_t q_total = 0, snap_total = 0;
+  for (int32_t j : all) {
+    nbands[j] = bound[j] >= band_min ? BandCount(JobPixBounds(ctx->h_geo[j]), band_rows) : 1;
+    qb[j] = q_total;
+    q_total += (int64_t)nbands[j] * ctx->h_geo[j].edges;
+  }
+  auto assign_snaps = [&](const std::vector<int32_t>& js) {
+    for (int32_t j : js) {
+      sb[j] = snap_total;
+      snap_total += (int64_t)(nbands[j] - 1) * SnapStride(tier[j] * kWave);
+    }
+  };
+  assign_snaps(all);
+  if (ctx->qstate.Ensure(sizeof(EdgeRec) * std::max<int64_t>(q_total, 1)) != hipSuccess ||
+      ctx->q_base.Ensure(sizeof(int64_t) * nj) != hipSuccess ||
+      ctx->snap_base.Ensure(sizeof(int64_t) * nj) != hipSuccess ||
+      ctx->band_count.Ensure(sizeof(int32_t) * nj) != hipSuccess ||
+      ctx->snaps.Ensure(std::max<int64_t>(snap_total, 1)) != hipSuccess ||
+      ctx->wcov.Ensure(sizeof(JobCov) * nj) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  HIP_TRY(hipMemcpyAsync(ctx->q_base.p, qb.data(), sizeof(int64_t) * nj, hipMemcpyHostToDevice,
+                         st));
+  HIP_TRY(hipMemcpyAsync(ctx->snap_base.p, sb.data(), sizeof(int64_t) * nj,
+                         hipMemcpyHostToDevice, st));
+  wp.qstate = ctx->qstate.as<EdgeRec>();
+  wp.q_base = ctx->q_base.as<int64_t>();
+  wp.snaps = ctx->snaps.as<uint8_t>();
+  wp.snap_base = ctx->snap_base.as<int64_t>();
+  HIP_TRY(hipMemcpyAsync(ctx->band_count.p, nbands.data(), sizeof(int32_t) * nj,
+                         hipMemcpyHostToDevice, st));
+  wp.band_count = ctx->band_count.as<int32_t>();
   std::vector<int64_t> pbase(nj, 0);
   std::vector<int32_t> pcap(nj, 0);
   int64_t plane_total = 0;
@@ -998,60 +1054,86 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
       plane_total += pcap[j];
     }
   };
-  auto read_cov = [&]() -> hipr_result {
-    HIP_TRY(hipMemcpyAsync(ctx->h_cov, ctx->cov.p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
+  auto read_cov = [&](DevBuf* src, JobCov* dst) -> hipr_result {
+    HIP_TRY(hipMemcpyAsync(dst, src->p, sizeof(JobCov) * nj, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return HIPR_SUCCESS;
   };
-  auto limit_error = [&](const std::vector<int32_t>& js, int mask) -> hipr_result {
+  auto limit_error = [&](const JobCov* c, const std::vector<int32_t>& js,
+                         int mask) -> hipr_result {
     for (int32_t j : js) {
-      if (ctx->h_cov[j].flags & mask) {
+      if (c[j].flags & mask) {
         std::fprintf(stderr, "skity_hip: draw %d exceeds a device limit (flags %d)\n",
-                     f.jobs[j].path, ctx->h_cov[j].flags);
+                     f.jobs[j].path, c[j].flags);
         return HIPR_ERROR_CAPACITY;
       }
     }
     return HIPR_SUCCESS;
   };
-  // One walk pass: the 4-slot jobs on the main stream, the 16-slot jobs on
-  // the side stream; the main stream waits for both.
+  // Work items of a pass: one per job (count, schedule) or one per row band
+  // (write), the bands of the longest walks first so they start first.
+  auto make_items = [&](const std::vector<int32_t>& js, int kmode) {
+    std::vector<int32_t> it;
+    if (kmode != kWalkWrite) {
+      for (int32_t j : js) it.push_back(j << 4);
+      return it;
+    }
+    std::vector<std::pair<int64_t, int32_t>> order;
+    for (int32_t j : js) {
+      PixBounds pb = JobPixBounds(ctx->h_geo[j]);
+      int64_t work = (int64_t)std::max(bound[j], 1) * std::max(pb.b - pb.t, 1) / nbands[j];
+      order.push_back({-work, j});
+    }
+    std::stable_sort(order.begin(), order.end());
+    for (auto& o : order)
+      for (int b = 0; b < nbands[o.second]; b++) it.push_back((o.second << 4) | b);
+    return it;
+  };
+  // One walk pass: the 4-slot jobs on the main stream, the 8- and 16-slot
+  // jobs on the side stream; the main stream waits for both.
   auto walk_pass = [&](int kmode, const std::vector<int32_t>& sm,
                        const std::vector<int32_t>& wd) -> hipr_result {
     hipr_result rr;
     WalkParams ws = wp, ww = wp;
-    if (!sm.empty()) {
-      if ((rr = Upload(&ctx->sub_list, sm.data(), sm.size(), st)) != HIPR_SUCCESS) return rr;
+    std::vector<int32_t> ism = make_items(sm, kmode);
+    if (!ism.empty()) {
+      if ((rr = Upload(&ctx->sub_list, ism.data(), ism.size(), st)) != HIPR_SUCCESS) return rr;
       ws.walk_list = ctx->sub_list.as<int32_t>();
-      ws.n_walk = (int32_t)sm.size();
+      ws.n_walk = (int32_t)ism.size();
     }
     if (!wd.empty()) {
-      // 8 slots for lists within 512 edges, 16 above; both on the side stream
-      std::vector<int32_t> w8, w16;
-      for (int32_t j : wd) (bound[j] <= 8 * kWave ? w8 : w16).push_back(j);
+      std::vector<int32_t> j8, j16;
+      for (int32_t j : wd) (tier[j] <= 8 ? j8 : j16).push_back(j);
+      std::vector<int32_t> w8 = make_items(j8, kmode), w16 = make_items(j16, kmode);
       std::vector<int32_t> ordered = w8;
       ordered.insert(ordered.end(), w16.begin(), w16.end());
       if ((rr = Upload(&ctx->big_list, ordered.data(), ordered.size(), st)) != HIPR_SUCCESS)
         return rr;
       HIP_TRY(hipEventRecord(ctx->ev_fork, st));
       HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+      auto launch = [&](auto k_tag, const WalkParams& w) -> hipr_result {
+        constexpr int KK = decltype(k_tag)::value;
+        return kmode == kWalkCount   ? LaunchWalk<KK, kWalkCount>(ctx, w, ctx->side)
+               : kmode == kWalkWrite ? LaunchWalk<KK, kWalkWrite>(ctx, w, ctx->side)
+                                     : LaunchWalk<KK, kWalkSchedule>(ctx, w, ctx->side);
+      };
       if (!w8.empty()) {
         WalkParams w = ww;
         w.walk_list = ctx->big_list.as<int32_t>();
         w.n_walk = (int32_t)w8.size();
-        rr = kmode == 0 ? LaunchWalk<8, 0>(ctx, w, ctx->side) : LaunchWalk<8, 1>(ctx, w, ctx->side);
-        if (rr != HIPR_SUCCESS) return rr;
+        if ((rr = launch(std::integral_constant<int, 8>{}, w)) != HIPR_SUCCESS) return rr;
       }
       if (!w16.empty()) {
         WalkParams w = ww;
         w.walk_list = ctx->big_list.as<int32_t>() + w8.size();
         w.n_walk = (int32_t)w16.size();
-        rr = kmode == 0 ? LaunchWalk<16, 0>(ctx, w, ctx->side)
-                        : LaunchWalk<16, 1>(ctx, w, ctx->side);
-        if (rr != HIPR_SUCCESS) return rr;
+        if ((rr = launch(std::integral_constant<int, 16>{}, w)) != HIPR_SUCCESS) return rr;
       }
     }
-    if (!sm.empty()) {
-      rr = kmode == 0 ? LaunchWalk<4, 0>(ctx, ws, st) : LaunchWalk<4, 1>(ctx, ws, st);
+    if (!ism.empty()) {
+      rr = kmode == kWalkCount   ? LaunchWalk<4, kWalkCount>(ctx, ws, st)
+           : kmode == kWalkWrite ? LaunchWalk<4, kWalkWrite>(ctx, ws, st)
+                                 : LaunchWalk<4, kWalkSchedule>(ctx, ws, st);
       if (rr != HIPR_SUCCESS) return rr;
     }
     if (!wd.empty()) {
@@ -1060,89 +1142,141 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     }
     return HIPR_SUCCESS;
   };
-  auto sort_list = [&](const std::vector<int32_t>& js, DevBuf* dl) -> hipr_result {
-    if (js.empty()) return HIPR_SUCCESS;
-    hipr_result rr = Upload(dl, js.data(), js.size(), st);
-    if (rr != HIPR_SUCCESS) return rr;
-    return SortGather(f, dl->as<int32_t>(), (int)js.size());
-  };
-  // A 4-slot job whose list still outgrew 256 edges (flag 2) moves to the
-  // 16-slot walk, which counts it; the bound makes this a safety net.
-  auto move_overflowed = [&]() -> hipr_result {
+  // A 4-slot job whose list still outgrew 256 edges (flag 2) is counted again
+  // by the 16-slot walk (recording its snapshots there); the bound makes this
+  // a safety net.  `from` holds the flags of the pass that overflowed.
+  auto move_overflowed = [&](const JobCov* from, std::vector<int32_t>* moved_out) -> hipr_result {
     std::vector<int32_t> moved, keep;
-    for (int32_t j : small) (ctx->h_cov[j].flags & 2 ? moved : keep).push_back(j);
+    for (int32_t j : small) (from[j].flags & 2 ? moved : keep).push_back(j);
     if (moved.empty()) return HIPR_SUCCESS;
     small.swap(keep);
+    for (int32_t j : moved) tier[j] = 16;
+    assign_snaps(moved);
+    if (ctx->snaps.EnsureKeep(std::max<int64_t>(snap_total, 1), ctx->snaps.cap, st) !=
+        hipSuccess)
+      return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+    wp.snaps = ctx->snaps.as<uint8_t>();
+    HIP_TRY(hipMemcpyAsync(ctx->snap_base.p, sb.data(), sizeof(int64_t) * nj,
+                           hipMemcpyHostToDevice, st));
     hipr_result rr;
-    if ((rr = sort_list(moved, &ctx->gather_list)) != HIPR_SUCCESS) return rr;
-    if ((rr = walk_pass(0, {}, moved)) != HIPR_SUCCESS) return rr;
-    if ((rr = read_cov()) != HIPR_SUCCESS) return rr;
-    if ((rr = limit_error(moved, 2 | 16 | 32)) != HIPR_SUCCESS) return rr;
+    if ((rr = walk_pass(kWalkCount, {}, moved)) != HIPR_SUCCESS) return rr;
+    if ((rr = read_cov(&ctx->cov, ctx->h_cov)) != HIPR_SUCCESS) return rr;
+    if ((rr = limit_error(ctx->h_cov, moved, 2 | 16 | 32)) != HIPR_SUCCESS) return rr;
     wide.insert(wide.end(), moved.begin(), moved.end());
+    if (moved_out) *moved_out = moved;
     return HIPR_SUCCESS;
   };
+  // Before a write pass over `js`: maps at -1 (all jobs) and the jobs'
+  // write-pass coverage records at zero.
+  auto prepare_write = [&](const std::vector<int32_t>* js) -> hipr_result {
+    if (!js) {
+      hipLaunchKernelGGL(FillKernel, dim3(1024), dim3(256), 0, st, ctx->map0.as<int32_t>(),
+                         total_map, -1);
+      hipLaunchKernelGGL(FillKernel, dim3(1024), dim3(256), 0, st, ctx->map1.as<int32_t>(),
+                         total_map, -1);
+      hipLaunchKernelGGL(FillKernel, dim3(256), dim3(256), 0, st, (int32_t*)ctx->wcov.p,
+                         (int64_t)nj * (int64_t)(sizeof(JobCov) / 4), 0);
+    } else if (!js->empty()) {
+      hipr_result rr = Upload(&ctx->gather_list, js->data(), js->size(), st);
+      if (rr != HIPR_SUCCESS) return rr;
+      hipLaunchKernelGGL(ResetCovKernel, dim3((unsigned)((js->size() + 255) / 256)), dim3(256), 0,
+                         st, ctx->gather_list.as<int32_t>(), (int)js->size(),
+                         ctx->wcov.as<JobCov>());
+    }
+    HIP_TRY(hipGetLastError());
+    return HIPR_SUCCESS;
+  };
+  JobCov* hw = ctx->h_wcov;
   if (mode == 0) {
-    if ((r = walk_pass(0, small, wide)) != HIPR_SUCCESS) return r;
-    if ((r = read_cov()) != HIPR_SUCCESS) return r;
-    if ((r = limit_error(wide, 2)) != HIPR_SUCCESS) return r;
-    if ((r = limit_error(all, 16 | 32)) != HIPR_SUCCESS) return r;
-    if ((r = move_overflowed()) != HIPR_SUCCESS) return r;
+    // count pass over every job (recording the band snapshots), then planes
+    // sized exactly
+    if ((r = walk_pass(kWalkCount, small, wide)) != HIPR_SUCCESS) return r;
+    if ((r = read_cov(&ctx->cov, ctx->h_cov)) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(ctx->h_cov, wide, 2)) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(ctx->h_cov, all, 16 | 32)) != HIPR_SUCCESS) return r;
+    if ((r = move_overflowed(ctx->h_cov, nullptr)) != HIPR_SUCCESS) return r;
     for (int32_t j : all) pcap[j] = ctx->h_cov[j].planes;
-    assign_planes(all);
-    if ((r = upload_planes()) != HIPR_SUCCESS) return r;
-    if ((r = SortGather(f, pl->d_walk.as<int32_t>(), nwalk)) != HIPR_SUCCESS) return r;
   } else {
+    // schedule pass (snapshots only) over the jobs with more than one band;
+    // planes from the estimates
+    std::vector<int32_t> ssm, swd, sall;
+    for (int32_t j : small)
+      if (nbands[j] > 1) ssm.push_back(j);
+    for (int32_t j : wide)
+      if (nbands[j] > 1) swd.push_back(j);
+    sall = ssm;
+    sall.insert(sall.end(), swd.begin(), swd.end());
+    if (!sall.empty()) {
+      if ((r = walk_pass(kWalkSchedule, ssm, swd)) != HIPR_SUCCESS) return r;
+      if ((r = read_cov(&ctx->cov, ctx->h_cov)) != HIPR_SUCCESS) return r;
+      if ((r = limit_error(ctx->h_cov, swd, 2)) != HIPR_SUCCESS) return r;
+      if ((r = limit_error(ctx->h_cov, sall, 16 | 32)) != HIPR_SUCCESS) return r;
+      // (only scheduled jobs carry schedule-pass flags)
+      std::vector<int32_t> saved_small = small;
+      small = ssm;
+      std::vector<int32_t> moved;
+      if ((r = move_overflowed(ctx->h_cov, &moved)) != HIPR_SUCCESS) return r;
+      std::vector<int32_t> rest;
+      for (int32_t j : saved_small)
+        if (nbands[j] <= 1 || tier[j] == 4) rest.push_back(j);
+      small = rest;
+      // moved jobs were counted exactly; their estimate is replaced below
+      for (int32_t j : moved) test[j] = std::max(test[j], ctx->h_cov[j].planes);
+    }
     for (int32_t j : all) pcap[j] = test[j];
-    assign_planes(all);
-    if ((r = upload_planes()) != HIPR_SUCCESS) return r;
   }
+  assign_planes(all);
+  if ((r = upload_planes()) != HIPR_SUCCESS) return r;
   wp.plane_base = ctx->plane_base.as<int64_t>();
   wp.plane_cap = ctx->plane_cap.as<int32_t>();
   wp.map0 = ctx->map0.as<int32_t>();
   wp.map1 = ctx->map1.as<int32_t>();
   wp.planes = ctx->planes.as<uint8_t>();
-  if ((r = walk_pass(1, small, wide)) != HIPR_SUCCESS) return r;
-  if ((r = read_cov()) != HIPR_SUCCESS) return r;
+  wp.cov = ctx->wcov.as<JobCov>();
+  if ((r = prepare_write(nullptr)) != HIPR_SUCCESS) return r;
+  if ((r = walk_pass(kWalkWrite, small, wide)) != HIPR_SUCCESS) return r;
+  if ((r = read_cov(&ctx->wcov, hw)) != HIPR_SUCCESS) return r;
   if (mode == 0) {
-    if ((r = limit_error(all, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(hw, all, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
   } else {
-    if ((r = limit_error(wide, 2)) != HIPR_SUCCESS) return r;
-    if ((r = limit_error(all, 16 | 32)) != HIPR_SUCCESS) return r;
-    // moved jobs were counted exactly by the 16-slot walk; they are redone
-    // below as under-estimates if their planes exceed the estimate
-    std::vector<int32_t> before = wide;
-    if ((r = move_overflowed()) != HIPR_SUCCESS) return r;
-    for (size_t i = before.size(); i < wide.size(); i++) pcap[wide[i]] = -1;
-  }
-  if (mode == 1) {
+    if ((r = limit_error(hw, wide, 2)) != HIPR_SUCCESS) return r;
+    if ((r = limit_error(hw, all, 16 | 32)) != HIPR_SUCCESS) return r;
+    // a one-band 4-slot job (not scheduled) whose list outgrew 256 edges is
+    // counted by the 16-slot walk and redone below with its exact planes
+    std::vector<int32_t> moved;
+    if ((r = move_overflowed(hw, &moved)) != HIPR_SUCCESS) return r;
+    for (int32_t j : moved) {
+      hw[j].planes = ctx->h_cov[j].planes;
+      pcap[j] = -1;
+    }
     // exact fallback: the write pass counted the planes of every
     // under-estimated job; they get exactly that many, appended, and are
     // walked again
     std::vector<int32_t> redo_small, redo_wide;
     for (int32_t j : small)
-      if (ctx->h_cov[j].planes > pcap[j]) redo_small.push_back(j);
+      if (hw[j].planes > pcap[j]) redo_small.push_back(j);
     for (int32_t j : wide)
-      if (ctx->h_cov[j].planes > pcap[j]) redo_wide.push_back(j);
+      if (hw[j].planes > pcap[j]) redo_wide.push_back(j);
     if (!redo_small.empty() || !redo_wide.empty()) {
       f.stats.fallbacks += (int64_t)(redo_small.size() + redo_wide.size());
       std::vector<int32_t> redo = redo_small;
       redo.insert(redo.end(), redo_wide.begin(), redo_wide.end());
-      for (int32_t j : redo) pcap[j] = ctx->h_cov[j].planes;
+      for (int32_t j : redo) pcap[j] = hw[j].planes;
       assign_planes(redo);
       if ((r = upload_planes()) != HIPR_SUCCESS) return r;
       wp.planes = ctx->planes.as<uint8_t>();
-      if ((r = sort_list(redo, &ctx->gather_list)) != HIPR_SUCCESS) return r;
-      if ((r = walk_pass(1, redo_small, redo_wide)) != HIPR_SUCCESS) return r;
-      if ((r = read_cov()) != HIPR_SUCCESS) return r;
-      if ((r = limit_error(redo, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
+      if ((r = prepare_write(&redo)) != HIPR_SUCCESS) return r;
+      if ((r = walk_pass(kWalkWrite, redo_small, redo_wide)) != HIPR_SUCCESS) return r;
+      if ((r = read_cov(&ctx->wcov, hw)) != HIPR_SUCCESS) return r;
+      if ((r = limit_error(hw, redo, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
     }
   }
   int64_t max_active = 0;
   for (int j = 0; j < nj; j++) {
     if (f.jobs[j].kind == kJobImage) continue;
-    f.stats.tiles += ctx->h_cov[j].tiles;
-    f.stats.tile_planes += ctx->h_cov[j].planes;
-    max_active = std::max<int64_t>(max_active, ctx->h_cov[j].max_active);
+    f.stats.tiles += hw[j].tiles;
+    f.stats.tile_planes += hw[j].planes;
+    max_active = std::max<int64_t>(max_active, hw[j].max_active);
   }
   f.stats.max_active = max_active;
   surf->job_edges.assign(nj, -1);
@@ -1154,17 +1288,17 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   for (int j = 0; j < nj; j++) {
     if (f.jobs[j].kind == kJobImage) continue;
     surf->job_edges[j] = ctx->h_geo[j].edges;
-    surf->job_planes[j] = ctx->h_cov[j].planes;
-    surf->job_steps[j] = ctx->h_cov[j].steps;
-    surf->job_time[j] = ctx->h_cov[j].time_10ns;
-    surf->job_active[j] = ctx->h_cov[j].max_active;
-    for (int k = 0; k < 5; k++) surf->job_phase[(size_t)j * 5 + k] = ctx->h_cov[j].phase[k];
+    surf->job_planes[j] = hw[j].planes;
+    surf->job_steps[j] = hw[j].steps;
+    surf->job_time[j] = hw[j].time_10ns;
+    surf->job_active[j] = hw[j].max_active;
+    for (int k = 0; k < 5; k++) surf->job_phase[(size_t)j * 5 + k] = hw[j].phase[k];
   }
   // ---- composite: layers, blur, canvas ----
   CompositeParams cp{};
   cp.n_canvases = (int)f.canvases.size();
   cp.jobs = f.d_jobs;
-  cp.cov = ctx->cov.as<JobCov>();
+  cp.cov = ctx->wcov.as<JobCov>();
   cp.map_base = ctx->map_base.as<int64_t>();
   cp.map0 = ctx->map0.as<int32_t>();
   cp.map1 = ctx->map1.as<int32_t>();
@@ -1407,7 +1541,9 @@ static void FreeContext(hipr_context ctx) {
                     &ctx->s_verbs, &ctx->s_pts, &ctx->s_w, &ctx->layers, &ctx->blur_rows,
                     &ctx->blur_cols, &ctx->blur_rb, &ctx->blur_cb, &ctx->blur_tmp,
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
-                    &ctx->d_jobs, &ctx->run_jobs, &ctx->counters};
+                    &ctx->d_jobs, &ctx->run_jobs, &ctx->counters, &ctx->wcov,
+                    &ctx->qstate, &ctx->q_base, &ctx->snaps, &ctx->snap_base,
+                    &ctx->band_count};
   for (DevBuf* b : bufs) b->Free();
   DevBuf* mlp[] = {&ctx->mlp_prm, &ctx->mlp_ops, &ctx->mlp_feats, &ctx->mlp_preds,
                    &ctx->mlp_targets, &ctx->mlp_grads, &ctx->mlp_loss, &ctx->mlp_m,
@@ -1415,6 +1551,7 @@ static void FreeContext(hipr_context ctx) {
   for (DevBuf* b : mlp) b->Free();
   if (ctx->h_geo) (void)hipHostFree(ctx->h_geo);
   if (ctx->h_cov) (void)hipHostFree(ctx->h_cov);
+  if (ctx->h_wcov) (void)hipHostFree(ctx->h_wcov);
   if (ctx->h_so) (void)hipHostFree(ctx->h_so);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(ctx->side);

@@ -115,10 +115,13 @@ class Context:
             pass
 
 
-def _options(mode, edge_estimates, tile_estimates, blur_impl=abi.BLUR_IMPL_MFMA):
+def _options(mode, edge_estimates, tile_estimates, blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0,
+             band_min_edges=0):
     opt = abi.Options()
     opt.mode = mode
     opt.blur_impl = blur_impl
+    opt.band_rows = band_rows
+    opt.band_min_edges = band_min_edges
     keep = []
     if edge_estimates is not None:
         e = np.ascontiguousarray(edge_estimates, np.int32)
@@ -168,9 +171,13 @@ class Surface:
         check(lib().hipr_surface_clear(self._h, rgba), "hipr_surface_clear")
 
     def draw(self, scene, mode=0, edge_estimates=None, tile_estimates=None,
-             blur_impl=abi.BLUR_IMPL_MFMA):
-        """Draws a scene (a Scene pointer or a Prepared scene) onto the surface."""
-        opt, keep = _options(mode, edge_estimates, tile_estimates, blur_impl)
+             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0):
+        """Draws a scene (a Scene pointer or a Prepared scene) onto the surface.
+        band_rows: rows per walk band (0 = the library default, < 0 = none);
+        band_min_edges: in the estimate modes, band only paths whose active
+        edge bound reaches this (0 = default, < 0 = all)."""
+        opt, keep = _options(mode, edge_estimates, tile_estimates, blur_impl, band_rows,
+                             band_min_edges)
         if isinstance(scene, Prepared):
             check(lib().hipr_surface_draw_prepared(self._h, scene.handle, ctypes.byref(opt)),
                   "hipr_surface_draw_prepared")

@@ -129,3 +129,42 @@ def test_presized_mode_matches_exact(ctx):
     assert np.array_equal(ample, ref)
     assert st_s["fallbacks"] > 0
     assert st_a["fallbacks"] == 0
+
+
+@pytest.mark.parametrize("band_rows", [16, 48, 0])
+def test_row_bands_match_single_band(ctx, band_rows):
+    """The write pass restarts each row band from the snapshot the count pass
+    took at a tile-row boundary; the bands together draw exactly what one
+    sequential walk per path draws, with the same plane and tile counts."""
+    sc = workload(2, 48, seed=31)
+    ref, st_ref = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=0, band_rows=-1)
+    got, st = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=0, band_rows=band_rows)
+    print(f"band_rows {band_rows}: planes {st['tile_planes']} vs {st_ref['tile_planes']}",
+          flush=True)
+    assert np.array_equal(got, ref)
+    assert st["tile_planes"] == st_ref["tile_planes"]
+    assert st["tiles"] == st_ref["tiles"]
+    want = O.render(sc.scene, sc.width, sc.height)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("band_rows,band_min_edges", [(16, -1), (0, -1), (32, 100)])
+def test_row_bands_with_estimates(ctx, band_rows, band_min_edges):
+    """Estimates mode with banding on: the schedule pass records the
+    snapshots, the bands of a job share its estimated plane range, and short
+    estimates fall back."""
+    sc = workload(2, 40, seed=32)
+    n = sc.n_paths
+    kw = dict(band_rows=band_rows, band_min_edges=band_min_edges)
+    ref, _ = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=0, band_rows=-1)
+    short, st_s = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=1,
+                             edge_estimates=np.full(n, 3000, np.int32),
+                             tile_estimates=np.full(n, 40, np.int32), **kw)
+    ample, st_a = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=1,
+                             edge_estimates=np.full(n, 20000, np.int32),
+                             tile_estimates=np.full(n, 2 * 16384, np.int32), **kw)
+    print(f"fallbacks: short {st_s['fallbacks']}, ample {st_a['fallbacks']}", flush=True)
+    assert np.array_equal(short, ref)
+    assert np.array_equal(ample, ref)
+    assert st_s["fallbacks"] > 0
+    assert st_a["fallbacks"] == 0
