@@ -288,6 +288,33 @@ HIPR_API hipr_result hipr_surface_draw_prepared(hipr_surface surface,
                                                 hipr_prepared prepared,
                                                 const hipr_options* options);
 
+/* Batched form: many canvases, each with its own recorded scene, drawn by
+ * one pipeline run (one launch per stage for all of them).  Replaces a loop
+ * of per-canvas Canvas::Flush calls (canvas.hpp:365-424) over independent
+ * canvases.  A surface appears at most once; the surfaces and scenes added
+ * must stay alive until the batch is cleared or destroyed.  Path indices of
+ * the HIPR_MODE_ESTIMATES arrays run over the pairs in order. */
+typedef struct hipr_batch_s* hipr_batch;
+HIPR_API hipr_result hipr_batch_create(hipr_context ctx, hipr_batch* out);
+HIPR_API void hipr_batch_destroy(hipr_batch batch);
+HIPR_API hipr_result hipr_batch_add(hipr_batch batch, hipr_surface surface,
+                                    hipr_prepared scene);
+HIPR_API hipr_result hipr_batch_clear(hipr_batch batch);
+HIPR_API hipr_result hipr_batch_draw(hipr_batch batch, const hipr_options* options);
+/* The work split used when options leave band_rows / band_min_edges at 0. */
+HIPR_API hipr_result hipr_batch_set_split(hipr_batch batch, int32_t band_rows,
+                                          int32_t band_min_edges);
+/* Autotuner: draws the batch with each candidate work split (rows per walk
+ * band, banded paths) in `mode` (EXACT or LEARNED), keeps the fastest by
+ * device time and returns it.  The targets' contents are left undefined
+ * (clear them before drawing). */
+HIPR_API hipr_result hipr_batch_autotune(hipr_batch batch, int32_t mode, int32_t repeats,
+                                         int32_t* out_band_rows,
+                                         int32_t* out_band_min_edges);
+HIPR_API hipr_result hipr_batch_get_stats(hipr_batch batch, hipr_stats* out);
+HIPR_API hipr_result hipr_batch_get_stage_times(hipr_batch batch, double* ms,
+                                                int32_t* launches, int32_t n_stages);
+
 /* --- stage entry points used by the parity tests -------------------------
  * Stage (a): flatten + edge build (Stroke::QuadPath, src/geometry/
  * stroke.cc:914-962 and SWEdgeBuilder::BuildEdges, src/render/sw/

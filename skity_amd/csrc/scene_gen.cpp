@@ -271,6 +271,67 @@ void GenGradientBlur(sg_scene_s* s, Pcg32& rng, int n) {
   }
 }
 
+// One canvas of the batched uneven family: side 2^U[8,12], paths
+// round(10^U[lo_p, hi_p]), segments per path round(10^U[0.602, hi_s]) of
+// lines / quads / cubics at 40 / 30 / 30 %, control points uniform in the
+// canvas, 50% even-odd; paints 70% solid, 30% linear or radial gradient
+// (2-16 stops, clamp / repeat / mirror).  Full scale is hi_p = 4.699
+// (50,000 paths) and hi_s = 4 (10,000 segments); the scaled form caps both.
+void GenUneven(sg_scene_s* s, Pcg32& rng, int n, float lo_p, float hi_p, float hi_s) {
+  int side = 1 << rng.Int(8, 12);
+  s->width = side;
+  s->height = side;
+  int paths = n > 0 ? n : (int)std::lround(std::pow(10.f, rng.Uniform(lo_p, hi_p)));
+  auto P = [&]() { return rng.Uniform(0.f, (float)side); };
+  for (int i = 0; i < paths; i++) {
+    Builder b;
+    int segs = (int)std::lround(std::pow(10.f, rng.Uniform(0.602f, hi_s)));
+    b.Move(P(), P());
+    for (int k = 0; k < segs; k++) {
+      float u = rng.Unit();
+      if (u < 0.4f) {
+        float x = P(), y = P();
+        b.Line(x, y);
+      } else if (u < 0.7f) {
+        float x1 = P(), y1 = P(), x2 = P(), y2 = P();
+        b.Quad(x1, y1, x2, y2);
+      } else {
+        float x1 = P(), y1 = P(), x2 = P(), y2 = P(), x3 = P(), y3 = P();
+        b.Cubic(x1, y1, x2, y2, x3, y3);
+      }
+    }
+    b.Close();
+    hipr_paint paint = SolidPaint(rng, 77);
+    if (rng.Unit() < 0.3f) {
+      hipr_gradient g{};
+      g.type = rng.Int(0, 1) ? HIPR_GRADIENT_RADIAL : HIPR_GRADIENT_LINEAR;
+      g.tile_mode = rng.Int(0, 2);
+      g.color_count = rng.Int(2, 16);
+      g.has_offsets = 1;
+      std::vector<float> offs(g.color_count);
+      for (auto& o : offs) o = rng.Unit();
+      std::sort(offs.begin(), offs.end());
+      for (int c = 0; c < g.color_count; c++) {
+        g.offsets[c] = offs[c];
+        for (int ch = 0; ch < 4; ch++) g.colors[c][ch] = rng.Unit();
+      }
+      g.points[0] = P();
+      g.points[1] = P();
+      if (g.type == HIPR_GRADIENT_LINEAR) {
+        g.points[2] = P();
+        g.points[3] = P();
+      } else {
+        g.radius[0] = rng.Uniform(0.05f, 0.5f) * side;
+      }
+      g.local_matrix[0] = 1.f;
+      g.local_matrix[4] = 1.f;
+      paint.gradient = (int32_t)s->gradients.size();
+      s->gradients.push_back(g);
+    }
+    s->Add(b, rng.Int(0, 1) ? HIPR_FILL_EVEN_ODD : HIPR_FILL_WINDING, paint);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -287,6 +348,12 @@ sg_scene sg_generate(int32_t workload, uint64_t seed, int32_t n_paths) {
       break;
     case 3:
       GenGradientBlur(s, rng, n_paths > 0 ? n_paths : 1024);
+      break;
+    case 4:  // one canvas of the batched uneven family, full scale
+      GenUneven(s, rng, n_paths, 2.f, 4.699f, 4.f);
+      break;
+    case 5:  // the same family scaled: 10-1,000 paths of 4-316 segments
+      GenUneven(s, rng, n_paths, 1.f, 3.f, 2.5f);
       break;
     default:
       delete s;

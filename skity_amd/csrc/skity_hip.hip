@@ -302,12 +302,26 @@ struct hipr_prepared_s {
   std::vector<int32_t> walk_jobs;    // job indices that are rasterised
   std::vector<int32_t> stroke_jobs;  // job indices that need an outline
   int64_t layer_bytes = 0;
+  int32_t n_targets = 1;             // canvases [0, n_targets) are draw targets
+  int64_t n_verbs = 0, n_pts = 0, n_weights = 0;
   DevBuf verbs, pts, weights, scene_paints, d_jobs, d_paints, d_grads, d_walk, d_stroke;
   ~hipr_prepared_s() {
     DevBuf* b[] = {&verbs, &pts, &weights, &scene_paints, &d_jobs, &d_paints, &d_grads,
                    &d_walk, &d_stroke};
     for (DevBuf* x : b) x->Free();
   }
+};
+
+// Many (surface, prepared scene) pairs drawn by one pipeline run: the plans
+// are merged into one (`merged`), whose canvases [0, pairs) are the pairs'
+// surfaces and whose jobs are the pairs' jobs in pair order.
+struct hipr_batch_s {
+  hipr_context ctx = nullptr;
+  std::vector<std::pair<hipr_surface, hipr_prepared>> pairs;
+  hipr_prepared_s merged;
+  bool dirty = true;        // pairs changed since the last merge
+  hipr_surface_s sink;      // statistics of the last draw (no pixels)
+  int32_t tuned_band_rows = 0, tuned_band_min_edges = 0;  // 0: library defaults
 };
 
 namespace {
@@ -851,8 +865,14 @@ hipr_result RunPresize(Frame& f, std::vector<int32_t>* eest, std::vector<int32_t
   return HIPR_SUCCESS;
 }
 
-hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_options* opt) {
+// Draws a planned scene whose canvases [0, targets.size()) are the given
+// surfaces (the others are mask-filter layers); statistics, stage times and
+// per-job profiles go to `surf`.
+hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface surf,
+                        hipr_prepared_s* pl, const hipr_options* opt) {
   hipr_context ctx = surf->ctx;
+  const int n_targets = (int)targets.size();
+  if (n_targets != pl->n_targets) return HIPR_ERROR_INVALID_ARGUMENT;
   Frame f;
   f.ctx = ctx;
   f.pl = pl;
@@ -885,12 +905,14 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
     return HIPR_SUCCESS;
   }
   // canvases: the target and the mask-filter layers
-  f.canvases[0].pixels = surf->pixels;
-  f.canvases[0].width = surf->width;
-  f.canvases[0].height = surf->height;
+  for (int t = 0; t < n_targets; t++) {
+    f.canvases[t].pixels = targets[t]->pixels;
+    f.canvases[t].width = targets[t]->width;
+    f.canvases[t].height = targets[t]->height;
+  }
   if (pl->layer_bytes > 0) {
     if (ctx->layers.Ensure(pl->layer_bytes) != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-    for (size_t c = 1; c < f.canvases.size(); c++)
+    for (size_t c = n_targets; c < f.canvases.size(); c++)
       f.canvases[c].pixels = ctx->layers.as<uint8_t>() + (intptr_t)pl->canvases[c].pixels;
     HIP_TRY(hipMemsetAsync(ctx->layers.p, 0, pl->layer_bytes, st));
   }
@@ -1306,9 +1328,10 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   cp.planes = ctx->planes.as<uint8_t>();
   cp.paints = pl->d_paints.as<PaintDev>();
   cp.grads = pl->d_grads.as<GradDev>();
-  int main_tiles = f.canvases[0].tiles_x * f.canvases[0].tiles_y;
-  if (f.canvases.size() > 1) {
-    std::vector<Canvas> lc(f.canvases.begin() + 1, f.canvases.end());
+  int main_tiles = 0;
+  for (int t = 0; t < n_targets; t++) main_tiles += f.canvases[t].tiles_x * f.canvases[t].tiles_y;
+  if ((int)f.canvases.size() > n_targets) {
+    std::vector<Canvas> lc(f.canvases.begin() + n_targets, f.canvases.end());
     int layer_tiles = 0;
     for (auto& c : lc) {
       c.tile_begin -= main_tiles;
@@ -1370,7 +1393,7 @@ hipr_result RunPipeline(hipr_surface surf, hipr_prepared_s* pl, const hipr_optio
   }
   CompositeParams mp = cp;
   mp.canvases = ctx->canvases.as<Canvas>();
-  mp.n_canvases = 1;
+  mp.n_canvases = n_targets;
   mp.total_tiles = main_tiles;
   if (ctx->counters.Ensure(2 * sizeof(unsigned long long)) != hipSuccess)
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
@@ -1451,6 +1474,9 @@ hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s**
     BuildPlan(scene, pl);
     int n = scene->n_paths;
     int nv = scene->verb_offsets[n], np = scene->point_offsets[n], nw = scene->weight_offsets[n];
+    pl->n_verbs = nv;
+    pl->n_pts = np;
+    pl->n_weights = nw;
     if ((r = Upload(&pl->verbs, scene->verbs, nv, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->pts, scene->points, 2 * (size_t)np, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->weights, scene->weights, nw, st)) != HIPR_SUCCESS ||
@@ -1470,6 +1496,127 @@ hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s**
   }
   ctx->refs++;  // released by hipr_prepared_destroy
   *out = pl;
+  return HIPR_SUCCESS;
+}
+
+// Concatenates the plans of a batch's pairs into b->merged: job, paint,
+// gradient, canvas and path indices and geometry offsets are rebased; the
+// geometry is gathered with device-to-device copies.
+hipr_result MergeBatch(hipr_batch b) {
+  hipr_context ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  hipr_prepared_s& m = b->merged;
+  m.ctx = ctx;
+  m.jobs.clear();
+  m.canvases.clear();
+  m.paints.clear();
+  m.grads.clear();
+  m.blurs.clear();
+  m.layer_of_blur.clear();
+  m.walk_jobs.clear();
+  m.stroke_jobs.clear();
+  const int T = (int)b->pairs.size();
+  m.n_targets = T;
+  std::vector<int> layer_base(T);
+  int n_layers = 0;
+  for (int p = 0; p < T; p++) {
+    layer_base[p] = T + n_layers;
+    const hipr_prepared_s* s = b->pairs[p].second;
+    n_layers += s->canvases.empty() ? 0 : (int)s->canvases.size() - 1;
+  }
+  m.canvases.assign(T + n_layers, Canvas{});
+  int64_t voff = 0, poff = 0, woff = 0, lbytes = 0;
+  int path_off = 0, paint_off = 0, grad_off = 0, job_off = 0;
+  for (int p = 0; p < T; p++) {
+    const hipr_prepared_s* s = b->pairs[p].second;
+    auto canv = [&](int c) { return c == 0 ? p : layer_base[p] + c - 1; };
+    if (voff + s->n_verbs > INT32_MAX || poff + s->n_pts > INT32_MAX ||
+        woff + s->n_weights > INT32_MAX)
+      return HIPR_ERROR_CAPACITY;
+    for (const Job& src : s->jobs) {
+      Job jb = src;
+      jb.path += path_off;
+      jb.canvas = canv(jb.canvas);
+      if (jb.kind != kJobImage) {
+        jb.verb_begin += (int32_t)voff;
+        jb.verb_end += (int32_t)voff;
+        jb.pt_begin += (int32_t)poff;
+        jb.w_begin += (int32_t)woff;
+      } else {
+        jb.image = canv(jb.image);
+      }
+      jb.paint += paint_off;
+      m.jobs.push_back(jb);
+    }
+    for (const PaintDev& src : s->paints) {
+      PaintDev q = src;
+      if (q.kind == kPaintGradient) q.grad += grad_off;
+      if (q.kind == kPaintImage) q.layer = canv(q.layer);
+      m.paints.push_back(q);
+    }
+    m.grads.insert(m.grads.end(), s->grads.begin(), s->grads.end());
+    for (size_t c = 0; c < s->canvases.size(); c++) {
+      Canvas cv = s->canvases[c];
+      cv.job_begin += job_off;
+      cv.job_end += job_off;
+      if (c > 0) cv.pixels = (uint8_t*)((intptr_t)cv.pixels + lbytes);
+      m.canvases[canv((int)c)] = cv;
+    }
+    if (s->canvases.empty()) {  // an empty scene draws nothing on its target
+      m.canvases[p].job_begin = m.canvases[p].job_end = job_off;
+    }
+    m.blurs.insert(m.blurs.end(), s->blurs.begin(), s->blurs.end());
+    for (int32_t l : s->layer_of_blur) m.layer_of_blur.push_back(canv(l));
+    for (int32_t j : s->walk_jobs) m.walk_jobs.push_back(j + job_off);
+    for (int32_t j : s->stroke_jobs) m.stroke_jobs.push_back(j + job_off);
+    voff += s->n_verbs;
+    poff += s->n_pts;
+    woff += s->n_weights;
+    lbytes += s->layer_bytes;
+    path_off += s->n_paths;
+    paint_off += (int)s->paints.size();
+    grad_off += (int)s->grads.size();
+    job_off += (int)s->jobs.size();
+  }
+  m.n_paths = path_off;
+  m.layer_bytes = lbytes;
+  m.n_verbs = voff;
+  m.n_pts = poff;
+  m.n_weights = woff;
+  if (m.verbs.Ensure(std::max<int64_t>(voff, 1)) != hipSuccess ||
+      m.pts.Ensure(8 * std::max<int64_t>(poff, 1)) != hipSuccess ||
+      m.weights.Ensure(4 * std::max<int64_t>(woff, 1)) != hipSuccess ||
+      m.scene_paints.Ensure(sizeof(hipr_paint) * std::max(path_off, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  int64_t v = 0, q = 0, w = 0, pa = 0;
+  for (int p = 0; p < T; p++) {
+    const hipr_prepared_s* s = b->pairs[p].second;
+    if (s->n_verbs)
+      HIP_TRY(hipMemcpyAsync(m.verbs.as<uint8_t>() + v, s->verbs.p, s->n_verbs,
+                             hipMemcpyDeviceToDevice, st));
+    if (s->n_pts)
+      HIP_TRY(hipMemcpyAsync(m.pts.as<uint8_t>() + 8 * q, s->pts.p, 8 * s->n_pts,
+                             hipMemcpyDeviceToDevice, st));
+    if (s->n_weights)
+      HIP_TRY(hipMemcpyAsync(m.weights.as<uint8_t>() + 4 * w, s->weights.p, 4 * s->n_weights,
+                             hipMemcpyDeviceToDevice, st));
+    if (s->n_paths)
+      HIP_TRY(hipMemcpyAsync(m.scene_paints.as<hipr_paint>() + pa, s->scene_paints.p,
+                             sizeof(hipr_paint) * s->n_paths, hipMemcpyDeviceToDevice, st));
+    v += s->n_verbs;
+    q += s->n_pts;
+    w += s->n_weights;
+    pa += s->n_paths;
+  }
+  hipr_result r;
+  if ((r = Upload(&m.d_jobs, m.jobs.data(), m.jobs.size(), st)) != HIPR_SUCCESS ||
+      (r = Upload(&m.d_paints, m.paints.data(), m.paints.size(), st)) != HIPR_SUCCESS ||
+      (r = Upload(&m.d_grads, m.grads.data(), m.grads.size(), st)) != HIPR_SUCCESS ||
+      (r = Upload(&m.d_walk, m.walk_jobs.data(), m.walk_jobs.size(), st)) != HIPR_SUCCESS ||
+      (r = Upload(&m.d_stroke, m.stroke_jobs.data(), m.stroke_jobs.size(), st)) != HIPR_SUCCESS)
+    return r;
+  HIP_TRY(hipStreamSynchronize(st));
+  b->dirty = false;
   return HIPR_SUCCESS;
 }
 
@@ -1874,7 +2021,129 @@ HIPR_API hipr_result hipr_surface_draw_prepared(hipr_surface surf, hipr_prepared
   if (!surf || !p) return HIPR_ERROR_INVALID_HANDLE;
   if (p->ctx != surf->ctx) return HIPR_ERROR_INVALID_ARGUMENT;
   HIP_TRY(hipSetDevice(surf->ctx->device));
-  return RunPipeline(surf, p, options);
+  return RunPipeline({surf}, surf, p, options);
+}
+
+HIPR_API hipr_result hipr_batch_create(hipr_context ctx, hipr_batch* out) {
+  if (!out) return HIPR_ERROR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  hipr_batch b = new (std::nothrow) hipr_batch_s();
+  if (!b) return HIPR_ERROR_OUT_OF_HOST_MEMORY;
+  b->ctx = ctx;
+  b->merged.ctx = ctx;
+  b->sink.ctx = ctx;
+  ctx->refs++;  // released by hipr_batch_destroy
+  *out = b;
+  return HIPR_SUCCESS;
+}
+
+HIPR_API void hipr_batch_destroy(hipr_batch b) {
+  if (!b) return;
+  hipr_context ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete b;
+  ReleaseContext(ctx);
+}
+
+HIPR_API hipr_result hipr_batch_add(hipr_batch b, hipr_surface surf, hipr_prepared scene) {
+  if (!b || !surf || !scene) return HIPR_ERROR_INVALID_HANDLE;
+  if (surf->ctx != b->ctx || scene->ctx != b->ctx) return HIPR_ERROR_INVALID_ARGUMENT;
+  for (const auto& pr : b->pairs)
+    if (pr.first == surf) return HIPR_ERROR_INVALID_ARGUMENT;  // one scene per surface
+  if (b->pairs.size() >= (size_t)(1 << 20)) return HIPR_ERROR_CAPACITY;
+  b->pairs.push_back({surf, scene});
+  b->dirty = true;
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_batch_clear(hipr_batch b) {
+  if (!b) return HIPR_ERROR_INVALID_HANDLE;
+  b->pairs.clear();
+  b->dirty = true;
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_batch_draw(hipr_batch b, const hipr_options* options) {
+  if (!b) return HIPR_ERROR_INVALID_HANDLE;
+  if (b->pairs.empty()) return HIPR_SUCCESS;
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  hipr_result r;
+  if (b->dirty && (r = MergeBatch(b)) != HIPR_SUCCESS) return r;
+  hipr_options opt{};
+  if (options) opt = *options;
+  if (opt.band_rows == 0) opt.band_rows = b->tuned_band_rows;
+  if (opt.band_min_edges == 0) opt.band_min_edges = b->tuned_band_min_edges;
+  std::vector<hipr_surface> targets;
+  for (const auto& pr : b->pairs) targets.push_back(pr.first);
+  return RunPipeline(targets, &b->sink, &b->merged, &opt);
+}
+
+HIPR_API hipr_result hipr_batch_set_split(hipr_batch b, int32_t band_rows, int32_t band_min_edges) {
+  if (!b) return HIPR_ERROR_INVALID_HANDLE;
+  b->tuned_band_rows = band_rows;
+  b->tuned_band_min_edges = band_min_edges;
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_batch_autotune(hipr_batch b, int32_t mode, int32_t repeats,
+                                         int32_t* out_band_rows, int32_t* out_band_min_edges) {
+  if (!b) return HIPR_ERROR_INVALID_HANDLE;
+  if (repeats < 1 || repeats > 100 || mode < HIPR_MODE_EXACT || mode > HIPR_MODE_LEARNED ||
+      mode == HIPR_MODE_ESTIMATES)
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  if (b->pairs.empty()) return HIPR_SUCCESS;
+  // Work splits: rows per walk band, and (estimate modes) which paths are
+  // banded.  Each candidate is drawn once to warm up, then `repeats` times;
+  // the device time of its fastest draw decides.
+  const int32_t rows[] = {-1, 64, 128, 256};
+  const int32_t mins[] = {INT32_MAX, 4 * kWave + 1, -1};
+  struct Cand {
+    int32_t rows, min_edges;
+  };
+  std::vector<Cand> cands;
+  for (int32_t r : rows) {
+    if (mode == HIPR_MODE_EXACT || r < 0) {
+      cands.push_back({r, 0});
+    } else {
+      for (int32_t m : mins) cands.push_back({r, m});
+    }
+  }
+  double best_ms = 1e30;
+  Cand best = cands[0];
+  for (const Cand& c : cands) {
+    hipr_options opt{};
+    opt.mode = mode;
+    opt.band_rows = c.rows;
+    opt.band_min_edges = c.min_edges;
+    double t = 1e30;
+    for (int k = 0; k <= repeats; k++) {
+      hipr_result r = hipr_batch_draw(b, &opt);
+      if (r != HIPR_SUCCESS) return r;
+      if (k > 0) t = std::min(t, b->sink.stats.ms_total);
+    }
+    if (t < best_ms) {
+      best_ms = t;
+      best = c;
+    }
+  }
+  b->tuned_band_rows = best.rows;
+  b->tuned_band_min_edges = best.min_edges;
+  if (out_band_rows) *out_band_rows = best.rows;
+  if (out_band_min_edges) *out_band_min_edges = best.min_edges;
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_batch_get_stats(hipr_batch b, hipr_stats* out) {
+  if (!b) return HIPR_ERROR_INVALID_HANDLE;
+  return hipr_surface_get_stats(&b->sink, out);
+}
+
+HIPR_API hipr_result hipr_batch_get_stage_times(hipr_batch b, double* ms, int32_t* launches,
+                                                int32_t n_stages) {
+  if (!b) return HIPR_ERROR_INVALID_HANDLE;
+  return hipr_surface_get_stage_times(&b->sink, ms, launches, n_stages);
 }
 
 HIPR_API hipr_result hipr_surface_draw_scene(hipr_surface surf, const hipr_scene* scene,
@@ -1884,7 +2153,7 @@ HIPR_API hipr_result hipr_surface_draw_scene(hipr_surface surf, const hipr_scene
   HIP_TRY(hipSetDevice(surf->ctx->device));
   hipr_result r = Prepare(surf->ctx, scene, &p);
   if (r != HIPR_SUCCESS) return r;
-  r = RunPipeline(surf, p, options);
+  r = RunPipeline({surf}, surf, p, options);
   hipr_prepared_destroy(p);
   return r;
 }

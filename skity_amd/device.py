@@ -60,6 +60,28 @@ def lib():
         L.hipr_stackblur_ex.argtypes = [c_ctx, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.hipr_stackblur_ex.restype = ctypes.c_int
+        c_batch = ctypes.c_void_p
+        L.hipr_batch_create.argtypes = [c_ctx, ctypes.POINTER(c_batch)]
+        L.hipr_batch_create.restype = ctypes.c_int
+        L.hipr_batch_destroy.argtypes = [c_batch]
+        L.hipr_batch_destroy.restype = None
+        L.hipr_batch_add.argtypes = [c_batch, c_surf, ctypes.c_void_p]
+        L.hipr_batch_add.restype = ctypes.c_int
+        L.hipr_batch_clear.argtypes = [c_batch]
+        L.hipr_batch_clear.restype = ctypes.c_int
+        L.hipr_batch_draw.argtypes = [c_batch, ctypes.POINTER(abi.Options)]
+        L.hipr_batch_draw.restype = ctypes.c_int
+        L.hipr_batch_set_split.argtypes = [c_batch, ctypes.c_int32, ctypes.c_int32]
+        L.hipr_batch_set_split.restype = ctypes.c_int
+        L.hipr_batch_autotune.argtypes = [c_batch, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_int32),
+                                          ctypes.POINTER(ctypes.c_int32)]
+        L.hipr_batch_autotune.restype = ctypes.c_int
+        L.hipr_batch_get_stats.argtypes = [c_batch, ctypes.POINTER(abi.Stats)]
+        L.hipr_batch_get_stats.restype = ctypes.c_int
+        L.hipr_batch_get_stage_times.argtypes = [c_batch, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_int32]
+        L.hipr_batch_get_stage_times.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -218,6 +240,70 @@ class Surface:
         if self._h:
             lib().hipr_surface_destroy(self._h)
             self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many (surface, prepared scene) pairs drawn by one pipeline run
+    (hipr_batch_*)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        self._pairs = []  # keeps the surfaces and scenes alive
+        check(lib().hipr_batch_create(ctx.handle, ctypes.byref(self._h)), "hipr_batch_create")
+
+    def add(self, surface, prepared):
+        check(lib().hipr_batch_add(self._h, surface._h, prepared.handle), "hipr_batch_add")
+        self._pairs.append((surface, prepared))
+
+    def clear(self):
+        check(lib().hipr_batch_clear(self._h), "hipr_batch_clear")
+        self._pairs = []
+
+    def set_split(self, band_rows, band_min_edges):
+        check(lib().hipr_batch_set_split(self._h, band_rows, band_min_edges),
+              "hipr_batch_set_split")
+
+    def autotune(self, mode, repeats=2):
+        """Picks the fastest work split for `mode` (hipr_batch_autotune);
+        returns (band_rows, band_min_edges).  Leaves the targets' contents
+        undefined."""
+        r, m = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().hipr_batch_autotune(self._h, mode, repeats, ctypes.byref(r), ctypes.byref(m)),
+              "hipr_batch_autotune")
+        return r.value, m.value
+
+    def draw(self, mode=0, edge_estimates=None, tile_estimates=None,
+             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0):
+        opt, keep = _options(mode, edge_estimates, tile_estimates, blur_impl, band_rows,
+                             band_min_edges)
+        check(lib().hipr_batch_draw(self._h, ctypes.byref(opt)), "hipr_batch_draw")
+        del keep
+
+    def stats(self):
+        s = abi.Stats()
+        check(lib().hipr_batch_get_stats(self._h, ctypes.byref(s)), "hipr_batch_get_stats")
+        return {k: getattr(s, k) for k, _ in abi.Stats._fields_}
+
+    def stage_times(self):
+        n = len(abi.STAGES)
+        ms = np.zeros(n, np.float64)
+        cnt = np.zeros(n, np.int32)
+        check(lib().hipr_batch_get_stage_times(self._h, ms.ctypes.data, cnt.ctypes.data, n),
+              "hipr_batch_get_stage_times")
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(abi.STAGES)}
+
+    def close(self):
+        if self._h:
+            lib().hipr_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+        self._pairs = []
 
     def __del__(self):
         try:

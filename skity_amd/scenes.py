@@ -74,7 +74,11 @@ class SceneHandle:
 
 def generate(workload, n_paths=0, seed=None):
     """Workload 1: plain fill (800x600, 64 polygons); 2: curves + strokes
-    (2048^2, 4096 paths); 3: gradients + blur (4096^2, 1024 paths)."""
+    (2048^2, 4096 paths); 3: gradients + blur (4096^2, 1024 paths); 4: one
+    canvas of the batched uneven family (side 256-4096, 100-50,000 paths of
+    4-10,000 line/quad/cubic segments, 30% gradients); 5: the same family
+    scaled to 10-1,000 paths of 4-316 segments.  n_paths > 0 overrides the
+    path count."""
     if seed is None:
         seed = SEED_BASE + workload
     h = _load().sg_generate(workload, seed, n_paths)

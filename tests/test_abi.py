@@ -48,3 +48,28 @@ def test_scene_generator_matches_header_and_workload_spec():
     assert 0.17 < frac < 0.23
     assert width[style == abi.STYLE_STROKE].min() >= 1.0
     assert width[style == abi.STYLE_STROKE].max() <= 16.0
+
+
+def test_uneven_generator_spans_the_family():
+    """Workload 5 (scaled batched uneven family): square canvases of side
+    2^8..2^12, 10-1,000 paths, segment counts 4-316 with lines, quads and
+    cubics at about 40/30/30 %, about 30% gradient paints."""
+    import numpy as np
+    from skity_amd import scenes
+    sides, paths, grads, verbs = set(), [], 0, np.zeros(8, np.int64)
+    for seed in range(40):
+        sc = scenes.generate(5, seed=1000 + seed)
+        s = sc.scene.contents
+        assert sc.width == sc.height and sc.width in (256, 512, 1024, 2048, 4096)
+        sides.add(sc.width)
+        paths.append(s.n_paths)
+        assert 10 <= s.n_paths <= 1000
+        n_v = s.verb_offsets[s.n_paths]
+        v = np.ctypeslib.as_array(s.verbs, shape=(n_v,))
+        verbs += np.bincount(v, minlength=8)[:8]
+        grads += sum(1 for p in range(s.n_paths) if s.paints[p].gradient >= 0)
+    assert len(sides) >= 4
+    seg = verbs[abi.VERB_LINE] + verbs[abi.VERB_QUAD] + verbs[abi.VERB_CUBIC]
+    assert abs(verbs[abi.VERB_LINE] / seg - 0.4) < 0.02
+    assert abs(verbs[abi.VERB_CUBIC] / seg - 0.3) < 0.02
+    assert abs(grads / sum(paths) - 0.3) < 0.03
