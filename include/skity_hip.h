@@ -163,7 +163,12 @@ typedef struct hipr_options {
   int32_t band_min_edges;         /* estimate modes: band only the paths whose
                                      active-edge bound reaches this; 0 = default
                                      (no banding), < 0 = every path */
+  int32_t flags;                  /* HIPR_FLAG_* */
 } hipr_options;
+enum {
+  HIPR_FLAG_NO_GRAPH = 1,   /* launch the walk passes directly, not as hipGraphs */
+  HIPR_FLAG_PERSISTENT = 2  /* persistent walk kernels on an atomic work queue   */
+};
 
 /* --- context / surface ---------------------------------------------------
  * Replaces Canvas::MakeSoftwareCanvas(Bitmap*) (src/render/sw/sw_canvas.cc:
@@ -301,16 +306,17 @@ HIPR_API hipr_result hipr_batch_add(hipr_batch batch, hipr_surface surface,
                                     hipr_prepared scene);
 HIPR_API hipr_result hipr_batch_clear(hipr_batch batch);
 HIPR_API hipr_result hipr_batch_draw(hipr_batch batch, const hipr_options* options);
-/* The work split used when options leave band_rows / band_min_edges at 0. */
+/* The work split used when options leave band_rows / band_min_edges /
+ * flags at 0. */
 HIPR_API hipr_result hipr_batch_set_split(hipr_batch batch, int32_t band_rows,
-                                          int32_t band_min_edges);
+                                          int32_t band_min_edges, int32_t flags);
 /* Autotuner: draws the batch with each candidate work split (rows per walk
- * band, banded paths) in `mode` (EXACT or LEARNED), keeps the fastest by
- * device time and returns it.  The targets' contents are left undefined
- * (clear them before drawing). */
+ * band, banded paths, persistent or dispatched walk kernels) in `mode`
+ * (EXACT or LEARNED), keeps the fastest by device time and returns it.  The
+ * targets' contents are left undefined (clear them before drawing). */
 HIPR_API hipr_result hipr_batch_autotune(hipr_batch batch, int32_t mode, int32_t repeats,
                                          int32_t* out_band_rows,
-                                         int32_t* out_band_min_edges);
+                                         int32_t* out_band_min_edges, int32_t* out_flags);
 HIPR_API hipr_result hipr_batch_get_stats(hipr_batch batch, hipr_stats* out);
 HIPR_API hipr_result hipr_batch_get_stage_times(hipr_batch batch, double* ms,
                                                 int32_t* launches, int32_t n_stages);

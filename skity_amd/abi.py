@@ -96,6 +96,7 @@ class Stats(ctypes.Structure):
 
 
 BLUR_IMPL_MFMA, BLUR_IMPL_LINES = 0, 1
+FLAG_NO_GRAPH, FLAG_PERSISTENT = 1, 2
 
 
 class Options(ctypes.Structure):
@@ -106,6 +107,7 @@ class Options(ctypes.Structure):
         ("blur_impl", ctypes.c_int32),
         ("band_rows", ctypes.c_int32),
         ("band_min_edges", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
     ]
 
 

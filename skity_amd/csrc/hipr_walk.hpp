@@ -75,6 +75,7 @@ struct WalkParams {
   int32_t* map1;
   JobCov* cov;
   int32_t lds_width;        // row buffer capacity in pixels
+  int32_t* queue;           // persistent form: item counter (starts at 0), else null
 };
 
 // Job pixel bounds as RastePath computes them (sw_raster.cc:741-754).
@@ -191,14 +192,11 @@ __device__ void BitonicSortSlots(int64_t (&key)[K], int N, int lane) {
 // a job run concurrently.  Count and schedule passes: one wave per job, band
 // 0, walking every row and recording the band snapshots.
 template <int K, int kMode>
-__global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
+__device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
   constexpr int CAP = K * kWave;
   constexpr bool kWrite = kMode == kWalkWrite;
   constexpr bool kBlit = kMode != kWalkSchedule;
-  extern __shared__ int32_t lds[];
   const int lane = Lane();
-  const int wi = blockIdx.x;
-  if (wi >= P.n_walk) return;
   const uint64_t t_start = wall_clock64();
   const int item = P.walk_list[wi];
   const int j = item >> 4;
@@ -1251,6 +1249,28 @@ __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
       for (int k = 0; k < 5; k++) c.phase[k] = (int32_t)(ph[k] < 0x7FFFFFFF ? ph[k] : 0x7FFFFFFF);
       P.cov[j] = c;
     }
+  }
+}
+
+// The walk kernel.  Without a queue, workgroup i walks item i (the
+// dispatcher hands out items in list order).  With a queue (persistent
+// form), a grid sized to the resident waves takes items from an atomic
+// counter until the list is exhausted; every wave leaves when the counter
+// passes n_walk.
+template <int K, int kMode>
+__global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
+  extern __shared__ int32_t lds[];
+  if (!P.queue) {
+    if ((int)blockIdx.x < P.n_walk) WalkItem<K, kMode>(P, blockIdx.x, lds);
+    return;
+  }
+  for (;;) {
+    int got = 0;
+    if (Lane() == 0) got = atomicAdd(P.queue, 1);
+    const int wi = __builtin_amdgcn_readlane(got, 0);
+    if (wi >= P.n_walk) return;
+    WalkItem<K, kMode>(P, wi, lds);
+    WaveSync();
   }
 }
 

@@ -249,6 +249,10 @@ struct hipr_context_s {
   DevBuf plane_base, plane_cap, planes, map_base, map0, map1, sub_list, big_list, bound,
       gather_list;
   DevBuf wcov, qstate, q_base, snaps, snap_base, band_count;  // write-pass coverage, row bands
+  DevBuf queues;                          // persistent walk: item counters
+  int n_cus = 0;                          // compute units of the device
+  std::vector<hipGraph_t> graphs;         // graphs of the walk passes in flight
+  std::vector<hipGraphExec_t> graph_execs;
   DevBuf stroke_out, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w;
   DevBuf layers, blur_rows, blur_cols, blur_rb, blur_cb, blur_tmp, blur_jobs, blur_lines;
   // debug/utility uploads
@@ -322,6 +326,7 @@ struct hipr_batch_s {
   bool dirty = true;        // pairs changed since the last merge
   hipr_surface_s sink;      // statistics of the last draw (no pixels)
   int32_t tuned_band_rows = 0, tuned_band_min_edges = 0;  // 0: library defaults
+  int32_t tuned_flags = 0;
 };
 
 namespace {
@@ -628,18 +633,39 @@ void SetTiles(std::vector<Canvas>* cvs) {
 constexpr int kBandRows = 128;
 
 template <int K, int kMode>
-hipr_result LaunchWalk(hipr_context ctx, const WalkParams& P, hipStream_t st) {
+hipr_result LaunchWalk(hipr_context ctx, WalkParams P, hipStream_t st, bool timed,
+                       int32_t* queue, double share) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
   size_t lds = (size_t)((P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 6 * K * kWave) *
                sizeof(int32_t);
-  STAGE_LAUNCH_ON(ctx, st,
-                  kMode == kWalkCount   ? HIPR_STAGE_WALK_COUNT
-                  : kMode == kWalkWrite ? HIPR_STAGE_WALK
-                                        : HIPR_STAGE_WALK_SCHEDULE,
-                  hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(P.n_walk), dim3(64), lds,
-                                     st, P));
+  unsigned grid = (unsigned)P.n_walk;
+  P.queue = queue;
+  if (queue) {
+    // persistent form: as many waves as fit at once (this tier's share)
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, WalkKernel<K, kMode>, kWave,
+                                                         lds));
+    int64_t cap = (int64_t)std::max(per_cu, 1) * std::max(ctx->n_cus, 1);
+    cap = std::max<int64_t>(1, (int64_t)((double)cap * share));
+    grid = (unsigned)std::min<int64_t>(P.n_walk, cap);
+  }
+  const int stage = kMode == kWalkCount   ? HIPR_STAGE_WALK_COUNT
+                    : kMode == kWalkWrite ? HIPR_STAGE_WALK
+                                          : HIPR_STAGE_WALK_SCHEDULE;
+  hipEvent_t b = timed ? StageBegin(ctx, st) : nullptr;
+  hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(grid), dim3(kWave), lds, st, P);
   HIP_TRY(hipGetLastError());
+  if (timed) StageEnd(ctx, stage, b, st);
   return HIPR_SUCCESS;
+}
+
+// Graphs launched by the walk passes; every pass is followed by a stream
+// synchronisation, so they are released at the start of the next pass.
+void ReleaseGraphs(hipr_context ctx) {
+  for (hipGraphExec_t e : ctx->graph_execs) (void)hipGraphExecDestroy(e);
+  for (hipGraph_t g : ctx->graphs) (void)hipGraphDestroy(g);
+  ctx->graph_execs.clear();
+  ctx->graphs.clear();
 }
 
 // ---- the device pipeline -----------------------------------------------------
@@ -1113,55 +1139,116 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   };
   // One walk pass: the 4-slot jobs on the main stream, the 8- and 16-slot
   // jobs on the side stream; the main stream waits for both.
+  // One walk pass: the 4-slot jobs on the main stream, the 8- and 16-slot
+  // jobs on the side stream; the main stream waits for both.  The item lists
+  // are uploaded first; the launches (queue reset, fork, tier kernels, join)
+  // are captured into a hipGraph and launched as one unless the caller asks
+  // for direct launches.
+  const int run_flags = opt ? opt->flags : 0;
+  const bool use_graph = (run_flags & HIPR_FLAG_NO_GRAPH) == 0;
+  const bool persistent = (run_flags & HIPR_FLAG_PERSISTENT) != 0;
+  if (persistent && ctx->queues.Ensure(4 * sizeof(int32_t)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   auto walk_pass = [&](int kmode, const std::vector<int32_t>& sm,
                        const std::vector<int32_t>& wd) -> hipr_result {
     hipr_result rr;
-    WalkParams ws = wp, ww = wp;
+    ReleaseGraphs(ctx);
+    WalkParams ws = wp, w8p = wp, w16p = wp;
     std::vector<int32_t> ism = make_items(sm, kmode);
+    std::vector<int32_t> j8, j16;
+    for (int32_t j : wd) (tier[j] <= 8 ? j8 : j16).push_back(j);
+    std::vector<int32_t> w8 = make_items(j8, kmode), w16 = make_items(j16, kmode);
     if (!ism.empty()) {
       if ((rr = Upload(&ctx->sub_list, ism.data(), ism.size(), st)) != HIPR_SUCCESS) return rr;
       ws.walk_list = ctx->sub_list.as<int32_t>();
       ws.n_walk = (int32_t)ism.size();
     }
-    if (!wd.empty()) {
-      std::vector<int32_t> j8, j16;
-      for (int32_t j : wd) (tier[j] <= 8 ? j8 : j16).push_back(j);
-      std::vector<int32_t> w8 = make_items(j8, kmode), w16 = make_items(j16, kmode);
+    if (!w8.empty() || !w16.empty()) {
       std::vector<int32_t> ordered = w8;
       ordered.insert(ordered.end(), w16.begin(), w16.end());
       if ((rr = Upload(&ctx->big_list, ordered.data(), ordered.size(), st)) != HIPR_SUCCESS)
         return rr;
-      HIP_TRY(hipEventRecord(ctx->ev_fork, st));
-      HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-      auto launch = [&](auto k_tag, const WalkParams& w) -> hipr_result {
-        constexpr int KK = decltype(k_tag)::value;
-        return kmode == kWalkCount   ? LaunchWalk<KK, kWalkCount>(ctx, w, ctx->side)
-               : kmode == kWalkWrite ? LaunchWalk<KK, kWalkWrite>(ctx, w, ctx->side)
-                                     : LaunchWalk<KK, kWalkSchedule>(ctx, w, ctx->side);
-      };
+      w8p.walk_list = ctx->big_list.as<int32_t>();
+      w8p.n_walk = (int32_t)w8.size();
+      w16p.walk_list = ctx->big_list.as<int32_t>() + w8.size();
+      w16p.n_walk = (int32_t)w16.size();
+    }
+    // persistent form: each tier's share of the resident waves follows its
+    // estimated work (active-edge bound x rows per item)
+    auto tier_work = [&](const std::vector<int32_t>& js) {
+      double w = 0;
+      for (int32_t j : js) {
+        PixBounds pb = JobPixBounds(ctx->h_geo[j]);
+        w += (double)std::max(bound[j], 1) * std::max(pb.b - pb.t, 1);
+      }
+      return w;
+    };
+    const double work4 = persistent ? tier_work(sm) : 0, work8 = persistent ? tier_work(j8) : 0,
+                 work16 = persistent ? tier_work(j16) : 0;
+    const double n_all = std::max(work4 + work8 + work16, 1.0);
+    int32_t* q = persistent ? ctx->queues.as<int32_t>() : nullptr;
+    const bool timed = !use_graph;
+    auto launches = [&]() -> hipr_result {
+      hipr_result lr;
+      if (q) hipLaunchKernelGGL(FillKernel, dim3(1), dim3(kWave), 0, st, q, (int64_t)3, 0);
+      const bool side = !w8.empty() || !w16.empty();
+      if (side) {
+        HIP_TRY(hipEventRecord(ctx->ev_fork, st));
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+      }
       if (!w8.empty()) {
-        WalkParams w = ww;
-        w.walk_list = ctx->big_list.as<int32_t>();
-        w.n_walk = (int32_t)w8.size();
-        if ((rr = launch(std::integral_constant<int, 8>{}, w)) != HIPR_SUCCESS) return rr;
+        lr = kmode == kWalkCount
+                 ? LaunchWalk<8, kWalkCount>(ctx, w8p, ctx->side, timed, q ? q + 1 : nullptr,
+                                             work8 / n_all)
+             : kmode == kWalkWrite
+                 ? LaunchWalk<8, kWalkWrite>(ctx, w8p, ctx->side, timed, q ? q + 1 : nullptr,
+                                             work8 / n_all)
+                 : LaunchWalk<8, kWalkSchedule>(ctx, w8p, ctx->side, timed,
+                                                q ? q + 1 : nullptr, work8 / n_all);
+        if (lr != HIPR_SUCCESS) return lr;
       }
       if (!w16.empty()) {
-        WalkParams w = ww;
-        w.walk_list = ctx->big_list.as<int32_t>() + w8.size();
-        w.n_walk = (int32_t)w16.size();
-        if ((rr = launch(std::integral_constant<int, 16>{}, w)) != HIPR_SUCCESS) return rr;
+        lr = kmode == kWalkCount
+                 ? LaunchWalk<16, kWalkCount>(ctx, w16p, ctx->side, timed, q ? q + 2 : nullptr,
+                                              work16 / n_all)
+             : kmode == kWalkWrite
+                 ? LaunchWalk<16, kWalkWrite>(ctx, w16p, ctx->side, timed, q ? q + 2 : nullptr,
+                                              work16 / n_all)
+                 : LaunchWalk<16, kWalkSchedule>(ctx, w16p, ctx->side, timed,
+                                                 q ? q + 2 : nullptr, work16 / n_all);
+        if (lr != HIPR_SUCCESS) return lr;
       }
-    }
-    if (!ism.empty()) {
-      rr = kmode == kWalkCount   ? LaunchWalk<4, kWalkCount>(ctx, ws, st)
-           : kmode == kWalkWrite ? LaunchWalk<4, kWalkWrite>(ctx, ws, st)
-                                 : LaunchWalk<4, kWalkSchedule>(ctx, ws, st);
-      if (rr != HIPR_SUCCESS) return rr;
-    }
-    if (!wd.empty()) {
-      HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
-      HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
-    }
+      if (!ism.empty()) {
+        lr = kmode == kWalkCount
+                 ? LaunchWalk<4, kWalkCount>(ctx, ws, st, timed, q, work4 / n_all)
+             : kmode == kWalkWrite
+                 ? LaunchWalk<4, kWalkWrite>(ctx, ws, st, timed, q, work4 / n_all)
+                 : LaunchWalk<4, kWalkSchedule>(ctx, ws, st, timed, q, work4 / n_all);
+        if (lr != HIPR_SUCCESS) return lr;
+      }
+      if (side) {
+        HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
+        HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
+      }
+      return HIPR_SUCCESS;
+    };
+    if (!use_graph) return launches();
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    rr = launches();
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(st, &g);
+    if (g) ctx->graphs.push_back(g);
+    if (rr != HIPR_SUCCESS) return rr;
+    HIP_TRY(ce);
+    hipGraphExec_t ge = nullptr;
+    HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    ctx->graph_execs.push_back(ge);
+    const int stage = kmode == kWalkCount   ? HIPR_STAGE_WALK_COUNT
+                      : kmode == kWalkWrite ? HIPR_STAGE_WALK
+                                            : HIPR_STAGE_WALK_SCHEDULE;
+    hipEvent_t b = StageBegin(ctx, st);
+    HIP_TRY(hipGraphLaunch(ge, st));
+    StageEnd(ctx, stage, b, st);
     return HIPR_SUCCESS;
   };
   // A 4-slot job whose list still outgrew 256 edges (flag 2) is counted again
@@ -1656,7 +1743,10 @@ HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out) {
   hipr_context ctx = new (std::nothrow) hipr_context_s();
   if (!ctx) return HIPR_ERROR_OUT_OF_HOST_MEMORY;
   ctx->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+  if (hipSetDevice(device) != hipSuccess ||
+      hipDeviceGetAttribute(&ctx->n_cus, hipDeviceAttributeMultiprocessorCount, device) !=
+          hipSuccess ||
+      hipStreamCreate(&ctx->stream) != hipSuccess ||
       hipStreamCreate(&ctx->side) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
@@ -1690,7 +1780,8 @@ static void FreeContext(hipr_context ctx) {
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
                     &ctx->d_jobs, &ctx->run_jobs, &ctx->counters, &ctx->wcov,
                     &ctx->qstate, &ctx->q_base, &ctx->snaps, &ctx->snap_base,
-                    &ctx->band_count};
+                    &ctx->band_count, &ctx->queues};
+  ReleaseGraphs(ctx);
   for (DevBuf* b : bufs) b->Free();
   DevBuf* mlp[] = {&ctx->mlp_prm, &ctx->mlp_ops, &ctx->mlp_feats, &ctx->mlp_preds,
                    &ctx->mlp_targets, &ctx->mlp_grads, &ctx->mlp_loss, &ctx->mlp_m,
@@ -2075,20 +2166,24 @@ HIPR_API hipr_result hipr_batch_draw(hipr_batch b, const hipr_options* options) 
   if (options) opt = *options;
   if (opt.band_rows == 0) opt.band_rows = b->tuned_band_rows;
   if (opt.band_min_edges == 0) opt.band_min_edges = b->tuned_band_min_edges;
+  if (opt.flags == 0) opt.flags = b->tuned_flags;
   std::vector<hipr_surface> targets;
   for (const auto& pr : b->pairs) targets.push_back(pr.first);
   return RunPipeline(targets, &b->sink, &b->merged, &opt);
 }
 
-HIPR_API hipr_result hipr_batch_set_split(hipr_batch b, int32_t band_rows, int32_t band_min_edges) {
+HIPR_API hipr_result hipr_batch_set_split(hipr_batch b, int32_t band_rows, int32_t band_min_edges,
+                                          int32_t flags) {
   if (!b) return HIPR_ERROR_INVALID_HANDLE;
   b->tuned_band_rows = band_rows;
   b->tuned_band_min_edges = band_min_edges;
+  b->tuned_flags = flags;
   return HIPR_SUCCESS;
 }
 
 HIPR_API hipr_result hipr_batch_autotune(hipr_batch b, int32_t mode, int32_t repeats,
-                                         int32_t* out_band_rows, int32_t* out_band_min_edges) {
+                                         int32_t* out_band_rows, int32_t* out_band_min_edges,
+                                         int32_t* out_flags) {
   if (!b) return HIPR_ERROR_INVALID_HANDLE;
   if (repeats < 1 || repeats > 100 || mode < HIPR_MODE_EXACT || mode > HIPR_MODE_LEARNED ||
       mode == HIPR_MODE_ESTIMATES)
@@ -2100,14 +2195,16 @@ HIPR_API hipr_result hipr_batch_autotune(hipr_batch b, int32_t mode, int32_t rep
   const int32_t rows[] = {-1, 64, 128, 256};
   const int32_t mins[] = {INT32_MAX, 4 * kWave + 1, -1};
   struct Cand {
-    int32_t rows, min_edges;
+    int32_t rows, min_edges, flags;
   };
   std::vector<Cand> cands;
-  for (int32_t r : rows) {
-    if (mode == HIPR_MODE_EXACT || r < 0) {
-      cands.push_back({r, 0});
-    } else {
-      for (int32_t m : mins) cands.push_back({r, m});
+  for (int32_t f : {0, (int32_t)HIPR_FLAG_PERSISTENT}) {
+    for (int32_t r : rows) {
+      if (mode == HIPR_MODE_EXACT || r < 0) {
+        cands.push_back({r, 0, f});
+      } else {
+        for (int32_t m : mins) cands.push_back({r, m, f});
+      }
     }
   }
   double best_ms = 1e30;
@@ -2117,6 +2214,7 @@ HIPR_API hipr_result hipr_batch_autotune(hipr_batch b, int32_t mode, int32_t rep
     opt.mode = mode;
     opt.band_rows = c.rows;
     opt.band_min_edges = c.min_edges;
+    opt.flags = c.flags;
     double t = 1e30;
     for (int k = 0; k <= repeats; k++) {
       hipr_result r = hipr_batch_draw(b, &opt);
@@ -2130,8 +2228,10 @@ HIPR_API hipr_result hipr_batch_autotune(hipr_batch b, int32_t mode, int32_t rep
   }
   b->tuned_band_rows = best.rows;
   b->tuned_band_min_edges = best.min_edges;
+  b->tuned_flags = best.flags;
   if (out_band_rows) *out_band_rows = best.rows;
   if (out_band_min_edges) *out_band_min_edges = best.min_edges;
+  if (out_flags) *out_flags = best.flags;
   return HIPR_SUCCESS;
 }
 

@@ -71,9 +71,11 @@ def lib():
         L.hipr_batch_clear.restype = ctypes.c_int
         L.hipr_batch_draw.argtypes = [c_batch, ctypes.POINTER(abi.Options)]
         L.hipr_batch_draw.restype = ctypes.c_int
-        L.hipr_batch_set_split.argtypes = [c_batch, ctypes.c_int32, ctypes.c_int32]
+        L.hipr_batch_set_split.argtypes = [c_batch, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int32]
         L.hipr_batch_set_split.restype = ctypes.c_int
         L.hipr_batch_autotune.argtypes = [c_batch, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_int32),
                                           ctypes.POINTER(ctypes.c_int32),
                                           ctypes.POINTER(ctypes.c_int32)]
         L.hipr_batch_autotune.restype = ctypes.c_int
@@ -138,8 +140,9 @@ class Context:
 
 
 def _options(mode, edge_estimates, tile_estimates, blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0,
-             band_min_edges=0):
+             band_min_edges=0, flags=0):
     opt = abi.Options()
+    opt.flags = flags
     opt.mode = mode
     opt.blur_impl = blur_impl
     opt.band_rows = band_rows
@@ -193,13 +196,13 @@ class Surface:
         check(lib().hipr_surface_clear(self._h, rgba), "hipr_surface_clear")
 
     def draw(self, scene, mode=0, edge_estimates=None, tile_estimates=None,
-             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0):
+             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0, flags=0):
         """Draws a scene (a Scene pointer or a Prepared scene) onto the surface.
         band_rows: rows per walk band (0 = the library default, < 0 = none);
         band_min_edges: in the estimate modes, band only paths whose active
-        edge bound reaches this (0 = default, < 0 = all)."""
+        edge bound reaches this (0 = default, < 0 = all); flags: FLAG_*."""
         opt, keep = _options(mode, edge_estimates, tile_estimates, blur_impl, band_rows,
-                             band_min_edges)
+                             band_min_edges, flags)
         if isinstance(scene, Prepared):
             check(lib().hipr_surface_draw_prepared(self._h, scene.handle, ctypes.byref(opt)),
                   "hipr_surface_draw_prepared")
@@ -266,23 +269,24 @@ class Batch:
         check(lib().hipr_batch_clear(self._h), "hipr_batch_clear")
         self._pairs = []
 
-    def set_split(self, band_rows, band_min_edges):
-        check(lib().hipr_batch_set_split(self._h, band_rows, band_min_edges),
+    def set_split(self, band_rows, band_min_edges, flags=0):
+        check(lib().hipr_batch_set_split(self._h, band_rows, band_min_edges, flags),
               "hipr_batch_set_split")
 
     def autotune(self, mode, repeats=2):
         """Picks the fastest work split for `mode` (hipr_batch_autotune);
-        returns (band_rows, band_min_edges).  Leaves the targets' contents
-        undefined."""
-        r, m = ctypes.c_int32(), ctypes.c_int32()
-        check(lib().hipr_batch_autotune(self._h, mode, repeats, ctypes.byref(r), ctypes.byref(m)),
+        returns (band_rows, band_min_edges, flags).  Leaves the targets'
+        contents undefined."""
+        r, m, f = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().hipr_batch_autotune(self._h, mode, repeats, ctypes.byref(r), ctypes.byref(m),
+                                        ctypes.byref(f)),
               "hipr_batch_autotune")
-        return r.value, m.value
+        return r.value, m.value, f.value
 
     def draw(self, mode=0, edge_estimates=None, tile_estimates=None,
-             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0):
+             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0, flags=0):
         opt, keep = _options(mode, edge_estimates, tile_estimates, blur_impl, band_rows,
-                             band_min_edges)
+                             band_min_edges, flags)
         check(lib().hipr_batch_draw(self._h, ctypes.byref(opt)), "hipr_batch_draw")
         del keep
 

@@ -168,3 +168,21 @@ def test_row_bands_with_estimates(ctx, band_rows, band_min_edges):
     assert np.array_equal(ample, ref)
     assert st_s["fallbacks"] > 0
     assert st_a["fallbacks"] == 0
+
+
+@pytest.mark.parametrize("flags", [abi.FLAG_NO_GRAPH, abi.FLAG_PERSISTENT,
+                                   abi.FLAG_PERSISTENT | abi.FLAG_NO_GRAPH])
+def test_walk_launch_forms_match(ctx, flags):
+    """The walk passes drawn as hipGraphs (default), as direct launches, and
+    as persistent kernels on an atomic item queue give the same pixels."""
+    sc = workload(2, 40, seed=33)
+    n = sc.n_paths
+    ref, st_ref = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=0)
+    got, st = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=0, flags=flags)
+    assert np.array_equal(got, ref)
+    assert st["tile_planes"] == st_ref["tile_planes"]
+    est, _ = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=1, flags=flags,
+                        band_min_edges=-1,
+                        edge_estimates=np.full(n, 3000, np.int32),
+                        tile_estimates=np.full(n, 60, np.int32))
+    assert np.array_equal(est, ref)
