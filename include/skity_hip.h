@@ -162,7 +162,7 @@ typedef struct hipr_options {
                                      < 0 = one band per path */
   int32_t band_min_edges;         /* estimate modes: band only the paths whose
                                      active-edge bound reaches this; 0 = default
-                                     (no banding), < 0 = every path */
+                                     (none), < 0 = every path */
   int32_t flags;                  /* HIPR_FLAG_* */
 } hipr_options;
 enum {

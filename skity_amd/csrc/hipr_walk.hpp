@@ -1260,6 +1260,10 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
 template <int K, int kMode>
 __global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
   extern __shared__ int32_t lds[];
+  // The 8- and 16-slot walks hold the paths with the widest active lists,
+  // the longest sequential walks of a frame: their waves take precedence in
+  // instruction issue over the 4-slot waves sharing their SIMD.
+  if (K >= 8) __builtin_amdgcn_s_setprio(2);
   if (!P.queue) {
     if ((int)blockIdx.x < P.n_walk) WalkItem<K, kMode>(P, blockIdx.x, lds);
     return;

@@ -241,8 +241,9 @@ struct hipr_context_s {
   int refs = 1;  // the handle itself + live surfaces and prepared scenes
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;  // wide walk jobs run here, beside the main stream
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t side = nullptr;  // walk tiers run here and on side2, beside the main stream
+  hipStream_t side2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // per-draw device buffers (grow-only)
   DevBuf canvases, layer_canvases, geo, cov, edge_base, edge_cap, edges, wedges, gkeys;
@@ -631,12 +632,18 @@ void SetTiles(std::vector<Canvas>* cvs) {
 // Nominal rows per walk band (the write pass runs the bands of a path
 // concurrently).
 constexpr int kBandRows = 128;
+// Estimate modes: paths whose active-edge bound reaches this are banded.
+// Off by default: the schedule pass banding needs costs more than the
+// banded write saves on a full chip (measured on curves + strokes, 4096
+// paths); the autotuner tries thresholds.
+constexpr int kBandMinEdges = INT32_MAX;
 
 template <int K, int kMode>
 hipr_result LaunchWalk(hipr_context ctx, WalkParams P, hipStream_t st, bool timed,
                        int32_t* queue, double share) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
-  size_t lds = (size_t)((P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 6 * K * kWave) *
+  // row deltas, two tile-slot rows, five scratch arrays of K * 64 entries
+  size_t lds = (size_t)((P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 5 * K * kWave) *
                sizeof(int32_t);
   unsigned grid = (unsigned)P.n_walk;
   P.queue = queue;
@@ -1044,12 +1051,12 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   // Row bands: each job's rows are split into bands walked concurrently by
   // the write pass; the count or schedule pass records where they start.
   const int band_rows = opt && opt->band_rows != 0 ? std::max(opt->band_rows, 0) : kBandRows;
-  // In the estimate modes banding costs a schedule pass, more than the write
-  // pass gains on a full chip, so it is off unless asked for.
+  // In the estimate modes banding costs a schedule pass, so only the paths
+  // with the widest active lists (the longest walks) are banded.
   const int band_min =
       mode == 0 ? 0
       : (opt && opt->band_min_edges != 0) ? std::max(opt->band_min_edges, 0)
-                                          : INT32_MAX;
+                                          : kBandMinEdges;
   std::vector<int32_t> nbands(nj, 1);
   std::vector<int64_t> qb(nj, 0), sb(nj, 0);
   int64_t q_total = 0, snap_total = 0;
@@ -1147,88 +1154,78 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   const int run_flags = opt ? opt->flags : 0;
   const bool use_graph = (run_flags & HIPR_FLAG_NO_GRAPH) == 0;
   const bool persistent = (run_flags & HIPR_FLAG_PERSISTENT) != 0;
-  if (persistent && ctx->queues.Ensure(4 * sizeof(int32_t)) != hipSuccess)
-    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-  auto walk_pass = [&](int kmode, const std::vector<int32_t>& sm,
-                       const std::vector<int32_t>& wd) -> hipr_result {
+  // A launch plan: walk kernels, each on one of three streams (0 = main);
+  // launches on one stream run in order, the streams concurrently.  The item
+  // lists are uploaded first; the launches (queue reset, fork, kernels, join)
+  // are captured into a hipGraph and launched as one unless the caller asks
+  // for direct launches.
+  struct WalkLaunch {
+    int kmode, K, stream;
+    std::vector<int32_t> jobs;
+  };
+  auto run_plan = [&](const std::vector<WalkLaunch>& plan) -> hipr_result {
     hipr_result rr;
     ReleaseGraphs(ctx);
-    WalkParams ws = wp, w8p = wp, w16p = wp;
-    std::vector<int32_t> ism = make_items(sm, kmode);
-    std::vector<int32_t> j8, j16;
-    for (int32_t j : wd) (tier[j] <= 8 ? j8 : j16).push_back(j);
-    std::vector<int32_t> w8 = make_items(j8, kmode), w16 = make_items(j16, kmode);
-    if (!ism.empty()) {
-      if ((rr = Upload(&ctx->sub_list, ism.data(), ism.size(), st)) != HIPR_SUCCESS) return rr;
-      ws.walk_list = ctx->sub_list.as<int32_t>();
-      ws.n_walk = (int32_t)ism.size();
-    }
-    if (!w8.empty() || !w16.empty()) {
-      std::vector<int32_t> ordered = w8;
-      ordered.insert(ordered.end(), w16.begin(), w16.end());
-      if ((rr = Upload(&ctx->big_list, ordered.data(), ordered.size(), st)) != HIPR_SUCCESS)
-        return rr;
-      w8p.walk_list = ctx->big_list.as<int32_t>();
-      w8p.n_walk = (int32_t)w8.size();
-      w16p.walk_list = ctx->big_list.as<int32_t>() + w8.size();
-      w16p.n_walk = (int32_t)w16.size();
-    }
-    // persistent form: each tier's share of the resident waves follows its
-    // estimated work (active-edge bound x rows per item)
-    auto tier_work = [&](const std::vector<int32_t>& js) {
+    std::vector<int32_t> all_items;
+    std::vector<size_t> off, cnt;
+    std::vector<double> work;
+    double work_all = 0;
+    for (const WalkLaunch& l : plan) {
+      std::vector<int32_t> it = make_items(l.jobs, l.kmode);
+      off.push_back(all_items.size());
+      cnt.push_back(it.size());
+      all_items.insert(all_items.end(), it.begin(), it.end());
       double w = 0;
-      for (int32_t j : js) {
-        PixBounds pb = JobPixBounds(ctx->h_geo[j]);
-        w += (double)std::max(bound[j], 1) * std::max(pb.b - pb.t, 1);
+      if (persistent) {
+        for (int32_t j : l.jobs) {
+          PixBounds pb = JobPixBounds(ctx->h_geo[j]);
+          w += (double)std::max(bound[j], 1) * std::max(pb.b - pb.t, 1);
+        }
       }
-      return w;
-    };
-    const double work4 = persistent ? tier_work(sm) : 0, work8 = persistent ? tier_work(j8) : 0,
-                 work16 = persistent ? tier_work(j16) : 0;
-    const double n_all = std::max(work4 + work8 + work16, 1.0);
+      work.push_back(w);
+      work_all += w;
+    }
+    if (all_items.empty()) return HIPR_SUCCESS;
+    if ((rr = Upload(&ctx->sub_list, all_items.data(), all_items.size(), st)) != HIPR_SUCCESS)
+      return rr;
+    if (persistent && ctx->queues.Ensure(sizeof(int32_t) * (plan.size() + 1)) != hipSuccess)
+      return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
     int32_t* q = persistent ? ctx->queues.as<int32_t>() : nullptr;
+    hipStream_t streams[3] = {st, ctx->side, ctx->side2};
+    hipEvent_t joins[3] = {nullptr, ctx->ev_join, ctx->ev_join2};
+    bool used[3] = {true, false, false};
+    for (size_t i = 0; i < plan.size(); i++)
+      if (cnt[i]) used[plan[i].stream] = true;
     const bool timed = !use_graph;
     auto launches = [&]() -> hipr_result {
-      hipr_result lr;
-      if (q) hipLaunchKernelGGL(FillKernel, dim3(1), dim3(kWave), 0, st, q, (int64_t)3, 0);
-      const bool side = !w8.empty() || !w16.empty();
-      if (side) {
-        HIP_TRY(hipEventRecord(ctx->ev_fork, st));
-        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-      }
-      if (!w8.empty()) {
-        lr = kmode == kWalkCount
-                 ? LaunchWalk<8, kWalkCount>(ctx, w8p, ctx->side, timed, q ? q + 1 : nullptr,
-                                             work8 / n_all)
-             : kmode == kWalkWrite
-                 ? LaunchWalk<8, kWalkWrite>(ctx, w8p, ctx->side, timed, q ? q + 1 : nullptr,
-                                             work8 / n_all)
-                 : LaunchWalk<8, kWalkSchedule>(ctx, w8p, ctx->side, timed,
-                                                q ? q + 1 : nullptr, work8 / n_all);
+      if (q)
+        hipLaunchKernelGGL(FillKernel, dim3(1), dim3(kWave), 0, st, q, (int64_t)plan.size(), 0);
+      HIP_TRY(hipEventRecord(ctx->ev_fork, st));
+      for (int k = 1; k < 3; k++)
+        if (used[k]) HIP_TRY(hipStreamWaitEvent(streams[k], ctx->ev_fork, 0));
+      for (size_t i = 0; i < plan.size(); i++) {
+        if (!cnt[i]) continue;
+        const WalkLaunch& l = plan[i];
+        WalkParams w = wp;
+        w.walk_list = ctx->sub_list.as<int32_t>() + off[i];
+        w.n_walk = (int32_t)cnt[i];
+        w.cov = l.kmode == kWalkWrite ? ctx->wcov.as<JobCov>() : ctx->cov.as<JobCov>();
+        int32_t* qi = q ? q + i : nullptr;
+        double share = work_all > 0 ? work[i] / work_all : 1.0;
+        hipStream_t ls = streams[l.stream];
+        hipr_result lr;
+#define HIPR_WALK_K(KK)                                                             \
+  (l.kmode == kWalkCount   ? LaunchWalk<KK, kWalkCount>(ctx, w, ls, timed, qi, share) \
+   : l.kmode == kWalkWrite ? LaunchWalk<KK, kWalkWrite>(ctx, w, ls, timed, qi, share) \
+                           : LaunchWalk<KK, kWalkSchedule>(ctx, w, ls, timed, qi, share))
+        lr = l.K == 4 ? HIPR_WALK_K(4) : l.K == 8 ? HIPR_WALK_K(8) : HIPR_WALK_K(16);
+#undef HIPR_WALK_K
         if (lr != HIPR_SUCCESS) return lr;
       }
-      if (!w16.empty()) {
-        lr = kmode == kWalkCount
-                 ? LaunchWalk<16, kWalkCount>(ctx, w16p, ctx->side, timed, q ? q + 2 : nullptr,
-                                              work16 / n_all)
-             : kmode == kWalkWrite
-                 ? LaunchWalk<16, kWalkWrite>(ctx, w16p, ctx->side, timed, q ? q + 2 : nullptr,
-                                              work16 / n_all)
-                 : LaunchWalk<16, kWalkSchedule>(ctx, w16p, ctx->side, timed,
-                                                 q ? q + 2 : nullptr, work16 / n_all);
-        if (lr != HIPR_SUCCESS) return lr;
-      }
-      if (!ism.empty()) {
-        lr = kmode == kWalkCount
-                 ? LaunchWalk<4, kWalkCount>(ctx, ws, st, timed, q, work4 / n_all)
-             : kmode == kWalkWrite
-                 ? LaunchWalk<4, kWalkWrite>(ctx, ws, st, timed, q, work4 / n_all)
-                 : LaunchWalk<4, kWalkSchedule>(ctx, ws, st, timed, q, work4 / n_all);
-        if (lr != HIPR_SUCCESS) return lr;
-      }
-      if (side) {
-        HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
-        HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
+      for (int k = 1; k < 3; k++) {
+        if (!used[k]) continue;
+        HIP_TRY(hipEventRecord(joins[k], streams[k]));
+        HIP_TRY(hipStreamWaitEvent(st, joins[k], 0));
       }
       return HIPR_SUCCESS;
     };
@@ -1243,13 +1240,25 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     hipGraphExec_t ge = nullptr;
     HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     ctx->graph_execs.push_back(ge);
-    const int stage = kmode == kWalkCount   ? HIPR_STAGE_WALK_COUNT
-                      : kmode == kWalkWrite ? HIPR_STAGE_WALK
-                                            : HIPR_STAGE_WALK_SCHEDULE;
+    int stage = HIPR_STAGE_WALK;
+    for (const WalkLaunch& l : plan)
+      if (l.kmode != kWalkWrite)
+        stage = l.kmode == kWalkCount ? HIPR_STAGE_WALK_COUNT : HIPR_STAGE_WALK_SCHEDULE;
+    bool any_write = false;
+    for (const WalkLaunch& l : plan) any_write |= l.kmode == kWalkWrite;
+    if (any_write) stage = HIPR_STAGE_WALK;  // a pass that writes planes counts as the walk
     hipEvent_t b = StageBegin(ctx, st);
     HIP_TRY(hipGraphLaunch(ge, st));
     StageEnd(ctx, stage, b, st);
     return HIPR_SUCCESS;
+  };
+  // One walk pass of one mode: the 4-slot jobs on the main stream, the 8-
+  // and 16-slot jobs on a side stream.
+  auto walk_pass = [&](int kmode, const std::vector<int32_t>& sm,
+                       const std::vector<int32_t>& wd) -> hipr_result {
+    std::vector<int32_t> j8, j16;
+    for (int32_t j : wd) (tier[j] <= 8 ? j8 : j16).push_back(j);
+    return run_plan({{kmode, 8, 1, j8}, {kmode, 16, 1, j16}, {kmode, 4, 0, sm}});
   };
   // A 4-slot job whose list still outgrew 256 edges (flag 2) is counted again
   // by the 16-slot walk (recording its snapshots there); the bound makes this
@@ -1306,32 +1315,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     if ((r = move_overflowed(ctx->h_cov, nullptr)) != HIPR_SUCCESS) return r;
     for (int32_t j : all) pcap[j] = ctx->h_cov[j].planes;
   } else {
-    // schedule pass (snapshots only) over the jobs with more than one band;
     // planes from the estimates
-    std::vector<int32_t> ssm, swd, sall;
-    for (int32_t j : small)
-      if (nbands[j] > 1) ssm.push_back(j);
-    for (int32_t j : wide)
-      if (nbands[j] > 1) swd.push_back(j);
-    sall = ssm;
-    sall.insert(sall.end(), swd.begin(), swd.end());
-    if (!sall.empty()) {
-      if ((r = walk_pass(kWalkSchedule, ssm, swd)) != HIPR_SUCCESS) return r;
-      if ((r = read_cov(&ctx->cov, ctx->h_cov)) != HIPR_SUCCESS) return r;
-      if ((r = limit_error(ctx->h_cov, swd, 2)) != HIPR_SUCCESS) return r;
-      if ((r = limit_error(ctx->h_cov, sall, 16 | 32)) != HIPR_SUCCESS) return r;
-      // (only scheduled jobs carry schedule-pass flags)
-      std::vector<int32_t> saved_small = small;
-      small = ssm;
-      std::vector<int32_t> moved;
-      if ((r = move_overflowed(ctx->h_cov, &moved)) != HIPR_SUCCESS) return r;
-      std::vector<int32_t> rest;
-      for (int32_t j : saved_small)
-        if (nbands[j] <= 1 || tier[j] == 4) rest.push_back(j);
-      small = rest;
-      // moved jobs were counted exactly; their estimate is replaced below
-      for (int32_t j : moved) test[j] = std::max(test[j], ctx->h_cov[j].planes);
-    }
     for (int32_t j : all) pcap[j] = test[j];
   }
   assign_planes(all);
@@ -1343,7 +1327,32 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   wp.planes = ctx->planes.as<uint8_t>();
   wp.cov = ctx->wcov.as<JobCov>();
   if ((r = prepare_write(nullptr)) != HIPR_SUCCESS) return r;
-  if ((r = walk_pass(kWalkWrite, small, wide)) != HIPR_SUCCESS) return r;
+  if (mode == 0) {
+    if ((r = walk_pass(kWalkWrite, small, wide)) != HIPR_SUCCESS) return r;
+  } else {
+    // The banded (longest) walks need a schedule pass before their write
+    // pass; both run on side streams while the main stream writes the other
+    // jobs, so the schedule pass overlaps the bulk of the writing.
+    std::vector<int32_t> b4, u4, b8, u8, b16, u16;
+    for (int32_t j : small) (nbands[j] > 1 ? b4 : u4).push_back(j);
+    for (int32_t j : wide) {
+      if (tier[j] <= 8) {
+        (nbands[j] > 1 ? b8 : u8).push_back(j);
+      } else {
+        (nbands[j] > 1 ? b16 : u16).push_back(j);
+      }
+    }
+    if ((r = run_plan({{kWalkSchedule, 8, 2, b8},
+                       {kWalkWrite, 8, 2, b8},
+                       {kWalkSchedule, 16, 2, b16},
+                       {kWalkWrite, 16, 2, b16},
+                       {kWalkWrite, 8, 2, u8},
+                       {kWalkWrite, 16, 2, u16},
+                       {kWalkSchedule, 4, 1, b4},
+                       {kWalkWrite, 4, 1, b4},
+                       {kWalkWrite, 4, 0, u4}})) != HIPR_SUCCESS)
+      return r;
+  }
   if ((r = read_cov(&ctx->wcov, hw)) != HIPR_SUCCESS) return r;
   if (mode == 0) {
     if ((r = limit_error(hw, all, 1 | 2 | 16 | 32)) != HIPR_SUCCESS) return r;
@@ -1747,9 +1756,10 @@ HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out) {
       hipDeviceGetAttribute(&ctx->n_cus, hipDeviceAttributeMultiprocessorCount, device) !=
           hipSuccess ||
       hipStreamCreate(&ctx->stream) != hipSuccess ||
-      hipStreamCreate(&ctx->side) != hipSuccess ||
+      hipStreamCreate(&ctx->side) != hipSuccess || hipStreamCreate(&ctx->side2) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join2, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     delete ctx;
     return HIPR_ERROR_INITIALIZATION_FAILED;
@@ -1797,7 +1807,10 @@ static void FreeContext(hipr_context ctx) {
   (void)hipEventDestroy(ctx->ev1);
   (void)hipEventDestroy(ctx->ev_fork);
   (void)hipEventDestroy(ctx->ev_join);
+  (void)hipEventDestroy(ctx->ev_join2);
+  (void)hipStreamSynchronize(ctx->side2);
   (void)hipStreamDestroy(ctx->side);
+  (void)hipStreamDestroy(ctx->side2);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -2193,7 +2206,7 @@ HIPR_API hipr_result hipr_batch_autotune(hipr_batch b, int32_t mode, int32_t rep
   // banded.  Each candidate is drawn once to warm up, then `repeats` times;
   // the device time of its fastest draw decides.
   const int32_t rows[] = {-1, 64, 128, 256};
-  const int32_t mins[] = {INT32_MAX, 4 * kWave + 1, -1};
+  const int32_t mins[] = {INT32_MAX, 4 * kWave + 1, 3 * kWave + 1, 2 * kWave + 1, -1};
   struct Cand {
     int32_t rows, min_edges, flags;
   };
