@@ -142,6 +142,9 @@ typedef struct hipr_stats {
   double ms_total;        /* device time of the whole draw                  */
   int64_t blends;         /* pixel blend operations on the target (F)       */
   int64_t pixels;         /* distinct target pixels touched (U)             */
+  int64_t underestimates; /* estimate modes: paths whose tile planes
+                             exceeded their estimate (served from the shared
+                             overflow pool, or re-run when it ran out)      */
 } hipr_stats;
 
 /* Execution options.  mode 0 = plain exact two-pass (count, scan, write);

@@ -92,6 +92,7 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("blends", ctypes.c_int64),
         ("pixels", ctypes.c_int64),
+        ("underestimates", ctypes.c_int64),
     ]
 
 

@@ -154,7 +154,6 @@ struct CompositeParams {
   const int64_t* map_base;
   const int32_t* map0;
   const int32_t* map1;
-  const int64_t* plane_base;
   const uint8_t* planes;
   const PaintDev* paints;
   const GradDev* grads;
@@ -263,12 +262,11 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
         }
         continue;
       }
-      int64_t pbase = P.plane_base[jj];
-      const uint32_t* pl0 =
-          (const uint32_t*)(P.planes + (pbase + q0) * kTilePlaneBytes) + lane;
+      // the maps hold absolute plane numbers
+      const uint32_t* pl0 = (const uint32_t*)(P.planes + (int64_t)q0 * kTilePlaneBytes) + lane;
       uint32_t cov0 = *pl0;
       uint32_t cov1 = 0;
-      if (q1 >= 0) cov1 = *((const uint32_t*)(P.planes + (pbase + q1) * kTilePlaneBytes) + lane);
+      if (q1 >= 0) cov1 = *((const uint32_t*)(P.planes + (int64_t)q1 * kTilePlaneBytes) + lane);
       if (pd.kind == kPaintSolid) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {

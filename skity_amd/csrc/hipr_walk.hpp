@@ -30,6 +30,8 @@
 namespace hipr {
 
 constexpr int kTilePlaneBytes = kTile * kTile;
+// Tile slot of an allocation that found no plane (capacity and pool spent).
+constexpr int32_t kNoPlane = 0x7FFFFFFF;
 
 // Walk modes: count the tile planes, write them, or only follow the step
 // schedule (no coverage) to record the row-band snapshots.
@@ -67,9 +69,14 @@ struct WalkParams {
   const int64_t* snap_base; // per job: byte offset of band 1's snapshot
   const int32_t* band_count;  // per job: row bands (1 = one sequential walk)
   const Canvas* canvases;
-  const int64_t* plane_base;
+  const int64_t* plane_base;  // per job: first plane of its estimated range
   const int32_t* plane_cap;
   uint8_t* planes;
+  // Shared overflow pool (estimate modes): a job's planes beyond its
+  // capacity come from planes [pool_base, pool_base + pool_cap), taken with
+  // an atomic counter (starts at 0).  Maps hold absolute plane numbers.
+  int32_t* pool_ctr;
+  int32_t pool_base, pool_cap;
   const int64_t* map_base;
   int32_t* map0;
   int32_t* map1;
@@ -192,7 +199,7 @@ __device__ void BitonicSortSlots(int64_t (&key)[K], int N, int lane) {
 // a job run concurrently.  Count and schedule passes: one wave per job, band
 // 0, walking every row and recording the band snapshots.
 template <int K, int kMode>
-__device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
+__device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
   constexpr int CAP = K * kWave;
   constexpr bool kWrite = kMode == kWalkWrite;
   constexpr bool kBlit = kMode != kWalkSchedule;
@@ -215,8 +222,20 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
   if (pb.empty || tr.ntc == 0 || nE == 0 || W > P.lds_width) {
     if (!pb.empty && W > P.lds_width) flags |= 16;
     if (lane == 0 && rband == 0) {
-      JobCov c{pb.l, pb.t, pb.r, pb.b, 0, 0, 0, flags, 0, 0, {0, 0, 0, 0, 0}};
-      P.cov[j] = c;
+      // field by field: a struct copy would go through scratch memory
+      JobCov* c = &P.cov[j];
+      c->left = pb.l;
+      c->top = pb.t;
+      c->right = pb.r;
+      c->bottom = pb.b;
+      c->tiles = 0;
+      c->planes = 0;
+      c->max_active = 0;
+      c->flags = flags;
+      c->steps = 0;
+      c->time_10ns = 0;
+#pragma unroll
+      for (int k = 0; k < 5; k++) c->phase[k] = 0;
     }
     return;
   }
@@ -290,9 +309,9 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
       int s0 = L.slot0[c], s1 = L.slot1[c];
       // Only planes inside this job's allocation are published; an overflow
       // is reported through flags and the job is redone by the exact path.
-      if (kWrite && s0 >= 0 && s0 < pcap) {
+      if (kWrite && s0 >= 0 && s0 != kNoPlane) {
         map0[(int64_t)trow * tr.ntc + c] = s0;
-        map1[(int64_t)trow * tr.ntc + c] = s1 < pcap ? s1 : -1;
+        map1[(int64_t)trow * tr.ntc + c] = (s1 >= 0 && s1 != kNoPlane) ? s1 : -1;
       }
       L.slot0[c] = -1;
       L.slot1[c] = -1;
@@ -382,11 +401,31 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
       }
       int first_slot = slot_base + rincl - req;
       next_slot += rtot;
+      // Count pass: tile slots are job-relative.  Write pass: absolute plane
+      // numbers, from the job's range or (past its capacity) from the pool.
+      int a0 = first_slot, a1 = first_slot + (need0 ? 1 : 0);
+      if (kWrite) {
+        const int over_from = slot_base > pcap ? slot_base : pcap;
+        const int over = slot_base + rtot - over_from;
+        int pool0 = 0;
+        if (over > 0) {
+          int got = 0;
+          if (lane == 0) got = atomicAdd(P.pool_ctr, over);
+          pool0 = __builtin_amdgcn_readlane(got, 0);
+        }
+        auto to_abs = [&](int sl) -> int {
+          if (sl < pcap) return (int)(pbase + sl);
+          int pi = pool0 + (sl - over_from);
+          return pi < P.pool_cap ? P.pool_base + pi : kNoPlane;
+        };
+        a0 = to_abs(a0);
+        a1 = to_abs(a1);
+      }
       if (need0) {
-        L.slot0[tcol] = first_slot;
+        L.slot0[tcol] = a0;
         n_tiles++;
       }
-      if (need1) L.slot1[tcol] = first_slot + (need0 ? 1 : 0);
+      if (need1) L.slot1[tcol] = a1;
       WaveSync();
       cur0 = in_cols ? L.slot0[tcol] : -1;
       cur1 = in_cols ? L.slot1[tcol] : -1;
@@ -396,29 +435,25 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
         bool new0 = __shfl(need0 ? 1 : 0, g * 4, kWave) != 0;
         bool new1 = __shfl(need1 ? 1 : 0, g * 4, kWave) != 0;
         if (in_cols) {
-          if (new0 && cur0 >= 0) {
-            if (cur0 < pcap) {
-              uint32_t* pl = (uint32_t*)(planes + (pbase + cur0) * kTilePlaneBytes);
-              for (int k = 0; k < 16; k++) pl[k * 4 + sub] = 0;
-            }
+          if (new0 && cur0 >= 0 && cur0 != kNoPlane) {
+            uint32_t* pl = (uint32_t*)(planes + (int64_t)cur0 * kTilePlaneBytes);
+            for (int k = 0; k < 16; k++) pl[k * 4 + sub] = 0;
           }
-          if (new1 && cur1 >= 0) {
-            if (cur1 < pcap) {
-              uint32_t* pl = (uint32_t*)(planes + (pbase + cur1) * kTilePlaneBytes);
-              for (int k = 0; k < 16; k++) pl[k * 4 + sub] = 0;
-            }
+          if (new1 && cur1 >= 0 && cur1 != kNoPlane) {
+            uint32_t* pl = (uint32_t*)(planes + (int64_t)cur1 * kTilePlaneBytes);
+            for (int k = 0; k < 16; k++) pl[k * 4 + sub] = 0;
           }
           if (chunk0 && cur0 >= 0) {
-            if (cur0 < pcap) {
-              uint32_t* pl = (uint32_t*)(planes + (pbase + cur0) * kTilePlaneBytes);
+            if (cur0 != kNoPlane) {
+              uint32_t* pl = (uint32_t*)(planes + (int64_t)cur0 * kTilePlaneBytes);
               pl[rowk * 4 + sub] = c0;
             } else {
               flags |= 1;
             }
           }
           if (chunk1 && cur1 >= 0) {
-            if (cur1 < pcap) {
-              uint32_t* pl = (uint32_t*)(planes + (pbase + cur1) * kTilePlaneBytes);
+            if (cur1 != kNoPlane) {
+              uint32_t* pl = (uint32_t*)(planes + (int64_t)cur1 * kTilePlaneBytes);
               pl[rowk * 4 + sub] = c1;
             } else {
               flags |= 1;
@@ -671,8 +706,8 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
       Fixed nxo = ShiftDown(xo, xo), ndx = ShiftDown(DX[s], DX[s]);
       int next_p = p + 1;
       if (s + 1 < K) {
-        Fixed n0x = __shfl(X[s + 1 < K ? s + 1 : s], 0, kWave);
-        Fixed n0dx = __shfl(DX[s + 1 < K ? s + 1 : s], 0, kWave);
+        Fixed n0x = __builtin_amdgcn_readlane(X[s + 1 < K ? s + 1 : s], 0);
+        Fixed n0dx = __builtin_amdgcn_readlane(DX[s + 1 < K ? s + 1 : s], 0);
         if (lane == kWave - 1) {
           nxo = n0x;
           ndx = n0dx;
@@ -808,6 +843,22 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
       }
       m = cnt_carry;
       bool all_sorted = __ballot(!sorted) == 0;
+      if (all_sorted && m == n) {
+        // every edge survives in place: the predecessor of element p is
+        // p - 1 (check_intersection against the previous element)
+        bool f = false;
+        Fixed carry_v = 0;
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          if (s >= S) break;
+          int p = s * kWave + lane;
+          Fixed v = X[s] + DX[s];
+          Fixed pv = ShiftUp(v, carry_v);
+          if (p >= 1 && p < n && pv > v) f = true;
+          carry_v = __builtin_amdgcn_readlane(v, kWave - 1);
+        }
+        flag_isect = __ballot(f) != 0;
+      } else {
       // write survivors to scratch in survivor order
 #pragma unroll
       for (int s = 0; s < K; s++) {
@@ -872,31 +923,22 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
             }
           }
           // odd phase: pairs (2t+1, 2t+2); lane 63 of slot s pairs with
-          // lane 0 of slot s+1
-          // the slot-boundary elements, read before any of them moves
-          Fixed b0x[K], b0dx[K], b0dy[K], b0ly[K], b63x[K], b63dx[K], b63dy[K], b63ly[K];
-          int b0o[K], b0m[K], b63o[K], b63m[K];
-#pragma unroll
-          for (int s = 0; s < K; s++) {
-            b0x[s] = __builtin_amdgcn_readlane(X[s], 0);
-            b0o[s] = __builtin_amdgcn_readlane(O[s], 0);
-            b0dx[s] = __builtin_amdgcn_readlane(DX[s], 0);
-            b0dy[s] = __builtin_amdgcn_readlane(DY[s], 0);
-            b0ly[s] = __builtin_amdgcn_readlane(LY[s], 0);
-            b0m[s] = __builtin_amdgcn_readlane(M[s], 0);
-            b63x[s] = __builtin_amdgcn_readlane(X[s], kWave - 1);
-            b63o[s] = __builtin_amdgcn_readlane(O[s], kWave - 1);
-            b63dx[s] = __builtin_amdgcn_readlane(DX[s], kWave - 1);
-            b63dy[s] = __builtin_amdgcn_readlane(DY[s], kWave - 1);
-            b63ly[s] = __builtin_amdgcn_readlane(LY[s], kWave - 1);
-            b63m[s] = __builtin_amdgcn_readlane(M[s], kWave - 1);
-          }
+          // lane 0 of slot s+1.  Slot s+1 has not moved yet when slot s is
+          // processed; lane 63 of slot s-1 is kept from before it moved.
+          Fixed p63x = 0, p63dx = 0, p63dy = 0, p63ly = 0;
+          int p63o = 0, p63m = 0;
 #pragma unroll
           for (int s = 0; s < K; s++) {
             if (s >= S2) break;
             int p = s * kWave + lane;
             bool left = (lane & 1) == 1;  // odd position = left of its pair
             int q = left ? p + 1 : p - 1;
+            const Fixed c63x = __builtin_amdgcn_readlane(X[s], kWave - 1);
+            const Fixed c63dx = __builtin_amdgcn_readlane(DX[s], kWave - 1);
+            const Fixed c63dy = __builtin_amdgcn_readlane(DY[s], kWave - 1);
+            const Fixed c63ly = __builtin_amdgcn_readlane(LY[s], kWave - 1);
+            const int c63o = __builtin_amdgcn_readlane(O[s], kWave - 1);
+            const int c63m = __builtin_amdgcn_readlane(M[s], kWave - 1);
             Fixed dnx = ShiftDown(X[s], 0), upx = ShiftUp(X[s], 0);
             int dno = ShiftDown(O[s], 0), upo = ShiftUp(O[s], 0);
             Fixed dndx = ShiftDown(DX[s], 0), updx = ShiftUp(DX[s], 0);
@@ -904,25 +946,30 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
             Fixed dnly = ShiftDown(LY[s], 0), uply = ShiftUp(LY[s], 0);
             int dnm = ShiftDown(M[s], 0), upm = ShiftUp(M[s], 0);
             // cross-slot partners
-            {
+            if (s + 1 < K) {
               const int t = s + 1 < K ? s + 1 : s;
-              const int u = s > 0 ? s - 1 : s;
+              const Fixed n0x = __builtin_amdgcn_readlane(X[t], 0);
+              const Fixed n0dx = __builtin_amdgcn_readlane(DX[t], 0);
+              const Fixed n0dy = __builtin_amdgcn_readlane(DY[t], 0);
+              const Fixed n0ly = __builtin_amdgcn_readlane(LY[t], 0);
+              const int n0o = __builtin_amdgcn_readlane(O[t], 0);
+              const int n0m = __builtin_amdgcn_readlane(M[t], 0);
               if (lane == kWave - 1) {
-                dnx = b0x[t];
-                dno = b0o[t];
-                dndx = b0dx[t];
-                dndy = b0dy[t];
-                dnly = b0ly[t];
-                dnm = b0m[t];
+                dnx = n0x;
+                dno = n0o;
+                dndx = n0dx;
+                dndy = n0dy;
+                dnly = n0ly;
+                dnm = n0m;
               }
-              if (lane == 0) {
-                upx = b63x[u];
-                upo = b63o[u];
-                updx = b63dx[u];
-                updy = b63dy[u];
-                uply = b63ly[u];
-                upm = b63m[u];
-              }
+            }
+            if (lane == 0) {
+              upx = p63x;
+              upo = p63o;
+              updx = p63dx;
+              updy = p63dy;
+              uply = p63ly;
+              upm = p63m;
             }
             bool both = p < m && q >= 0 && q < m;
             bool take;
@@ -949,6 +996,12 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
               }
               swapped = true;
             }
+            p63x = c63x;
+            p63dx = c63dx;
+            p63dy = c63dy;
+            p63ly = c63ly;
+            p63o = c63o;
+            p63m = c63m;
           }
           done = __ballot(swapped) == 0;
         }
@@ -1018,6 +1071,7 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
         flag_isect = __ballot(f) != 0;
       }
       WaveSync();
+      }  // survivors moved
     }
     n = m;
     {
@@ -1244,10 +1298,19 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
       for (int k = 0; k < 5; k++)
         atomicAdd(&c->phase[k], (int32_t)(ph[k] < 0x7FFFFFFF ? ph[k] : 0x7FFFFFFF));
     } else {
-      JobCov c{pb.l, pb.t, pb.r, pb.b, n_tiles, next_slot, max_active, flags, guard, dt32};
+      JobCov* c = &P.cov[j];
+      c->left = pb.l;
+      c->top = pb.t;
+      c->right = pb.r;
+      c->bottom = pb.b;
+      c->tiles = n_tiles;
+      c->planes = next_slot;
+      c->max_active = max_active;
+      c->flags = flags;
+      c->steps = guard;
+      c->time_10ns = dt32;
 #pragma unroll
-      for (int k = 0; k < 5; k++) c.phase[k] = (int32_t)(ph[k] < 0x7FFFFFFF ? ph[k] : 0x7FFFFFFF);
-      P.cov[j] = c;
+      for (int k = 0; k < 5; k++) c->phase[k] = (int32_t)(ph[k] < 0x7FFFFFFF ? ph[k] : 0x7FFFFFFF);
     }
   }
 }
@@ -1257,23 +1320,26 @@ __device__ void WalkItem(const WalkParams& P, int wi, int32_t* lds) {
 // form), a grid sized to the resident waves takes items from an atomic
 // counter until the list is exhausted; every wave leaves when the counter
 // passes n_walk.
+// The 4-slot walk is held to 256 registers (two waves per SIMD), which it
+// meets without spilling; the wider walks take what they need.
 template <int K, int kMode>
-__global__ void __launch_bounds__(64) WalkKernel(WalkParams P) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K == 4 ? 2 : 1)))
+WalkKernel(WalkParams P) {
   extern __shared__ int32_t lds[];
   // The 8- and 16-slot walks hold the paths with the widest active lists,
   // the longest sequential walks of a frame: their waves take precedence in
   // instruction issue over the 4-slot waves sharing their SIMD.
   if (K >= 8) __builtin_amdgcn_s_setprio(2);
-  if (!P.queue) {
-    if ((int)blockIdx.x < P.n_walk) WalkItem<K, kMode>(P, blockIdx.x, lds);
-    return;
-  }
-  for (;;) {
+  // one call site, so the item body is inlined once
+  auto take = [&]() -> int {
+    if (!P.queue) return (int)blockIdx.x;
     int got = 0;
     if (Lane() == 0) got = atomicAdd(P.queue, 1);
-    const int wi = __builtin_amdgcn_readlane(got, 0);
-    if (wi >= P.n_walk) return;
+    return __builtin_amdgcn_readlane(got, 0);
+  };
+  for (int wi = take(); wi < P.n_walk; wi = take()) {
     WalkItem<K, kMode>(P, wi, lds);
+    if (!P.queue) break;
     WaveSync();
   }
 }

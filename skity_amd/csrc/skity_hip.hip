@@ -251,6 +251,7 @@ struct hipr_context_s {
       gather_list;
   DevBuf wcov, qstate, q_base, snaps, snap_base, band_count;  // write-pass coverage, row bands
   DevBuf queues;                          // persistent walk: item counters
+  DevBuf pool_ctr;                        // overflow plane pool counter
   int n_cus = 0;                          // compute units of the device
   std::vector<hipGraph_t> graphs;         // graphs of the walk passes in flight
   std::vector<hipGraphExec_t> graph_execs;
@@ -1094,6 +1095,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   std::vector<int32_t> pcap(nj, 0);
   int64_t plane_total = 0;
   auto upload_planes = [&]() -> hipr_result {
+    if (plane_total >= kNoPlane) return HIPR_ERROR_CAPACITY;  // plane numbers are int32
     if (ctx->planes.EnsureKeep((size_t)kTilePlaneBytes * std::max<int64_t>(plane_total, 1),
                                ctx->planes.cap, st) != hipSuccess)
       return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
@@ -1294,6 +1296,8 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
                          total_map, -1);
       hipLaunchKernelGGL(FillKernel, dim3(256), dim3(256), 0, st, (int32_t*)ctx->wcov.p,
                          (int64_t)nj * (int64_t)(sizeof(JobCov) / 4), 0);
+      hipLaunchKernelGGL(FillKernel, dim3(1), dim3(kWave), 0, st, ctx->pool_ctr.as<int32_t>(),
+                         (int64_t)1, 0);
     } else if (!js->empty()) {
       hipr_result rr = Upload(&ctx->gather_list, js->data(), js->size(), st);
       if (rr != HIPR_SUCCESS) return rr;
@@ -1319,7 +1323,17 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     for (int32_t j : all) pcap[j] = test[j];
   }
   assign_planes(all);
+  // Estimate modes: a shared pool after the estimated ranges serves the
+  // planes of under-estimated jobs; only a job that finds the pool spent is
+  // walked again.
+  const int64_t pool_base = plane_total;
+  const int64_t pool_cap = mode == 0 ? 0 : std::max<int64_t>(plane_total / 4, 1 << 16);
+  plane_total += pool_cap;
+  if (ctx->pool_ctr.Ensure(sizeof(int32_t)) != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   if ((r = upload_planes()) != HIPR_SUCCESS) return r;
+  wp.pool_ctr = ctx->pool_ctr.as<int32_t>();
+  wp.pool_base = (int32_t)pool_base;
+  wp.pool_cap = (int32_t)pool_cap;
   wp.plane_base = ctx->plane_base.as<int64_t>();
   wp.plane_cap = ctx->plane_cap.as<int32_t>();
   wp.map0 = ctx->map0.as<int32_t>();
@@ -1367,14 +1381,17 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
       hw[j].planes = ctx->h_cov[j].planes;
       pcap[j] = -1;
     }
-    // exact fallback: the write pass counted the planes of every
-    // under-estimated job; they get exactly that many, appended, and are
-    // walked again
+    // Under-estimated jobs took their extra planes from the pool.  Exact
+    // fallback for the jobs that found it spent (flag 1) and the moved
+    // ones: the write pass counted their planes; they get exactly that many,
+    // appended, and are walked again.
+    for (int32_t j : all)
+      if (hw[j].planes > test[j]) f.stats.underestimates++;
     std::vector<int32_t> redo_small, redo_wide;
     for (int32_t j : small)
-      if (hw[j].planes > pcap[j]) redo_small.push_back(j);
+      if ((hw[j].flags & 1) || pcap[j] < 0) redo_small.push_back(j);
     for (int32_t j : wide)
-      if (hw[j].planes > pcap[j]) redo_wide.push_back(j);
+      if ((hw[j].flags & 1) || pcap[j] < 0) redo_wide.push_back(j);
     if (!redo_small.empty() || !redo_wide.empty()) {
       f.stats.fallbacks += (int64_t)(redo_small.size() + redo_wide.size());
       std::vector<int32_t> redo = redo_small;
@@ -1420,7 +1437,6 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   cp.map_base = ctx->map_base.as<int64_t>();
   cp.map0 = ctx->map0.as<int32_t>();
   cp.map1 = ctx->map1.as<int32_t>();
-  cp.plane_base = ctx->plane_base.as<int64_t>();
   cp.planes = ctx->planes.as<uint8_t>();
   cp.paints = pl->d_paints.as<PaintDev>();
   cp.grads = pl->d_grads.as<GradDev>();
@@ -1790,7 +1806,7 @@ static void FreeContext(hipr_context ctx) {
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
                     &ctx->d_jobs, &ctx->run_jobs, &ctx->counters, &ctx->wcov,
                     &ctx->qstate, &ctx->q_base, &ctx->snaps, &ctx->snap_base,
-                    &ctx->band_count, &ctx->queues};
+                    &ctx->band_count, &ctx->queues, &ctx->pool_ctr};
   ReleaseGraphs(ctx);
   for (DevBuf* b : bufs) b->Free();
   DevBuf* mlp[] = {&ctx->mlp_prm, &ctx->mlp_ops, &ctx->mlp_feats, &ctx->mlp_preds,

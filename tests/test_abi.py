@@ -30,7 +30,7 @@ def test_struct_sizes_match_header_layout():
     # hipr_paint: 8 floats + 9 x 4-byte fields
     assert ctypes.sizeof(abi.Paint) == 68
     assert ctypes.sizeof(abi.Gradient) == 4 * (4 + 4 + 2 + 6 + 64 + 16)
-    assert ctypes.sizeof(abi.Stats) == 7 * 8 + 8 + 2 * 8
+    assert ctypes.sizeof(abi.Stats) == 7 * 8 + 8 + 3 * 8
 
 
 def test_scene_generator_matches_header_and_workload_spec():

@@ -127,8 +127,34 @@ def test_presized_mode_matches_exact(ctx):
     print(f"fallbacks: short {st_s['fallbacks']}, ample {st_a['fallbacks']}", flush=True)
     assert np.array_equal(short, ref)
     assert np.array_equal(ample, ref)
-    assert st_s["fallbacks"] > 0
-    assert st_a["fallbacks"] == 0
+    assert st_s["fallbacks"] > 0 and st_s["underestimates"] > 0
+    assert st_a["fallbacks"] == 0 and st_a["underestimates"] == 0
+
+
+def test_overflow_pool_and_exact_fallback(ctx):
+    """Plane estimates of 1 per path: the shared overflow pool serves the
+    first under-estimated paths and the exact path re-walks those that find
+    it spent; edge estimates are ample, so no flatten fallback occurs.  A
+    mild under-estimate is absorbed by the pool alone."""
+    sc = workload(2, 64, seed=27)
+    n = sc.n_paths
+    ref, st_ref = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=0)
+    edges = np.full(n, 20000, np.int32)
+    spent, st1 = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=1, edge_estimates=edges,
+                            tile_estimates=np.ones(n, np.int32))
+    print("pool spent:", st1["underestimates"], "under-estimated,", st1["fallbacks"], "re-walked",
+          flush=True)
+    assert np.array_equal(spent, ref)
+    assert st1["underestimates"] > st1["raster_paths"] // 2
+    assert st1["fallbacks"] > 0
+    assert st1["tile_planes"] == st_ref["tile_planes"]
+    # a uniform estimate of 80% of the mean plane count: the pool holds the rest
+    mild, st2 = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=1, edge_estimates=edges,
+                           tile_estimates=np.full(n, max(1, st_ref["tile_planes"] * 4 // (5 * n)),
+                                                  np.int32))
+    print("mild:", st2["underestimates"], "under-estimated,", st2["fallbacks"], "re-walked",
+          flush=True)
+    assert np.array_equal(mild, ref)
 
 
 @pytest.mark.parametrize("band_rows", [16, 48, 0])
@@ -166,8 +192,8 @@ def test_row_bands_with_estimates(ctx, band_rows, band_min_edges):
     print(f"fallbacks: short {st_s['fallbacks']}, ample {st_a['fallbacks']}", flush=True)
     assert np.array_equal(short, ref)
     assert np.array_equal(ample, ref)
-    assert st_s["fallbacks"] > 0
-    assert st_a["fallbacks"] == 0
+    assert st_s["underestimates"] > 0
+    assert st_a["fallbacks"] == 0 and st_a["underestimates"] == 0
 
 
 @pytest.mark.parametrize("flags", [abi.FLAG_NO_GRAPH, abi.FLAG_PERSISTENT,
