@@ -119,6 +119,116 @@ def run_form(surf, prep, mode, steps, warmup, torch):
     return wall, stage_tot, st
 
 
+def blur_work(sc):
+    """Per blurred draw of a scene (fills): filter-bitmap area A_f and radius
+    r, as HandleFilter / ApproximateFilteredBounds size it; returns
+    (algorithmic FLOPs = sum 2 passes x 4 ch x 2(2r+1) x A_f, bytes = sum 16 A_f)."""
+    s = sc.scene.contents
+    flops, nbytes = 0, 0
+    for p in range(s.n_paths):
+        pt = s.paints[p]
+        if pt.blur_radius <= 0:
+            continue
+        p0, p1 = s.point_offsets[p], s.point_offsets[p + 1]
+        v0, v1 = s.verb_offsets[p], s.verb_offsets[p + 1]
+        if v1 - v0 > 1 and s.verbs[v1 - 1] == 0:
+            p1 -= 1
+        xs = np.array([s.points[2 * i] for i in range(p0, p1)], np.float32)
+        ys = np.array([s.points[2 * i + 1] for i in range(p0, p1)], np.float32)
+        r = np.float32(pt.blur_radius)
+        w = int(np.ceil(xs.max() + r) - np.floor(xs.min() - r))
+        h = int(np.ceil(ys.max() + r) - np.floor(ys.min() - r))
+        rad = min(int(np.floor(float(r) + 0.5)), 254)  # std::round of a positive radius
+        if w <= 0 or h <= 0 or rad <= 1:
+            continue
+        flops += 2 * 4 * 2 * (2 * rad + 1) * w * h
+        nbytes += 16 * w * h
+    return flops, nbytes
+
+
+def blur_leg(ctx, torch, steps):
+    """Gradients + blur: 1,024 cubic paths with linear/radial/sweep gradients
+    and normal mask-filter blur (sigma 1-32) on 4096^2; the blur runs as
+    MFMA banded products."""
+    from skity_amd import abi, device, scenes
+    sc = scenes.generate(3)
+    prep = device.Prepared(ctx, sc.scene)
+    surf = device.Surface(ctx, sc.width, sc.height)
+    wall, stage_tot, st = run_form(surf, prep, abi.MODE_EXACT, steps, 1, torch)
+    flops, nbytes = blur_work(sc)
+    blur_ms = stage_tot.get("blur", [0.0, 0])[0] / steps
+    out = {"workload": "gradients+blur: 1024 cubic paths, 2-16 stop gradients, blur sigma 1-32, "
+                       "4096x4096", "paths": sc.n_paths,
+           "paths_per_s": round(sc.n_paths / (wall / steps), 1),
+           "ms_per_step": round(wall * 1e3 / steps, 3),
+           "mpix_per_s": round(st["blends"] / (wall / steps) / 1e6, 1),
+           "blur_ms_per_step": round(blur_ms, 3),
+           "blur_alg_tflops": round(flops / (blur_ms * 1e-3) / 1e12, 3) if blur_ms else None,
+           "blur_alg_gbs": round(nbytes / (blur_ms * 1e-3) / 1e9, 1) if blur_ms else None,
+           "blur_alg_flops": flops, "blur_alg_bytes": nbytes}
+    surf.close()
+    prep.close()
+    return out
+
+
+def batched_leg(ctx, torch, steps, n_canvases=64):
+    """Batched uneven scenes (scaled family): n canvases of side 256-4096 with
+    10-1,000 paths of 4-316 line/quad/cubic segments each, drawn per canvas
+    (plain: one exact pipeline run per canvas) and as one batch (learned
+    pre-sizing, autotuned work split)."""
+    from skity_amd import abi, device, scenes
+    scs = [scenes.generate(5, seed=scenes.SEED_BASE + 500 + i) for i in range(n_canvases)]
+    preps = [device.Prepared(ctx, sc.scene) for sc in scs]
+    surfs = [device.Surface(ctx, sc.width, sc.height) for sc in scs]
+    n_paths = sum(sc.n_paths for sc in scs)
+    batch = device.Batch(ctx)
+    for su, pr in zip(surfs, preps):
+        batch.add(su, pr)
+    tuned = batch.autotune(abi.MODE_LEARNED, repeats=1)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        for su in surfs:
+            su.finish()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    def plain():
+        for su, pr in zip(surfs, preps):
+            su.clear(0)
+            su.draw(pr, mode=abi.MODE_EXACT)
+
+    def batched():
+        for su in surfs:
+            su.clear(0)
+        batch.draw(mode=abi.MODE_LEARNED)
+
+    t_plain = timed(plain)
+    t_batch = timed(batched)
+    st = batch.stats()
+    out = {"workload": f"batched uneven (scaled): {n_canvases} canvases 256^2-4096^2, "
+                       "10-1,000 paths of 4-316 segments each",
+           "canvases": n_canvases, "paths": n_paths,
+           "plain_paths_per_s": round(n_paths / t_plain, 1),
+           "plain_ms_per_step": round(t_plain * 1e3, 3),
+           "batched_learned_paths_per_s": round(n_paths / t_batch, 1),
+           "batched_learned_ms_per_step": round(t_batch * 1e3, 3),
+           "speedup": round(t_plain / t_batch, 3),
+           "mpix_per_s": round(st["blends"] / t_batch / 1e6, 1),
+           "fallbacks": st["fallbacks"], "underestimates": st["underestimates"],
+           "autotuned": {"band_rows": tuned[0], "band_min_edges": tuned[1], "flags": tuned[2]}}
+    batch.close()
+    for su in surfs:
+        su.close()
+    for pr in preps:
+        pr.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +239,8 @@ def main():
                     help="budget of the CPU baseline sample (0 = skip)")
     ap.add_argument("--forms", default="exact,learned",
                     help="comma list of forms to time: exact, learned")
+    ap.add_argument("--legs", default="batched,blur",
+                    help="extra workloads reported beside the headline: batched, blur")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -165,6 +277,7 @@ def main():
             "ms_per_step": round(wall * 1e3 / args.steps, 3),
             "mpix_per_s": round(st["blends"] / (wall / args.steps) / 1e6, 1),
             "fallbacks_per_frame": st["fallbacks_per_frame"],
+            "underestimates_per_frame": st["underestimates"],
             "stages": stages}
     best = max(out_forms, key=lambda f: out_forms[f]["paths_per_s"])
     wall, stage_tot, st = results[best]
@@ -199,6 +312,13 @@ def main():
         "roofline": roof,
         "device": device_info(torch),
     }
+    legs = {}
+    leg_names = [x for x in args.legs.split(",") if x]
+    if "batched" in leg_names and "learned" in forms:
+        legs["batched_uneven"] = batched_leg(ctx, torch, max(2, args.steps // 4))
+    if "blur" in leg_names:
+        legs["gradients_blur"] = blur_leg(ctx, torch, max(2, args.steps // 4))
+    out["legs"] = legs
     if args.cpu_seconds > 0:
         cb = cpu_baseline(sc, args.cpu_seconds)
         cb["value"] = round(cb["value"], 2)
