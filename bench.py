@@ -49,23 +49,53 @@ def geometry_bytes(sc):
 
 
 PMC_SUMMARY = os.path.join(HERE, "profiles", "pmc_curves_strokes_4096.json")
-STAGE_KERNEL = {"walk": "hipr::WalkKernel<4, 1>", "walk_count": "hipr::WalkKernel<4, 0>",
-                "composite": "hipr::CompositeKernel", "flatten": "hipr::FlattenKernel<1>",
-                "sort": "hipr::SortGatherKernel"}
+STAGE_KERNELS = {"walk": ("hipr::WalkKernel<4, 1>", "hipr::WalkKernel<8, 1>",
+                          "hipr::WalkKernel<16, 1>"),
+                 "walk_count": ("hipr::WalkKernel<4, 0>", "hipr::WalkKernel<8, 0>",
+                                "hipr::WalkKernel<16, 0>"),
+                 "composite": ("hipr::CompositeKernel",), "flatten": ("hipr::FlattenKernel<1>",),
+                 "sort": ("hipr::SortGatherKernel",)}
 
 
-def measured_traffic(stage):
-    """HBM bytes per launch of the stage's main kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (counter
-    values scaled by the copy-kernel calibration), or None."""
-    if not os.path.exists(PMC_SUMMARY) or stage not in STAGE_KERNEL:
+def measured_traffic(stage, form):
+    """HBM bytes of one launch of a stage (all its kernels in the frame of
+    `form`), from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload (counter values scaled by the copy-kernel calibration), or
+    None."""
+    if not os.path.exists(PMC_SUMMARY) or stage not in STAGE_KERNELS:
         return None
     with open(PMC_SUMMARY) as fh:
         js = json.load(fh)
-    for d in js["dispatches"]:
-        if d["kernel"] == STAGE_KERNEL[stage]:
-            return int((d["fetch_mib"] + d["write_mib"]) * 1024 * 1024)
-    return None
+    disp = js.get("frames", {}).get(form, [])
+    hits = [d for d in disp if d["kernel"] in STAGE_KERNELS[stage]]
+    if not hits:
+        return None
+    return int(sum(d["fetch_mib"] + d["write_mib"] for d in hits) * 1024 * 1024)
+
+
+def pmc_probe():
+    """The program the committed counter summaries profile: one exact frame
+    and one learned frame of the workload (walk passes launched directly, so
+    every kernel is its own dispatch), then a 256 MiB device-to-device copy
+    whose counters calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    from skity_amd import abi, device, presizer, scenes
+    sc = scenes.generate(WORKLOAD)
+    ctx = device.Context(0)
+    prep = device.Prepared(ctx, sc.scene)
+    surf = device.Surface(ctx, sc.width, sc.height)
+    prm, margin = presizer.load()
+    presizer.install(ctx, prm, margin)
+    surf.clear(0)
+    surf.draw(prep, mode=abi.MODE_EXACT, flags=abi.FLAG_NO_GRAPH)
+    surf.clear(0)
+    surf.draw(prep, mode=abi.MODE_LEARNED, flags=abi.FLAG_NO_GRAPH)
+    surf.finish()
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("pmc probe done", flush=True)
 
 
 def cpu_baseline(sc, budget_s):
@@ -241,7 +271,12 @@ def main():
                     help="comma list of forms to time: exact, learned")
     ap.add_argument("--legs", default="batched,blur",
                     help="extra workloads reported beside the headline: batched, blur")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run the short program the rocprofv3 counter summaries profile")
     args = ap.parse_args()
+    if args.pmc_probe:
+        pmc_probe()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: every box of this build has a single MI355X "
@@ -290,7 +325,7 @@ def main():
         achieved = alg / (avg_launch_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(dom), "alg_bytes_per_launch": int(alg),
+                "traffic": measured_traffic(dom, best), "alg_bytes_per_launch": int(alg),
                 "avg_launch_ms": round(avg_launch_ms, 4)}
     ref = out_forms.get("exact", out_forms[best])
     out = {

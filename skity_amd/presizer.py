@@ -162,13 +162,16 @@ def load(path=WEIGHTS):
 
 
 def training_scenes(n_scenes, seed0=0x7A11):
-    """Curves + strokes scenes of 1024 paths at fresh seeds (never the bench
-    seed), plus plain-fill scenes."""
+    """Curves + strokes scenes of 1024 paths at fresh seeds (never a bench
+    seed), plain-fill scenes, and canvases of the batched uneven family
+    (256^2-4096^2, lines / quads / cubics, gradients)."""
     out = []
     for i in range(n_scenes):
         out.append(scenes.generate(2, n_paths=1024, seed=seed0 + i))
         if i % 4 == 0:
             out.append(scenes.generate(1, seed=seed0 + 1000 + i))
+        for k in range(4):
+            out.append(scenes.generate(5, seed=seed0 + 2000 + 4 * i + k))
     return out
 
 
