@@ -143,6 +143,7 @@ struct WalkLds {
   int32_t* d;      // packed row deltas (RowSink)
   int32_t* slot0;  // per tile column of the current band
   int32_t* slot1;
+  int32_t* rows;   // per tile column: rows written, plane 0 (bits 0-15), plane 1 (16-31)
   int32_t* sX;     // permutation scratch (CAP entries each)
   int32_t* sDX;
   int32_t* sDY;
@@ -250,7 +251,8 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   L.d = lds;
   L.slot0 = L.d + (P.lds_width + 1);
   L.slot1 = L.slot0 + (P.lds_width / kTile + 2);
-  L.sX = L.slot1 + (P.lds_width / kTile + 2);
+  L.rows = L.slot1 + (P.lds_width / kTile + 2);
+  L.sX = L.rows + (P.lds_width / kTile + 2);
   L.sDX = L.sX + CAP;
   L.sDY = L.sDX + CAP;
   L.sLY = L.sDY + CAP;
@@ -261,6 +263,7 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   for (int i = lane; i < tr.ntc; i += kWave) {
     L.slot0[i] = -1;
     L.slot1[i] = -1;
+    L.rows[i] = 0;
   }
   WaveSync();
 
@@ -302,19 +305,36 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   };
 
   // Finish a band: publish the tile slots of every column to the job map.
+  // Each plane row is written once: with coverage by the row flush, or with
+  // zeros here for the rows the band never wrote.
   auto finish_band = [&]() {
     if (band < 0) return;
     int trow = band - tr.tr0;
     for (int c = lane; c < tr.ntc; c += kWave) {
       int s0 = L.slot0[c], s1 = L.slot1[c];
-      // Only planes inside this job's allocation are published; an overflow
-      // is reported through flags and the job is redone by the exact path.
+      // Only planes that were found are published; a job that found none is
+      // reported through flags and redone by the exact path.
       if (kWrite && s0 >= 0 && s0 != kNoPlane) {
         map0[(int64_t)trow * tr.ntc + c] = s0;
         map1[(int64_t)trow * tr.ntc + c] = (s1 >= 0 && s1 != kNoPlane) ? s1 : -1;
       }
+      if (kWrite) {
+        const uint32_t written = (uint32_t)L.rows[c];
+        const int4 zero = {0, 0, 0, 0};
+        if (s0 >= 0 && s0 != kNoPlane) {
+          int4* pl = (int4*)(planes + (int64_t)s0 * kTilePlaneBytes);
+          for (int k = 0; k < kTile; k++)
+            if (!((written >> k) & 1)) pl[k] = zero;
+        }
+        if (s1 >= 0 && s1 != kNoPlane) {
+          int4* pl = (int4*)(planes + (int64_t)s1 * kTilePlaneBytes);
+          for (int k = 0; k < kTile; k++)
+            if (!((written >> (16 + k)) & 1)) pl[k] = zero;
+        }
+      }
       L.slot0[c] = -1;
       L.slot1[c] = -1;
+      L.rows[c] = 0;
     }
     WaveSync();
   };
@@ -432,16 +452,12 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
       if (kWrite) {
         int sub = lane & 3;
         int rowk = r & 15;
-        bool new0 = __shfl(need0 ? 1 : 0, g * 4, kWave) != 0;
-        bool new1 = __shfl(need1 ? 1 : 0, g * 4, kWave) != 0;
         if (in_cols) {
-          if (new0 && cur0 >= 0 && cur0 != kNoPlane) {
-            uint32_t* pl = (uint32_t*)(planes + (int64_t)cur0 * kTilePlaneBytes);
-            for (int k = 0; k < 16; k++) pl[k * 4 + sub] = 0;
-          }
-          if (new1 && cur1 >= 0 && cur1 != kNoPlane) {
-            uint32_t* pl = (uint32_t*)(planes + (int64_t)cur1 * kTilePlaneBytes);
-            for (int k = 0; k < 16; k++) pl[k * 4 + sub] = 0;
+          if (sub == 0) {
+            // rows written in this band (the other rows are zeroed when the
+            // band ends); one lane per tile column
+            int wr = (chunk0 && cur0 >= 0 ? 1 << rowk : 0) | (chunk1 && cur1 >= 0 ? 1 << (16 + rowk) : 0);
+            if (wr) L.rows[tcol] |= wr;
           }
           if (chunk0 && cur0 >= 0) {
             if (cur0 != kNoPlane) {
@@ -496,14 +512,25 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
       int cnt = __popcll(b);
       if (k + cnt > CAP) return false;
       if (ready) {
-        EdgeRec e = E[idx];
-        if (EdgeIsQuad(e.packed)) Q[idx] = e;  // the quad's state from here on
+        const EdgeRec* er = &E[idx];
+        const int32_t packed = er->packed;
+        if (EdgeIsQuad(packed)) {
+          // the quad's state from here on; four 16-byte moves (a struct
+          // copy would be staged through scratch memory)
+          const int4* src = (const int4*)er;
+          int4* dst = (int4*)&Q[idx];
+          const int4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
+          dst[0] = q0;
+          dst[1] = q1;
+          dst[2] = q2;
+          dst[3] = q3;
+        }
         int p = k + lane;
-        L.sX[p] = e.x;
-        L.sDX[p] = e.dx;
-        L.sDY[p] = e.dy;
-        L.sLY[p] = e.lower_y;
-        L.sM[p] = MakeMeta(EdgeWinding(e.packed), EdgeIsQuad(e.packed), idx);
+        L.sX[p] = er->x;
+        L.sDX[p] = er->dx;
+        L.sDY[p] = er->dy;
+        L.sLY[p] = er->lower_y;
+        L.sM[p] = MakeMeta(EdgeWinding(packed), EdgeIsQuad(packed), idx);
       }
       k += cnt;
       if (cnt < kWave) break;

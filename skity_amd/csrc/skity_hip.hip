@@ -643,8 +643,9 @@ template <int K, int kMode>
 hipr_result LaunchWalk(hipr_context ctx, WalkParams P, hipStream_t st, bool timed,
                        int32_t* queue, double share) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
-  // row deltas, two tile-slot rows, five scratch arrays of K * 64 entries
-  size_t lds = (size_t)((P.lds_width + 1) + 2 * (P.lds_width / kTile + 2) + 5 * K * kWave) *
+  // row deltas, three per-tile-column rows (two slots, written rows), five
+  // scratch arrays of K * 64 entries
+  size_t lds = (size_t)((P.lds_width + 1) + 3 * (P.lds_width / kTile + 2) + 5 * K * kWave) *
                sizeof(int32_t);
   unsigned grid = (unsigned)P.n_walk;
   P.queue = queue;
