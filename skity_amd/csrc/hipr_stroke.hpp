@@ -485,8 +485,11 @@ struct Stroker {
     max_inner_p = Max(max_inner_p, inner->np);
     max_inner_w = Max(max_inner_w, inner->nw);
     if (inner->nv == 0) return;
-    if (!outer->verbs) {
-      // count mode: the reversal appends every non-move verb and point
+    if (!outer->verbs || inner->nv > inner->cap_v || inner->np > inner->cap_p ||
+        inner->nw > inner->cap_w) {
+      // count mode, or an inner contour that outgrew its range (the job is
+      // written again into exact ranges, and its stored contour is
+      // incomplete): the reversal appends every non-move verb and point
       outer->nv += inner->nv - 1;
       outer->np += inner->np - 1;
       outer->nw += inner->nw;
@@ -769,14 +772,16 @@ struct StrokeOut {
 };
 
 // Outline of one stroke job.  mode 0 counts (fills so's sizes); mode 1
-// writes into the regions `so` describes.
+// writes into the regions `so` describes; mode 2 writes into them as far as
+// they reach and reports the full sizes (a region too small shows as a size
+// above its capacity, and nothing is written past it).
 HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc scene,
                               StrokeOut* so_io, int mode, uint8_t* o_verbs, float* o_pts,
                               float* o_w, uint8_t* s_verbs, float* s_pts, float* s_w) {
   using namespace stroke;
   StrokeOut so = *so_io;
   PathW outer, inner;
-  if (mode == 1) {
+  if (mode != 0) {
     outer.Init(o_verbs + so.verb_begin, o_pts + 2 * (int64_t)so.pt_begin, o_w + so.w_begin,
                so.nv, so.np, so.nw);
     inner.Init(s_verbs + so.scratch_begin, s_pts + 2 * (int64_t)so.scratch_begin,
@@ -863,7 +868,7 @@ HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc s
     }
   }
   st->FinishContour(false);
-  if (mode == 0) {
+  if (mode != 1) {
     so.nv = outer.nv;
     so.np = outer.np;
     so.nw = outer.nw;
@@ -885,7 +890,7 @@ __global__ void __launch_bounds__(64)
   const Job job = jobs[list[li]];
   StrokeOut so = outs[li];
   StrokeOne(job, paints[job.path], scene, &so, kMode, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w);
-  if (kMode == 0) outs[li] = so;
+  if (kMode != 1) outs[li] = so;
 }
 
 }  // namespace hipr

@@ -251,6 +251,7 @@ struct hipr_context_s {
       gather_list;
   DevBuf wcov, qstate, q_base, snaps, snap_base, band_count;  // write-pass coverage, row bands
   DevBuf queues;                          // persistent walk: item counters
+  DevBuf stroke_redo, stroke_redo_out;    // stroke jobs written again (estimate forms)
   DevBuf pool_ctr;                        // overflow plane pool counter
   int n_cus = 0;                          // compute units of the device
   std::vector<hipGraph_t> graphs;         // graphs of the walk passes in flight
@@ -307,6 +308,7 @@ struct hipr_prepared_s {
   std::vector<int32_t> layer_of_blur;
   std::vector<int32_t> walk_jobs;    // job indices that are rasterised
   std::vector<int32_t> stroke_jobs;  // job indices that need an outline
+  std::vector<int32_t> stroke_quads; // per stroke job: segments of its quad path
   int64_t layer_bytes = 0;
   int32_t n_targets = 1;             // canvases [0, n_targets) are draw targets
   int64_t n_verbs = 0, n_pts = 0, n_weights = 0;
@@ -537,6 +539,34 @@ void AddShapeJobs(const hipr_scene* s, int p, int canvas, float tx, float ty, co
   }
 }
 
+// Segments of a path's quad path (Stroke::QuadPath, stroke.cc:914-962):
+// lines and quads one each, conics two, cubics Cubic::ToQuads's count, plus
+// a closing line per contour.  Sizes the single-pass stroke outlines.
+int32_t QuadPathSegments(const hipr_scene* s, int p) {
+  int64_t n = 0;
+  int pi = s->point_offsets[p];
+  for (int v = s->verb_offsets[p]; v < s->verb_offsets[p + 1]; v++) {
+    int verb = s->verbs[v];
+    if (verb == HIPR_VERB_LINE || verb == HIPR_VERB_QUAD) {
+      n += 1;
+    } else if (verb == HIPR_VERB_CONIC) {
+      n += 2;
+    } else if (verb == HIPR_VERB_CUBIC) {
+      P2 c[4] = {{s->points[2 * (pi - 1)], s->points[2 * (pi - 1) + 1]},
+                 {s->points[2 * pi], s->points[2 * pi + 1]},
+                 {s->points[2 * (pi + 1)], s->points[2 * (pi + 1) + 1]},
+                 {s->points[2 * (pi + 2)], s->points[2 * (pi + 2) + 1]}};
+      CubicQ q;
+      CubicSetup(c, &q);
+      n += q.n;
+    } else if (verb == HIPR_VERB_CLOSE) {
+      n += 1;
+    }
+    pi += VerbPointCount(verb);
+  }
+  return (int32_t)std::min<int64_t>(n + 1, 1 << 24);
+}
+
 void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
   std::vector<std::vector<Job>> canvas_jobs(1);
   pl->canvases.push_back(Canvas{});
@@ -689,7 +719,13 @@ struct Frame {
   hipr_stats stats{};
 };
 
-hipr_result RunStrokes(Frame& f) {
+// Stroke outlines of the frame's stroke jobs.  Exact form: a count pass,
+// then a write pass into exact ranges.  Estimate forms (`single_pass`): one
+// write pass into ranges sized from each job's quad-path segment count
+// (an outline has at most ~12 verbs, ~19 points and ~0.7 conic weights per
+// segment on the workloads measured); a job whose outline outgrew its range
+// reports its exact sizes and is written again into an exact range.
+hipr_result RunStrokes(Frame& f, bool single_pass) {
   hipr_context ctx = f.ctx;
   hipr_prepared_s* pl = f.pl;
   int ns = (int)pl->stroke_jobs.size();
@@ -701,18 +737,8 @@ hipr_result RunStrokes(Frame& f) {
   // One stroke job per wavefront: the stroker is sequential, data-dependent
   // code, and 64 different paths in one wave would serialise their branches.
   int blocks = ns;
-  STAGE_LAUNCH(ctx, HIPR_STAGE_STROKE,
-    hipLaunchKernelGGL((StrokeKernel<0>), dim3(blocks), dim3(1), 0, st, pl->d_jobs.as<Job>(),
-                     pl->d_stroke.as<int32_t>(), ns, gs, pl->scene_paints.as<hipr_paint>(),
-                     ctx->stroke_out.as<StrokeOut>(), (uint8_t*)nullptr, (float*)nullptr,
-                     (float*)nullptr, (uint8_t*)nullptr, (float*)nullptr, (float*)nullptr));
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(ctx->h_so, ctx->stroke_out.p, sizeof(StrokeOut) * ns,
-                         hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
   int64_t tv = 0, tp = 0, tw = 0, ts = 0;
-  for (int i = 0; i < ns; i++) {
-    StrokeOut& so = ctx->h_so[i];
+  auto place = [&](StrokeOut& so) {
     so.verb_begin = (int32_t)tv;
     so.pt_begin = (int32_t)tp;
     so.w_begin = (int32_t)tw;
@@ -721,30 +747,104 @@ hipr_result RunStrokes(Frame& f) {
     tp += so.np;
     tw += so.nw;
     ts += std::max(so.inner_nv, std::max(so.inner_np, so.inner_nw));
+  };
+  auto ensure_outlines = [&](bool keep) -> hipr_result {
+    if (tv > INT32_MAX || tp > INT32_MAX || ts > INT32_MAX) return HIPR_ERROR_CAPACITY;
+    DevBuf* bufs[6] = {&ctx->o_verbs, &ctx->o_pts, &ctx->o_w, &ctx->s_verbs, &ctx->s_pts,
+                       &ctx->s_w};
+    const int64_t need[6] = {tv, 8 * tp, 4 * tw, ts, 8 * ts, 4 * ts};
+    for (int i = 0; i < 6; i++) {
+      size_t bytes = (size_t)std::max<int64_t>(need[i], 1);
+      hipError_t e = keep ? bufs[i]->EnsureKeep(bytes, bufs[i]->cap, st) : bufs[i]->Ensure(bytes);
+      if (e != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+    }
+    return HIPR_SUCCESS;
+  };
+  auto launch = [&](auto mode_tag, const int32_t* list, int n, StrokeOut* outs) -> hipr_result {
+    constexpr int kM = decltype(mode_tag)::value;
+    STAGE_LAUNCH(ctx, HIPR_STAGE_STROKE,
+      hipLaunchKernelGGL((StrokeKernel<kM>), dim3(n), dim3(1), 0, st, pl->d_jobs.as<Job>(), list,
+                         n, gs, pl->scene_paints.as<hipr_paint>(), outs,
+                         kM == 0 ? nullptr : ctx->o_verbs.as<uint8_t>(),
+                         kM == 0 ? nullptr : ctx->o_pts.as<float>(),
+                         kM == 0 ? nullptr : ctx->o_w.as<float>(),
+                         kM == 0 ? nullptr : ctx->s_verbs.as<uint8_t>(),
+                         kM == 0 ? nullptr : ctx->s_pts.as<float>(),
+                         kM == 0 ? nullptr : ctx->s_w.as<float>()));
+    HIP_TRY(hipGetLastError());
+    return HIPR_SUCCESS;
+  };
+  auto set_job = [&](int i, const StrokeOut& so) {
     Job& jb = f.jobs[pl->stroke_jobs[i]];
     jb.verb_begin = so.verb_begin;
     jb.verb_end = so.verb_begin + so.nv;
     jb.pt_begin = so.pt_begin;
     jb.w_begin = so.w_begin;
+  };
+  hipr_result r;
+  if (!single_pass) {
+    if ((r = launch(std::integral_constant<int, 0>{}, pl->d_stroke.as<int32_t>(), blocks,
+                    ctx->stroke_out.as<StrokeOut>())) != HIPR_SUCCESS)
+      return r;
+    HIP_TRY(hipMemcpyAsync(ctx->h_so, ctx->stroke_out.p, sizeof(StrokeOut) * ns,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < ns; i++) {
+      place(ctx->h_so[i]);
+      set_job(i, ctx->h_so[i]);
+    }
+    if ((r = ensure_outlines(false)) != HIPR_SUCCESS) return r;
+    HIP_TRY(hipMemcpyAsync(ctx->stroke_out.p, ctx->h_so, sizeof(StrokeOut) * ns,
+                           hipMemcpyHostToDevice, st));
+    return launch(std::integral_constant<int, 1>{}, pl->d_stroke.as<int32_t>(), blocks,
+                  ctx->stroke_out.as<StrokeOut>());
   }
-  if (tv > INT32_MAX || tp > INT32_MAX) return HIPR_ERROR_CAPACITY;
-  if (ctx->o_verbs.Ensure(std::max<int64_t>(tv, 1)) != hipSuccess ||
-      ctx->o_pts.Ensure(8 * std::max<int64_t>(tp, 1)) != hipSuccess ||
-      ctx->o_w.Ensure(4 * std::max<int64_t>(tw, 1)) != hipSuccess ||
-      ctx->s_verbs.Ensure(std::max<int64_t>(ts, 1)) != hipSuccess ||
-      ctx->s_pts.Ensure(8 * std::max<int64_t>(ts, 1)) != hipSuccess ||
-      ctx->s_w.Ensure(4 * std::max<int64_t>(ts, 1)) != hipSuccess)
-    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-  HIP_TRY(hipMemcpyAsync(ctx->stroke_out.p, ctx->h_so, sizeof(StrokeOut) * ns,
+  std::vector<StrokeOut> cap(ns);
+  for (int i = 0; i < ns; i++) {
+    const int64_t q = pl->stroke_quads.empty() ? 64 : pl->stroke_quads[i];
+    StrokeOut so{};
+    so.nv = (int32_t)std::min<int64_t>(12 * q + 64, INT32_MAX / 4);
+    so.np = (int32_t)std::min<int64_t>(20 * q + 64, INT32_MAX / 4);
+    so.nw = (int32_t)std::min<int64_t>(q + 16, INT32_MAX / 4);
+    so.inner_nv = so.nv;
+    so.inner_np = so.np;
+    so.inner_nw = so.nw;
+    place(so);
+    cap[i] = so;
+  }
+  if ((r = ensure_outlines(false)) != HIPR_SUCCESS) return r;
+  HIP_TRY(hipMemcpyAsync(ctx->stroke_out.p, cap.data(), sizeof(StrokeOut) * ns,
                          hipMemcpyHostToDevice, st));
-  STAGE_LAUNCH(ctx, HIPR_STAGE_STROKE,
-    hipLaunchKernelGGL((StrokeKernel<1>), dim3(blocks), dim3(1), 0, st, pl->d_jobs.as<Job>(),
-                     pl->d_stroke.as<int32_t>(), ns, gs, pl->scene_paints.as<hipr_paint>(),
-                     ctx->stroke_out.as<StrokeOut>(), ctx->o_verbs.as<uint8_t>(),
-                     ctx->o_pts.as<float>(), ctx->o_w.as<float>(), ctx->s_verbs.as<uint8_t>(),
-                     ctx->s_pts.as<float>(), ctx->s_w.as<float>()));
-  HIP_TRY(hipGetLastError());
-  return HIPR_SUCCESS;
+  if ((r = launch(std::integral_constant<int, 2>{}, pl->d_stroke.as<int32_t>(), blocks,
+                  ctx->stroke_out.as<StrokeOut>())) != HIPR_SUCCESS)
+    return r;
+  HIP_TRY(hipMemcpyAsync(ctx->h_so, ctx->stroke_out.p, sizeof(StrokeOut) * ns,
+                         hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int32_t> redo_list;
+  std::vector<StrokeOut> redo_out;
+  for (int i = 0; i < ns; i++) {
+    StrokeOut so = ctx->h_so[i];  // ranges as placed, sizes as found
+    const StrokeOut& c = cap[i];
+    bool fits = so.nv <= c.nv && so.np <= c.np && so.nw <= c.nw && so.inner_nv <= c.inner_nv &&
+                so.inner_np <= c.inner_np && so.inner_nw <= c.inner_nw;
+    if (fits) {
+      set_job(i, so);
+      continue;
+    }
+    place(so);  // an exact range after the others
+    set_job(i, so);
+    redo_list.push_back(pl->stroke_jobs[i]);
+    redo_out.push_back(so);
+  }
+  if (redo_list.empty()) return HIPR_SUCCESS;
+  f.stats.fallbacks += (int64_t)redo_list.size();
+  if ((r = ensure_outlines(true)) != HIPR_SUCCESS) return r;
+  if ((r = Upload(&ctx->stroke_redo, redo_list.data(), redo_list.size(), st)) != HIPR_SUCCESS ||
+      (r = Upload(&ctx->stroke_redo_out, redo_out.data(), redo_out.size(), st)) != HIPR_SUCCESS)
+    return r;
+  return launch(std::integral_constant<int, 1>{}, ctx->stroke_redo.as<int32_t>(),
+                (int)redo_list.size(), ctx->stroke_redo_out.as<StrokeOut>());
 }
 
 // Flatten + sort for the jobs in `list` (indices into f.jobs).
@@ -964,7 +1064,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
       ctx->map_base.Ensure(sizeof(int64_t) * nj) != hipSuccess)
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   // ---- stage (a): stroke outlines, flatten, sort ----
-  if ((r = RunStrokes(f)) != HIPR_SUCCESS) return r;
+  if ((r = RunStrokes(f, mode != HIPR_MODE_EXACT)) != HIPR_SUCCESS) return r;
   // the prepared job table keeps the source ranges; stroke jobs are redirected
   // to their outlines in a per-draw copy
   f.d_jobs = pl->d_jobs.as<Job>();
@@ -1590,6 +1690,7 @@ hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s**
     pl->n_verbs = nv;
     pl->n_pts = np;
     pl->n_weights = nw;
+    for (int32_t j : pl->stroke_jobs) pl->stroke_quads.push_back(QuadPathSegments(scene, pl->jobs[j].path));
     if ((r = Upload(&pl->verbs, scene->verbs, nv, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->pts, scene->points, 2 * (size_t)np, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->weights, scene->weights, nw, st)) != HIPR_SUCCESS ||
@@ -1628,6 +1729,7 @@ hipr_result MergeBatch(hipr_batch b) {
   m.layer_of_blur.clear();
   m.walk_jobs.clear();
   m.stroke_jobs.clear();
+  m.stroke_quads.clear();
   const int T = (int)b->pairs.size();
   m.n_targets = T;
   std::vector<int> layer_base(T);
@@ -1682,6 +1784,7 @@ hipr_result MergeBatch(hipr_batch b) {
     for (int32_t l : s->layer_of_blur) m.layer_of_blur.push_back(canv(l));
     for (int32_t j : s->walk_jobs) m.walk_jobs.push_back(j + job_off);
     for (int32_t j : s->stroke_jobs) m.stroke_jobs.push_back(j + job_off);
+    m.stroke_quads.insert(m.stroke_quads.end(), s->stroke_quads.begin(), s->stroke_quads.end());
     voff += s->n_verbs;
     poff += s->n_pts;
     woff += s->n_weights;
@@ -1807,7 +1910,8 @@ static void FreeContext(hipr_context ctx) {
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
                     &ctx->d_jobs, &ctx->run_jobs, &ctx->counters, &ctx->wcov,
                     &ctx->qstate, &ctx->q_base, &ctx->snaps, &ctx->snap_base,
-                    &ctx->band_count, &ctx->queues, &ctx->pool_ctr};
+                    &ctx->band_count, &ctx->queues, &ctx->pool_ctr, &ctx->stroke_redo,
+                    &ctx->stroke_redo_out};
   ReleaseGraphs(ctx);
   for (DevBuf* b : bufs) b->Free();
   DevBuf* mlp[] = {&ctx->mlp_prm, &ctx->mlp_ops, &ctx->mlp_feats, &ctx->mlp_preds,

@@ -212,3 +212,32 @@ def test_walk_launch_forms_match(ctx, flags):
                         edge_estimates=np.full(n, 3000, np.int32),
                         tile_estimates=np.full(n, 60, np.int32))
     assert np.array_equal(est, ref)
+
+
+def test_single_pass_stroke_outgrows_its_range(ctx):
+    """Estimate forms stroke in one pass into ranges sized from the quad-path
+    segment count.  A zigzag with round joins needs 238 conic weights where
+    its range holds 138: the job reports its exact sizes, is written again
+    into an exact range (a fallback), and the pixels equal the exact form's
+    and the oracle's."""
+    c = Canvas(512, 400)
+    p = Path().move_to(10, 20)
+    for i in range(120):
+        p.line_to(12 + 4 * i, 20 + (340 if i % 2 == 0 else 0))
+    zig = Paint((0.8, 0.2, 0.1, 0.9)).set_style(abi.STYLE_STROKE).set_stroke_width(9.0)
+    zig.set_stroke_join(abi.JOIN_ROUND)
+    c.draw_path(p, zig)
+    easy = Paint((0.1, 0.5, 0.9, 0.7)).set_style(abi.STYLE_STROKE).set_stroke_width(5.0)
+    c.draw_path(Path().add_circle(250, 200, 90), easy)
+    sc = c.scene()
+    n = sc.scene.contents.n_paths
+    ref, _ = render_gpu(ctx, sc.scene, 512, 400, mode=0)
+    got, st = render_gpu(ctx, sc.scene, 512, 400, mode=1,
+                         edge_estimates=np.full(n, 50000, np.int32),
+                         tile_estimates=np.full(n, 2 * 1024, np.int32))
+    print("fallbacks", st["fallbacks"], flush=True)
+    assert st["fallbacks"] == 1
+    assert np.array_equal(got, ref)
+    want = O.render(sc.scene, 512, 400)
+    mx, cnt = diff_report(got, want)
+    assert mx <= 1 and cnt == 0, f"max diff {mx}, {cnt} pixels"
