@@ -165,7 +165,8 @@ typedef struct hipr_options {
                                      < 0 = one band per path */
   int32_t band_min_edges;         /* estimate modes: band only the paths whose
                                      active-edge bound reaches this; 0 = default
-                                     (none), < 0 = every path */
+                                     (257: the 8- and 16-slot walks), < 0 =
+                                     every path, a huge value = none */
   int32_t flags;                  /* HIPR_FLAG_* */
 } hipr_options;
 enum {

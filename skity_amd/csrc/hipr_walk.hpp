@@ -1347,10 +1347,12 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
 // form), a grid sized to the resident waves takes items from an atomic
 // counter until the list is exhausted; every wave leaves when the counter
 // passes n_walk.
-// The 4-slot walk is held to 256 registers (two waves per SIMD), which it
-// meets without spilling; the wider walks take what they need.
+// The 4- and 8-slot walks are held to 256 registers (two waves per SIMD):
+// the 4-slot walk fits without spilling; the 8-slot walk spills, which its
+// row bands more than make up for, since 4-slot waves can then share its
+// SIMDs.  The 16-slot walk takes what it needs.
 template <int K, int kMode>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K == 4 ? 2 : 1)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 2 : 1)))
 WalkKernel(WalkParams P) {
   extern __shared__ int32_t lds[];
   // The 8- and 16-slot walks hold the paths with the widest active lists,

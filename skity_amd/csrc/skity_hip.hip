@@ -663,11 +663,12 @@ void SetTiles(std::vector<Canvas>* cvs) {
 // Nominal rows per walk band (the write pass runs the bands of a path
 // concurrently).
 constexpr int kBandRows = 128;
-// Estimate modes: paths whose active-edge bound reaches this are banded.
-// Off by default: the schedule pass banding needs costs more than the
-// banded write saves on a full chip (measured on curves + strokes, 4096
-// paths); the autotuner tries thresholds.
-constexpr int kBandMinEdges = INT32_MAX;
+// Estimate modes: paths whose active-edge bound reaches this are banded:
+// the paths of the 8- and 16-slot walks, the longest sequential walks of a
+// frame, whose schedule pass and banded write run beside the 4-slot writes.
+// (Banding every path costs more schedule work than it saves; the
+// autotuner tries other thresholds.)
+constexpr int kBandMinEdges = 4 * kWave + 1;
 
 template <int K, int kMode>
 hipr_result LaunchWalk(hipr_context ctx, WalkParams P, hipStream_t st, bool timed,
