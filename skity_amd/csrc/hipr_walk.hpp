@@ -909,22 +909,24 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
         flag_isect = __ballot(f) != 0;
         load_from_scratch(m);
       } else {
-        // Stable re-sort by x: the survivors are reloaded in list order and
-        // sorted by (x, list position) with odd-even transposition (a step
-        // only swaps edges that crossed, so few passes are needed); a
-        // pathological order falls back to ranking by counting.
-        load_from_scratch(m);
+        // Stable re-sort by x.  The survivors stay in scratch in list order;
+        // only the keys (x, list position) are sorted in registers, by
+        // odd-even transposition (a step only swaps edges that crossed, so
+        // one or two passes usually do), with a bitonic network when eight
+        // passes do not; then the other fields are gathered by position.
         int O[K];
 #pragma unroll
-        for (int s = 0; s < K; s++) O[s] = s * kWave + lane;
+        for (int s = 0; s < K; s++) {
+          int p = s * kWave + lane;
+          O[s] = p;
+          if (p < m) X[s] = L.sX[p];
+        }
         const int S2 = (m + kWave - 1) / kWave;
-        // compare-exchange of element p (this lane, slot s) with partner data
         auto gt_key = [](Fixed xa, int oa, Fixed xb, int ob) {
           return xa > xb || (xa == xb && oa > ob);
         };
-        bool done = false;
-        for (int pass = 0; pass < 8 && !done; pass++) {
-          bool swapped = false;
+        bool sorted = false;
+        for (int pass = 0; pass < 8 && !sorted; pass++) {
           // even phase: pairs (2t, 2t+1) are lanes (2t, 2t+1) of one slot
 #pragma unroll
           for (int s = 0; s < K; s++) {
@@ -936,24 +938,16 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
             bool both = p < m && q < m;
             bool left = (lane & 1) == 0;
             bool take = both && (left ? gt_key(X[s], O[s], ox, oo) : gt_key(ox, oo, X[s], O[s]));
-            Fixed odx = __shfl_xor(DX[s], 1, kWave), ody = __shfl_xor(DY[s], 1, kWave);
-            Fixed oly = __shfl_xor(LY[s], 1, kWave);
-            int om = __shfl_xor(M[s], 1, kWave);
             if (take) {
               X[s] = ox;
               O[s] = oo;
-              DX[s] = odx;
-              DY[s] = ody;
-              LY[s] = oly;
-              M[s] = om;
-              swapped = true;
             }
           }
           // odd phase: pairs (2t+1, 2t+2); lane 63 of slot s pairs with
           // lane 0 of slot s+1.  Slot s+1 has not moved yet when slot s is
           // processed; lane 63 of slot s-1 is kept from before it moved.
-          Fixed p63x = 0, p63dx = 0, p63dy = 0, p63ly = 0;
-          int p63o = 0, p63m = 0;
+          Fixed p63x = 0;
+          int p63o = 0;
 #pragma unroll
           for (int s = 0; s < K; s++) {
             if (s >= S2) break;
@@ -961,91 +955,49 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
             bool left = (lane & 1) == 1;  // odd position = left of its pair
             int q = left ? p + 1 : p - 1;
             const Fixed c63x = __builtin_amdgcn_readlane(X[s], kWave - 1);
-            const Fixed c63dx = __builtin_amdgcn_readlane(DX[s], kWave - 1);
-            const Fixed c63dy = __builtin_amdgcn_readlane(DY[s], kWave - 1);
-            const Fixed c63ly = __builtin_amdgcn_readlane(LY[s], kWave - 1);
             const int c63o = __builtin_amdgcn_readlane(O[s], kWave - 1);
-            const int c63m = __builtin_amdgcn_readlane(M[s], kWave - 1);
             Fixed dnx = ShiftDown(X[s], 0), upx = ShiftUp(X[s], 0);
             int dno = ShiftDown(O[s], 0), upo = ShiftUp(O[s], 0);
-            Fixed dndx = ShiftDown(DX[s], 0), updx = ShiftUp(DX[s], 0);
-            Fixed dndy = ShiftDown(DY[s], 0), updy = ShiftUp(DY[s], 0);
-            Fixed dnly = ShiftDown(LY[s], 0), uply = ShiftUp(LY[s], 0);
-            int dnm = ShiftDown(M[s], 0), upm = ShiftUp(M[s], 0);
-            // cross-slot partners
             if (s + 1 < K) {
               const int t = s + 1 < K ? s + 1 : s;
               const Fixed n0x = __builtin_amdgcn_readlane(X[t], 0);
-              const Fixed n0dx = __builtin_amdgcn_readlane(DX[t], 0);
-              const Fixed n0dy = __builtin_amdgcn_readlane(DY[t], 0);
-              const Fixed n0ly = __builtin_amdgcn_readlane(LY[t], 0);
               const int n0o = __builtin_amdgcn_readlane(O[t], 0);
-              const int n0m = __builtin_amdgcn_readlane(M[t], 0);
               if (lane == kWave - 1) {
                 dnx = n0x;
                 dno = n0o;
-                dndx = n0dx;
-                dndy = n0dy;
-                dnly = n0ly;
-                dnm = n0m;
               }
             }
             if (lane == 0) {
               upx = p63x;
               upo = p63o;
-              updx = p63dx;
-              updy = p63dy;
-              uply = p63ly;
-              upm = p63m;
             }
             bool both = p < m && q >= 0 && q < m;
-            bool take;
-            if (left) {
-              take = both && gt_key(X[s], O[s], dnx, dno);
-            } else {
-              take = both && gt_key(upx, upo, X[s], O[s]);
-            }
+            bool take = both && (left ? gt_key(X[s], O[s], dnx, dno) : gt_key(upx, upo, X[s], O[s]));
             if (take) {
-              if (left) {
-                X[s] = dnx;
-                O[s] = dno;
-                DX[s] = dndx;
-                DY[s] = dndy;
-                LY[s] = dnly;
-                M[s] = dnm;
-              } else {
-                X[s] = upx;
-                O[s] = upo;
-                DX[s] = updx;
-                DY[s] = updy;
-                LY[s] = uply;
-                M[s] = upm;
-              }
-              swapped = true;
+              X[s] = left ? dnx : upx;
+              O[s] = left ? dno : upo;
             }
             p63x = c63x;
-            p63dx = c63dx;
-            p63dy = c63dy;
-            p63ly = c63ly;
             p63o = c63o;
-            p63m = c63m;
           }
-          done = __ballot(swapped) == 0;
-        }
-        WaveSync();
-        if (!done) {
-          // bitonic sort of the keys (x, list position); the other fields
-          // are parked in scratch by list position and gathered back
+          // sorted when every element exceeds its predecessor
+          bool bad = false;
+          Fixed cx = 0;
+          int co = 0;
 #pragma unroll
           for (int s = 0; s < K; s++) {
-            int c = s * kWave + lane;
-            if (c < m) {
-              L.sDX[O[s]] = DX[s];
-              L.sDY[O[s]] = DY[s];
-              L.sLY[O[s]] = LY[s];
-              L.sM[O[s]] = M[s];
-            }
+            if (s >= S2) break;
+            int p = s * kWave + lane;
+            Fixed px = ShiftUp(X[s], cx);
+            int po = ShiftUp(O[s], co);
+            if (p >= 1 && p < m && gt_key(px, po, X[s], O[s])) bad = true;
+            cx = __builtin_amdgcn_readlane(X[s], kWave - 1);
+            co = __builtin_amdgcn_readlane(O[s], kWave - 1);
           }
+          sorted = __ballot(bad) == 0;
+        }
+        if (!sorted) {
+          // bitonic sort of the keys (x, list position)
           int64_t key[K];
 #pragma unroll
           for (int s = 0; s < K; s++) {
@@ -1057,21 +1009,27 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
           int N = kWave;
           while (N < m) N <<= 1;
           BitonicSortSlots<K>(key, N, lane);
-          WaveSync();
 #pragma unroll
           for (int s = 0; s < K; s++) {
             int c = s * kWave + lane;
             if (c < m) {
               X[s] = (Fixed)(key[s] >> 32);
               O[s] = (int)(uint32_t)((uint64_t)key[s] & 0xFFFFFFFFull);
-              DX[s] = L.sDX[O[s]];
-              DY[s] = L.sDY[O[s]];
-              LY[s] = L.sLY[O[s]];
-              M[s] = L.sM[O[s]];
             }
           }
-          WaveSync();
         }
+        // the other fields, by list position
+#pragma unroll
+        for (int s = 0; s < K; s++) {
+          int c = s * kWave + lane;
+          if (c < m) {
+            DX[s] = L.sDX[O[s]];
+            DY[s] = L.sDY[O[s]];
+            LY[s] = L.sLY[O[s]];
+            M[s] = L.sM[O[s]];
+          }
+        }
+        WaveSync();
         // check_intersection with the predecessor at insertion time: the
         // nearest earlier element of the sorted list whose list position is
         // smaller (edges that moved in front of it were processed later).
