@@ -409,46 +409,49 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
       int cur1 = in_cols ? L.slot1[tcol] : -1;
       bool need0 = in_cols && leader && chunk0 && cur0 < 0;
       bool need1 = in_cols && leader && chunk1 && cur1 < 0;
-      int req = (need0 ? 1 : 0) + (need1 ? 1 : 0);
-      int rincl = ScanSum(req);
-      int rtot = __builtin_amdgcn_readlane(rincl, kWave - 1);
-      int slot_base = next_slot;
-      if (kWrite && rtot > 0) {
-        // the bands of a job share its plane range
-        int got = 0;
-        if (lane == 0) got = atomicAdd(&P.cov[j].planes, rtot);
-        slot_base = __builtin_amdgcn_readlane(got, 0);
-      }
-      int first_slot = slot_base + rincl - req;
-      next_slot += rtot;
-      // Count pass: tile slots are job-relative.  Write pass: absolute plane
-      // numbers, from the job's range or (past its capacity) from the pool.
-      int a0 = first_slot, a1 = first_slot + (need0 ? 1 : 0);
-      if (kWrite) {
-        const int over_from = slot_base > pcap ? slot_base : pcap;
-        const int over = slot_base + rtot - over_from;
-        int pool0 = 0;
-        if (over > 0) {
+      // most chunks already hold their planes: allocate only when needed
+      if (__ballot(need0 || need1) != 0) {
+        int req = (need0 ? 1 : 0) + (need1 ? 1 : 0);
+        int rincl = ScanSum(req);
+        int rtot = __builtin_amdgcn_readlane(rincl, kWave - 1);
+        int slot_base = next_slot;
+        if (kWrite && rtot > 0) {
+          // the bands of a job share its plane range
           int got = 0;
-          if (lane == 0) got = atomicAdd(P.pool_ctr, over);
-          pool0 = __builtin_amdgcn_readlane(got, 0);
+          if (lane == 0) got = atomicAdd(&P.cov[j].planes, rtot);
+          slot_base = __builtin_amdgcn_readlane(got, 0);
         }
-        auto to_abs = [&](int sl) -> int {
-          if (sl < pcap) return (int)(pbase + sl);
-          int pi = pool0 + (sl - over_from);
-          return pi < P.pool_cap ? P.pool_base + pi : kNoPlane;
-        };
-        a0 = to_abs(a0);
-        a1 = to_abs(a1);
+        int first_slot = slot_base + rincl - req;
+        next_slot += rtot;
+        // Count pass: tile slots are job-relative.  Write pass: absolute plane
+        // numbers, from the job's range or (past its capacity) from the pool.
+        int a0 = first_slot, a1 = first_slot + (need0 ? 1 : 0);
+        if (kWrite) {
+          const int over_from = slot_base > pcap ? slot_base : pcap;
+          const int over = slot_base + rtot - over_from;
+          int pool0 = 0;
+          if (over > 0) {
+            int got = 0;
+            if (lane == 0) got = atomicAdd(P.pool_ctr, over);
+            pool0 = __builtin_amdgcn_readlane(got, 0);
+          }
+          auto to_abs = [&](int sl) -> int {
+            if (sl < pcap) return (int)(pbase + sl);
+            int pi = pool0 + (sl - over_from);
+            return pi < P.pool_cap ? P.pool_base + pi : kNoPlane;
+          };
+          a0 = to_abs(a0);
+          a1 = to_abs(a1);
+        }
+        if (need0) {
+          L.slot0[tcol] = a0;
+          n_tiles++;
+        }
+        if (need1) L.slot1[tcol] = a1;
+        WaveSync();
+        cur0 = in_cols ? L.slot0[tcol] : -1;
+        cur1 = in_cols ? L.slot1[tcol] : -1;
       }
-      if (need0) {
-        L.slot0[tcol] = a0;
-        n_tiles++;
-      }
-      if (need1) L.slot1[tcol] = a1;
-      WaveSync();
-      cur0 = in_cols ? L.slot0[tcol] : -1;
-      cur1 = in_cols ? L.slot1[tcol] : -1;
       if (kWrite) {
         int sub = lane & 3;
         int rowk = r & 15;
@@ -1151,14 +1154,15 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
           has_pred = true;
         } else if (Pi > 0) {
           int q = Pi - 1;
-          int s = q / kWave, ln = q % kWave;
+          // q is the same for every lane: read the element from its slot
+          const int s = __builtin_amdgcn_readfirstlane(q / kWave);
+          const int ln = __builtin_amdgcn_readfirstlane(q % kWave);
           Fixed ax = 0, adx = 0;
 #pragma unroll
           for (int t = 0; t < K; t++) {
-            Fixed vx = __shfl(X[t], ln, kWave), vdx = __shfl(DX[t], ln, kWave);
             if (t == s) {
-              ax = vx;
-              adx = vdx;
+              ax = __builtin_amdgcn_readlane(X[t], ln);
+              adx = __builtin_amdgcn_readlane(DX[t], ln);
             }
           }
           predx = ax;
