@@ -19,6 +19,9 @@
 // The stroker is host-and-device code: the library runs it on the device;
 // a host build of the same functions checks it against the oracle on the CPU.
 #define HIPR_SD __host__ __device__
+// Inlined into the stroke kernel so the stroker's state stays in registers
+// (a called function would keep it in scratch memory).
+#define HIPR_SDI __host__ __device__ __attribute__((always_inline)) inline
 
 namespace hipr {
 
@@ -275,10 +278,13 @@ struct Stroker {
   bool join_completed;
   int depth;
   int stroke_type;
-  QC stack[kRecursiveLimits + 3];
-  int state[kRecursiveLimits + 3];
+  QC* stack;   // the quad-fit recursion, kRecursiveLimits + 3 levels
+  int* state;  // (caller-provided: LDS in the stroke kernel)
 
-  HIPR_SD void Setup(float r, float ml, int cp, int jn, PathW* o, PathW* in) {
+  HIPR_SD void Setup(float r, float ml, int cp, int jn, PathW* o, PathW* in, QC* stack_mem,
+                     int* state_mem) {
+    stack = stack_mem;
+    state = state_mem;
     radius = r;
     miter_limit = ml;
     cap = cp;
@@ -354,7 +360,7 @@ struct Stroker {
       for (int jj = 0; jj < 3; jj++) pts[i][jj] = MApply(m, pts[i][jj]);
     return count;
   }
-  HIPR_SD void Join(V before_un, V pivot, V after_un) {
+  HIPR_SDI void Join(V before_un, V pivot, V after_un) {
     PathW* o = outer;
     PathW* in = inner;
     if (join == HIPR_JOIN_BEVEL) {
@@ -431,7 +437,7 @@ struct Stroker {
     InnerJoin(in, pivot, after);
   }
   // ---- caps (stroke.cc:163-194) ----
-  HIPR_SD void Cap(V pivot, V normal, V stop) {
+  HIPR_SDI void Cap(V pivot, V normal, V stop) {
     if (cap == HIPR_CAP_BUTT) {
       outer->LineTo(stop);
     } else if (cap == HIPR_CAP_SQUARE) {
@@ -479,7 +485,7 @@ struct Stroker {
     seg_count += 1;
   }
   // Path::ReversePathTo of the inner contour onto the outer path.
-  HIPR_SD void ReverseInner() {
+  HIPR_SDI void ReverseInner() {
     // the inner path is largest here (after the closing join or the end cap)
     max_inner_v = Max(max_inner_v, inner->nv);
     max_inner_p = Max(max_inner_p, inner->np);
@@ -513,7 +519,7 @@ struct Stroker {
       }
     }
   }
-  HIPR_SD void FinishContour(bool close) {
+  HIPR_SDI void FinishContour(bool close) {
     if (seg_count > 0) {
       if (close) {
         Join(prev_unit, prev_pt, first_unit);
@@ -533,13 +539,13 @@ struct Stroker {
     inner->Reset();
     seg_count = -1;
   }
-  HIPR_SD void MoveTo(V pt) {
+  HIPR_SDI void MoveTo(V pt) {
     if (seg_count > 0) FinishContour(false);
     seg_count = 0;
     first_pt = prev_pt = pt;
     join_completed = false;
   }
-  HIPR_SD void LineTo(V curr) {
+  HIPR_SDI void LineTo(V curr) {
     bool teeny = NearlyZero(prev_pt.x - curr.x, kNearlyZero * inv_res) &&
                  NearlyZero(prev_pt.y - curr.y, kNearlyZero * inv_res);
     if (teeny && cap == HIPR_CAP_BUTT) return;
@@ -648,7 +654,7 @@ struct Stroker {
     return CloseEnough(qp->quad, ray, qp);
   }
   // PathStroker::QuadStroke with its recursion as an explicit stack.
-  HIPR_SD void QuadStroke(const V q[3]) {
+  HIPR_SDI void QuadStroke(const V q[3]) {
     int sp = 0;
     stack[0].Init(0, 1);
     state[0] = 0;
@@ -728,7 +734,7 @@ struct Stroker {
     *reduction = QuadEval(q, t);
     return 3;
   }
-  HIPR_SD void QuadTo(V c, V p2) {
+  HIPR_SDI void QuadTo(V c, V p2) {
     V quad[3] = {prev_pt, c, p2};
     V reduction;
     int rt = CheckQuadLinear(quad, &reduction);
@@ -764,6 +770,14 @@ struct Stroker {
 }  // namespace stroke
 
 // Output region of one stroke job.
+namespace stroke {
+// Storage of the quad-fit recursion of one stroke.
+struct StrokeStack {
+  QC quads[kRecursiveLimits + 3];
+  int state[kRecursiveLimits + 3];
+};
+}  // namespace stroke
+
 struct StrokeOut {
   int32_t verb_begin, pt_begin, w_begin;   // in the outline arrays
   int32_t nv, np, nw;                      // sizes (count mode result)
@@ -777,7 +791,8 @@ struct StrokeOut {
 // above its capacity, and nothing is written past it).
 HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc scene,
                               StrokeOut* so_io, int mode, uint8_t* o_verbs, float* o_pts,
-                              float* o_w, uint8_t* s_verbs, float* s_pts, float* s_w) {
+                              float* o_w, uint8_t* s_verbs, float* s_pts, float* s_w,
+                              stroke::StrokeStack* stack_mem) {
   using namespace stroke;
   StrokeOut so = *so_io;
   PathW outer, inner;
@@ -793,7 +808,7 @@ HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc s
   Stroker stk;
   Stroker* st = &stk;
   st->Setup(paint.stroke_width * 0.5f, paint.miter_limit, paint.cap, paint.join, &outer,
-            &inner);
+            &inner, stack_mem->quads, stack_mem->state);
   // Iterate the source path as its quad path (Stroke::QuadPath) and feed the
   // quad path through Path::Iter(quad, false) into the stroker.
   const uint8_t* verbs = scene.verbs + job.verb_begin;
@@ -804,11 +819,11 @@ HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc s
   V src_move{0, 0}, src_last{0, 0};  // Iter over the source
   V q_move{0, 0}, q_last{0, 0};      // Iter over the quad path
   auto P = [&](int i) { return V{pts[2 * i], pts[2 * i + 1]}; };
-  auto q_line = [&](V p) {
+  auto q_line = [&](V p) __attribute__((always_inline)) {
     st->LineTo(p);
     q_last = p;
   };
-  auto q_quad = [&](V c, V p) {
+  auto q_quad = [&](V c, V p) __attribute__((always_inline)) {
     st->QuadTo(c, p);
     q_last = p;
   };
@@ -885,11 +900,14 @@ __global__ void __launch_bounds__(64)
                  GeomSrc scene, const hipr_paint* __restrict__ paints, StrokeOut* outs,
                  uint8_t* o_verbs, float* o_pts, float* o_w, uint8_t* s_verbs, float* s_pts,
                  float* s_w) {
+  // one thread per workgroup: the recursion stack lives in the group's LDS
+  __shared__ stroke::StrokeStack stack_mem;
   int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= n) return;
   const Job job = jobs[list[li]];
   StrokeOut so = outs[li];
-  StrokeOne(job, paints[job.path], scene, &so, kMode, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w);
+  StrokeOne(job, paints[job.path], scene, &so, kMode, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w,
+            &stack_mem);
   if (kMode != 1) outs[li] = so;
 }
 

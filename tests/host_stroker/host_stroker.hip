@@ -16,7 +16,9 @@ int64_t hs_stroke_outline(const hipr_scene* s, int32_t p, uint8_t* verbs, int64_
   job.w_begin = s->weight_offsets[p];
   GeomSrc g{s->verbs, s->points, s->weights};
   StrokeOut so{};
-  StrokeOne(job, s->paints[p], g, &so, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  stroke::StrokeStack stack;
+  StrokeOne(job, s->paints[p], g, &so, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+            &stack);
   int scratch = so.inner_nv > so.inner_np ? so.inner_nv : so.inner_np;
   if (so.inner_nw > scratch) scratch = so.inner_nw;
   std::vector<uint8_t> ov(so.nv + 1), sv(scratch + 1);
@@ -24,7 +26,7 @@ int64_t hs_stroke_outline(const hipr_scene* s, int32_t p, uint8_t* verbs, int64_
   StrokeOut w = so;
   w.verb_begin = w.pt_begin = w.w_begin = w.scratch_begin = 0;
   StrokeOne(job, s->paints[p], g, &w, 1, ov.data(), op.data(), ow.data(), sv.data(), sp.data(),
-            sw.data());
+            sw.data(), &stack);
   for (int i = 0; i < so.nv && i < vcap; i++) verbs[i] = ov[i];
   for (int i = 0; i < 2 * so.np && i < 2 * pcap; i++) pts[i] = op[i];
   for (int i = 0; i < so.nw && i < wcap; i++) ws[i] = ow[i];
