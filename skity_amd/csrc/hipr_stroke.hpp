@@ -1,8 +1,9 @@
 // hipr_stroke.hpp -- stage (a): stroke outlines on the device.
 //
 // Stroke::StrokePath / PathStroker (src/geometry/stroke.cc:15-901) run by one
-// thread per stroked draw over the quad path of its source
-// (Stroke::QuadPath, stroke.cc:914-962, produced on the fly).  The recursion
+// wavefront per stroked draw over the quad path of its source
+// (Stroke::QuadPath, stroke.cc:914-962, produced on the fly); the lanes
+// stroke chunks of quad segments ahead of the stroker.  The recursion
 // of PathStroker::QuadStroke (depth <= 33) is an explicit stack; caps and
 // joins, the unit-arc conics of round joins (Conic::BuildUnitArc,
 // conic.cc:72-167) and the reversal of the inner offset path
@@ -18,7 +19,7 @@
 
 // The stroker is host-and-device code: the library runs it on the device;
 // a host build of the same functions checks it against the oracle on the CPU.
-#define HIPR_SD __host__ __device__
+#define HIPR_SD __host__ __device__ __attribute__((always_inline))
 // Inlined into the stroke kernel so the stroker's state stays in registers
 // (a called function would keep it in scratch memory).
 #define HIPR_SDI __host__ __device__ __attribute__((always_inline)) inline
@@ -267,6 +268,23 @@ struct QC {
   }
 };
 
+// One quad segment's outer and inner offset curves, stroked ahead of the
+// stroker from a presumed start point: n[side] quads of 4 floats (control
+// point, end point) at q[side].  ok = 0 when a side needed more recursion
+// levels or quads than were kept.
+struct SpecView {
+  V start;
+  int ok;
+  int n[2];
+  const float* q[2];
+};
+HIPR_SD inline bool SameBits(V a, V b) {
+  uint32_t u[4];
+  __builtin_memcpy(&u[0], &a, 8);
+  __builtin_memcpy(&u[2], &b, 8);
+  return u[0] == u[2] && u[1] == u[3];
+}
+
 struct Stroker {
   float radius, miter_limit, inv_res, inv_res_sq;
   int cap, join;
@@ -278,13 +296,19 @@ struct Stroker {
   bool join_completed;
   int depth;
   int stroke_type;
-  QC* stack;   // the quad-fit recursion, kRecursiveLimits + 3 levels
-  int* state;  // (caller-provided: LDS in the stroke kernel)
+  QC* stack;         // the quad-fit recursion (caller-provided: LDS in the
+  int* state;        // stroke kernel); level i at [i * stack_stride]
+  int stack_stride;
+  int stack_cap;     // levels; a recursion that needs more sets `overflow`
+  bool overflow;
 
   HIPR_SD void Setup(float r, float ml, int cp, int jn, PathW* o, PathW* in, QC* stack_mem,
                      int* state_mem) {
     stack = stack_mem;
     state = state_mem;
+    stack_stride = 1;
+    stack_cap = kRecursiveLimits + 3;  // never reached: depth <= kRecursiveLimits
+    overflow = false;
     radius = r;
     miter_limit = ml;
     cap = cp;
@@ -360,15 +384,26 @@ struct Stroker {
       for (int jj = 0; jj < 3; jj++) pts[i][jj] = MApply(m, pts[i][jj]);
     return count;
   }
+  // The two paths trade contents where the join's outer side is the inner
+  // path (a runtime choice of path object would keep both in memory).
+  HIPR_SDI void SwapPaths() {
+    PathW t = *outer;
+    *outer = *inner;
+    *inner = t;
+  }
   HIPR_SDI void Join(V before_un, V pivot, V after_un) {
+    bool sw = false;
+    JoinSides(before_un, pivot, after_un, &sw);
+    if (sw) SwapPaths();
+  }
+  HIPR_SDI void JoinSides(V before_un, V pivot, V after_un, bool* sw) {
     PathW* o = outer;
     PathW* in = inner;
     if (join == HIPR_JOIN_BEVEL) {
       V after = Mul(after_un, radius);
       if (!(before_un.x * after_un.y > before_un.y * after_un.x)) {
-        PathW* t = o;
-        o = in;
-        in = t;
+        SwapPaths();
+        *sw = true;
         after = V{-after.x, -after.y};
       }
       o->LineTo(V{pivot.x + after.x, pivot.y + after.y});
@@ -382,9 +417,8 @@ struct Stroker {
       V before = before_un, after = after_un;
       bool ccw = false;
       if (!(before.x * after.y > before.y * after.x)) {
-        PathW* t = o;
-        o = in;
-        in = t;
+        SwapPaths();
+        *sw = true;
         before = V{-before.x, -before.y};
         after = V{-after.x, -after.y};
         ccw = true;
@@ -412,9 +446,8 @@ struct Stroker {
       return;
     }
     if (!(before.x * after.y > before.y * after.x)) {
-      PathW* t = o;
-      o = in;
-      in = t;
+      SwapPaths();
+      *sw = true;
       before = V{-before.x, -before.y};
       after = V{-after.x, -after.y};
     }
@@ -654,16 +687,17 @@ struct Stroker {
     return CloseEnough(qp->quad, ray, qp);
   }
   // PathStroker::QuadStroke with its recursion as an explicit stack.
-  HIPR_SDI void QuadStroke(const V q[3]) {
+  HIPR_SDI void QuadStroke(const V q[3], PathW* path) {
+    const int s = stack_stride;
     int sp = 0;
     stack[0].Init(0, 1);
     state[0] = 0;
     while (sp >= 0) {
-      QC* node = &stack[sp];
-      if (state[sp] == 0) {
+      QC* node = &stack[sp * s];
+      int* ns = &state[sp * s];
+      if (*ns == 0) {
         int rt = Compare(q, node);
         if (rt == 2) {
-          PathW* path = stroke_type == 1 ? outer : inner;
           path->QuadTo(node->quad[1], node->quad[2]);
           sp--;
           continue;
@@ -673,14 +707,18 @@ struct Stroker {
           continue;
         }
         if (++depth > kRecursiveLimits) return;  // every level returns false
-        state[sp] = 1;
-        stack[sp + 1].InitWithStart(*node);
-        state[sp + 1] = 0;
+        if (sp + 1 >= stack_cap) {
+          overflow = true;
+          return;
+        }
+        *ns = 1;
+        stack[(sp + 1) * s].InitWithStart(*node);
+        state[(sp + 1) * s] = 0;
         sp++;
-      } else if (state[sp] == 1) {
-        state[sp] = 2;
-        stack[sp + 1].InitWithEnd(*node);
-        state[sp + 1] = 0;
+      } else if (*ns == 1) {
+        *ns = 2;
+        stack[(sp + 1) * s].InitWithEnd(*node);
+        state[(sp + 1) * s] = 0;
         sp++;
       } else {
         --depth;
@@ -734,7 +772,11 @@ struct Stroker {
     *reduction = QuadEval(q, t);
     return 3;
   }
-  HIPR_SDI void QuadTo(V c, V p2) {
+  // `spec`, when given, holds this segment's two offset curves stroked
+  // ahead from a presumed start point; they stand in for the two QuadStroke
+  // calls when that start is the stroker's current point and the recursion
+  // depth is zero (the state QuadStroke would start from).
+  HIPR_SDI void QuadTo(V c, V p2, const SpecView* spec = nullptr) {
     V quad[3] = {prev_pt, c, p2};
     V reduction;
     int rt = CheckQuadLinear(quad, &reduction);
@@ -755,10 +797,21 @@ struct Stroker {
       LineTo(p2);
       return;
     }
-    stroke_type = 1;
-    QuadStroke(quad);
-    stroke_type = -1;
-    QuadStroke(quad);
+    if (spec && spec->ok && depth == 0 && SameBits(prev_pt, spec->start)) {
+      for (int k = 0; k < spec->n[0]; k++) {
+        const float* f = spec->q[0] + 4 * k;
+        outer->QuadTo(V{f[0], f[1]}, V{f[2], f[3]});
+      }
+      for (int k = 0; k < spec->n[1]; k++) {
+        const float* f = spec->q[1] + 4 * k;
+        inner->QuadTo(V{f[0], f[1]}, V{f[2], f[3]});
+      }
+    } else {
+      stroke_type = 1;
+      QuadStroke(quad, outer);
+      stroke_type = -1;
+      QuadStroke(quad, inner);
+    }
     if (!NormalUnit(quad[1], quad[2], &nBC, &uBC)) {
       nBC = nAB;
       uBC = uAB;
@@ -769,15 +822,160 @@ struct Stroker {
 
 }  // namespace stroke
 
-// Output region of one stroke job.
 namespace stroke {
 // Storage of the quad-fit recursion of one stroke.
 struct StrokeStack {
   QC quads[kRecursiveLimits + 3];
   int state[kRecursiveLimits + 3];
 };
+
+// The stroker calls of Path::Iter(quad path, false) over the quad path of a
+// source path (Stroke::QuadPath, stroke.cc:914-962), one at a time: the quad
+// path is produced on the fly and never stored.
+enum OpKind { kOpMove, kOpLine, kOpQuad, kOpClose };
+struct Op {
+  int kind;
+  V s, c, p;  // quad: start (the previous quad-path point), control, end; move, line: p
+};
+struct OpIter {
+  const uint8_t* verbs;
+  const float* pts;
+  const float* w;
+  int nv, k, pi, wi;
+  V src_move, src_last;  // Iter over the source
+  V q_move, q_last;      // Iter over the quad path
+  int sub, nsub;         // quads of the conic or cubic in progress: emitted, total
+  bool cubic;
+  int close_step;        // 1, 2: closing lines still to check; 3: the close
+  P2 q5[5];
+  CubicQ cq;
+
+  HIPR_SDI void Init(const Job& job, GeomSrc scene) {
+    verbs = scene.verbs + job.verb_begin;
+    pts = scene.pts + 2 * (int64_t)job.pt_begin;
+    w = scene.weights + job.w_begin;
+    nv = job.verb_end - job.verb_begin;
+    k = pi = wi = 0;
+    src_move = src_last = q_move = q_last = V{0, 0};
+    sub = nsub = 0;
+    cubic = false;
+    close_step = 0;
+  }
+  HIPR_SDI V P(int i) const { return V{pts[2 * i], pts[2 * i + 1]}; }
+  HIPR_SDI void Line(Op* op, V p) {
+    op->kind = kOpLine;
+    op->p = p;
+    q_last = p;
+  }
+  HIPR_SDI void Quad(Op* op, V c, V p) {
+    op->kind = kOpQuad;
+    op->s = q_last;
+    op->c = c;
+    op->p = p;
+    q_last = p;
+  }
+  // The next stroker call; false at the end of the path.
+  HIPR_SDI bool Next(Op* op) {
+    for (;;) {
+      if (sub < nsub) {
+        int i = sub++;
+        if (cubic) {
+          P2 c, e;
+          CubicQuad(cq, i, &c, &e);
+          Quad(op, V{c.x, c.y}, V{e.x, e.y});
+        } else {
+          Quad(op, V{q5[2 * i + 1].x, q5[2 * i + 1].y}, V{q5[2 * i + 2].x, q5[2 * i + 2].y});
+        }
+        return true;
+      }
+      if (close_step != 0) {
+        // Iter(source).AutoClose emits the closing line into the quad path,
+        // then Iter(quad path) closes with a line if its last point differs.
+        if (close_step == 1) {
+          close_step = 2;
+          if (!Eq(src_last, src_move)) {
+            src_last = src_move;
+            Line(op, src_move);
+            return true;
+          }
+        }
+        if (close_step == 2) {
+          close_step = 3;
+          if (!Eq(q_last, q_move)) {
+            Line(op, q_move);
+            return true;
+          }
+        }
+        close_step = 0;
+        src_last = src_move;
+        op->kind = kOpClose;
+        return true;
+      }
+      if (k >= nv) return false;
+      int v = verbs[k++];
+      if (v == HIPR_VERB_MOVE) {
+        if (k == nv) return false;  // trailing move: Iter returns kDone
+        src_move = src_last = P(pi);
+        pi++;
+        q_move = q_last = src_move;
+        op->kind = kOpMove;
+        op->p = q_move;
+        return true;
+      } else if (v == HIPR_VERB_LINE) {
+        V e = P(pi);
+        pi++;
+        src_last = e;
+        Line(op, e);
+        return true;
+      } else if (v == HIPR_VERB_QUAD) {
+        V c = P(pi), e = P(pi + 1);
+        pi += 2;
+        src_last = e;
+        Quad(op, c, e);
+        return true;
+      } else if (v == HIPR_VERB_CONIC) {
+        P2 src3[3] = {{pts[2 * (pi - 1)], pts[2 * (pi - 1) + 1]},
+                      {pts[2 * pi], pts[2 * pi + 1]},
+                      {pts[2 * (pi + 1)], pts[2 * (pi + 1) + 1]}};
+        ConicToQuads(src3, w[wi], q5);
+        wi++;
+        pi += 2;
+        src_last = V{src3[2].x, src3[2].y};
+        cubic = false;
+        sub = 0;
+        nsub = 2;
+      } else if (v == HIPR_VERB_CUBIC) {
+        P2 src4[4] = {{pts[2 * (pi - 1)], pts[2 * (pi - 1) + 1]},
+                      {pts[2 * pi], pts[2 * pi + 1]},
+                      {pts[2 * (pi + 1)], pts[2 * (pi + 1) + 1]},
+                      {pts[2 * (pi + 2)], pts[2 * (pi + 2) + 1]}};
+        pi += 3;
+        CubicSetup(src4, &cq);
+        src_last = V{src4[3].x, src4[3].y};
+        cubic = true;
+        sub = 0;
+        nsub = cq.n;
+      } else if (v == HIPR_VERB_CLOSE) {
+        close_step = 1;
+      }
+    }
+  }
+};
+
+HIPR_SDI void Apply(Stroker* st, const Op& op) {
+  if (op.kind == kOpMove) {
+    st->MoveTo(op.p);
+  } else if (op.kind == kOpLine) {
+    st->LineTo(op.p);
+  } else if (op.kind == kOpQuad) {
+    st->QuadTo(op.c, op.p);
+  } else {
+    st->FinishContour(true);
+  }
+}
 }  // namespace stroke
 
+// Output region of one stroke job.
 struct StrokeOut {
   int32_t verb_begin, pt_begin, w_begin;   // in the outline arrays
   int32_t nv, np, nw;                      // sizes (count mode result)
@@ -788,11 +986,13 @@ struct StrokeOut {
 // Outline of one stroke job.  mode 0 counts (fills so's sizes); mode 1
 // writes into the regions `so` describes; mode 2 writes into them as far as
 // they reach and reports the full sizes (a region too small shows as a size
-// above its capacity, and nothing is written past it).
-HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc scene,
-                              StrokeOut* so_io, int mode, uint8_t* o_verbs, float* o_pts,
-                              float* o_w, uint8_t* s_verbs, float* s_pts, float* s_w,
-                              stroke::StrokeStack* stack_mem) {
+// above its capacity, and nothing is written past it).  `feed` drives the
+// stroker through the quad path.
+template <class Feed>
+HIPR_SDI void StrokeOutline(const Job& job, const hipr_paint& paint, GeomSrc scene,
+                            StrokeOut* so_io, int mode, uint8_t* o_verbs, float* o_pts,
+                            float* o_w, uint8_t* s_verbs, float* s_pts, float* s_w,
+                            stroke::StrokeStack* stack_mem, Feed feed) {
   using namespace stroke;
   StrokeOut so = *so_io;
   PathW outer, inner;
@@ -806,109 +1006,178 @@ HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc s
     inner.Init(nullptr, nullptr, nullptr, 0, 0, 0);
   }
   Stroker stk;
-  Stroker* st = &stk;
-  st->Setup(paint.stroke_width * 0.5f, paint.miter_limit, paint.cap, paint.join, &outer,
+  stk.Setup(paint.stroke_width * 0.5f, paint.miter_limit, paint.cap, paint.join, &outer,
             &inner, stack_mem->quads, stack_mem->state);
-  // Iterate the source path as its quad path (Stroke::QuadPath) and feed the
-  // quad path through Path::Iter(quad, false) into the stroker.
-  const uint8_t* verbs = scene.verbs + job.verb_begin;
-  const float* pts = scene.pts + 2 * (int64_t)job.pt_begin;
-  const float* w = scene.weights + job.w_begin;
-  int nv = job.verb_end - job.verb_begin;
-  int pi = 0, wi = 0;
-  V src_move{0, 0}, src_last{0, 0};  // Iter over the source
-  V q_move{0, 0}, q_last{0, 0};      // Iter over the quad path
-  auto P = [&](int i) { return V{pts[2 * i], pts[2 * i + 1]}; };
-  auto q_line = [&](V p) __attribute__((always_inline)) {
-    st->LineTo(p);
-    q_last = p;
-  };
-  auto q_quad = [&](V c, V p) __attribute__((always_inline)) {
-    st->QuadTo(c, p);
-    q_last = p;
-  };
-  for (int k = 0; k < nv; k++) {
-    int v = verbs[k];
-    if (v == HIPR_VERB_MOVE) {
-      if (k == nv - 1) break;  // trailing move: Iter returns kDone
-      src_move = src_last = P(pi);
-      pi++;
-      q_move = q_last = src_move;
-      st->MoveTo(q_move);
-    } else if (v == HIPR_VERB_LINE) {
-      V e = P(pi);
-      pi++;
-      src_last = e;
-      q_line(e);
-    } else if (v == HIPR_VERB_QUAD) {
-      V c = P(pi), e = P(pi + 1);
-      pi += 2;
-      src_last = e;
-      q_quad(c, e);
-    } else if (v == HIPR_VERB_CONIC) {
-      P2 src3[3] = {{pts[2 * (pi - 1)], pts[2 * (pi - 1) + 1]},
-                    {pts[2 * pi], pts[2 * pi + 1]},
-                    {pts[2 * (pi + 1)], pts[2 * (pi + 1) + 1]}};
-      P2 q5[5];
-      ConicToQuads(src3, w[wi], q5);
-      wi++;
-      pi += 2;
-      src_last = V{src3[2].x, src3[2].y};
-      q_quad(V{q5[1].x, q5[1].y}, V{q5[2].x, q5[2].y});
-      q_quad(V{q5[3].x, q5[3].y}, V{q5[4].x, q5[4].y});
-    } else if (v == HIPR_VERB_CUBIC) {
-      P2 src4[4] = {{pts[2 * (pi - 1)], pts[2 * (pi - 1) + 1]},
-                    {pts[2 * pi], pts[2 * pi + 1]},
-                    {pts[2 * (pi + 1)], pts[2 * (pi + 1) + 1]},
-                    {pts[2 * (pi + 2)], pts[2 * (pi + 2) + 1]}};
-      pi += 3;
-      CubicQ cq;
-      CubicSetup(src4, &cq);
-      for (int i = 0; i < cq.n; i++) {
-        P2 c, e;
-        CubicQuad(cq, i, &c, &e);
-        q_quad(V{c.x, c.y}, V{e.x, e.y});
-      }
-      src_last = V{src4[3].x, src4[3].y};
-    } else if (v == HIPR_VERB_CLOSE) {
-      // Iter(source).AutoClose emits the closing line into the quad path,
-      // then Iter(quad path) closes with a line if its last point differs.
-      if (!Eq(src_last, src_move)) {
-        q_line(src_move);
-        src_last = src_move;
-      }
-      if (!Eq(q_last, q_move)) q_line(q_move);
-      st->FinishContour(true);
-      src_last = src_move;
-    }
-  }
-  st->FinishContour(false);
+  OpIter it;
+  it.Init(job, scene);
+  feed(&stk, &it);
+  stk.FinishContour(false);
   if (mode != 1) {
     so.nv = outer.nv;
     so.np = outer.np;
     so.nw = outer.nw;
-    so.inner_nv = Max(st->max_inner_v, 1);
-    so.inner_np = Max(st->max_inner_p, 1);
-    so.inner_nw = Max(st->max_inner_w, 1);
+    so.inner_nv = Max(stk.max_inner_v, 1);
+    so.inner_np = Max(stk.max_inner_p, 1);
+    so.inner_nw = Max(stk.max_inner_w, 1);
     *so_io = so;
   }
 }
 
+// The stroker run call by call on one thread.
+HIPR_SD inline void StrokeOne(const Job& job, const hipr_paint& paint, GeomSrc scene,
+                              StrokeOut* so_io, int mode, uint8_t* o_verbs, float* o_pts,
+                              float* o_w, uint8_t* s_verbs, float* s_pts, float* s_w,
+                              stroke::StrokeStack* stack_mem) {
+  StrokeOutline(job, paint, scene, so_io, mode, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w,
+                stack_mem, [](stroke::Stroker* st, stroke::OpIter* it) {
+                  stroke::Op op;
+                  while (it->Next(&op)) stroke::Apply(st, op);
+                });
+}
+
+namespace stroke {
+// One wavefront per stroke.  The quad path is taken in chunks of up to
+// kSpecLanes quad segments; lane i strokes segment i's two offset curves
+// (PathStroker::QuadStroke) from the segment's start point in the quad
+// path, with a recursion stack of kSpecDepth levels in LDS; then the whole
+// wave runs the stroker over the chunk call by call, taking a lane's curves
+// in place of its own QuadStroke calls where QuadTo accepts them.  A segment
+// whose curves needed deeper recursion or more quads than a lane keeps, or
+// whose start differs from the stroker's point, is stroked there as before,
+// so the outline is the one-thread stroker's.
+constexpr int kSpecLanes = 64;
+constexpr int kSpecDepth = 6;
+constexpr int kSpecQuads = 8;
+constexpr int kChunkOps = 128;
+constexpr int kSpecStride = 2 * 4 * kSpecQuads + 1;  // floats per lane (odd: no bank conflicts)
+struct SpecStack {
+  QC quads[kSpecDepth * kSpecLanes];  // level i of lane l at [i * kSpecLanes + l]
+  int state[kSpecDepth * kSpecLanes];
+};
+struct WaveLds {
+  union {
+    StrokeStack serial;  // the stroker's own recursion (call by call)
+    SpecStack spec;      // the lanes' recursions (ahead of it)
+  } stack;
+  float out[kSpecLanes * kSpecStride];
+  int ok[kSpecLanes];
+  int n[2][kSpecLanes];
+  int quad_op[kSpecLanes];  // chunk position of each quad segment
+  Op ops[kChunkOps];
+};
+
+// Lane `lane`'s segment: both offset curves into its LDS output.
+HIPR_SDI void StrokeAhead(const Stroker& cfg, WaveLds* L, int lane) {
+  const Op o = L->ops[L->quad_op[lane]];
+  const V quad[3] = {o.s, o.c, o.p};
+  Stroker sp;
+  sp.radius = cfg.radius;
+  sp.miter_limit = cfg.miter_limit;
+  sp.inv_res = cfg.inv_res;
+  sp.inv_res_sq = cfg.inv_res_sq;
+  sp.cap = cfg.cap;
+  sp.join = cfg.join;
+  sp.outer = sp.inner = nullptr;
+  sp.stack = &L->stack.spec.quads[lane];
+  sp.state = &L->stack.spec.state[lane];
+  sp.stack_stride = kSpecLanes;
+  sp.stack_cap = kSpecDepth;
+  float* out = &L->out[lane * kSpecStride];
+  int ok = 0, n0 = 0, n1 = 0;
+  V reduction;
+  auto side = [&](int type, float* dst, int* n) __attribute__((always_inline)) {
+    PathW pw;
+    pw.Init(nullptr, dst, nullptr, 0, 2 * kSpecQuads, 0);
+    pw.move_open = pw.any_move = true;  // no injected move
+    sp.stroke_type = type;
+    sp.depth = 0;
+    sp.overflow = false;
+    sp.QuadStroke(quad, &pw);
+    *n = pw.nv;
+    return !sp.overflow && sp.depth == 0 && pw.nv <= kSpecQuads;
+  };
+  if (sp.CheckQuadLinear(quad, &reduction) == 2) {
+    ok = side(1, out, &n0);
+    ok = side(-1, out + 4 * kSpecQuads, &n1) && ok;
+  }
+  L->ok[lane] = ok;
+  L->n[0][lane] = n0;
+  L->n[1][lane] = n1;
+}
+
+// The chunked feed.  `writer` marks the one lane that stores a chunk's calls,
+// `sync()` orders the LDS between the phases, and `ahead(nq)` runs
+// StrokeAhead for lanes [0, nq) (the wave's lanes on the device; a loop in
+// the host build that checks this feed).
+template <class Sync, class Ahead>
+HIPR_SDI void FeedChunks(Stroker* st, OpIter* it, WaveLds* L, bool writer, Sync sync,
+                         Ahead ahead) {
+  bool more = true;
+  while (more) {
+    int n = 0, nq = 0;
+    while (n < kChunkOps && nq < kSpecLanes) {
+      Op op;
+      if (!it->Next(&op)) {
+        more = false;
+        break;
+      }
+      if (writer) {
+        L->ops[n] = op;
+        if (op.kind == kOpQuad) L->quad_op[nq] = n;
+      }
+      nq += op.kind == kOpQuad;
+      n++;
+    }
+    sync();
+    ahead(nq);
+    sync();
+    int q = 0;
+    for (int i = 0; i < n; i++) {
+      const Op op = L->ops[i];
+      if (op.kind == kOpQuad) {
+        SpecView sv;
+        sv.start = op.s;
+        sv.ok = L->ok[q];
+        sv.n[0] = L->n[0][q];
+        sv.n[1] = L->n[1][q];
+        sv.q[0] = &L->out[q * kSpecStride];
+        sv.q[1] = sv.q[0] + 4 * kSpecQuads;
+        st->QuadTo(op.c, op.p, &sv);
+        q++;
+      } else {
+        Apply(st, op);
+      }
+    }
+    sync();  // the next chunk reuses the LDS
+  }
+}
+}  // namespace stroke
+
+// One wavefront per stroked draw; the launch order puts the longest quad
+// paths first.  Every lane runs the stroker on the same values, so the
+// outline stores of the 64 lanes carry one value per address.
 template <int kMode>
 __global__ void __launch_bounds__(64)
     StrokeKernel(const Job* __restrict__ jobs, const int32_t* __restrict__ list, int n,
                  GeomSrc scene, const hipr_paint* __restrict__ paints, StrokeOut* outs,
                  uint8_t* o_verbs, float* o_pts, float* o_w, uint8_t* s_verbs, float* s_pts,
                  float* s_w) {
-  // one thread per workgroup: the recursion stack lives in the group's LDS
-  __shared__ stroke::StrokeStack stack_mem;
-  int li = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ stroke::WaveLds lds;
+  const int li = blockIdx.x;
   if (li >= n) return;
   const Job job = jobs[list[li]];
   StrokeOut so = outs[li];
-  StrokeOne(job, paints[job.path], scene, &so, kMode, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w,
-            &stack_mem);
-  if (kMode != 1) outs[li] = so;
+  StrokeOutline(job, paints[job.path], scene, &so, kMode, o_verbs, o_pts, o_w, s_verbs, s_pts,
+                s_w, &lds.stack.serial,
+                [&](stroke::Stroker* st, stroke::OpIter* it) __attribute__((always_inline)) {
+                  const int lane = threadIdx.x;
+                  stroke::FeedChunks(
+                      st, it, &lds, lane == 0, []() { __syncthreads(); },
+                      [&](int nq) {
+                        if (lane < nq) stroke::StrokeAhead(*st, &lds, lane);
+                      });
+                });
+  if (kMode != 1 && threadIdx.x == 0) outs[li] = so;
 }
 
 }  // namespace hipr

@@ -567,6 +567,25 @@ int32_t QuadPathSegments(const hipr_scene* s, int p) {
   return (int32_t)std::min<int64_t>(n + 1, 1 << 24);
 }
 
+// Stroke jobs in decreasing order of quad-path length: the stroke kernel's
+// workgroups start in launch order, so the longest outlines start first.
+void LongestStrokesFirst(hipr_prepared_s* pl) {
+  const size_t n = pl->stroke_jobs.size();
+  if (pl->stroke_quads.size() != n) return;
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return pl->stroke_quads[a] > pl->stroke_quads[b];
+  });
+  std::vector<int32_t> jobs(n), quads(n);
+  for (size_t i = 0; i < n; i++) {
+    jobs[i] = pl->stroke_jobs[order[i]];
+    quads[i] = pl->stroke_quads[order[i]];
+  }
+  pl->stroke_jobs.swap(jobs);
+  pl->stroke_quads.swap(quads);
+}
+
 void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
   std::vector<std::vector<Job>> canvas_jobs(1);
   pl->canvases.push_back(Canvas{});
@@ -735,8 +754,9 @@ hipr_result RunStrokes(Frame& f, bool single_pass) {
   if (ctx->stroke_out.Ensure(sizeof(StrokeOut) * ns) != hipSuccess)
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   GeomSrc gs{pl->verbs.as<uint8_t>(), pl->pts.as<float>(), pl->weights.as<float>()};
-  // One stroke job per wavefront: the stroker is sequential, data-dependent
-  // code, and 64 different paths in one wave would serialise their branches.
+  // One stroke job per wavefront (its lanes stroke quad segments ahead of
+  // the stroker): 64 different paths in one wave would serialise their
+  // branches.
   int blocks = ns;
   int64_t tv = 0, tp = 0, tw = 0, ts = 0;
   auto place = [&](StrokeOut& so) {
@@ -764,7 +784,7 @@ hipr_result RunStrokes(Frame& f, bool single_pass) {
   auto launch = [&](auto mode_tag, const int32_t* list, int n, StrokeOut* outs) -> hipr_result {
     constexpr int kM = decltype(mode_tag)::value;
     STAGE_LAUNCH(ctx, HIPR_STAGE_STROKE,
-      hipLaunchKernelGGL((StrokeKernel<kM>), dim3(n), dim3(1), 0, st, pl->d_jobs.as<Job>(), list,
+      hipLaunchKernelGGL((StrokeKernel<kM>), dim3(n), dim3(64), 0, st, pl->d_jobs.as<Job>(), list,
                          n, gs, pl->scene_paints.as<hipr_paint>(), outs,
                          kM == 0 ? nullptr : ctx->o_verbs.as<uint8_t>(),
                          kM == 0 ? nullptr : ctx->o_pts.as<float>(),
@@ -1692,6 +1712,7 @@ hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s**
     pl->n_pts = np;
     pl->n_weights = nw;
     for (int32_t j : pl->stroke_jobs) pl->stroke_quads.push_back(QuadPathSegments(scene, pl->jobs[j].path));
+    LongestStrokesFirst(pl);
     if ((r = Upload(&pl->verbs, scene->verbs, nv, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->pts, scene->points, 2 * (size_t)np, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->weights, scene->weights, nw, st)) != HIPR_SUCCESS ||
@@ -1825,6 +1846,7 @@ hipr_result MergeBatch(hipr_batch b) {
     w += s->n_weights;
     pa += s->n_paths;
   }
+  LongestStrokesFirst(&m);
   hipr_result r;
   if ((r = Upload(&m.d_jobs, m.jobs.data(), m.jobs.size(), st)) != HIPR_SUCCESS ||
       (r = Upload(&m.d_paints, m.paints.data(), m.paints.size(), st)) != HIPR_SUCCESS ||

@@ -1,5 +1,6 @@
-"""The device stroker (skity_amd/csrc/hipr_stroke.hpp, StrokeOne) compiled
-for the host and compared with the oracle's PathStroker restatement: the
+"""The device stroker (skity_amd/csrc/hipr_stroke.hpp: StrokeOne, and the
+stroke kernel's chunked feed) compiled for the host and compared with the
+oracle's PathStroker restatement: the
 outlines (verbs, points, conic weights) must be bit-identical.  This runs the
 product's stroke code on the CPU, so a stroker bug shows up here without a
 GPU; the GPU tests then check the same code on the device."""
@@ -35,7 +36,8 @@ def host():
         os.replace(LIB + ".tmp", LIB)
     lib = ctypes.CDLL(LIB)
     SP = ctypes.POINTER(abi.Scene)
-    for L, name in ((lib, "hs_stroke_outline"), (O.lib(), "orc_stroke_outline")):
+    for L, name in ((lib, "hs_stroke_outline"), (lib, "hs_stroke_outline_chunked"),
+                    (O.lib(), "orc_stroke_outline")):
         fn = getattr(L, name)
         fn.restype = ctypes.c_int64
         fn.argtypes = [SP, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
@@ -54,11 +56,16 @@ def _outline(fn, scene, p):
 
 
 def _same(host, scene, p):
-    a = _outline(host.hs_stroke_outline, scene, p)
+    """The device stroker, run call by call and through the stroke kernel's
+    chunked feed (segments stroked ahead by lanes), against the oracle."""
     b = _outline(O.lib().orc_stroke_outline, scene, p)
-    return (np.array_equal(a[0], b[0]) and a[1].shape == b[1].shape
-            and np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
-            and np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32)))
+    for fn in (host.hs_stroke_outline, host.hs_stroke_outline_chunked):
+        a = _outline(fn, scene, p)
+        if not (np.array_equal(a[0], b[0]) and a[1].shape == b[1].shape
+                and np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+                and np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))):
+            return False
+    return True
 
 
 @pytest.mark.parametrize("join", [abi.JOIN_MITER, abi.JOIN_ROUND, abi.JOIN_BEVEL])
