@@ -45,7 +45,8 @@ enum WalkMode : int { kWalkCount = 0, kWalkWrite = 1, kWalkSchedule = 2 };
 // row.  Restarted from that state, the walk of a band is the same computation
 // as the sequential walk over those rows, and every tile row belongs to one
 // band.
-constexpr int kMaxBands = 16;  // work items are job << 4 | band
+constexpr int kBandBits = 5;  // work items are job << kBandBits | band
+constexpr int kMaxBands = 1 << kBandBits;
 struct SnapHeader {
   int32_t y, nny, pp, n;  // y = kMaxS32: the band has no rows
   int32_t pad[12];
@@ -207,7 +208,7 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   const int lane = Lane();
   const uint64_t t_start = wall_clock64();
   const int item = P.walk_list[wi];
-  const int j = item >> 4;
+  const int j = item >> kBandBits;
   const int rband = kWrite ? (item & (kMaxBands - 1)) : 0;
   const Job job = P.jobs[j];
   const JobGeo geo = P.geo[j];

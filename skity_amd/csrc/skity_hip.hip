@@ -1048,6 +1048,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   if (mode == HIPR_MODE_ESTIMATES && (!opt->edge_estimates || !opt->tile_estimates))
     return HIPR_ERROR_INVALID_ARGUMENT;
   if (mode == HIPR_MODE_LEARNED && !ctx->has_presizer) return HIPR_ERROR_INVALID_ARGUMENT;
+  if (nj >= (1 << (31 - kBandBits))) return HIPR_ERROR_CAPACITY;  // walk items: job, band
   ctx->ev_used = 0;
   ctx->marks.clear();
   for (int k = 0; k < HIPR_STAGE_COUNT; k++) {
@@ -1254,7 +1255,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   auto make_items = [&](const std::vector<int32_t>& js, int kmode) {
     std::vector<int32_t> it;
     if (kmode != kWalkWrite) {
-      for (int32_t j : js) it.push_back(j << 4);
+      for (int32_t j : js) it.push_back(j << kBandBits);
       return it;
     }
     std::vector<std::pair<int64_t, int32_t>> order;
@@ -1265,7 +1266,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     }
     std::stable_sort(order.begin(), order.end());
     for (auto& o : order)
-      for (int b = 0; b < nbands[o.second]; b++) it.push_back((o.second << 4) | b);
+      for (int b = 0; b < nbands[o.second]; b++) it.push_back((o.second << kBandBits) | b);
     return it;
   };
   // One walk pass: the 4-slot jobs on the main stream, the 8- and 16-slot
