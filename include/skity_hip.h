@@ -161,8 +161,9 @@ typedef struct hipr_options {
   const int32_t* edge_estimates;  /* n_paths or NULL */
   const int32_t* tile_estimates;  /* n_paths or NULL */
   int32_t blur_impl;              /* HIPR_BLUR_IMPL_*; 0 = MFMA */
-  int32_t band_rows;              /* rows per walk band: 0 = default (128),
-                                     < 0 = one band per path */
+  int32_t band_rows;              /* rows per walk band: 0 = default (128
+                                     exact, 64 estimate modes), < 0 = one
+                                     band per path */
   int32_t band_min_edges;         /* estimate modes: band only the paths whose
                                      active-edge bound reaches this; 0 = default
                                      (257: the 8- and 16-slot walks), < 0 =

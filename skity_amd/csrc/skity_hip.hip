@@ -680,8 +680,13 @@ void SetTiles(std::vector<Canvas>* cvs) {
 }
 
 // Nominal rows per walk band (the write pass runs the bands of a path
-// concurrently).
-constexpr int kBandRows = 128;
+// concurrently).  Exact form: every path is banded by its count pass;
+// estimate forms: only the widest paths, after a schedule pass, whose
+// banded write then ends the walk stage, so shorter bands pay there
+// (curves + strokes at 2048^2: learned 155.0 -> 152.6 ms with 64 rows;
+// exact 305 -> 308 ms).
+constexpr int kBandRowsExact = 128;
+constexpr int kBandRowsEstimates = 64;
 // Estimate modes: paths whose active-edge bound reaches this are banded:
 // the paths of the 8- and 16-slot walks, the longest sequential walks of a
 // frame, whose schedule pass and banded write run beside the 4-slot writes.
@@ -1174,7 +1179,9 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   }
   // Row bands: each job's rows are split into bands walked concurrently by
   // the write pass; the count or schedule pass records where they start.
-  const int band_rows = opt && opt->band_rows != 0 ? std::max(opt->band_rows, 0) : kBandRows;
+  const int band_rows = opt && opt->band_rows != 0 ? std::max(opt->band_rows, 0)
+                        : mode == 0                    ? kBandRowsExact
+                                                       : kBandRowsEstimates;
   // In the estimate modes banding costs a schedule pass, so only the paths
   // with the widest active lists (the longest walks) are banded.
   const int band_min =
