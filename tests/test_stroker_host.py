@@ -104,3 +104,46 @@ def test_forced_stroke_parameters(host):
         s.paints[p].miter_limit = [4.0, 1.0, 10.0][p % 3]
     bad = [p for p in range(s.n_paths) if not _same(host, sc.scene, p)]
     assert not bad, bad[:10]
+
+
+@pytest.mark.parametrize("cap", [abi.CAP_BUTT, abi.CAP_ROUND, abi.CAP_SQUARE])
+def test_degenerate_segments(host, cap):
+    """Segments the stroker skips or reduces: near-zero lines (skipped with a
+    butt cap, so the next quad starts from an earlier point than the quad
+    path's), repeated points, a quad folded back on itself, a cusp cubic and
+    a width far above the curve size (deep quad-fit recursion)."""
+    c = Canvas(200, 160)
+    for k, width in enumerate((3.0, 40.0)):
+        p = Paint().set_style(abi.STYLE_STROKE).set_stroke_width(width)
+        p.set_stroke_join(k % 3).set_stroke_cap(cap)
+        path = (Path().move_to(20, 20).line_to(20.00001, 20.0).quad_to(60, 5, 90, 40)
+                .line_to(90, 40).line_to(90.000003, 40.000002).quad_to(95, 45, 90, 40)
+                .quad_to(130, 80, 60, 60).cubic_to(140, 140, 20, 140, 100, 60)
+                .quad_to(100.0001, 60.0001, 100.0002, 60.0).line_to(150, 150)
+                .conic_to(180, 150, 180, 120, 0.3).close())
+        c.draw_path(path, p)
+        c.draw_path(Path().move_to(5, 150).quad_to(6, 149, 5, 150).line_to(5, 150), p)
+    sc = c.scene()
+    for i in range(sc.n_paths):
+        assert _same(host, sc.scene, i), (cap, i)
+
+
+def test_recursion_limit_carries_over(host):
+    """Curves with coordinates near 1e6 and a 1 px width make PathStroker's
+    quad fit give up after 33 levels; its depth then stays raised for the
+    rest of the path (stroke.cc:552-564, reset only in the constructor,
+    stroke.cc:849), which the chunked feed must not paper over."""
+    import random
+    rng = random.Random(5)
+    c = Canvas(64, 64)
+    for _ in range(24):
+        p = Paint().set_style(abi.STYLE_STROKE).set_stroke_width(1.0)
+
+        def r():
+            return rng.uniform(-1e6, 1e6)
+        c.draw_path(Path().move_to(r(), r()).quad_to(r(), r(), r(), r()).quad_to(r(), r(), r(), r())
+                    .cubic_to(r(), r(), r(), r(), r(), r()).line_to(r(), r())
+                    .quad_to(r(), r(), r(), r()), p)
+    sc = c.scene()
+    bad = [i for i in range(sc.n_paths) if not _same(host, sc.scene, i)]
+    assert not bad, bad
