@@ -241,3 +241,35 @@ def test_single_pass_stroke_outgrows_its_range(ctx):
     want = O.render(sc.scene, 512, 400)
     mx, cnt = diff_report(got, want)
     assert mx <= 1 and cnt == 0, f"max diff {mx}, {cnt} pixels"
+
+
+@pytest.mark.parametrize("cap", [abi.CAP_BUTT, abi.CAP_ROUND, abi.CAP_SQUARE])
+@pytest.mark.parametrize("mode", [abi.MODE_EXACT, abi.MODE_ESTIMATES])
+def test_stroke_degenerate_segments(ctx, cap, mode):
+    """Segments the stroker skips or reduces (near-zero lines, repeated
+    points, a folded quad, a cusp cubic, a width far above the curve size):
+    the stroke kernel's lanes stroke segments ahead from the quad path's
+    points, and the stroker must fall back where its own point differs.
+    Exact form (count + write) and the single-pass form of the estimate
+    modes."""
+    c = Canvas(200, 160)
+    for k, width in enumerate((3.0, 40.0)):
+        p = Paint((0.9, 0.3, 0.1, 0.7)).set_style(abi.STYLE_STROKE).set_stroke_width(width)
+        p.set_stroke_join(k % 3).set_stroke_cap(cap)
+        path = (Path().move_to(20, 20).line_to(20.00001, 20.0).quad_to(60, 5, 90, 40)
+                .line_to(90, 40).line_to(90.000003, 40.000002).quad_to(95, 45, 90, 40)
+                .quad_to(130, 80, 60, 60).cubic_to(140, 140, 20, 140, 100, 60)
+                .quad_to(100.0001, 60.0001, 100.0002, 60.0).line_to(150, 150)
+                .conic_to(180, 150, 180, 120, 0.3).close())
+        c.draw_path(path, p)
+        c.draw_path(Path().move_to(5, 150).quad_to(6, 149, 5, 150).line_to(5, 150), p)
+    sc = c.scene()
+    kw = {}
+    if mode == abi.MODE_ESTIMATES:
+        n = sc.n_paths
+        kw = dict(mode=mode, edge_estimates=np.full(n, 4096, np.int32),
+                  tile_estimates=np.full(n, 256, np.int32))
+    got, _ = render_gpu(ctx, sc.scene, 200, 160, **kw)
+    want = O.render(sc.scene, 200, 160)
+    mx, cnt = diff_report(got, want)
+    assert mx <= 1 and cnt == 0, f"max diff {mx}, {cnt} pixels"
