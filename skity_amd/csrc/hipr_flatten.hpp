@@ -151,15 +151,16 @@ __device__ __forceinline__ P2 SegmentQEnd(int verb, const float* pts, int pi) {
   return P(pi + VerbPoints(verb) - 1);
 }
 
-// mode 0: count edges and bounds; mode 1: also write edge records.
+// mode 0: count edges and bounds; mode 1: also write edge records.  Work
+// item i is job list[i], or job i without a list.
 template <int kMode>
 __global__ void __launch_bounds__(64)
-    FlattenKernel(const Job* __restrict__ jobs, int n_jobs, GeomSrc scene,
+    FlattenKernel(const Job* __restrict__ jobs, int n_items, GeomSrc scene,
                   GeomSrc outline, const int64_t* __restrict__ edge_base,
                   const int32_t* __restrict__ edge_cap, EdgeRec* __restrict__ edges,
-                  JobGeo* __restrict__ geo) {
-  int j = blockIdx.x;
-  if (j >= n_jobs) return;
+                  JobGeo* __restrict__ geo, const int32_t* __restrict__ list) {
+  if ((int)blockIdx.x >= n_items) return;
+  const int j = list ? list[blockIdx.x] : (int)blockIdx.x;
   const Job job = jobs[j];
   JobGeo g;
   if (job.kind == kJobImage) {

@@ -251,6 +251,7 @@ struct hipr_context_s {
       gather_list;
   DevBuf wcov, qstate, q_base, snaps, snap_base, band_count;  // write-pass coverage, row bands
   DevBuf queues;                          // persistent walk: item counters
+  DevBuf flatten_redo;                    // jobs whose edges outgrew their estimate
   DevBuf stroke_redo, stroke_redo_out;    // stroke jobs written again (estimate forms)
   DevBuf pool_ctr;                        // overflow plane pool counter
   int n_cus = 0;                          // compute units of the device
@@ -888,7 +889,7 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
     STAGE_LAUNCH(ctx, HIPR_STAGE_FLATTEN_COUNT,
     hipLaunchKernelGGL((FlattenKernel<0>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj,
                        gs, go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                       (EdgeRec*)nullptr, ctx->geo.as<JobGeo>()));
+                       (EdgeRec*)nullptr, ctx->geo.as<JobGeo>(), nullptr));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * nj, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -907,7 +908,7 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
   STAGE_LAUNCH(ctx, HIPR_STAGE_FLATTEN,
     hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj, gs,
                      go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                     ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>()));
+                     ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>(), nullptr));
   HIP_TRY(hipGetLastError());
   if (mode == 1) {
     // one read-back: exact counts, bounds and overflow flags
@@ -924,18 +925,22 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
         (*ecap)[j] = ctx->h_geo[j].edges;
         total += (*ecap)[j];
       }
-      if (ctx->edges.Ensure(sizeof(EdgeRec) * std::max<int64_t>(total, 1)) != hipSuccess)
+      // the other jobs' edges stay where they are (kept if the buffer grows)
+      const size_t written = sizeof(EdgeRec) * (size_t)(*ebase)[nj];
+      if (ctx->edges.EnsureKeep(sizeof(EdgeRec) * std::max<int64_t>(total, 1), written, st) !=
+          hipSuccess)
         return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
-      // (Ensure may have reallocated: the redone jobs rewrite every edge
-      // they own; the others are rewritten too, cheaply, by re-running.)
       HIP_TRY(hipMemcpyAsync(ctx->edge_base.p, ebase->data(), sizeof(int64_t) * nj,
                              hipMemcpyHostToDevice, st));
       HIP_TRY(hipMemcpyAsync(ctx->edge_cap.p, ecap->data(), sizeof(int32_t) * nj,
                              hipMemcpyHostToDevice, st));
+      hipr_result rr = Upload(&ctx->flatten_redo, redo.data(), redo.size(), st);
+      if (rr != HIPR_SUCCESS) return rr;
       STAGE_LAUNCH(ctx, HIPR_STAGE_FLATTEN,
-    hipLaunchKernelGGL((FlattenKernel<1>), dim3(nj), dim3(64), 0, st, f.d_jobs, nj,
-                         gs, go, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                         ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>()));
+    hipLaunchKernelGGL((FlattenKernel<1>), dim3((unsigned)redo.size()), dim3(64), 0, st, f.d_jobs,
+                         (int)redo.size(), gs, go, ctx->edge_base.as<int64_t>(),
+                         ctx->edge_cap.as<int32_t>(), ctx->edges.as<EdgeRec>(),
+                         ctx->geo.as<JobGeo>(), ctx->flatten_redo.as<int32_t>()));
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * nj, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
@@ -1941,7 +1946,7 @@ static void FreeContext(hipr_context ctx) {
                     &ctx->blur_jobs, &ctx->blur_lines, &ctx->d_verbs, &ctx->d_pts, &ctx->d_w,
                     &ctx->d_jobs, &ctx->run_jobs, &ctx->counters, &ctx->wcov,
                     &ctx->qstate, &ctx->q_base, &ctx->snaps, &ctx->snap_base,
-                    &ctx->band_count, &ctx->queues, &ctx->pool_ctr, &ctx->stroke_redo,
+                    &ctx->band_count, &ctx->queues, &ctx->flatten_redo, &ctx->pool_ctr, &ctx->stroke_redo,
                     &ctx->stroke_redo_out};
   ReleaseGraphs(ctx);
   for (DevBuf* b : bufs) b->Free();
@@ -2460,7 +2465,7 @@ HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx, const hipr_scene* 
   if (n > 0) {
     hipLaunchKernelGGL((FlattenKernel<0>), dim3(n), dim3(64), 0, st, ctx->d_jobs.as<Job>(), n, gs,
                        gs, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                       (EdgeRec*)nullptr, ctx->geo.as<JobGeo>());
+                       (EdgeRec*)nullptr, ctx->geo.as<JobGeo>(), nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->h_geo, ctx->geo.p, sizeof(JobGeo) * n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -2481,7 +2486,7 @@ HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx, const hipr_scene* 
   if (n > 0) {
     hipLaunchKernelGGL((FlattenKernel<1>), dim3(n), dim3(64), 0, st, ctx->d_jobs.as<Job>(), n, gs,
                        gs, ctx->edge_base.as<int64_t>(), ctx->edge_cap.as<int32_t>(),
-                       ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>());
+                       ctx->edges.as<EdgeRec>(), ctx->geo.as<JobGeo>(), nullptr);
     HIP_TRY(hipGetLastError());
   }
   if (total > 0)
