@@ -330,8 +330,10 @@ HIPR_API hipr_result hipr_batch_get_stage_times(hipr_batch batch, double* ms,
  * Stage (a): flatten + edge build (Stroke::QuadPath, src/geometry/
  * stroke.cc:914-962 and SWEdgeBuilder::BuildEdges, src/render/sw/
  * sw_edge.cc:299-336) for every path of `scene`, returning the edge records
- * (16 int32 per edge, layout in DESIGN.md) in emission order and per-path
- * edge offsets (n_paths + 1). */
+ * in emission order and per-path edge offsets (n_paths + 1).  An edge is 16
+ * int32: x, snapped_x, dx, dy, upper_x, upper_y, lower_y (16.16 fixed
+ * point), packed (winding | curve count << 8 | curve shift << 16 | quad <<
+ * 24), then the quad state qx, qy, qdx, qdy, qddx, qddy, q_last_x, q_last_y. */
 HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx,
                                             const hipr_scene* scene,
                                             int32_t* edge_offsets,

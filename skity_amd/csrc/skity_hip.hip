@@ -440,7 +440,8 @@ hipr_result ValidateScene(const hipr_scene* s) {
       return HIPR_ERROR_INVALID_ARGUMENT;
     if (need_w && !s->weights) return HIPR_ERROR_INVALID_ARGUMENT;
     for (int i = s->point_offsets[p]; i < s->point_offsets[p + 1]; i++) {
-      // non-finite points are rejected (SURVEY: Rect::SetBoundsCheck cannot)
+      // non-finite points are rejected (Rect::SetBoundsCheck, src/geometry/
+      // rect.cc:12-49, only tests isinf of 0 * x products, which a NaN passes)
       if (!std::isfinite(s->points[2 * i]) || !std::isfinite(s->points[2 * i + 1]))
         return HIPR_ERROR_INVALID_ARGUMENT;
     }
