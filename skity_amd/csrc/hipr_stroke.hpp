@@ -277,6 +277,12 @@ struct SpecView {
   int ok;
   int n[2];
   const float* q[2];
+  // from the same start: CheckQuadLinear's result and reduction point, and
+  // NormalUnit of the legs start -> control (ab) and control -> end (bc)
+  int linear;
+  V reduction;
+  int ok_ab, ok_bc;
+  V n_ab, u_ab, n_bc, u_bc;
 };
 HIPR_SD inline bool SameBits(V a, V b) {
   uint32_t u[4];
@@ -492,9 +498,19 @@ struct Stroker {
     *normal = Mul(*unit, radius);
     return true;
   }
-  HIPR_SD bool PreJoinTo(V curr, V* normal, V* unit) {
+  // `ahead`: the normal and unit of prev_pt -> curr computed ahead (ok_ab,
+  // n_ab, u_ab of a SpecView whose start is prev_pt), or null.
+  HIPR_SD bool PreJoinTo(V curr, V* normal, V* unit, const SpecView* ahead = nullptr) {
     float px = prev_pt.x, py = prev_pt.y;
-    if (!NormalUnit(prev_pt, curr, normal, unit)) {
+    bool ok;
+    if (ahead) {
+      ok = ahead->ok_ab != 0;
+      *normal = ahead->n_ab;
+      *unit = ahead->u_ab;
+    } else {
+      ok = NormalUnit(prev_pt, curr, normal, unit);
+    }
+    if (!ok) {
       if (cap == HIPR_CAP_BUTT) return false;
       *normal = V{radius, 0};
       *unit = V{1, 0};
@@ -778,8 +794,17 @@ struct Stroker {
   // depth is zero (the state QuadStroke would start from).
   HIPR_SDI void QuadTo(V c, V p2, const SpecView* spec = nullptr) {
     V quad[3] = {prev_pt, c, p2};
+    // values computed ahead from this very start point stand in for the
+    // stroker's own calls
+    const SpecView* ahead = spec && SameBits(prev_pt, spec->start) ? spec : nullptr;
     V reduction;
-    int rt = CheckQuadLinear(quad, &reduction);
+    int rt;
+    if (ahead) {
+      rt = ahead->linear;
+      reduction = ahead->reduction;
+    } else {
+      rt = CheckQuadLinear(quad, &reduction);
+    }
     if (rt == 0 || rt == 1) {
       LineTo(p2);
       return;
@@ -793,11 +818,11 @@ struct Stroker {
       return;
     }
     V nAB, uAB, nBC, uBC;
-    if (!PreJoinTo(c, &nAB, &uAB)) {
+    if (!PreJoinTo(c, &nAB, &uAB, ahead)) {
       LineTo(p2);
       return;
     }
-    if (spec && spec->ok && depth == 0 && SameBits(prev_pt, spec->start)) {
+    if (ahead && ahead->ok && depth == 0) {
       for (int k = 0; k < spec->n[0]; k++) {
         const float* f = spec->q[0] + 4 * k;
         outer->QuadTo(V{f[0], f[1]}, V{f[2], f[3]});
@@ -812,7 +837,15 @@ struct Stroker {
       stroke_type = -1;
       QuadStroke(quad, inner);
     }
-    if (!NormalUnit(quad[1], quad[2], &nBC, &uBC)) {
+    bool ok_bc;
+    if (ahead) {
+      ok_bc = ahead->ok_bc != 0;
+      nBC = ahead->n_bc;
+      uBC = ahead->u_bc;
+    } else {
+      ok_bc = NormalUnit(quad[1], quad[2], &nBC, &uBC);
+    }
+    if (!ok_bc) {
       nBC = nAB;
       uBC = uAB;
     }
@@ -1062,6 +1095,8 @@ struct WaveLds {
   float out[kSpecLanes * kSpecStride];
   int ok[kSpecLanes];
   int n[2][kSpecLanes];
+  int linear[kSpecLanes], ok_ab[kSpecLanes], ok_bc[kSpecLanes];
+  V leg[5][kSpecLanes];  // reduction, n_ab, u_ab, n_bc, u_bc
   int quad_op[kSpecLanes];  // chunk position of each quad segment
   Op ops[kChunkOps];
 };
@@ -1096,13 +1131,26 @@ HIPR_SDI void StrokeAhead(const Stroker& cfg, WaveLds* L, int lane) {
     *n = pw.nv;
     return !sp.overflow && sp.depth == 0 && pw.nv <= kSpecQuads;
   };
-  if (sp.CheckQuadLinear(quad, &reduction) == 2) {
+  reduction = V{0, 0};
+  const int linear = sp.CheckQuadLinear(quad, &reduction);
+  if (linear == 2) {
     ok = side(1, out, &n0);
     ok = side(-1, out + 4 * kSpecQuads, &n1) && ok;
   }
+  V n_ab{0, 0}, u_ab{0, 0}, n_bc{0, 0}, u_bc{0, 0};
+  const bool ok_ab = sp.NormalUnit(o.s, o.c, &n_ab, &u_ab);
+  const bool ok_bc = sp.NormalUnit(o.c, o.p, &n_bc, &u_bc);
   L->ok[lane] = ok;
   L->n[0][lane] = n0;
   L->n[1][lane] = n1;
+  L->linear[lane] = linear;
+  L->ok_ab[lane] = ok_ab;
+  L->ok_bc[lane] = ok_bc;
+  L->leg[0][lane] = reduction;
+  L->leg[1][lane] = n_ab;
+  L->leg[2][lane] = u_ab;
+  L->leg[3][lane] = n_bc;
+  L->leg[4][lane] = u_bc;
 }
 
 // The chunked feed.  `writer` marks the one lane that stores a chunk's calls,
@@ -1142,6 +1190,14 @@ HIPR_SDI void FeedChunks(Stroker* st, OpIter* it, WaveLds* L, bool writer, Sync 
         sv.n[1] = L->n[1][q];
         sv.q[0] = &L->out[q * kSpecStride];
         sv.q[1] = sv.q[0] + 4 * kSpecQuads;
+        sv.linear = L->linear[q];
+        sv.ok_ab = L->ok_ab[q];
+        sv.ok_bc = L->ok_bc[q];
+        sv.reduction = L->leg[0][q];
+        sv.n_ab = L->leg[1][q];
+        sv.u_ab = L->leg[2][q];
+        sv.n_bc = L->leg[3][q];
+        sv.u_bc = L->leg[4][q];
         st->QuadTo(op.c, op.p, &sv);
         q++;
       } else {
