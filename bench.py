@@ -41,11 +41,25 @@ def algorithmic_bytes(stage, st, geom_bytes, canvas_px):
 
 
 def geometry_bytes(sc):
-    """G of the scene: 8 B per point, 1 B per verb, 4 B per conic weight and
-    a 32-byte draw record per path."""
+    """G of the scene: 8 B per point, 1 B per verb, 4 B per conic weight, a
+    32-byte draw record per path and 20 B per gradient stop (the scene has
+    no canvas matrix, so no 24-byte matrix records)."""
     s = sc.scene.contents
     n = s.n_paths
-    return 8 * s.point_offsets[n] + s.verb_offsets[n] + 4 * s.weight_offsets[n] + 32 * n
+    stops = 0
+    for p in range(n):
+        g = s.paints[p].gradient
+        if g >= 0:
+            stops += s.gradients[g].color_count
+    return (8 * s.point_offsets[n] + s.verb_offsets[n] + 4 * s.weight_offsets[n] + 32 * n
+            + 20 * stops)
+
+
+def frame_alg_bytes(sc, st, blur_bytes=0):
+    """Algorithmic bytes of one frame: G + 8 U + B (G the scene's geometry and
+    paints, U the distinct target pixels the frame touches, each read and
+    written once, B 16 bytes per pixel of every blurred layer)."""
+    return geometry_bytes(sc) + 8 * st["pixels"] + blur_bytes
 
 
 PMC_SUMMARY = os.path.join(HERE, "profiles", "pmc_curves_strokes_4096.json")
@@ -316,17 +330,36 @@ def main():
             "stages": stages}
     best = max(out_forms, key=lambda f: out_forms[f]["paths_per_s"])
     wall, stage_tot, st = results[best]
-    # dominant kernel of the reported form and its roofline
+    # The timed frames are checked: the learned frame must equal the exact
+    # frame byte for byte.
+    same = None
+    if "exact" in results and "learned" in results:
+        frames = {}
+        for form, mode in (("exact", abi.MODE_EXACT), ("learned", abi.MODE_LEARNED)):
+            surf.clear(0)
+            surf.draw(prep, mode=mode)
+            frames[form] = surf.read()
+        same = bool(np.array_equal(frames["exact"], frames["learned"]))
+        if not same:
+            nd = int((frames["exact"] != frames["learned"]).any(-1).sum())
+            print(f"bench: the learned frame differs from the exact frame at {nd} pixels",
+                  file=sys.stderr, flush=True)
+            sys.exit(3)
+    # Roofline of the dominant stage (the walk pass): the frame's algorithmic
+    # bytes G + 8 U + B over that stage's average launch time, against HBM.
     dom = max((k for k in stage_tot if stage_tot[k][1]), key=lambda k: stage_tot[k][0])
     avg_launch_ms = stage_tot[dom][0] / stage_tot[dom][1]
-    alg = algorithmic_bytes(dom, st, geom, sc.width * sc.height)
-    roof = None
-    if alg is not None:
-        achieved = alg / (avg_launch_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": measured_traffic(dom, best), "alg_bytes_per_launch": int(alg),
-                "avg_launch_ms": round(avg_launch_ms, 4)}
+    alg = frame_alg_bytes(sc, st)
+    model = algorithmic_bytes(dom, st, geom, sc.width * sc.height)
+    achieved = alg / (avg_launch_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 7),
+            "traffic": measured_traffic(dom, best),
+            "traffic_source": "committed profile " + os.path.relpath(PMC_SUMMARY, HERE),
+            "alg_bytes": int(alg), "alg_bytes_def": "G + 8 U + B per frame",
+            "avg_launch_ms": round(avg_launch_ms, 4),
+            "traffic_model_bytes": int(model) if model is not None else None,
+            "traffic_model_def": "32 B per edge + 256 B per tile plane (walk)"}
     ref = out_forms.get("exact", out_forms[best])
     out = {
         "metric": METRIC, "value": out_forms[best]["paths_per_s"], "unit": "paths/s",
@@ -340,6 +373,7 @@ def main():
                    "seed": scenes.SEED_BASE + WORKLOAD, "form": best},
         "mpix_per_s": out_forms[best]["mpix_per_s"],
         "speedup_vs_exact": round(out_forms[best]["paths_per_s"] / ref["paths_per_s"], 3),
+        "learned_equals_exact": same,
         "frame": {"edges": st["edges"], "tile_planes": st["tile_planes"],
                   "blends": st["blends"], "pixels": st["pixels"],
                   "device_ms": round(st["ms_total"], 3)},

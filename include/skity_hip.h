@@ -242,6 +242,35 @@ HIPR_API hipr_result hipr_surface_get_job_counts(hipr_surface surface, int32_t* 
 HIPR_API hipr_result hipr_surface_get_job_profile(hipr_surface surface, int32_t* steps,
                                                   int32_t* time_10ns, int32_t* phases,
                                                   int32_t* max_active, int32_t n_jobs);
+/* Raster jobs of the last draw on the surface, in draw order: the DrawPath
+ * call each comes from, its kind (0 fill, 1 stroke outline, 2 composite of a
+ * blurred mask-filter layer) and its canvas (0 the surface, >= 1 the layers).
+ * *n_jobs receives the count; at most `cap` entries are written to each
+ * non-NULL array.  No reference counterpart: SWCanvas::OnDrawPath
+ * (src/render/sw/sw_canvas.cc:357-411) makes the same rasterisations one
+ * after another without naming them. */
+HIPR_API hipr_result hipr_surface_get_jobs(hipr_surface surface, int32_t* path,
+                                          int32_t* kind, int32_t* canvas, int32_t cap,
+                                          int32_t* n_jobs);
+/* Edge records of one job of the last draw, in emission order, 16 int32 each
+ * (the hipr_debug_build_edges layout): the edges SWEdgeBuilder::BuildEdges
+ * (src/render/sw/sw_edge.cc:299-336) builds for the fill or the stroke outline.
+ * *n_edges receives the count; edges may be NULL to query it.  Valid until
+ * the next draw on the surface's context (then HIPR_ERROR_INVALID_ARGUMENT). */
+HIPR_API hipr_result hipr_surface_get_job_edges(hipr_surface surface, int32_t job,
+                                               int32_t* edges, int64_t cap,
+                                               int64_t* n_edges);
+/* Tile bins of the last draw: for every job, the 16x16 tiles of its canvas
+ * that hold coverage, as codes ((tile_y * tiles_x + tile_x) << 1 | plane),
+ * ascending; offsets[j]..offsets[j+1] (n_jobs + 1 entries) delimit job j.
+ * Plane 0 holds each pixel's first non-zero cover, plane 1 its accumulated
+ * cover where a directly emitted span came first (the two span kinds of
+ * RealSpanBuilder / SpanBuilder, src/render/sw/sw_raster.cc:19-136).  The
+ * software backend has spans, not tiles; the bins are this integer function
+ * of its spans.  bins may be NULL to query the sizes.  Valid until the next
+ * draw on the surface's context. */
+HIPR_API hipr_result hipr_surface_get_tile_bins(hipr_surface surface, int64_t* offsets,
+                                               int32_t* bins, int64_t cap, int32_t n_jobs);
 /* Predictions (n x 2, log1p units) for host-side features. */
 HIPR_API hipr_result hipr_presizer_predict(hipr_context ctx, const float* params,
                                            const float* features, int64_t n, float* out);

@@ -22,6 +22,11 @@ extern "C" {
 ORC_API int orc_render_scene(const hipr_scene* scene, uint8_t* pixels,
                              int32_t width, int32_t height);
 
+/* orc_render_scene with the draws rasterised on `threads` threads and their
+ * spans applied in draw order per row band: the same image. */
+ORC_API int orc_render_scene_mt(const hipr_scene* scene, uint8_t* pixels,
+                                int32_t width, int32_t height, int32_t threads);
+
 /* Edge records of path p after Stroke::QuadPath + SWEdgeBuilder::BuildEdges,
  * 16 int32 per edge in the hipr_debug_build_edges layout.  stroke != 0 builds
  * the edges of the stroke outline instead of the fill.  Returns the number of
@@ -34,6 +39,16 @@ ORC_API int64_t orc_path_edges(const hipr_scene* scene, int32_t p,
 ORC_API int64_t orc_path_spans(const hipr_scene* scene, int32_t p,
                                int32_t stroke, int32_t* out, int64_t cap,
                                float* bounds);
+
+/* Tile bins of one raster (fill, or stroke outline if stroke != 0) of path p
+ * on a width x height canvas: 16x16 tiles, plane 0 = each pixel's first
+ * non-zero cover, plane 1 = its second (the accumulated cover after a direct
+ * one); codes ((tile_y * tiles_x + tile_x) << 1 | plane), ascending.
+ * *stacked receives the pixels with two direct covers (expected 0).
+ * Returns the bin count (only cap are written). */
+ORC_API int64_t orc_path_tile_bins(const hipr_scene* scene, int32_t p, int32_t stroke,
+                                   int32_t width, int32_t height, int32_t* out,
+                                   int64_t cap, int64_t* stacked);
 
 /* Number of quads Cubic::ToQuads produces for one cubic. */
 ORC_API int32_t orc_cubic_quad_count(const float* pts8);

@@ -8,6 +8,7 @@
 
 namespace orc {
 void RenderScene(const hipr_scene* s, uint8_t* px, int w, int h);
+void RenderSceneParallel(const hipr_scene* s, uint8_t* px, int w, int h, int threads);
 void PathSpans(const hipr_scene* s, int p, int stroke, std::vector<Span>* spans,
                float bounds[4], std::vector<Edge>* edges);
 void StackBlur(const uint8_t* src, uint8_t* dst, int w, int h, int radius);
@@ -60,6 +61,13 @@ int orc_render_scene(const hipr_scene* scene, uint8_t* pixels, int32_t width,
   return 0;
 }
 
+int orc_render_scene_mt(const hipr_scene* scene, uint8_t* pixels, int32_t width,
+                        int32_t height, int32_t threads) {
+  if (!scene || !pixels || width <= 0 || height <= 0) return -2;
+  RenderSceneParallel(scene, pixels, width, height, threads);
+  return 0;
+}
+
 int64_t orc_path_edges(const hipr_scene* scene, int32_t p, int32_t stroke,
                        int32_t* out, int64_t cap) {
   std::vector<Span> spans;
@@ -84,6 +92,57 @@ int64_t orc_path_spans(const hipr_scene* scene, int32_t p, int32_t stroke,
     out[4 * i + 3] = spans[i].cover;
   }
   return (int64_t)spans.size();
+}
+
+// Tile bins of one raster of path p on a width x height canvas (the
+// definition the device's 16x16 coverage tiles follow).  Per visible pixel,
+// the non-zero covers the reference's span builders emit for it, in
+// emission order (RealSpanBuilder spans first, then the row accumulator's
+// flush, sw_raster.cc:19-136); spans of cover 0 blend nothing and are left
+// out.  Plane 0 of a tile holds the first cover of each pixel, plane 1 the
+// second.  A tile's bin (plane k) exists iff some pixel of the tile has a
+// k-th cover.  Codes: ((tile_y * tiles_x + tile_x) << 1 | plane), ascending.
+// *stacked receives the number of pixels with more than one direct cover
+// (the device stores one direct cover per pixel; the reference never emits
+// two for one pixel, since full-row spans of different intervals are
+// disjoint and too-close edges are routed to the accumulator).
+int64_t orc_path_tile_bins(const hipr_scene* scene, int32_t p, int32_t stroke,
+                           int32_t width, int32_t height, int32_t* out, int64_t cap,
+                           int64_t* stacked) {
+  std::vector<Span> spans;
+  float b[4];
+  PathSpans(scene, p, stroke, &spans, b, nullptr);
+  const int tiles_x = (width + 15) / 16;
+  std::vector<uint8_t> direct_count((size_t)width * height, 0);
+  std::vector<uint8_t> has_acc((size_t)width * height, 0);
+  int64_t n_stacked = 0;
+  for (const Span& s : spans) {
+    if (s.cover == 0 || s.y < 0 || s.y >= height) continue;
+    int x0 = std::max(s.x, 0), x1 = std::min(s.x + s.len, width);
+    for (int x = x0; x < x1; x++) {
+      size_t i = (size_t)s.y * width + x;
+      if (s.accumulated) {
+        has_acc[i] = 1;
+      } else {
+        if (direct_count[i] == 1) n_stacked++;
+        if (direct_count[i] < 2) direct_count[i]++;
+      }
+    }
+  }
+  std::vector<int32_t> codes;
+  for (int y = 0; y < height; y++) {
+    for (int x = 0; x < width; x++) {
+      size_t i = (size_t)y * width + x;
+      int covers = (direct_count[i] ? 1 : 0) + (has_acc[i] ? 1 : 0);
+      int tile = (y >> 4) * tiles_x + (x >> 4);
+      for (int k = 0; k < covers; k++) codes.push_back(tile << 1 | k);
+    }
+  }
+  std::sort(codes.begin(), codes.end());
+  codes.erase(std::unique(codes.begin(), codes.end()), codes.end());
+  if (stacked) *stacked = n_stacked;
+  for (int64_t i = 0; i < (int64_t)codes.size() && i < cap; i++) out[i] = codes[i];
+  return (int64_t)codes.size();
 }
 
 int32_t orc_cubic_quad_count(const float* pts8) {

@@ -25,12 +25,18 @@ def lib():
         L = ctypes.CDLL(path)
         SP = ctypes.POINTER(abi.Scene)
         L.orc_render_scene.argtypes = [SP, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+        L.orc_render_scene_mt.argtypes = [SP, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32]
         L.orc_path_edges.restype = ctypes.c_int64
         L.orc_path_edges.argtypes = [SP, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                      ctypes.c_int64]
         L.orc_path_spans.restype = ctypes.c_int64
         L.orc_path_spans.argtypes = [SP, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                      ctypes.c_int64, ctypes.c_void_p]
+        L.orc_path_tile_bins.restype = ctypes.c_int64
+        L.orc_path_tile_bins.argtypes = [SP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_int64)]
         L.orc_cubic_quad_count.restype = ctypes.c_int32
         L.orc_cubic_quad_count.argtypes = [ctypes.c_void_p]
         L.orc_stackblur.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
@@ -58,6 +64,16 @@ def render(scene_ptr, width, height, base=None):
     return img
 
 
+def render_mt(scene_ptr, width, height, threads=None, base=None):
+    """render() on several threads (same image)."""
+    if threads is None:
+        threads = max(1, min(16, os.cpu_count() or 1))
+    img = np.zeros((height, width, 4), np.uint8) if base is None else base.copy()
+    rc = lib().orc_render_scene_mt(scene_ptr, img.ctypes.data, width, height, threads)
+    assert rc == 0
+    return img
+
+
 def path_edges(scene_ptr, p, stroke=0):
     n = lib().orc_path_edges(scene_ptr, p, stroke, None, 0)
     out = np.zeros((max(n, 1), 16), np.int32)
@@ -71,6 +87,17 @@ def path_spans(scene_ptr, p, stroke=0):
     out = np.zeros((max(n, 1), 4), np.int32)
     lib().orc_path_spans(scene_ptr, p, stroke, out.ctypes.data, n, b.ctypes.data)
     return out[:n], b
+
+
+def path_tile_bins(scene_ptr, p, stroke, width, height):
+    """Tile-bin codes of one raster of path p (see orc_path_tile_bins) and the
+    number of pixels with two direct covers."""
+    st = ctypes.c_int64()
+    n = lib().orc_path_tile_bins(scene_ptr, p, stroke, width, height, None, 0, ctypes.byref(st))
+    out = np.zeros(max(n, 1), np.int32)
+    lib().orc_path_tile_bins(scene_ptr, p, stroke, width, height, out.ctypes.data, n,
+                             ctypes.byref(st))
+    return out[:n], st.value
 
 
 def stackblur(img, radius):

@@ -478,9 +478,9 @@ struct SpanSink {
   float scan_top;
   std::vector<Span> spans;
   // RealSpanBuilder::BuildSpan (sw_raster.cc:26-42)
-  void Real(int x, int y, int width, int alpha) {
+  void Real(int x, int y, int width, int alpha, int accumulated = 0) {
     if (y < scan_top) return;
-    spans.push_back(Span{x, y, width, alpha});
+    spans.push_back(Span{x, y, width, alpha, accumulated});
   }
   void RealSpans(int x, int y, const Alpha* aa, int len) {
     if (y < scan_top) return;
@@ -506,7 +506,7 @@ struct SpanSink {
             break;
           }
         } while (alphas[curr]);
-        Real(left + (int)start, curr_y, (int)width, a);
+        Real(left + (int)start, curr_y, (int)width, a, 1);
       } else {
         curr++;
       }

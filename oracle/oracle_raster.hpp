@@ -60,6 +60,9 @@ struct Edge {
 
 struct Span {
   int32_t x, y, len, cover;
+  // 1 if the span comes from SpanBuilder's row accumulator (its flush), 0 if
+  // it went straight to RealSpanBuilder (sw_raster.cc:19-136).
+  int32_t accumulated;
 };
 
 // Convert a scene path into PathData.

@@ -5,8 +5,11 @@
 // HandleFilter).  Exposed through the C functions in oracle.h, which only
 // tests/, smoke() and the CPU-baseline leg of bench.py call.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "oracle.h"
@@ -545,6 +548,133 @@ void RenderScene(const hipr_scene* s, uint8_t* px, int w, int h) {
       PathData path = ScenePath(s, p);
       DrawPathNoFilter(s, path, paint, ident, &t);
     }
+  }
+}
+
+// RenderScene with the work split over threads; the result is the same image.
+// Rasterising a draw reads nothing but its own path and paint, so the draws
+// of a chunk are rasterised concurrently into span lists (and blurred
+// layers); the spans are then applied in draw order, each thread owning a
+// band of rows, which keeps every pixel's blend sequence (rows are
+// independent, and a stable sort by row keeps each row's emission order).
+namespace {
+struct SpanOp {
+  std::vector<Span> spans;  // sorted by row, emission order within a row
+  std::vector<size_t> row_start;  // first span of each row offset from y0
+  int y0 = 0;
+  Brush br;
+  std::vector<uint8_t> layer;  // pixels of an image brush
+};
+
+void FinishOp(SpanOp* op) {
+  std::stable_sort(op->spans.begin(), op->spans.end(),
+                   [](const Span& a, const Span& b) { return a.y < b.y; });
+  if (op->spans.empty()) return;
+  op->y0 = op->spans.front().y;
+  int y1 = op->spans.back().y + 1;
+  op->row_start.assign((size_t)(y1 - op->y0) + 1, 0);
+  size_t i = 0;
+  for (int y = op->y0; y <= y1; y++) {
+    while (i < op->spans.size() && op->spans[i].y < y) i++;
+    op->row_start[y - op->y0] = i;
+  }
+}
+
+void ShapeOps(const hipr_scene* s, int p, std::vector<std::unique_ptr<SpanOp>>* ops) {
+  const hipr_paint& paint = s->paints[p];
+  if (paint.blur_radius > 0) {
+    auto op = std::make_unique<SpanOp>();
+    int fb[4];
+    BlurredLayer(s, p, &op->layer, fb);
+    int W = fb[2] - fb[0], H = fb[3] - fb[1];
+    if (W <= 0 || H <= 0) return;
+    PathData rect;
+    float L = (float)fb[0], T = (float)fb[1], R = (float)fb[2], B = (float)fb[3];
+    rect.MoveTo(Pt{L, T});
+    rect.LineTo(Pt{R, T});
+    rect.LineTo(Pt{R, B});
+    rect.LineTo(Pt{L, B});
+    rect.Close();
+    float bounds[4];
+    RastePath(rect, &op->spans, bounds, nullptr);
+    op->br.kind = 2;
+    op->br.img = op->layer.data();
+    op->br.img_w = W;
+    op->br.img_h = H;
+    Aff inv;
+    Invert(Translate(L, T), &inv);
+    op->br.img_ptu = Mul(Scale(1.f / W, 1.f / H), inv);
+    float alpha_f = paint.style == HIPR_STYLE_STROKE ? paint.stroke_color[3] : paint.color[3];
+    op->br.global_alpha = (uint8_t)(255 * alpha_f);
+    FinishOp(op.get());
+    ops->push_back(std::move(op));
+    return;
+  }
+  PathData path = ScenePath(s, p);
+  Aff ident;
+  auto shape = [&](bool stroke) {
+    auto op = std::make_unique<SpanOp>();
+    op->br = MakeBrush(s, paint, ident, stroke);
+    float bounds[4];
+    if (!stroke) {
+      RastePath(path, &op->spans, bounds, nullptr);
+    } else {
+      PathData outline = StrokeOutline(QuadPath(path), paint);
+      RastePath(outline, &op->spans, bounds, nullptr);
+    }
+    FinishOp(op.get());
+    ops->push_back(std::move(op));
+  };
+  bool need_fill = paint.style != HIPR_STYLE_STROKE;
+  bool need_stroke = paint.style != HIPR_STYLE_FILL;
+  if (paint.style == HIPR_STYLE_STROKE_THEN_FILL) {
+    if (need_stroke) shape(true);
+    if (need_fill) shape(false);
+  } else {
+    if (need_fill) shape(false);
+    if (need_stroke) shape(true);
+  }
+}
+
+template <class F>
+void ParallelFor(int n, int threads, F&& fn) {
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; t++)
+    pool.emplace_back([&]() {
+      for (int i; (i = next.fetch_add(1)) < n;) fn(i);
+    });
+  for (auto& th : pool) th.join();
+}
+}  // namespace
+
+void RenderSceneParallel(const hipr_scene* s, uint8_t* px, int w, int h, int threads) {
+  if (threads <= 1) {
+    RenderScene(s, px, w, h);
+    return;
+  }
+  Target t{px, w, h};
+  const int chunk = 8 * threads;
+  constexpr int kBand = 16;
+  const int bands = (h + kBand - 1) / kBand;
+  for (int p0 = 0; p0 < s->n_paths; p0 += chunk) {
+    const int n = std::min(chunk, s->n_paths - p0);
+    std::vector<std::vector<std::unique_ptr<SpanOp>>> ops(n);
+    ParallelFor(n, threads, [&](int i) { ShapeOps(s, p0 + i, &ops[i]); });
+    ParallelFor(bands, threads, [&](int b) {
+      const int y0 = b * kBand, y1 = std::min(h, y0 + kBand);
+      for (auto& draw : ops) {
+        for (auto& op : draw) {
+          if (op->spans.empty()) continue;
+          const int oy1 = op->y0 + (int)op->row_start.size() - 1;
+          const int a = std::max(y0, op->y0), e = std::min(y1, oy1);
+          if (a >= e) continue;
+          std::vector<Span> rows(op->spans.begin() + op->row_start[a - op->y0],
+                                 op->spans.begin() + op->row_start[e - op->y0]);
+          ApplySpans(rows, op->br, &t);
+        }
+      }
+    });
   }
 }
 

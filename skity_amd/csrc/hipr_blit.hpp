@@ -23,21 +23,32 @@ typedef uint8_t Alpha;
 
 constexpr int kDirectCount = 1 << 12;  // one direct span, in the D field
 
+// The row covers the visible pixels [left, left + width) of the path's
+// canvas.  A span's two delta entries are clamped into [0, width]: deltas left
+// of the canvas fold into entry 0 and deltas right of it into entry `width`,
+// which no visible pixel's prefix sum reads, so every visible pixel sums the
+// same deltas as over the path's full width.
 struct RowSink {
   int32_t* d;
-  int32_t left;   // pixel x of delta index 0
+  int32_t left;   // pixel x of delta index 0 (the first visible column)
+  int32_t width;  // visible columns; entry `width` is the end-of-row entry
+  __device__ __forceinline__ int Clamp(int i) const {
+    return i < 0 ? 0 : (i > width ? width : i);
+  }
   __device__ __forceinline__ void Direct(int x, int len, int v) {
     if (v == 0 || len <= 0) return;
-    int i = x - left;
+    int i = Clamp(x - left), e = Clamp(x - left + len);
+    if (i == e) return;
     int inc = (kDirectCount + v) << 16;
     atomicAdd(&d[i], inc);
-    atomicAdd(&d[i + len], -inc);
+    atomicAdd(&d[e], -inc);
   }
   __device__ __forceinline__ void Acc(int x, int len, int v) {
     if (v == 0 || len <= 0) return;
-    int i = x - left;
+    int i = Clamp(x - left), e = Clamp(x - left + len);
+    if (i == e) return;
     atomicAdd(&d[i], v);
-    atomicAdd(&d[i + len], -v);
+    atomicAdd(&d[e], -v);
   }
   __device__ __forceinline__ void Put(bool direct, int x, int len, int v) {
     if (direct) {
@@ -140,9 +151,12 @@ __device__ void BlitAAARow(RowSink* s, Fixed ul, Fixed ur, Fixed ll, Fixed lr, F
   Fixed ab_l = ur - IntToFixed(uR), ab_r = lr - IntToFixed(uR);
   int ab_R = FixedCeilToInt(ab_r);
   // Run of equal full alphas between the two AA regions goes out as one
-  // delta span; every other pixel individually.
+  // delta span; every other pixel individually.  Pixels outside the visible
+  // columns add nothing any visible pixel reads (RowSink), so they are
+  // skipped.
   int run_begin = -1, run_len = 0;
-  for (int i = 0; i < len; i++) {
+  const int i_end = Min32(len, s->left + s->width - L);
+  for (int i = Max32(0, s->left - L); i < i_end; i++) {
     int x = L + i;
     Alpha a = full;
     bool touched = false;

@@ -12,6 +12,9 @@ namespace hipr {
 
 constexpr int kWave = 64;
 constexpr int kTile = 16;  // coverage tiles are 16x16 pixels
+// Widest canvas (and mask-filter layer): the walk keeps one LDS row of the
+// visible columns, 4 bytes per pixel.
+constexpr int kMaxCanvasSide = 16384;
 
 // Geometry of a set of paths in skity's Path layout (structure of arrays).
 struct GeomSrc {

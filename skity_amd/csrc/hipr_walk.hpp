@@ -87,15 +87,22 @@ struct WalkParams {
 };
 
 // Job pixel bounds as RastePath computes them (sw_raster.cc:741-754).
+// Device-space coordinates beyond +-kCoordLimit are refused before any walk
+// (the reference's float -> 26.6 conversion overflows there); the clamp keeps
+// every width and height of a refused job within int32.
+constexpr float kCoordLimit = 8388608.f;  // 2^23 px
 struct PixBounds {
   int l, t, r, b;
   bool empty;
 };
+__host__ __device__ inline float ClampCoord(float v) {
+  return v < -kCoordLimit ? -kCoordLimit : (v > kCoordLimit ? kCoordLimit : v);
+}
 __host__ __device__ inline PixBounds JobPixBounds(const JobGeo& g) {
   PixBounds pb;
   bool ok = g.l < g.r && g.t < g.b;
-  float l = ok ? floorf(g.l) : 0.f, t = ok ? floorf(g.t) : 0.f;
-  float r = ok ? ceilf(g.r) : 0.f, b = ok ? ceilf(g.b) : 0.f;
+  float l = ok ? floorf(ClampCoord(g.l)) : 0.f, t = ok ? floorf(ClampCoord(g.t)) : 0.f;
+  float r = ok ? ceilf(ClampCoord(g.r)) : 0.f, b = ok ? ceilf(ClampCoord(g.b)) : 0.f;
   pb.empty = !(l < r && t < b);
   pb.l = (int)l;
   pb.t = (int)t;
@@ -215,7 +222,11 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   const Canvas cv = P.canvases[job.canvas];
   PixBounds pb = JobPixBounds(geo);
   TileRange tr = JobTiles(pb, cv.width, cv.height);
-  int W = pb.r - pb.l;
+  // The LDS row holds the visible columns [ox, ox + W) of the canvas
+  // (RowSink folds the deltas of off-canvas pixels), so its width is bounded
+  // by the canvas, not by the path.
+  const int ox = pb.l > 0 ? pb.l : 0;
+  const int W = (pb.r < cv.width ? pb.r : cv.width) - ox;
   int flags = 0;
   int n_tiles = 0, max_active = 0;
   int nE = geo.edges;
@@ -279,7 +290,7 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   const int winding_mask = job.fill == HIPR_FILL_EVEN_ODD ? 1 : -1;
   const int cx0 = pb.l > 0 ? pb.l : 0, cx1 = pb.r < cv.width ? pb.r : cv.width;
   const int cy0 = pb.t > 0 ? pb.t : 0, cy1 = pb.b < cv.height ? pb.b : cv.height;
-  RowSink sink{L.d, pb.l};
+  RowSink sink{L.d, ox, W};
 
   // Active list registers.
   Fixed X[K], DX[K], DY[K], LY[K];
@@ -354,17 +365,17 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
       band = rb;
     }
     // canvas-aligned 16-pixel chunks; lane owns 4 pixels
-    int base = (pb.l + lo) & ~15;
-    int end = pb.l + hi;  // exclusive pixel bound of the dirty range
+    int base = (ox + lo) & ~15;
+    int end = ox + hi;  // exclusive pixel bound of the dirty range
     int carry = 0;
-    // deltas left of `base` are zero: base <= pb.l + lo
+    // deltas left of `base` are zero: base <= ox + lo
     for (int x0 = base; x0 <= end; x0 += 4 * kWave) {
       int px = x0 + 4 * lane;
       int32_t vS[4];
       int sS = 0;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        int i = px + q - pb.l;
+        int i = px + q - ox;
         int dv = 0;
         if (i >= 0 && i <= W) {
           dv = L.d[i];
@@ -759,10 +770,10 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
         bool no_real = full == 0xFF && (too_close_prev || too_close_next);
         BlitTrapezoidRow(&sink, left, right, next_left, next_right, ldy, DY[s], full,
                          no_real);
-        int lo = FixedFloorToInt(Min32(Min32(left, next_left), Min32(right, next_right))) - pb.l;
-        int hi = FixedCeilToInt(Max32(Max32(left, next_left), Max32(right, next_right))) - pb.l;
-        dmin = Min32(dmin, lo);
-        dmax = Max32(dmax, hi);
+        int lo = FixedFloorToInt(Min32(Min32(left, next_left), Min32(right, next_right))) - ox;
+        int hi = FixedCeilToInt(Max32(Max32(left, next_left), Max32(right, next_right))) - ox;
+        dmin = Min32(dmin, Max32(lo, 0));
+        dmax = Max32(dmax, Min32(hi, W));
       }
       // carries
       carry_w = __builtin_amdgcn_readlane(incl, kWave - 1);
@@ -792,8 +803,8 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
       if (lane == 0) {
         BlitTrapezoidRow(&sink, left, right_clip, Max32(left_clip, carry_lxn), right_clip,
                          carry_ldy, 0, full, full == 0xFF);
-        int lo = FixedFloorToInt(Min32(left, Max32(left_clip, carry_lxn))) - pb.l;
-        dmin = Min32(dmin, lo);
+        int lo = FixedFloorToInt(Min32(left, Max32(left_clip, carry_lxn))) - ox;
+        dmin = Min32(dmin, Max32(lo, 0));
         dmax = Max32(dmax, W);
       }
     }

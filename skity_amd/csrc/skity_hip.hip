@@ -282,6 +282,7 @@ struct hipr_context_s {
   JobCov* h_wcov = nullptr;
   StrokeOut* h_so = nullptr;
   size_t h_cap = 0;
+  uint64_t draw_serial = 0;  // pipeline runs so far (see hipr_surface_s)
 };
 
 struct hipr_surface_s {
@@ -293,6 +294,13 @@ struct hipr_surface_s {
   std::vector<int32_t> job_steps, job_time;    // walk profile of the last draw
   std::vector<int32_t> job_phase;              // 5 per job
   std::vector<int32_t> job_active;             // largest active edge list
+  // Where the last draw left each job's edges (emission order, context
+  // buffer `edges`) and tile maps (context buffers `map0`/`map1`); valid while
+  // draw_serial equals the context's.
+  uint64_t draw_serial = 0;
+  std::vector<int32_t> job_path, job_kind, job_canvas;
+  std::vector<int64_t> job_ebase, job_mbase;
+  std::vector<int32_t> job_tiles;  // per job: tc0, tr0, ntc, ntr, canvas tiles_x
   double stage_ms[HIPR_STAGE_COUNT] = {};
   int32_t stage_launches[HIPR_STAGE_COUNT] = {};
 };
@@ -1129,16 +1137,28 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   for (int j = 0; j < nj; j++) {
     int64_t msz = 0;
     if (f.jobs[j].kind != kJobImage) {
-      PixBounds pb = JobPixBounds(ctx->h_geo[j]);
-      if (!pb.empty) lds_width = std::max(lds_width, pb.r - pb.l);
+      const JobGeo& g = ctx->h_geo[j];
+      if (g.l < g.r && g.t < g.b &&
+          (std::fabs(g.l) > kCoordLimit || std::fabs(g.r) > kCoordLimit ||
+           std::fabs(g.t) > kCoordLimit || std::fabs(g.b) > kCoordLimit)) {
+        // beyond 2^23 px the reference's float -> 26.6 conversion
+        // ((int)(4x * 64), sw_edge.cc:24-29) overflows int32
+        std::fprintf(stderr, "skity_hip: draw %d has device coordinates beyond 2^23 px\n",
+                     f.jobs[j].path);
+        return HIPR_ERROR_NOT_SUPPORTED;
+      }
+      PixBounds pb = JobPixBounds(g);
       const Canvas& cv = f.canvases[f.jobs[j].canvas];
       TileRange tr = JobTiles(pb, cv.width, cv.height);
+      // the walk's row holds the job's visible columns only
+      if (tr.ntc > 0)
+        lds_width = std::max(lds_width, std::min(pb.r, cv.width) - std::max(pb.l, 0));
       msz = (int64_t)tr.ntc * tr.ntr;
     }
     mbase[j + 1] = mbase[j] + msz;
   }
   lds_width = (lds_width + 255) & ~255;
-  if (lds_width > 16384) return HIPR_ERROR_CAPACITY;  // 2 x 64 KiB row buffers
+  if (lds_width > kMaxCanvasSide) return HIPR_ERROR_CAPACITY;  // a layer wider than a canvas
   int64_t total_map = mbase[nj];
   if (ctx->map0.Ensure(sizeof(int32_t) * std::max<int64_t>(total_map, 1)) != hipSuccess ||
       ctx->map1.Ensure(sizeof(int32_t) * std::max<int64_t>(total_map, 1)) != hipSuccess)
@@ -1556,6 +1576,29 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   surf->job_time.assign(nj, 0);
   surf->job_phase.assign((size_t)nj * 5, 0);
   surf->job_active.assign(nj, 0);
+  surf->draw_serial = ++ctx->draw_serial;
+  surf->job_path.assign(nj, 0);
+  surf->job_kind.assign(nj, 0);
+  surf->job_canvas.assign(nj, 0);
+  surf->job_ebase.assign(nj, 0);
+  surf->job_mbase.assign(nj, 0);
+  surf->job_tiles.assign((size_t)nj * 5, 0);
+  for (int j = 0; j < nj; j++) {
+    surf->job_path[j] = f.jobs[j].path;
+    surf->job_kind[j] = f.jobs[j].kind;
+    surf->job_canvas[j] = f.jobs[j].canvas;
+    if (f.jobs[j].kind == kJobImage) continue;
+    surf->job_ebase[j] = ebase[j];
+    surf->job_mbase[j] = mbase[j];
+    const Canvas& cv = f.canvases[f.jobs[j].canvas];
+    TileRange tr = JobTiles(JobPixBounds(ctx->h_geo[j]), cv.width, cv.height);
+    int32_t* t = &surf->job_tiles[(size_t)j * 5];
+    t[0] = tr.tc0;
+    t[1] = tr.tr0;
+    t[2] = tr.ntc;
+    t[3] = tr.ntr;
+    t[4] = cv.tiles_x;
+  }
   for (int j = 0; j < nj; j++) {
     if (f.jobs[j].kind == kJobImage) continue;
     surf->job_edges[j] = ctx->h_geo[j].edges;
@@ -1979,6 +2022,7 @@ HIPR_API hipr_result hipr_surface_create(hipr_context ctx, int32_t width, int32_
   *out = nullptr;
   if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
   if (width <= 0 || height <= 0) return HIPR_ERROR_INVALID_ARGUMENT;
+  if (width > kMaxCanvasSide || height > kMaxCanvasSide) return HIPR_ERROR_NOT_SUPPORTED;
   hipr_surface s = new (std::nothrow) hipr_surface_s();
   if (!s) return HIPR_ERROR_OUT_OF_HOST_MEMORY;
   s->ctx = ctx;
@@ -2116,6 +2160,82 @@ HIPR_API hipr_result hipr_surface_get_job_profile(hipr_surface s, int32_t* steps
       for (int k = 0; k < 5; k++) phases[(size_t)j * 5 + k] = s->job_phase[(size_t)j * 5 + k];
   }
   return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_surface_get_jobs(hipr_surface s, int32_t* path, int32_t* kind,
+                                          int32_t* canvas, int32_t cap, int32_t* n_jobs) {
+  if (!s) return HIPR_ERROR_INVALID_HANDLE;
+  if (!n_jobs) return HIPR_ERROR_INVALID_ARGUMENT;
+  *n_jobs = (int32_t)s->job_path.size();
+  if (cap < 0) return HIPR_ERROR_INVALID_ARGUMENT;
+  for (int j = 0; j < std::min(cap, *n_jobs); j++) {
+    if (path) path[j] = s->job_path[j];
+    if (kind) kind[j] = s->job_kind[j];
+    if (canvas) canvas[j] = s->job_canvas[j];
+  }
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_surface_get_job_edges(hipr_surface s, int32_t job, int32_t* edges,
+                                               int64_t cap, int64_t* n_edges) {
+  if (!s) return HIPR_ERROR_INVALID_HANDLE;
+  if (!n_edges || job < 0 || job >= (int32_t)s->job_path.size())
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  hipr_context ctx = s->ctx;
+  if (s->draw_serial == 0 || s->draw_serial != ctx->draw_serial)
+    return HIPR_ERROR_INVALID_ARGUMENT;  // another draw reused the buffers
+  const int64_t n = s->job_kind[job] == kJobImage ? 0 : s->job_edges[job];
+  *n_edges = n;
+  if (!edges || n == 0) return HIPR_SUCCESS;
+  if (cap < n) return HIPR_ERROR_CAPACITY;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(edges, ctx->edges.as<EdgeRec>() + s->job_ebase[job], sizeof(EdgeRec) * n,
+                    hipMemcpyDeviceToHost));
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_surface_get_tile_bins(hipr_surface s, int64_t* offsets, int32_t* bins,
+                                               int64_t cap, int32_t n_jobs) {
+  if (!s) return HIPR_ERROR_INVALID_HANDLE;
+  if (!offsets || n_jobs != (int32_t)s->job_path.size()) return HIPR_ERROR_INVALID_ARGUMENT;
+  hipr_context ctx = s->ctx;
+  if (s->draw_serial == 0 || s->draw_serial != ctx->draw_serial)
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  int64_t total_map = 0;
+  for (int j = 0; j < n_jobs; j++) {
+    const int32_t* t = &s->job_tiles[(size_t)j * 5];
+    total_map = std::max<int64_t>(total_map, s->job_mbase[j] + (int64_t)t[2] * t[3]);
+  }
+  std::vector<int32_t> m0(total_map), m1(total_map);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (total_map > 0) {
+    HIP_TRY(hipMemcpy(m0.data(), ctx->map0.p, sizeof(int32_t) * total_map, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(m1.data(), ctx->map1.p, sizeof(int32_t) * total_map, hipMemcpyDeviceToHost));
+  }
+  int64_t k = 0;
+  for (int j = 0; j < n_jobs; j++) {
+    offsets[j] = k;
+    if (s->job_kind[j] == kJobImage) continue;
+    const int32_t* t = &s->job_tiles[(size_t)j * 5];
+    for (int r = 0; r < t[3]; r++) {
+      for (int c = 0; c < t[2]; c++) {
+        int64_t i = s->job_mbase[j] + (int64_t)r * t[2] + c;
+        int32_t code = ((t[1] + r) * t[4] + (t[0] + c)) << 1;
+        if (m0[i] >= 0) {
+          if (bins && k < cap) bins[k] = code;
+          k++;
+        }
+        if (m1[i] >= 0) {
+          if (bins && k < cap) bins[k] = code | 1;
+          k++;
+        }
+      }
+    }
+  }
+  offsets[n_jobs] = k;
+  return bins && k > cap ? HIPR_ERROR_CAPACITY : HIPR_SUCCESS;
 }
 
 HIPR_API hipr_result hipr_presizer_predict(hipr_context ctx, const float* params,

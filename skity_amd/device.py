@@ -60,6 +60,16 @@ def lib():
         L.hipr_stackblur_ex.argtypes = [c_ctx, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.hipr_stackblur_ex.restype = ctypes.c_int
+        L.hipr_surface_get_jobs.argtypes = [c_surf, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.POINTER(ctypes.c_int32)]
+        L.hipr_surface_get_jobs.restype = ctypes.c_int
+        L.hipr_surface_get_job_edges.argtypes = [c_surf, ctypes.c_int32, ctypes.c_void_p,
+                                                 ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.hipr_surface_get_job_edges.restype = ctypes.c_int
+        L.hipr_surface_get_tile_bins.argtypes = [c_surf, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_int64, ctypes.c_int32]
+        L.hipr_surface_get_tile_bins.restype = ctypes.c_int
         c_batch = ctypes.c_void_p
         L.hipr_batch_create.argtypes = [c_ctx, ctypes.POINTER(c_batch)]
         L.hipr_batch_create.restype = ctypes.c_int
@@ -229,6 +239,42 @@ class Surface:
         s = abi.Stats()
         check(lib().hipr_surface_get_stats(self._h, ctypes.byref(s)), "hipr_surface_get_stats")
         return {k: getattr(s, k) for k, _ in abi.Stats._fields_}
+
+    def jobs(self):
+        """(path, kind, canvas) int32 arrays of the last draw's raster jobs."""
+        n = ctypes.c_int32()
+        check(lib().hipr_surface_get_jobs(self._h, None, None, None, 0, ctypes.byref(n)),
+              "hipr_surface_get_jobs")
+        path, kind, canvas = (np.zeros(n.value, np.int32) for _ in range(3))
+        check(lib().hipr_surface_get_jobs(self._h, path.ctypes.data, kind.ctypes.data,
+                                          canvas.ctypes.data, n.value, ctypes.byref(n)),
+              "hipr_surface_get_jobs")
+        return path, kind, canvas
+
+    def job_edges(self, job):
+        """Edge records (n x 16 int32, emission order) of one job of the last
+        draw."""
+        n = ctypes.c_int64()
+        check(lib().hipr_surface_get_job_edges(self._h, job, None, 0, ctypes.byref(n)),
+              "hipr_surface_get_job_edges")
+        out = np.zeros((max(n.value, 1), 16), np.int32)
+        check(lib().hipr_surface_get_job_edges(self._h, job, out.ctypes.data, n.value,
+                                               ctypes.byref(n)),
+              "hipr_surface_get_job_edges")
+        return out[:n.value]
+
+    def tile_bins(self):
+        """Per job of the last draw, the sorted tile-bin codes
+        ((tile_y * tiles_x + tile_x) << 1 | plane) as a list of arrays."""
+        nj = len(self.jobs()[0])
+        offs = np.zeros(nj + 1, np.int64)
+        check(lib().hipr_surface_get_tile_bins(self._h, offs.ctypes.data, None, 0, nj),
+              "hipr_surface_get_tile_bins")
+        bins = np.zeros(max(int(offs[-1]), 1), np.int32)
+        check(lib().hipr_surface_get_tile_bins(self._h, offs.ctypes.data, bins.ctypes.data,
+                                               int(offs[-1]), nj),
+              "hipr_surface_get_tile_bins")
+        return [bins[offs[j]:offs[j + 1]] for j in range(nj)]
 
     def stage_times(self):
         """{stage: (ms, launches)} of the last draw (HIP events on the stream)."""

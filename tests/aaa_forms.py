@@ -1,0 +1,180 @@
+"""Closed forms of the analytic-AA coverage for shapes whose exact area the
+reference's fixed-point rules determine, used to pin the per-pixel integer
+arithmetic of the oracle and of the device walk.
+
+The rules the forms use are the documented model of skity's software
+rasteriser, not its code:
+  * x is kept in 1/256 px (float -> 26.6 of 4x, sw_edge.cc:24-29), y is
+    snapped to 1/4 px (SWEdge::SnapY, sw_edge.hpp:36-41);
+  * a pixel's horizontal cover is its covered area in 1/256 units (a full
+    pixel saturates the 8-bit alpha at 255);
+  * the half-plane of a 45-degree edge covers a triangle per pixel whose leg
+    is measured in 1/32 px, so the area is floor(n^2 / 8) for a leg of n/32 px
+    (partial_triangle_to_alpha, sw_raster.cc:272-278);
+  * a row covered over h quarters is walked in sub-steps of 1/4 or 1/2 row
+    (h = 1: 1/4, 2: 1/2, 3: 1/4 + 1/2, sw_raster.cc:584-591); each sub-step
+    of height s scales a pixel's cover by round(255 s) in {64, 128} as
+    (cover * that) >> 8 and the sub-steps' results add up (SpanBuilder's
+    saturating accumulation); a full row emits its covers once.
+Pixels of an opaque white paint then read (255 * cover) >> 8 per channel, or
+255 for a cover of 255 (AlphaMulQ, color_priv.hpp:62-68).
+
+A one-LSB slip in any of these steps (a rounding of the sub-step height, a
+scale by 255 instead of 256, a leg rounded instead of truncated) changes
+pixels these forms predict.
+"""
+import numpy as np
+
+from skity_amd import abi, canvas
+
+STEP_SCALE = {1: (64,), 2: (128,), 3: (64, 128)}  # sub-step alphas per quarters
+
+
+def _row_quarters(y0q, y1q, r):
+    """Quarters of pixel row r inside [y0q/4, y1q/4)."""
+    return max(0, min(y1q, 4 * r + 4) - max(y0q, 4 * r))
+
+
+# The rules above; tests substitute slipped variants to show that the forms
+# tell them apart.
+RULES = {
+    "step_scale": STEP_SCALE,                        # round(255 s) per sub-step
+    "scale": lambda c, s: (c * s) >> 8,              # sub-step scaling of a cover
+    "area": lambda f: 256 - f,                       # 1/256 cover of [f/256, 1)
+    "leg": lambda f: f // 8,                         # leg of f/256 px in 1/32 px
+    "tri": lambda n: (n * n) // 8,                   # triangle of leg n/32 px
+}
+
+
+def rect_covers(a, f0, b, f1, y0q, y1q, rules=RULES):
+    """{(x, y): cover} of the rectangle [a + f0/256, b + f1/256) x
+    [y0q/4, y1q/4), a <= b, 0 <= f0, f1 < 256, (a, f0) < (b, f1)."""
+    steps, scale = rules["step_scale"], rules["scale"]
+    horiz = {}  # x -> (cover, pixel entirely inside the rectangle's columns)
+    if a == b:
+        horiz[a] = (f1 - f0, False)
+    else:
+        first = a
+        if f0 > 0:
+            horiz[a] = (rules["area"](f0), False)
+            first = a + 1
+        for x in range(first, b):
+            horiz[x] = (255, True)
+        if f1 > 0:
+            horiz[b] = (f1, False)
+    out = {}
+    for r in range(y0q // 4, (y1q + 3) // 4):
+        h = _row_quarters(y0q, y1q, r)
+        if h == 0:
+            continue
+        for x, (c, inner) in horiz.items():
+            if h == 4:
+                v = c
+            elif inner:
+                # a run of whole pixels takes each sub-step's alpha as is
+                v = sum(steps[h])
+            else:
+                v = sum(scale(c, s) for s in steps[h])
+            if v:
+                out[(x, r)] = min(v, 255)
+    return out
+
+
+def parallelogram_covers(a, f, y0, w, h, direction, rules=RULES):
+    """{(x, y): cover} of the parallelogram whose left and right edges run at
+    45 degrees (direction +1: down-right, -1: down-left), top-left corner at
+    (a + f/256, y0), width w >= 2 px, h rows (y0, h integers)."""
+    n1 = rules["leg"](256 - f)  # leg of the first pixel's triangle, 1/32 px
+    n2 = rules["leg"](f)        # leg of the second pixel's triangle
+    tri = rules["tri"]
+    out = {}
+    for k in range(h):
+        # pixel column holding the left edge's x at the row's top (down-right)
+        # or bottom (down-left): the two pixels the edge crosses are A, A + 1
+        A = a + k if direction > 0 else a - k - 1
+        if f == 0:
+            out[(A, y0 + k)] = 128
+            for x in range(A + 1, A + w):
+                out[(x, y0 + k)] = 255
+            out[(A + w, y0 + k)] = 128
+        else:
+            out[(A, y0 + k)] = tri(n1)
+            out[(A + 1, y0 + k)] = 255 - tri(n2)
+            for x in range(A + 2, A + w):
+                out[(x, y0 + k)] = 255
+            out[(A + w, y0 + k)] = 255 - tri(n1)
+            out[(A + w + 1, y0 + k)] = tri(n2)
+    return {k: v for k, v in out.items() if v}
+
+
+def pixels_of(covers, width, height):
+    """RGBA8 image of opaque white drawn with the given covers."""
+    img = np.zeros((height, width, 4), np.uint8)
+    for (x, y), c in covers.items():
+        v = 255 if c >= 255 else (255 * c) >> 8
+        img[y, x] = v
+    return img
+
+
+WHITE = (1.0, 1.0, 1.0, 1.0)
+
+
+def rect_scene(seed=7, rules=RULES):
+    """Rectangles at 1/256-px x offsets and 1/4-px y offsets, including
+    slivers inside one row and rectangles within one pixel column; returns
+    (packed scene, expected covers)."""
+    rng = np.random.default_rng(seed)
+    cv = canvas.Canvas(256, 160)
+    want = {}
+    fracs = [0, 1, 2, 3, 5, 7, 8, 9, 31, 32, 33, 64, 100, 127, 128, 129, 200, 254, 255]
+    for i in range(40):
+        col, row = i % 10, i // 10
+        a = 4 + 25 * col
+        f0 = int(fracs[i % len(fracs)])
+        if i % 7 == 3:
+            b, f1 = a, int(rng.integers(f0 + 1, 256)) if f0 < 255 else 255
+            if f1 <= f0:
+                f0, f1 = 10, 200
+        else:
+            b, f1 = a + int(rng.integers(1, 12)), int(rng.choice(fracs))
+        y_base = 4 + 38 * row
+        if i % 5 == 4:
+            # a sliver inside one pixel row
+            r = y_base + int(rng.integers(0, 20))
+            t = int(rng.integers(0, 3))
+            u = int(rng.integers(t + 1, 4))
+            y0q, y1q = 4 * r + t, 4 * r + u
+        else:
+            y0q = 4 * y_base + int(rng.integers(0, 4))
+            y1q = y0q + int(rng.integers(1, 100))
+        p = canvas.Path()
+        p.add_rect(a + f0 / 256.0, y0q / 4.0, b + f1 / 256.0, y1q / 4.0)
+        cv.draw_path(p, canvas.Paint(WHITE))
+        want.update(rect_covers(a, f0, b, f1, y0q, y1q, rules))
+    return cv.scene(), want
+
+
+def parallelogram_scene(seed=9, rules=RULES):
+    """45-degree parallelograms at 1/256-px x offsets, both slant directions;
+    returns (packed scene, expected covers)."""
+    rng = np.random.default_rng(seed)
+    cv = canvas.Canvas(320, 160)
+    want = {}
+    fracs = [0, 1, 7, 8, 9, 15, 16, 17, 63, 64, 65, 127, 128, 129, 191, 200, 248, 255]
+    for i, f in enumerate(fracs):
+        col, row = i % 9, i // 9
+        direction = 1 if (i % 2 == 0) else -1
+        h = int(rng.integers(3, 12))
+        w = int(rng.integers(2, 7))
+        a = 8 + 34 * col + (0 if direction > 0 else h)
+        y0 = 6 + 70 * row + int(rng.integers(0, 20))
+        x0 = a + f / 256.0
+        p = canvas.Path()
+        p.move_to(x0, y0)
+        p.line_to(x0 + w, y0)
+        p.line_to(x0 + w + direction * h, y0 + h)
+        p.line_to(x0 + direction * h, y0 + h)
+        p.close()
+        cv.draw_path(p, canvas.Paint(WHITE))
+        want.update(parallelogram_covers(a, f, y0, w, h, direction, rules))
+    return cv.scene(), want

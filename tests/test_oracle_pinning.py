@@ -1,0 +1,111 @@
+"""Pins of the oracle that a plausible one-LSB or one-index mistake fails:
+the analytic-AA integer arithmetic against closed forms (tests/aaa_forms.py)
+and the tile-bin definition against the tiles its spans cover."""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from skity_amd import abi, canvas
+
+import aaa_forms as F
+from helpers import workload
+
+
+def _check(scene_and_want):
+    sc, want = scene_and_want
+    got = O.render(sc.scene, sc.width, sc.height)
+    exp = F.pixels_of(want, sc.width, sc.height)
+    bad = np.argwhere((got != exp).any(-1))
+    assert bad.size == 0, (
+        f"{len(bad)} pixels differ; first at (y, x) = {tuple(bad[0])}: oracle "
+        f"{got[tuple(bad[0])]} vs closed form {exp[tuple(bad[0])]}")
+
+
+def test_rectangles_at_subpixel_offsets():
+    _check(F.rect_scene(7))
+    _check(F.rect_scene(8))
+
+
+def test_45_degree_half_planes():
+    _check(F.parallelogram_scene(9))
+    _check(F.parallelogram_scene(10))
+
+
+SLIPS = {
+    "sub-step alpha truncated": {"step_scale": {1: (63,), 2: (127,), 3: (63, 127)}},
+    "sub-step scaling rounded": {"scale": lambda c, s: (c * s + 128) >> 8},
+    "cover scaled by 255": {"area": lambda f: ((256 - f) * 255) // 256},
+    "leg rounded": {"leg": lambda f: (f + 4) // 8},
+    "triangle area rounded": {"tri": lambda n: (n * n + 4) // 8},
+}
+
+
+@pytest.mark.parametrize("slip", sorted(SLIPS))
+def test_closed_forms_catch_one_lsb_slips(slip):
+    """A rasteriser with any of these one-LSB slips draws different pixels
+    on the pinning scenes, so the closed-form tests above would fail it."""
+    rules = dict(F.RULES, **SLIPS[slip])
+    diff = 0
+    for make, seed in ((F.rect_scene, 7), (F.rect_scene, 8), (F.parallelogram_scene, 9),
+                       (F.parallelogram_scene, 10)):
+        sc, good = make(seed)
+        _, bad = make(seed, rules)
+        diff += int((F.pixels_of(good, sc.width, sc.height) !=
+                     F.pixels_of(bad, sc.width, sc.height)).any(-1).sum())
+    assert diff > 0
+
+
+def test_rect_tile_bins_closed_form():
+    """An axis-aligned rectangle on whole rows: every pixel gets one direct
+    cover, so its bins are plane 0 of exactly the tiles of its pixel box."""
+    cv = canvas.Canvas(100, 70)
+    p = canvas.Path()
+    p.add_rect(3.5, 5, 40.25, 37)
+    cv.draw_path(p, canvas.Paint(F.WHITE))
+    sc = cv.scene()
+    bins, stacked = O.path_tile_bins(sc.scene, 0, 0, sc.width, sc.height)
+    tiles_x = (100 + 15) // 16
+    want = sorted(((ty * tiles_x + tx) << 1) for ty in range(5 // 16, 36 // 16 + 1)
+                  for tx in range(3 // 16, 40 // 16 + 1))
+    assert stacked == 0
+    assert list(bins) == want
+
+
+@pytest.mark.parametrize("wl,n", [(1, 0), (2, 12)])
+def test_tile_bins_cover_every_drawn_pixel(wl, n):
+    """Plane-0 bins hold every tile where the path alone draws a pixel, and
+    no bin lies outside the path's pixel bounds; no pixel gets two direct
+    covers."""
+    sc = workload(wl, n)
+    s = sc.scene.contents
+    tiles_x = (sc.width + 15) // 16
+    for p in range(s.n_paths):
+        stroke = 1 if s.paints[p].style == abi.STYLE_STROKE else 0
+        bins, stacked = O.path_tile_bins(sc.scene, p, stroke, sc.width, sc.height)
+        assert stacked == 0
+        assert np.all(np.diff(bins) > 0)
+        one = sc.slice(p, p + 1)
+        s1 = one.scene.contents
+        s1.paints[0].style = abi.STYLE_STROKE if stroke else abi.STYLE_FILL
+        s1.paints[0].color[3] = 1.0
+        s1.paints[0].stroke_color[3] = 1.0
+        img = O.render(one.scene, sc.width, sc.height)
+        ys, xs = np.nonzero(img[..., 3])
+        drawn = set(((ys >> 4) * tiles_x + (xs >> 4)).tolist())
+        plane0 = set((bins[(bins & 1) == 0] >> 1).tolist())
+        assert drawn <= plane0, f"path {p}: drawn tiles without a bin"
+        spans, bnd = O.path_spans(sc.scene, p, stroke)
+        l, t, r, b = (int(v) for v in bnd)
+        tx = (bins >> 1) % tiles_x
+        ty = (bins >> 1) // tiles_x
+        assert np.all(tx >= max(l, 0) >> 4) and np.all(tx <= (min(r, sc.width) - 1) >> 4)
+        assert np.all(ty >= max(t, 0) >> 4) and np.all(ty <= (min(b, sc.height) - 1) >> 4)
+
+
+@pytest.mark.parametrize("wl,n", [(1, 0), (2, 60), (3, 24)])
+def test_threaded_oracle_equals_serial(wl, n):
+    """The threaded render that checks full frames is the serial oracle."""
+    sc = workload(wl, n)
+    a = O.render(sc.scene, sc.width, sc.height)
+    b = O.render_mt(sc.scene, sc.width, sc.height, 4)
+    assert np.array_equal(a, b)
