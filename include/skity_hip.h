@@ -62,12 +62,30 @@ enum { HIPR_JOIN_MITER = 0, HIPR_JOIN_ROUND = 1, HIPR_JOIN_BEVEL = 2 };
 enum { HIPR_GRADIENT_LINEAR = 2, HIPR_GRADIENT_RADIAL = 3, HIPR_GRADIENT_SWEEP = 4 };
 /* TileMode (include/skity/graphic/tile_mode.hpp:13-30). */
 enum { HIPR_TILE_CLAMP = 0, HIPR_TILE_REPEAT = 1, HIPR_TILE_MIRROR = 2, HIPR_TILE_DECAL = 3 };
-/* BlurStyle (include/skity/effect/mask_filter.hpp:15-20). Only kNormal is
- * drawn by the device path. */
-enum { HIPR_BLUR_NORMAL = 1 };
-/* BlendMode: only kSrcOver is implemented on the device (value 3 in
- * include/skity/graphic/blend_mode.hpp). */
-enum { HIPR_BLEND_SRC_OVER = 3 };
+/* BlurStyle (include/skity/effect/mask_filter.hpp:15-20). */
+enum { HIPR_BLUR_NORMAL = 1, HIPR_BLUR_SOLID = 2, HIPR_BLUR_OUTER = 3, HIPR_BLUR_INNER = 4 };
+/* BlendMode (include/skity/graphic/blend_mode.hpp:45-80).  The software
+ * backend blends kClear .. kScreen and kSoftLight; every other mode falls back
+ * to kSrcOver in PorterDuffBlend (src/graphic/blend_mode.cc:125-134), and so
+ * does the device. */
+enum {
+  HIPR_BLEND_CLEAR = 0, HIPR_BLEND_SRC = 1, HIPR_BLEND_DST = 2, HIPR_BLEND_SRC_OVER = 3,
+  HIPR_BLEND_DST_OVER = 4, HIPR_BLEND_SRC_IN = 5, HIPR_BLEND_DST_IN = 6,
+  HIPR_BLEND_SRC_OUT = 7, HIPR_BLEND_DST_OUT = 8, HIPR_BLEND_SRC_ATOP = 9,
+  HIPR_BLEND_DST_ATOP = 10, HIPR_BLEND_XOR = 11, HIPR_BLEND_PLUS = 12,
+  HIPR_BLEND_MODULATE = 13, HIPR_BLEND_SCREEN = 14, HIPR_BLEND_SOFT_LIGHT = 21,
+  HIPR_BLEND_LAST = 28 /* kLuminosity */
+};
+/* AlphaType (include/skity/graphic/alpha_type.hpp:33-39) and ColorType
+ * (include/skity/graphic/color_type.hpp:16-23) of a surface.  Targets that
+ * are not premultiplied are converted on every blend
+ * (SWRenderTarget::BlendPixel, src/render/sw/sw_render_target.cc:12-34). */
+enum { HIPR_ALPHA_OPAQUE = 1, HIPR_ALPHA_PREMUL = 2, HIPR_ALPHA_UNPREMUL = 3 };
+enum { HIPR_COLOR_RGBA = 1, HIPR_COLOR_BGRA = 2 };
+/* What one entry of a hipr_scene draws: Canvas::DrawPath, or
+ * Canvas::DrawPaint (SWCanvas::OnDrawPaint, src/render/sw/sw_canvas.cc:
+ * 412-438: the paint over the whole target, whose entry has no verbs). */
+enum { HIPR_DRAW_PATH = 0, HIPR_DRAW_PAINT = 1 };
 
 #define HIPR_MAX_GRADIENT_STOPS 16
 
@@ -111,7 +129,11 @@ typedef struct hipr_gradient {
  * [point_offsets[p], point_offsets[p+1]) and conic weights
  * [weight_offsets[p], weight_offsets[p+1]).  The verb/point streams must be
  * what skity's Path builder produces (every contour starts with a move).
- * The canvas matrix is the identity. */
+ * `matrices` holds the canvas matrix of each draw (Canvas::GetTotalMatrix,
+ * src/render/canvas.cc:126-128), applied as SWRaster::RastePath does after
+ * the quad conversion (Path::CopyWithMatrix, src/graphic/path.cc:1274-1295).
+ * The fields after n_gradients may be left zero (identity matrices, all
+ * draws DrawPath). */
 typedef struct hipr_scene {
   int32_t n_paths;
   const uint8_t* verbs;
@@ -124,6 +146,11 @@ typedef struct hipr_scene {
   const hipr_paint* paints;      /* n_paths */
   const hipr_gradient* gradients;
   int32_t n_gradients;
+  /* n_paths x 6 floats {scale_x, skew_x, trans_x, skew_y, scale_y, trans_y}:
+   * x' = (scale_x * x + skew_x * y) + trans_x,
+   * y' = (skew_y * x + scale_y * y) + trans_y; NULL = identity. */
+  const float* matrices;
+  const uint8_t* draw_kinds;     /* n_paths, HIPR_DRAW_*; NULL = all paths */
 } hipr_scene;
 
 typedef struct hipr_context_s* hipr_context;
@@ -189,6 +216,14 @@ HIPR_API void hipr_context_destroy(hipr_context ctx);
  * MakeSoftwareCanvas returning null for an empty bitmap. */
 HIPR_API hipr_result hipr_surface_create(hipr_context ctx, int32_t width,
                                          int32_t height, hipr_surface* out);
+/* A target of the given HIPR_ALPHA_* and HIPR_COLOR_* types, the two
+ * properties of skity's Bitmap(width, height, alpha_type, color_type)
+ * (include/skity/graphic/bitmap.hpp:27-31, whose default is kUnpremul /
+ * kRGBA).  Pixels are 4 bytes, in the colour type's byte order; read and
+ * write calls move them as stored. */
+HIPR_API hipr_result hipr_surface_create_ex(hipr_context ctx, int32_t width, int32_t height,
+                                            int32_t alpha_type, int32_t color_type,
+                                            hipr_surface* out);
 HIPR_API void hipr_surface_destroy(hipr_surface surface);
 HIPR_API hipr_result hipr_surface_clear(hipr_surface surface, uint32_t rgba);
 /* Copy the surface (width*height*4 bytes) to host memory.

@@ -27,6 +27,12 @@ ORC_API int orc_render_scene(const hipr_scene* scene, uint8_t* pixels,
 ORC_API int orc_render_scene_mt(const hipr_scene* scene, uint8_t* pixels,
                                 int32_t width, int32_t height, int32_t threads);
 
+/* orc_render_scene onto a target of the given HIPR_ALPHA_* / HIPR_COLOR_*
+ * types (bytes in the colour type's order), on `threads` threads. */
+ORC_API int orc_render_scene_ex(const hipr_scene* scene, uint8_t* pixels, int32_t width,
+                                int32_t height, int32_t alpha_type, int32_t color_type,
+                                int32_t threads);
+
 /* Edge records of path p after Stroke::QuadPath + SWEdgeBuilder::BuildEdges,
  * 16 int32 per edge in the hipr_debug_build_edges layout.  stroke != 0 builds
  * the edges of the stroke outline instead of the fill.  Returns the number of

@@ -9,6 +9,8 @@
 namespace orc {
 void RenderScene(const hipr_scene* s, uint8_t* px, int w, int h);
 void RenderSceneParallel(const hipr_scene* s, uint8_t* px, int w, int h, int threads);
+void RenderSceneEx(const hipr_scene* s, uint8_t* px, int w, int h, int alpha_type,
+                   int color_type, int threads);
 void PathSpans(const hipr_scene* s, int p, int stroke, std::vector<Span>* spans,
                float bounds[4], std::vector<Edge>* edges);
 void StackBlur(const uint8_t* src, uint8_t* dst, int w, int h, int radius);
@@ -65,6 +67,16 @@ int orc_render_scene_mt(const hipr_scene* scene, uint8_t* pixels, int32_t width,
                         int32_t height, int32_t threads) {
   if (!scene || !pixels || width <= 0 || height <= 0) return -2;
   RenderSceneParallel(scene, pixels, width, height, threads);
+  return 0;
+}
+
+int orc_render_scene_ex(const hipr_scene* scene, uint8_t* pixels, int32_t width,
+                        int32_t height, int32_t alpha_type, int32_t color_type,
+                        int32_t threads) {
+  if (!scene || !pixels || width <= 0 || height <= 0) return -2;
+  if (alpha_type < HIPR_ALPHA_OPAQUE || alpha_type > HIPR_ALPHA_UNPREMUL) return -2;
+  if (color_type != HIPR_COLOR_RGBA && color_type != HIPR_COLOR_BGRA) return -2;
+  RenderSceneEx(scene, pixels, width, height, alpha_type, color_type, threads);
   return 0;
 }
 

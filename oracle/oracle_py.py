@@ -27,6 +27,8 @@ def lib():
         L.orc_render_scene.argtypes = [SP, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
         L.orc_render_scene_mt.argtypes = [SP, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32]
+        L.orc_render_scene_ex.argtypes = [SP, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.orc_path_edges.restype = ctypes.c_int64
         L.orc_path_edges.argtypes = [SP, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                      ctypes.c_int64]
@@ -70,6 +72,17 @@ def render_mt(scene_ptr, width, height, threads=None, base=None):
         threads = max(1, min(16, os.cpu_count() or 1))
     img = np.zeros((height, width, 4), np.uint8) if base is None else base.copy()
     rc = lib().orc_render_scene_mt(scene_ptr, img.ctypes.data, width, height, threads)
+    assert rc == 0
+    return img
+
+
+def render_ex(scene_ptr, width, height, alpha_type, color_type, base=None, threads=None):
+    """render() onto a target of the given abi.ALPHA_* / abi.COLOR_* types."""
+    if threads is None:
+        threads = max(1, min(16, os.cpu_count() or 1))
+    img = np.zeros((height, width, 4), np.uint8) if base is None else base.copy()
+    rc = lib().orc_render_scene_ex(scene_ptr, img.ctypes.data, width, height, alpha_type,
+                                   color_type, threads)
     assert rc == 0
     return img
 

@@ -1123,15 +1123,17 @@ void BuildEdges(const PathData& q, float top, float bottom,
 // (sw_raster.cc:679-729).  std::sort is not stable; equal (upper_y, x, dx)
 // keys keep their emission order here.
 void RastePath(const PathData& path, std::vector<Span>* spans,
-               float out_bounds[4], std::vector<Edge>* edges_out, float tx,
-               float ty) {
+               float out_bounds[4], std::vector<Edge>* edges_out, const float* m) {
   spans->clear();
   PathData q = QuadPath(path);
-  if (tx != 0.f || ty != 0.f) {
-    // glm mat4 * vec4(x, y, 0, 1) for a translation: (1*x + 0*y) + (0 + t).
+  if (m) {
+    // Path::CopyWithMatrix: glm mat4 * vec4(x, y, 0, 1) sums the column
+    // products pairwise, (m0 x + m1 y) + (m2 0 + m3 1), i.e.
+    // (sx x + kx y) + (0 + tx) with every operation rounded separately.
     for (auto& pt : q.pts) {
-      pt.x = pt.x + tx;
-      pt.y = pt.y + ty;
+      float x = pt.x, y = pt.y;
+      pt.x = (m[0] * x + m[1] * y) + (0.f + m[2]);
+      pt.y = (m[3] * x + m[4] * y) + (0.f + m[5]);
     }
   }
   float sb[4];

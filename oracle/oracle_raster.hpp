@@ -76,14 +76,15 @@ PathData QuadPath(const PathData& src);
 void BuildEdges(const PathData& qpath, float top, float bottom,
                 std::vector<Edge>* edges);
 
-// SWRaster::RastePath (src/render/sw/sw_raster.cc:731-786) with a canvas
-// matrix that is a pure translation (tx, ty), applied after the quad
-// conversion as Path::CopyWithMatrix does (src/graphic/path.cc:1274-1295).
+// SWRaster::RastePath (src/render/sw/sw_raster.cc:731-786) with an affine
+// canvas matrix m = {sx, kx, tx, ky, sy, ty} (nullptr: identity), applied
+// after the quad conversion as Path::CopyWithMatrix does
+// (src/graphic/path.cc:1274-1295, src/geometry/matrix.cc:462-465).
 // Returns the spans in emission order; *bounds receives the floored/ceiled
 // path bounds.
 void RastePath(const PathData& path, std::vector<Span>* spans,
                float bounds_ltrb[4], std::vector<Edge>* edges_out,
-               float tx = 0.f, float ty = 0.f);
+               const float* m = nullptr);
 
 // Path bounds (Path::ComputePtBounds, src/graphic/path.cc:1347-1360).
 bool PathBounds(const PathData& p, float ltrb[4]);

@@ -65,32 +65,140 @@ inline uint32_t PMSrcOver(uint32_t src, uint32_t dst) {
   return src + AlphaMulQ(dst, 256 - GetA(src));
 }
 
-// ---- premultiplied RGBA8 target: SWRenderTarget::BlendPixel with FastBlend
-// and PorterDuffBlend for kSrcOver (sw_render_target.cc:12-141;
-// blend_mode.cc:125-191); Bitmap::Get/SetPixel for kRGBA (bitmap.cc:25-85).
+// ---- targets: SWRenderTarget::BlendPixel = FastBlend, else PorterDuffBlend
+// with the target converted to and from premultiplied form when it is not
+// premultiplied (sw_render_target.cc:12-141; blend_mode.cc:93-191;
+// color_priv.cc:97-108); Bitmap::Get/SetPixel for kRGBA and kBGRA
+// (bitmap.cc:25-85).
+inline uint32_t Alpha255To256(uint32_t a) { return a + 1; }
+inline uint32_t PMColorMul(uint32_t s, uint32_t d) {
+  return SetARGB(MulDiv255Round(GetA(s), GetA(d)), MulDiv255Round(GetR(s), GetR(d)),
+                 MulDiv255Round(GetG(s), GetG(d)), MulDiv255Round(GetB(s), GetB(d)));
+}
+// PMColorToColor: per channel (scale * c + 2^23) >> 24 in 32-bit arithmetic,
+// scale = 255 * 2^24 / a rounded to nearest (the values of the reference's
+// UnPreMultiply table, color_priv.cc:16-96).
+inline uint32_t UnpremulScale(uint32_t a) {
+  return a == 0 ? 0u : (uint32_t)((((uint64_t)255 << 24) + a / 2) / a);
+}
+inline uint32_t PMColorToColor(uint32_t c) {
+  uint32_t a = GetA(c), sc = UnpremulScale(a);
+  auto ap = [&](uint32_t v) { return (uint32_t)(sc * v + (1u << 23)) >> 24; };
+  return SetARGB(a, ap(GetR(c)), ap(GetG(c)), ap(GetB(c)));
+}
+inline float SoftLightComponent(float sx, float sy, float dx, float dy) {
+  // blend_mode.cc:93-110, with the reference's order of float operations
+  if (2.f * sx <= sy) {
+    return dx * dx * (sy - 2 * sx) / dy + (1 - dy) * sx + dx * (-sy + 2 * sx + 1);
+  } else if (4.f * dx <= dy) {
+    float DSqd = dx * dx;
+    float DCub = DSqd * dx;
+    float DaSqd = dy * dy;
+    float DaCub = DaSqd * dy;
+    return (DaSqd * (sx - dx * (3 * sy - 6 * sx - 1)) + 12 * dy * DSqd * (sy - 2 * sx) -
+            16 * DCub * (sy - 2 * sx) - DaCub * sx) /
+           DaSqd;
+  } else {
+    return dx * (sy - 2 * sx + 1) + sx - std::sqrt(dy * dx) * (sy - 2 * sx) - dy * sx;
+  }
+}
+inline uint32_t SoftLight(uint32_t src, uint32_t dst) {
+  if (GetA(dst) == 0) return src;
+  float s[4] = {GetR(src) / 255.f, GetG(src) / 255.f, GetB(src) / 255.f, GetA(src) / 255.f};
+  float d[4] = {GetR(dst) / 255.f, GetG(dst) / 255.f, GetB(dst) / 255.f, GetA(dst) / 255.f};
+  float c[4] = {SoftLightComponent(s[0], s[3], d[0], d[3]),
+                SoftLightComponent(s[1], s[3], d[1], d[3]),
+                SoftLightComponent(s[2], s[3], d[2], d[3]), s[3] + (1 - s[3]) * d[3]};
+  return Color4fToColor(c);
+}
+uint32_t PorterDuffBlend(uint32_t src, uint32_t dst, int mode) {
+  if (mode > HIPR_BLEND_SCREEN && mode != HIPR_BLEND_SOFT_LIGHT) mode = HIPR_BLEND_SRC_OVER;
+  const uint32_t sa = GetA(src), da = GetA(dst);
+  switch (mode) {
+    case HIPR_BLEND_CLEAR: return 0;
+    case HIPR_BLEND_SRC: return src;
+    case HIPR_BLEND_DST: return dst;
+    case HIPR_BLEND_SRC_OVER: return sa == 0 ? dst : PMSrcOver(src, dst);
+    case HIPR_BLEND_DST_OVER: return da == 255 ? dst : PMSrcOver(dst, src);
+    case HIPR_BLEND_SRC_IN: return da == 255 ? src : AlphaMulQ(src, Alpha255To256(da));
+    case HIPR_BLEND_DST_IN: return sa == 255 ? dst : AlphaMulQ(dst, Alpha255To256(sa));
+    case HIPR_BLEND_SRC_OUT: return da == 0 ? src : AlphaMulQ(src, Alpha255To256(255 - da));
+    case HIPR_BLEND_DST_OUT: return sa == 0 ? dst : AlphaMulQ(dst, Alpha255To256(255 - sa));
+    case HIPR_BLEND_SRC_ATOP:
+      return AlphaMulQ(src, Alpha255To256(da)) + AlphaMulQ(dst, Alpha255To256(255 - sa));
+    case HIPR_BLEND_DST_ATOP:
+      return AlphaMulQ(dst, Alpha255To256(sa)) + AlphaMulQ(src, Alpha255To256(255 - da));
+    case HIPR_BLEND_XOR:
+      return AlphaMulQ(src, Alpha255To256(255 - da)) + AlphaMulQ(dst, Alpha255To256(255 - sa));
+    case HIPR_BLEND_PLUS:
+      return SetARGB(std::min(sa + da, 255u), std::min(GetR(src) + GetR(dst), 255u),
+                     std::min(GetG(src) + GetG(dst), 255u),
+                     std::min(GetB(src) + GetB(dst), 255u));
+    case HIPR_BLEND_MODULATE: return PMColorMul(src, dst);
+    case HIPR_BLEND_SCREEN: return src + dst - PMColorMul(src, dst);
+    case HIPR_BLEND_SOFT_LIGHT: return SoftLight(src, dst);
+    default: return dst;
+  }
+}
+
 struct Target {
   uint8_t* px;
   int w, h;
+  int alpha_type = HIPR_ALPHA_PREMUL;
+  int color_type = HIPR_COLOR_RGBA;
   uint32_t Get(int x, int y) const {
     const uint8_t* p = px + ((size_t)y * w + x) * 4;
-    return SetARGB(p[3], p[0], p[1], p[2]);
+    return color_type == HIPR_COLOR_BGRA ? SetARGB(p[3], p[2], p[1], p[0])
+                                         : SetARGB(p[3], p[0], p[1], p[2]);
   }
   void Set(int x, int y, uint32_t c) {
     uint8_t* p = px + ((size_t)y * w + x) * 4;
-    p[0] = (uint8_t)GetR(c);
+    bool bgra = color_type == HIPR_COLOR_BGRA;
+    p[0] = (uint8_t)(bgra ? GetB(c) : GetR(c));
     p[1] = (uint8_t)GetG(c);
-    p[2] = (uint8_t)GetB(c);
+    p[2] = (uint8_t)(bgra ? GetR(c) : GetB(c));
     p[3] = (uint8_t)GetA(c);
   }
-  void Blend(int x, int y, uint32_t src) {
+  void Blend(int x, int y, uint32_t src, int mode) {
     if ((unsigned)x >= (unsigned)w || (unsigned)y >= (unsigned)h) return;
-    uint32_t a = GetA(src);
-    if (a == 0) return;
-    if (a == 255) {
-      Set(x, y, src);
-      return;
+    // FastBlend (sw_render_target.cc:88-141)
+    const uint32_t a = GetA(src);
+    const uint8_t dst_a = px[((size_t)y * w + x) * 4 + 3];
+    switch (mode) {
+      case HIPR_BLEND_CLEAR: Set(x, y, 0); return;
+      case HIPR_BLEND_SRC: Set(x, y, src); return;
+      case HIPR_BLEND_DST: return;
+      case HIPR_BLEND_SRC_OVER:
+        if (a == 0) return;
+        if (a == 255) {
+          Set(x, y, src);
+          return;
+        }
+        break;
+      case HIPR_BLEND_DST_OVER:
+        if (dst_a == 255) return;
+        break;
+      case HIPR_BLEND_DST_IN:
+        if (a == 255) return;
+        if (a == 0) {
+          Set(x, y, 0);
+          return;
+        }
+        break;
+      case HIPR_BLEND_DST_OUT:
+        if (a == 0) return;
+        if (a == 255) {
+          Set(x, y, 0);
+          return;
+        }
+        break;
+      default:
+        break;
     }
-    Set(x, y, PMSrcOver(src, Get(x, y)));
+    const bool pm = alpha_type == HIPR_ALPHA_PREMUL;
+    uint32_t dst = pm ? Get(x, y) : ColorToPMColor(Get(x, y));
+    uint32_t r = PorterDuffBlend(src, dst, mode);
+    Set(x, y, pm ? r : PMColorToColor(r));
   }
 };
 
@@ -331,8 +439,10 @@ struct Brush {
 };
 
 // SWSpanBrush::Brush / BrushH (sw_span_brush.cc:66-138).
-void ApplySpans(const std::vector<Span>& spans, const Brush& br, Target* t) {
-  for (const Span& s : spans) {
+template <class It>
+void ApplySpans(It begin, It end, const Brush& br, int blend, Target* t) {
+  for (It it = begin; it != end; ++it) {
+    const Span& s = *it;
     int x = s.x, y = s.y, len = s.len;
     if (y < 0 || y >= t->h) continue;
     if (x >= t->w || x + len < 0) continue;
@@ -346,7 +456,7 @@ void ApplySpans(const std::vector<Span>& spans, const Brush& br, Target* t) {
     for (int l = 0; l < len; l++) {
       uint32_t c = br.Color(x + l, y);
       if (alpha != 255) c = AlphaMulQ(c, alpha);
-      t->Blend(x + l, y, c);
+      t->Blend(x + l, y, c, blend);
     }
   }
 }
@@ -427,43 +537,148 @@ Brush MakeBrush(const hipr_scene* s, const hipr_paint& paint, const Aff& ctm,
   return br;
 }
 
-// One fill or stroke rasterisation plus brush (sw_canvas.cc:374-407).
-void DrawShape(const hipr_scene* s, const PathData& path, const hipr_paint& paint,
-               bool stroke, const Aff& ctm, Target* t) {
-  Brush br = MakeBrush(s, paint, ctm, stroke);
-  std::vector<Span> spans;
-  float bounds[4];
-  if (!stroke) {
-    RastePath(path, &spans, bounds, nullptr, ctm.tx, ctm.ty);
-  } else {
-    PathData quad = QuadPath(path);
-    PathData outline = StrokeOutline(quad, paint);
-    RastePath(outline, &spans, bounds, nullptr, ctm.tx, ctm.ty);
+Aff DrawMatrix(const hipr_scene* s, int p) {
+  Aff m;
+  if (s->matrices) {
+    const float* f = s->matrices + 6 * (size_t)p;
+    m.sx = f[0];
+    m.kx = f[1];
+    m.tx = f[2];
+    m.ky = f[3];
+    m.sy = f[4];
+    m.ty = f[5];
   }
-  ApplySpans(spans, br, t);
+  return m;
+}
+void AffToArray(const Aff& m, float f[6]) {
+  f[0] = m.sx;
+  f[1] = m.kx;
+  f[2] = m.tx;
+  f[3] = m.ky;
+  f[4] = m.sy;
+  f[5] = m.ty;
 }
 
-// The canvas matrix is a translation here (identity on the main canvas,
-// -filter_bounds.LT on a mask-filter layer).
-void DrawPathNoFilter(const hipr_scene* s, const PathData& path,
-                      const hipr_paint& paint, const Aff& ctm, Target* t) {
+// One rasterisation and its brush: the spans sorted by row (emission order
+// within a row, which keeps each pixel's blend sequence), the first span of
+// each row, and the pixels of an image brush.
+struct SpanOp {
+  std::vector<Span> spans;
+  std::vector<size_t> row_start;  // first span of row y0 + i
+  int y0 = 0;
+  Brush br;
+  int blend = HIPR_BLEND_SRC_OVER;
+  std::vector<uint8_t> layer;
+};
+using Ops = std::vector<std::unique_ptr<SpanOp>>;
+
+void FinishOp(SpanOp* op) {
+  std::stable_sort(op->spans.begin(), op->spans.end(),
+                   [](const Span& a, const Span& b) { return a.y < b.y; });
+  if (op->spans.empty()) return;
+  op->y0 = op->spans.front().y;
+  int y1 = op->spans.back().y + 1;
+  op->row_start.assign((size_t)(y1 - op->y0) + 1, 0);
+  size_t i = 0;
+  for (int y = op->y0; y <= y1; y++) {
+    while (i < op->spans.size() && op->spans[i].y < y) i++;
+    op->row_start[y - op->y0] = i;
+  }
+}
+
+// Rows [y0, y1) of an op onto a target.
+void ApplyOp(const SpanOp& op, Target* t, int y0, int y1) {
+  if (op.spans.empty()) return;
+  const int oy1 = op.y0 + (int)op.row_start.size() - 1;
+  const int a = std::max(y0, op.y0), e = std::min(y1, oy1);
+  if (a >= e) return;
+  ApplySpans(op.spans.begin() + op.row_start[a - op.y0],
+             op.spans.begin() + op.row_start[e - op.y0], op.br, op.blend, t);
+}
+
+// The fill and/or stroke rasterisations of a path under canvas matrix `ctm`,
+// in paint order (sw_canvas.cc:357-411, paint_order.hpp:12-31).
+void ShapeOps(const hipr_scene* s, const PathData& path, const hipr_paint& paint,
+              const Aff& ctm, Ops* ops) {
+  float m6[6];
+  AffToArray(ctm, m6);
+  auto shape = [&](bool stroke) {
+    auto op = std::make_unique<SpanOp>();
+    op->br = MakeBrush(s, paint, ctm, stroke);
+    op->blend = paint.blend;
+    float bounds[4];
+    if (!stroke) {
+      RastePath(path, &op->spans, bounds, nullptr, m6);
+    } else {
+      PathData outline = StrokeOutline(QuadPath(path), paint);
+      RastePath(outline, &op->spans, bounds, nullptr, m6);
+    }
+    FinishOp(op.get());
+    ops->push_back(std::move(op));
+  };
   bool need_fill = paint.style != HIPR_STYLE_STROKE;
   bool need_stroke = paint.style != HIPR_STYLE_FILL;
   if (paint.style == HIPR_STYLE_STROKE_THEN_FILL) {
-    if (need_stroke) DrawShape(s, path, paint, true, ctm, t);
-    if (need_fill) DrawShape(s, path, paint, false, ctm, t);
+    if (need_stroke) shape(true);
+    if (need_fill) shape(false);
   } else {
-    if (need_fill) DrawShape(s, path, paint, false, ctm, t);
-    if (need_stroke) DrawShape(s, path, paint, true, ctm, t);
+    if (need_fill) shape(false);
+    if (need_stroke) shape(true);
+  }
+}
+
+void DrawOps(const Ops& ops, Target* t) {
+  for (const auto& op : ops) ApplyOp(*op, t, 0, t->h);
+}
+
+// The rectangle path of Path::AddRect(l, t, r, b) (path.cc:815-835).
+PathData RectPath(float L, float T, float R, float B) {
+  PathData rect;
+  rect.MoveTo(Pt{L, T});
+  rect.LineTo(Pt{R, T});
+  rect.LineTo(Pt{R, B});
+  rect.LineTo(Pt{L, B});
+  rect.Close();
+  return rect;
+}
+
+// MaskFilterOnFilter's post-passes for the other blur styles
+// (mask_filter.cc:60-103): raw is the layer before the blur.
+void BlurStylePass(int style, const std::vector<uint8_t>& raw, std::vector<uint8_t>* blurred) {
+  if (style == HIPR_BLUR_NORMAL) return;
+  const size_t n = raw.size() / 4;
+  for (size_t i = 0; i < n; i++) {
+    const uint8_t* r = &raw[4 * i];
+    uint8_t* b = &(*blurred)[4 * i];
+    uint32_t raw_a = r[3], blur_a = b[3];
+    if (style == HIPR_BLUR_SOLID) {
+      if (raw_a > 0) std::memcpy(b, r, 4);
+    } else if (style == HIPR_BLUR_OUTER) {
+      if (raw_a > 0 && raw_a >= blur_a) std::memset(b, 0, 4);
+    } else if (style == HIPR_BLUR_INNER) {
+      if (raw_a > 0) {
+        constexpr float a_factor = 1.f / 255.f;
+        uint32_t scale = (uint32_t)(a_factor * raw_a * blur_a);
+        uint32_t c = AlphaMulQ(SetARGB(b[3], b[0], b[1], b[2]), scale);
+        b[0] = (uint8_t)GetR(c);
+        b[1] = (uint8_t)GetG(c);
+        b[2] = (uint8_t)GetB(c);
+        b[3] = (uint8_t)GetA(c);
+      } else {
+        std::memset(b, 0, 4);
+      }
+    }
   }
 }
 
 }  // namespace
 
-// SWCanvas::HandleFilter + MaskFilterOnFilter(kNormal) + OnDrawImageRect
-// (sw_canvas.cc:135-144,641-677,797-826; mask_filter.cc:51-58;
-// image_filter.cc:33-41,184-194).  Returns the filter bounds and leaves the
-// blurred premultiplied image in *blurred (exposed for stage parity).
+// SWCanvas::HandleFilter + MaskFilterOnFilter + OnDrawImageRect
+// (sw_canvas.cc:135-144,641-677,797-826; mask_filter.cc:51-103;
+// image_filter.cc:33-41,184-194).  The layer is a fresh premultiplied RGBA
+// canvas translated by -filter_bounds.LT (the draw's canvas matrix does not
+// reach it); returns the filter bounds and leaves the blurred (and styled)
+// premultiplied image in *blurred.
 void BlurredLayer(const hipr_scene* s, int p, std::vector<uint8_t>* blurred,
                   int fb[4]) {
   const hipr_paint& paint = s->paints[p];
@@ -485,155 +700,70 @@ void BlurredLayer(const hipr_scene* s, int p, std::vector<uint8_t>* blurred,
   fb[2] = (int)rt;
   fb[3] = (int)bt;
   int W = (int)(rt - l), H = (int)(bt - tp);
+  blurred->clear();
+  if (W <= 0 || H <= 0) return;
   std::vector<uint8_t> layer((size_t)W * H * 4, 0);
   Target lt{layer.data(), W, H};
   hipr_paint work = paint;
   work.blur_radius = 0;
-  // The temporary canvas is translated by -filter_bounds.LT; its points go
-  // through the canvas matrix as x + (-L) after the quad conversion
-  // (sw_raster.cc:739-740; matrix.cc:462-465).
-  DrawPathNoFilter(s, path, work, Translate(-l, -tp), &lt);
+  Ops ops;
+  ShapeOps(s, path, work, Translate(-l, -tp), &ops);
+  DrawOps(ops, &lt);
   blurred->assign((size_t)W * H * 4, 0);
   int radius = (int)std::round(r);
   StackBlur(layer.data(), blurred->data(), W, H, radius);
+  BlurStylePass(paint.blur_style, layer, blurred);
 }
 
 namespace {
 
-void DrawPathBlurred(const hipr_scene* s, int p, Target* t) {
+// The rasterisations of draw p onto a tw x th target.
+void DrawEntryOps(const hipr_scene* s, int p, int tw, int th, Ops* ops) {
   const hipr_paint& paint = s->paints[p];
-  std::vector<uint8_t> blurred;
-  int fb[4];
-  BlurredLayer(s, p, &blurred, fb);
-  int W = fb[2] - fb[0], H = fb[3] - fb[1];
-  if (W <= 0 || H <= 0) return;
-  // OnDrawImageRect: rect path of the filter bounds filled with a decal
-  // nearest pixmap shader; paint alpha applied again as global alpha.
-  PathData rect;
-  float L = (float)fb[0], T = (float)fb[1], R = (float)fb[2], B = (float)fb[3];
-  rect.MoveTo(Pt{L, T});
-  rect.LineTo(Pt{R, T});
-  rect.LineTo(Pt{R, B});
-  rect.LineTo(Pt{L, B});
-  rect.Close();
-  std::vector<Span> spans;
-  float bounds[4];
-  RastePath(rect, &spans, bounds, nullptr);
-  Brush br;
-  br.kind = 2;
-  br.img = blurred.data();
-  br.img_w = W;
-  br.img_h = H;
-  Aff local = Translate(L, T);
-  Aff inv;
-  Invert(local, &inv);
-  br.img_ptu = Mul(Scale(1.f / W, 1.f / H), inv);
-  // Paint::GetAlphaF: the stroke colour's alpha for kStroke_Style, else the
-  // fill colour's (paint.cc:96-98).
-  float alpha_f = paint.style == HIPR_STYLE_STROKE ? paint.stroke_color[3] : paint.color[3];
-  br.global_alpha = (uint8_t)(255 * alpha_f);
-  ApplySpans(spans, br, t);
-}
-
-}  // namespace
-
-void RenderScene(const hipr_scene* s, uint8_t* px, int w, int h) {
-  Target t{px, w, h};
-  Aff ident;
-  for (int p = 0; p < s->n_paths; p++) {
-    const hipr_paint& paint = s->paints[p];
-    if (paint.blur_radius > 0) {
-      DrawPathBlurred(s, p, &t);
-    } else {
-      PathData path = ScenePath(s, p);
-      DrawPathNoFilter(s, path, paint, ident, &t);
-    }
+  const Aff ctm = DrawMatrix(s, p);
+  const int kind = s->draw_kinds ? s->draw_kinds[p] : HIPR_DRAW_PATH;
+  if (kind == HIPR_DRAW_PAINT) {
+    // OnDrawPaint: the target's rectangle, identity raster matrix, the
+    // paint's fill brush (its shader through the canvas matrix).
+    auto op = std::make_unique<SpanOp>();
+    float bounds[4];
+    RastePath(RectPath(0.f, 0.f, (float)tw, (float)th), &op->spans, bounds, nullptr);
+    op->br = MakeBrush(s, paint, ctm, false);
+    op->blend = paint.blend;
+    FinishOp(op.get());
+    ops->push_back(std::move(op));
+    return;
   }
-}
-
-// RenderScene with the work split over threads; the result is the same image.
-// Rasterising a draw reads nothing but its own path and paint, so the draws
-// of a chunk are rasterised concurrently into span lists (and blurred
-// layers); the spans are then applied in draw order, each thread owning a
-// band of rows, which keeps every pixel's blend sequence (rows are
-// independent, and a stable sort by row keeps each row's emission order).
-namespace {
-struct SpanOp {
-  std::vector<Span> spans;  // sorted by row, emission order within a row
-  std::vector<size_t> row_start;  // first span of each row offset from y0
-  int y0 = 0;
-  Brush br;
-  std::vector<uint8_t> layer;  // pixels of an image brush
-};
-
-void FinishOp(SpanOp* op) {
-  std::stable_sort(op->spans.begin(), op->spans.end(),
-                   [](const Span& a, const Span& b) { return a.y < b.y; });
-  if (op->spans.empty()) return;
-  op->y0 = op->spans.front().y;
-  int y1 = op->spans.back().y + 1;
-  op->row_start.assign((size_t)(y1 - op->y0) + 1, 0);
-  size_t i = 0;
-  for (int y = op->y0; y <= y1; y++) {
-    while (i < op->spans.size() && op->spans[i].y < y) i++;
-    op->row_start[y - op->y0] = i;
-  }
-}
-
-void ShapeOps(const hipr_scene* s, int p, std::vector<std::unique_ptr<SpanOp>>* ops) {
-  const hipr_paint& paint = s->paints[p];
   if (paint.blur_radius > 0) {
     auto op = std::make_unique<SpanOp>();
     int fb[4];
     BlurredLayer(s, p, &op->layer, fb);
     int W = fb[2] - fb[0], H = fb[3] - fb[1];
     if (W <= 0 || H <= 0) return;
-    PathData rect;
+    // OnDrawImageRect onto this canvas: the filter-bounds rectangle through
+    // the canvas matrix, a decal nearest pixmap shader whose local matrix
+    // places the image on the rectangle, and the paint (style set to fill,
+    // so its alpha is the fill colour's, Paint::GetAlphaF, paint.cc:96-98).
     float L = (float)fb[0], T = (float)fb[1], R = (float)fb[2], B = (float)fb[3];
-    rect.MoveTo(Pt{L, T});
-    rect.LineTo(Pt{R, T});
-    rect.LineTo(Pt{R, B});
-    rect.LineTo(Pt{L, B});
-    rect.Close();
+    float m6[6];
+    AffToArray(ctm, m6);
     float bounds[4];
-    RastePath(rect, &op->spans, bounds, nullptr);
+    RastePath(RectPath(L, T, R, B), &op->spans, bounds, nullptr, m6);
     op->br.kind = 2;
     op->br.img = op->layer.data();
     op->br.img_w = W;
     op->br.img_h = H;
-    Aff inv;
+    Aff inv, l2l;
     Invert(Translate(L, T), &inv);
-    op->br.img_ptu = Mul(Scale(1.f / W, 1.f / H), inv);
-    float alpha_f = paint.style == HIPR_STYLE_STROKE ? paint.stroke_color[3] : paint.color[3];
-    op->br.global_alpha = (uint8_t)(255 * alpha_f);
+    if (!Invert(ctm, &l2l)) l2l = Aff();
+    op->br.img_ptu = Mul(Mul(Scale(1.f / W, 1.f / H), inv), l2l);
+    op->br.global_alpha = (uint8_t)(255 * paint.color[3]);
+    op->blend = paint.blend;
     FinishOp(op.get());
     ops->push_back(std::move(op));
     return;
   }
-  PathData path = ScenePath(s, p);
-  Aff ident;
-  auto shape = [&](bool stroke) {
-    auto op = std::make_unique<SpanOp>();
-    op->br = MakeBrush(s, paint, ident, stroke);
-    float bounds[4];
-    if (!stroke) {
-      RastePath(path, &op->spans, bounds, nullptr);
-    } else {
-      PathData outline = StrokeOutline(QuadPath(path), paint);
-      RastePath(outline, &op->spans, bounds, nullptr);
-    }
-    FinishOp(op.get());
-    ops->push_back(std::move(op));
-  };
-  bool need_fill = paint.style != HIPR_STYLE_STROKE;
-  bool need_stroke = paint.style != HIPR_STYLE_FILL;
-  if (paint.style == HIPR_STYLE_STROKE_THEN_FILL) {
-    if (need_stroke) shape(true);
-    if (need_fill) shape(false);
-  } else {
-    if (need_fill) shape(false);
-    if (need_stroke) shape(true);
-  }
+  ShapeOps(s, ScenePath(s, p), paint, ctm, ops);
 }
 
 template <class F>
@@ -646,48 +776,59 @@ void ParallelFor(int n, int threads, F&& fn) {
     });
   for (auto& th : pool) th.join();
 }
+
 }  // namespace
 
-void RenderSceneParallel(const hipr_scene* s, uint8_t* px, int w, int h, int threads) {
+void RenderSceneEx(const hipr_scene* s, uint8_t* px, int w, int h, int alpha_type,
+                   int color_type, int threads) {
+  Target t{px, w, h, alpha_type, color_type};
   if (threads <= 1) {
-    RenderScene(s, px, w, h);
+    for (int p = 0; p < s->n_paths; p++) {
+      Ops ops;
+      DrawEntryOps(s, p, w, h, &ops);
+      DrawOps(ops, &t);
+    }
     return;
   }
-  Target t{px, w, h};
+  // Rasterising a draw reads nothing but its own path and paint, so the
+  // draws of a chunk are rasterised concurrently; their spans are then
+  // applied in draw order, each thread owning a band of rows (rows are
+  // independent): the same image as the serial loop.
   const int chunk = 8 * threads;
   constexpr int kBand = 16;
   const int bands = (h + kBand - 1) / kBand;
   for (int p0 = 0; p0 < s->n_paths; p0 += chunk) {
     const int n = std::min(chunk, s->n_paths - p0);
-    std::vector<std::vector<std::unique_ptr<SpanOp>>> ops(n);
-    ParallelFor(n, threads, [&](int i) { ShapeOps(s, p0 + i, &ops[i]); });
+    std::vector<Ops> ops(n);
+    ParallelFor(n, threads, [&](int i) { DrawEntryOps(s, p0 + i, w, h, &ops[i]); });
     ParallelFor(bands, threads, [&](int b) {
       const int y0 = b * kBand, y1 = std::min(h, y0 + kBand);
-      for (auto& draw : ops) {
-        for (auto& op : draw) {
-          if (op->spans.empty()) continue;
-          const int oy1 = op->y0 + (int)op->row_start.size() - 1;
-          const int a = std::max(y0, op->y0), e = std::min(y1, oy1);
-          if (a >= e) continue;
-          std::vector<Span> rows(op->spans.begin() + op->row_start[a - op->y0],
-                                 op->spans.begin() + op->row_start[e - op->y0]);
-          ApplySpans(rows, op->br, &t);
-        }
-      }
+      for (auto& draw : ops)
+        for (auto& op : draw) ApplyOp(*op, &t, y0, y1);
     });
   }
+}
+
+void RenderScene(const hipr_scene* s, uint8_t* px, int w, int h) {
+  RenderSceneEx(s, px, w, h, HIPR_ALPHA_PREMUL, HIPR_COLOR_RGBA, 1);
+}
+
+void RenderSceneParallel(const hipr_scene* s, uint8_t* px, int w, int h, int threads) {
+  RenderSceneEx(s, px, w, h, HIPR_ALPHA_PREMUL, HIPR_COLOR_RGBA, threads);
 }
 
 void PathSpans(const hipr_scene* s, int p, int stroke,
                std::vector<Span>* spans, float bounds[4],
                std::vector<Edge>* edges) {
   PathData path = ScenePath(s, p);
+  float m6[6];
+  AffToArray(DrawMatrix(s, p), m6);
   if (!stroke) {
-    RastePath(path, spans, bounds, edges);
+    RastePath(path, spans, bounds, edges, m6);
   } else {
     PathData quad = QuadPath(path);
     PathData outline = StrokeOutline(quad, s->paints[p]);
-    RastePath(outline, spans, bounds, edges);
+    RastePath(outline, spans, bounds, edges, m6);
   }
 }
 

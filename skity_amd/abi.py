@@ -18,8 +18,15 @@ CAP_BUTT, CAP_ROUND, CAP_SQUARE = range(3)
 JOIN_MITER, JOIN_ROUND, JOIN_BEVEL = range(3)
 GRADIENT_LINEAR, GRADIENT_RADIAL, GRADIENT_SWEEP = 2, 3, 4
 TILE_CLAMP, TILE_REPEAT, TILE_MIRROR, TILE_DECAL = range(4)
-BLUR_NORMAL = 1
-BLEND_SRC_OVER = 3
+BLUR_NORMAL, BLUR_SOLID, BLUR_OUTER, BLUR_INNER = 1, 2, 3, 4
+(BLEND_CLEAR, BLEND_SRC, BLEND_DST, BLEND_SRC_OVER, BLEND_DST_OVER, BLEND_SRC_IN,
+ BLEND_DST_IN, BLEND_SRC_OUT, BLEND_DST_OUT, BLEND_SRC_ATOP, BLEND_DST_ATOP, BLEND_XOR,
+ BLEND_PLUS, BLEND_MODULATE, BLEND_SCREEN) = range(15)
+BLEND_SOFT_LIGHT = 21
+BLEND_LAST = 28
+ALPHA_OPAQUE, ALPHA_PREMUL, ALPHA_UNPREMUL = 1, 2, 3
+COLOR_RGBA, COLOR_BGRA = 1, 2
+DRAW_PATH, DRAW_PAINT = 0, 1
 
 # hipr_stage, in enum order
 STAGES = ("stroke", "flatten_count", "flatten", "sort", "walk_count", "walk", "layers",
@@ -77,6 +84,8 @@ class Scene(ctypes.Structure):
         ("paints", ctypes.POINTER(Paint)),
         ("gradients", ctypes.POINTER(Gradient)),
         ("n_gradients", ctypes.c_int32),
+        ("matrices", ctypes.POINTER(ctypes.c_float)),
+        ("draw_kinds", ctypes.POINTER(ctypes.c_uint8)),
     ]
 
 

@@ -119,6 +119,34 @@ class Path:
         return self
 
 
+F32 = np.float32
+IDENTITY = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0)  # scale_x, skew_x, trans_x, skew_y, scale_y, trans_y
+
+
+def mat_mul(a, b):
+    """Matrix a * b (src/geometry/matrix.cc:450-458): glm's 4x4 product
+    restricted to affine operands, every float operation rounded to f32."""
+    if tuple(a) == IDENTITY:
+        return tuple(b)
+    if tuple(b) == IDENTITY:
+        return tuple(a)
+    asx, akx, atx, aky, asy, aty = (F32(v) for v in a)
+    bsx, bkx, btx, bky, bsy, bty = (F32(v) for v in b)
+    return (float(asx * bsx + akx * bky), float(asx * bkx + akx * bsy),
+            float((asx * btx + akx * bty) + atx), float(aky * bsx + asy * bky),
+            float(aky * bkx + asy * bsy), float((aky * btx + asy * bty) + aty))
+
+
+def mat_rotate(degrees, px=0.0, py=0.0):
+    """Rotation about (px, py) (Matrix::RotateDeg)."""
+    r = math.radians(degrees)
+    c, s = float(F32(math.cos(r))), float(F32(math.sin(r)))
+    rot = (c, -s, 0.0, s, c, 0.0)
+    if px == 0 and py == 0:
+        return rot
+    return mat_mul(mat_mul((1.0, 0.0, px, 0.0, 1.0, py), rot), (1.0, 0.0, -px, 0.0, 1.0, -py))
+
+
 class Paint:
     """The Paint fields the DrawPath path reads (include/skity/graphic/paint.hpp)."""
 
@@ -171,10 +199,14 @@ class Paint:
         self.p.miter_limit = m
         return self
 
-    def set_blur(self, radius):
-        """MaskFilter::MakeBlur(kNormal, radius)."""
+    def set_blur(self, radius, style=abi.BLUR_NORMAL):
+        """MaskFilter::MakeBlur(style, radius)."""
         self.p.blur_radius = radius
-        self.p.blur_style = abi.BLUR_NORMAL if radius > 0 else 0
+        self.p.blur_style = style if radius > 0 else 0
+        return self
+
+    def set_blend_mode(self, mode):
+        self.p.blend = mode
         return self
 
     def set_gradient(self, g):
@@ -184,14 +216,56 @@ class Paint:
 
 
 class Canvas:
-    """Records DrawPath calls; ``scene()`` packs them as a hipr_scene."""
+    """Records DrawPath / DrawPaint calls with the canvas matrix of each
+    (Canvas::Save/Restore/Translate/Scale/Rotate/Skew/Concat/SetMatrix,
+    src/render/canvas.cc:70-125, pre-concatenated as in
+    src/render/canvas_state.cc:18-45); ``scene()`` packs them as a
+    hipr_scene."""
 
     def __init__(self, width, height):
         self.width, self.height = width, height
         self.draws = []
+        self._stack = [IDENTITY]
 
+    # -- matrix state --
+    def save(self):
+        self._stack.append(self._stack[-1])
+        return len(self._stack) - 1
+
+    def restore(self):
+        if len(self._stack) > 1:
+            self._stack.pop()
+
+    def get_total_matrix(self):
+        return self._stack[-1]
+
+    def concat(self, m):
+        self._stack[-1] = mat_mul(self._stack[-1], m)
+
+    def set_matrix(self, m):
+        self._stack[-1] = tuple(float(F32(v)) for v in m)
+
+    def reset_matrix(self):
+        self._stack[-1] = IDENTITY
+
+    def translate(self, dx, dy):
+        self.concat((1.0, 0.0, dx, 0.0, 1.0, dy))
+
+    def scale(self, sx, sy):
+        self.concat((sx, 0.0, 0.0, 0.0, sy, 0.0))
+
+    def rotate(self, degrees, px=0.0, py=0.0):
+        self.concat(mat_rotate(degrees, px, py))
+
+    def skew(self, kx, ky):
+        self.concat((1.0, kx, 0.0, ky, 1.0, 0.0))
+
+    # -- draws --
     def draw_path(self, path, paint):
-        self.draws.append((path, paint))
+        self.draws.append((path, paint, self._stack[-1], abi.DRAW_PATH))
+
+    def draw_paint(self, paint):
+        self.draws.append((Path(), paint, self._stack[-1], abi.DRAW_PAINT))
 
     def scene(self):
         return PackedScene(self.draws, self.width, self.height)
@@ -209,7 +283,10 @@ class PackedScene:
         verbs, pts, ws, fills = [], [], [], []
         grads = []
         self.paints = (abi.Paint * max(n, 1))()
-        for i, (path, paint) in enumerate(draws):
+        draws = [d if len(d) == 4 else (d[0], d[1], IDENTITY, abi.DRAW_PATH) for d in draws]
+        mats = np.array([d[2] for d in draws] or [IDENTITY], np.float32).reshape(-1)
+        kinds = np.array([d[3] for d in draws] or [0], np.uint8)
+        for i, (path, paint, _, _) in enumerate(draws):
             verbs += path.verbs
             pts += path.points
             ws += path.weights
@@ -242,6 +319,11 @@ class PackedScene:
         s.paints = ctypes.cast(self.paints, ctypes.POINTER(abi.Paint))
         s.gradients = ctypes.cast(self.grads, ctypes.POINTER(abi.Gradient))
         s.n_gradients = len(grads)
+        self.mats, self.kinds = mats, kinds
+        if any(tuple(d[2]) != IDENTITY for d in draws):
+            s.matrices = mats.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        if any(d[3] != abi.DRAW_PATH for d in draws):
+            s.draw_kinds = kinds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         self._s = s
         self.scene = ctypes.pointer(s)
 

@@ -39,6 +39,48 @@ struct BlurJob {
   int32_t radius;  // already rounded and clamped to 254, >= 2
 };
 
+// MaskFilterOnFilter's post-passes for the blur styles other than kNormal
+// (src/effect/mask_filter.cc:60-103), per pixel of a layer: `raw` is the
+// layer before the blur, `blurred` the blur's output (updated in place).
+struct BlurStyleJob {
+  const uint8_t* raw;
+  uint8_t* blurred;
+  int64_t pixels;
+  int32_t style;  // HIPR_BLUR_SOLID / OUTER / INNER
+  int32_t pad;
+};
+__global__ void __launch_bounds__(256)
+    BlurStyleKernel(const BlurStyleJob* __restrict__ jobs, int n_jobs) {
+  for (int k = blockIdx.y; k < n_jobs; k += gridDim.y) {
+  const BlurStyleJob jb = jobs[k];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < jb.pixels;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t raw = ((const uint32_t*)jb.raw)[i];
+    uint32_t blur = ((const uint32_t*)jb.blurred)[i];
+    uint32_t raw_a = raw >> 24, blur_a = blur >> 24, out = blur;
+    if (jb.style == HIPR_BLUR_SOLID) {
+      if (raw_a > 0) out = raw;
+    } else if (jb.style == HIPR_BLUR_OUTER) {
+      if (raw_a > 0 && raw_a >= blur_a) out = 0;
+    } else if (jb.style == HIPR_BLUR_INNER) {
+      if (raw_a > 0) {
+        // AlphaMulQ(blur, a_factor * raw_a * blur_a): the float product,
+        // left to right, converted to the integer scale by truncation
+        const float a_factor = 1.f / 255.f;
+        uint32_t scale = (uint32_t)(a_factor * (float)raw_a * (float)blur_a);
+        const uint32_t mask = 0xFF00FF;
+        uint32_t rb = ((blur & mask) * scale) >> 8;
+        uint32_t ag = ((blur >> 8) & mask) * scale;
+        out = (rb & mask) | (ag & ~mask);
+      } else {
+        out = 0;
+      }
+    }
+    ((uint32_t*)jb.blurred)[i] = out;
+  }
+  }
+}
+
 // pass 0: rows (src -> dst); pass 1: columns (src -> dst).  One thread per
 // line; lines of all blur jobs are numbered consecutively (line_begin).
 __global__ void __launch_bounds__(256)

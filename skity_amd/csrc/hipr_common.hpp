@@ -23,22 +23,36 @@ struct GeomSrc {
   const float* weights;
 };
 
-enum JobKind : int32_t { kJobFill = 0, kJobStroke = 1, kJobImage = 2 };
+// kJobRect: coverage 255 over an integer rectangle of the canvas (the
+// paint's L, T, R, B), which is what RastePath gives an axis-aligned
+// rectangle with integer sides under the identity: a DrawPaint, or the
+// composite of a blurred layer drawn without a canvas matrix.  It is not
+// walked.
+enum JobKind : int32_t { kJobFill = 0, kJobStroke = 1, kJobRect = 2 };
+
+// An affine canvas matrix applied to quad-path points as Path::CopyWithMatrix
+// does (src/graphic/path.cc:1274-1295): glm's mat4 * vec4(x, y, 0, 1) adds the
+// column products pairwise, (sx x + kx y) + (0 + tx), each operation rounded.
+struct Affine {
+  float sx, kx, tx, ky, sy, ty;
+  __host__ __device__ P2 Map(P2 p) const {
+    return P2{(sx * p.x + kx * p.y) + (0.f + tx), (ky * p.x + sy * p.y) + (0.f + ty)};
+  }
+};
 
 // One rasterisation: a fill or stroke outline of a path, drawn onto canvas
-// `canvas` with the canvas translation (tx, ty) applied after the quad
-// conversion; or (kJobImage) the blurred layer `image` composited over its
-// filter bounds.
+// `canvas` through matrix `m` applied after the quad conversion; or
+// (kJobRect) a rectangle of full coverage.
 struct Job {
-  int32_t path;         // index of the DrawPath call (paint)
+  int32_t path;         // index of the draw (paint)
   int32_t canvas;
   int32_t kind;
   int32_t fill;         // HIPR_FILL_*
   int32_t geom;         // 0: scene arrays, 1: stroke outline arrays
   int32_t verb_begin, verb_end;
   int32_t pt_begin, w_begin;
-  float tx, ty;
-  int32_t image;        // layer canvas index for kJobImage, else -1
+  Affine m;
+  int32_t image;        // layer canvas index of an image paint, else -1
   int32_t paint;        // index into the device paint table
   int32_t pad;
 };
@@ -64,12 +78,19 @@ struct JobCov {
                         // row flushes, edge insertion (10 ns units)
 };
 
+// Canvas::format bits: the target is not premultiplied (kOpaque or
+// kUnpremul), and its bytes are B, G, R, A.
+constexpr int32_t kFormatUnpremul = 1;
+constexpr int32_t kFormatBGRA = 2;
+
 struct Canvas {
-  uint8_t* pixels;      // premultiplied RGBA8, row-major
+  uint8_t* pixels;      // RGBA8 (or BGRA8), row-major
   int32_t width, height;
   int32_t tiles_x, tiles_y;
   int32_t tile_begin;   // first global tile index
   int32_t job_begin, job_end;  // jobs drawn on this canvas, in order
+  int32_t format;              // kFormat* bits (0: premultiplied RGBA)
+  int32_t pad;
 };
 
 // ---- wave helpers ------------------------------------------------------------

@@ -34,8 +34,9 @@ struct PaintDev {
   int32_t ga;       // global alpha byte (cover & ga)
   int32_t grad;     // GradDev index
   int32_t layer;    // image: layer canvas index
-  int32_t L, T, R, B;            // image rect (filter bounds)
-  float isx, itx, isy, ity;      // image: device -> unit (scale, translate)
+  int32_t blend;    // HIPR_BLEND_* of the paint
+  int32_t L, T, R, B;  // kJobRect: the rectangle of full coverage
+  float im[6];      // image: device point -> unit square of the layer
 };
 
 __device__ __forceinline__ uint32_t AlphaMulQ(uint32_t c, uint32_t scale) {
@@ -70,6 +71,120 @@ __device__ __forceinline__ uint32_t PremulRGBA(const float c[4]) {
     b = MulDiv255Round(b, a);
   }
   return r | (g << 8) | (b << 16) | (a << 24);
+}
+
+// ---- SWRenderTarget::BlendPixel (sw_render_target.cc:12-141) ------------
+// Words hold one pixel with red in bits 0-7 and alpha in bits 24-31; every
+// operation is per channel, so the reference's ARGB register arithmetic
+// applies unchanged.
+__device__ __forceinline__ uint32_t Chan(uint32_t c, int k) { return (c >> (8 * k)) & 0xFF; }
+__device__ __forceinline__ uint32_t Pack(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+  return r | (g << 8) | (b << 16) | (a << 24);
+}
+// ColorToPMColor (color_priv.cc:103-106)
+__device__ __forceinline__ uint32_t PremulWord(uint32_t c) {
+  uint32_t a = c >> 24;
+  if (a == 255) return c;
+  return Pack(MulDiv255Round(Chan(c, 0), a), MulDiv255Round(Chan(c, 1), a),
+              MulDiv255Round(Chan(c, 2), a), a);
+}
+// PMColorToColor (color_priv.cc:97-103): scale = 255 * 2^24 / a rounded to
+// nearest, channel = (scale * c + 2^23) >> 24 in 32-bit arithmetic.
+__device__ __forceinline__ uint32_t UnpremulWord(uint32_t c) {
+  uint32_t a = c >> 24;
+  uint32_t sc = a == 0 ? 0u : ((255u << 24) + a / 2) / a;
+  auto ap = [&](uint32_t v) { return (sc * v + (1u << 23)) >> 24; };
+  return Pack(ap(Chan(c, 0)), ap(Chan(c, 1)), ap(Chan(c, 2)), a);
+}
+__device__ __forceinline__ uint32_t PMColorMulWord(uint32_t s, uint32_t d) {
+  return Pack(MulDiv255Round(Chan(s, 0), Chan(d, 0)), MulDiv255Round(Chan(s, 1), Chan(d, 1)),
+              MulDiv255Round(Chan(s, 2), Chan(d, 2)), MulDiv255Round(s >> 24, d >> 24));
+}
+// SoftLightComponent (blend_mode.cc:93-110), the reference's order of float
+// operations (this file is built without contraction).
+__device__ float SoftLightComponent(float sx, float sy, float dx, float dy) {
+  if (2.f * sx <= sy) {
+    return dx * dx * (sy - 2 * sx) / dy + (1 - dy) * sx + dx * (-sy + 2 * sx + 1);
+  } else if (4.f * dx <= dy) {
+    float DSqd = dx * dx;
+    float DCub = DSqd * dx;
+    float DaSqd = dy * dy;
+    float DaCub = DaSqd * dy;
+    return (DaSqd * (sx - dx * (3 * sy - 6 * sx - 1)) + 12 * dy * DSqd * (sy - 2 * sx) -
+            16 * DCub * (sy - 2 * sx) - DaCub * sx) /
+           DaSqd;
+  }
+  return dx * (sy - 2 * sx + 1) + sx - sqrtf(dy * dx) * (sy - 2 * sx) - dy * sx;
+}
+__device__ uint32_t SoftLightWord(uint32_t src, uint32_t dst) {
+  if ((dst >> 24) == 0) return src;
+  float sa = (float)(src >> 24) / 255.f, da = (float)(dst >> 24) / 255.f;
+  float c[4];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    c[k] = SoftLightComponent((float)Chan(src, k) / 255.f, sa, (float)Chan(dst, k) / 255.f, da);
+  c[3] = sa + (1 - sa) * da;
+  return Pack(UnitToByte(c[0]), UnitToByte(c[1]), UnitToByte(c[2]), UnitToByte(c[3]));
+}
+// PorterDuffBlend (blend_mode.cc:125-191): modes past kScreen other than
+// kSoftLight fall back to kSrcOver.
+__device__ uint32_t PorterDuffWord(uint32_t src, uint32_t dst, int mode) {
+  if (mode > HIPR_BLEND_SCREEN && mode != HIPR_BLEND_SOFT_LIGHT) mode = HIPR_BLEND_SRC_OVER;
+  const uint32_t sa = src >> 24, da = dst >> 24;
+  switch (mode) {
+    case HIPR_BLEND_CLEAR: return 0;
+    case HIPR_BLEND_SRC: return src;
+    case HIPR_BLEND_DST: return dst;
+    case HIPR_BLEND_SRC_OVER: return sa == 0 ? dst : src + AlphaMulQ(dst, 256 - sa);
+    case HIPR_BLEND_DST_OVER: return da == 255 ? dst : dst + AlphaMulQ(src, 256 - da);
+    case HIPR_BLEND_SRC_IN: return da == 255 ? src : AlphaMulQ(src, da + 1);
+    case HIPR_BLEND_DST_IN: return sa == 255 ? dst : AlphaMulQ(dst, sa + 1);
+    case HIPR_BLEND_SRC_OUT: return da == 0 ? src : AlphaMulQ(src, 256 - da);
+    case HIPR_BLEND_DST_OUT: return sa == 0 ? dst : AlphaMulQ(dst, 256 - sa);
+    case HIPR_BLEND_SRC_ATOP: return AlphaMulQ(src, da + 1) + AlphaMulQ(dst, 256 - sa);
+    case HIPR_BLEND_DST_ATOP: return AlphaMulQ(dst, sa + 1) + AlphaMulQ(src, 256 - da);
+    case HIPR_BLEND_XOR: return AlphaMulQ(src, 256 - da) + AlphaMulQ(dst, 256 - sa);
+    case HIPR_BLEND_PLUS:
+      return Pack(min(Chan(src, 0) + Chan(dst, 0), 255u), min(Chan(src, 1) + Chan(dst, 1), 255u),
+                  min(Chan(src, 2) + Chan(dst, 2), 255u), min(sa + da, 255u));
+    case HIPR_BLEND_MODULATE: return PMColorMulWord(src, dst);
+    case HIPR_BLEND_SCREEN: return src + dst - PMColorMulWord(src, dst);
+    case HIPR_BLEND_SOFT_LIGHT: return SoftLightWord(src, dst);
+    default: return dst;
+  }
+}
+// BlendPixel: FastBlend on the stored pixel, else PorterDuffBlend with a
+// target that is not premultiplied converted to and from premultiplied form.
+__device__ uint32_t BlendStored(uint32_t src, uint32_t dst, int mode, bool premul) {
+  const uint32_t a = src >> 24;
+  switch (mode) {
+    case HIPR_BLEND_CLEAR: return 0;
+    case HIPR_BLEND_SRC: return src;
+    case HIPR_BLEND_DST: return dst;
+    case HIPR_BLEND_SRC_OVER:
+      if (a == 0) return dst;
+      if (a == 255) return src;
+      break;
+    case HIPR_BLEND_DST_OVER:
+      if ((dst >> 24) == 255) return dst;
+      break;
+    case HIPR_BLEND_DST_IN:
+      if (a == 255) return dst;
+      if (a == 0) return 0;
+      break;
+    case HIPR_BLEND_DST_OUT:
+      if (a == 0) return dst;
+      if (a == 255) return 0;
+      break;
+    default:
+      break;
+  }
+  uint32_t d = premul ? dst : PremulWord(dst);
+  uint32_t r = PorterDuffWord(src, d, mode);
+  return premul ? r : UnpremulWord(r);
+}
+__device__ __forceinline__ uint32_t SwapRB(uint32_t v) {
+  return (v & 0xFF00FF00u) | ((v & 0xFFu) << 16) | ((v >> 16) & 0xFFu);
 }
 
 // RemapFloatTile (sw_span_brush.cc:21-32).
@@ -161,9 +276,36 @@ struct CompositeParams {
   unsigned long long* counters;
 };
 
-__device__ __forceinline__ uint32_t ApplyCover(uint32_t dst, uint32_t color, uint32_t cover) {
+// SWSpanBrush::BrushH for one pixel and cover (sw_span_brush.cc:108-138):
+// AlphaMulQ by the cover unless it is 255, then the target blend.
+__device__ __forceinline__ uint32_t ApplyCover(uint32_t dst, uint32_t color, uint32_t cover,
+                                               int mode, bool premul) {
   uint32_t src = cover != 255 ? AlphaMulQ(color, cover) : color;
-  return BlendSrcOver(src, dst);
+  if (mode == HIPR_BLEND_SRC_OVER && premul) return BlendSrcOver(src, dst);
+  return BlendStored(src, dst, mode, premul);
+}
+
+// PixmapBrush colour of a device pixel (sw_span_brush.cc:555-579): the
+// pixel centre mapped to the layer's unit square, decal outside, nearest
+// sample (BitmapSampler, bitmap_sampler.cc:26-40,85-108), then the
+// Color4fFromColor -> Color4fToColor round trip per channel.
+__device__ uint32_t ImageColor(const PaintDev& pd, const Canvas& layer, int x, int y) {
+  float cx = x + 0.5f, cy = y + 0.5f;
+  float u = cx * pd.im[0] + cy * pd.im[1] + pd.im[2];
+  float v = cx * pd.im[3] + cy * pd.im[4] + pd.im[5];
+  if (u < 0.0f || u >= 1.0f || v < 0.0f || v >= 1.0f) return 0;
+  uint32_t ix = (uint32_t)(u * (float)(uint32_t)layer.width);
+  uint32_t iy = (uint32_t)(v * (float)(uint32_t)layer.height);
+  ix = ix > (uint32_t)layer.width - 1 ? (uint32_t)layer.width - 1 : ix;
+  iy = iy > (uint32_t)layer.height - 1 ? (uint32_t)layer.height - 1 : iy;
+  uint32_t s = ((const uint32_t*)layer.pixels)[(int64_t)iy * layer.width + ix];
+  uint32_t o = 0;
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++) {
+    float f = (float)((s >> (8 * ch)) & 0xFF) / 255.f;
+    o |= UnitToByte(f) << (8 * ch);
+  }
+  return o;
 }
 
 __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
@@ -181,6 +323,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
     }
   }
   const Canvas cv = P.canvases[lo];
+  const bool premul = (cv.format & kFormatUnpremul) == 0;
+  const bool bgra = (cv.format & kFormatBGRA) != 0;
   int lt = t - cv.tile_begin;
   int tx = lt % cv.tiles_x, ty = lt / cv.tiles_x;
   int px0 = tx * kTile + 4 * (lane & 3);
@@ -192,6 +336,7 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
   for (int q = 0; q < 4; q++) {
     int x = px0 + q;
     pix[q] = (row_ok && x < cv.width) ? rowp[x] : 0;
+    if (bgra) pix[q] = SwapRB(pix[q]);
   }
   int blends = 0;
   uint32_t touched = 0;  // bit q: pixel q blended at least once
@@ -201,7 +346,7 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
     bool present = false;
     if (j < cv.job_end) {
       const Job job = P.jobs[j];
-      if (job.kind == kJobImage) {
+      if (job.kind == kJobRect) {
         const PaintDev pd = P.paints[job.paint];
         int x0 = tx * kTile, y0 = ty * kTile;
         present = pd.L < x0 + kTile && pd.R > x0 && pd.T < y0 + kTile && pd.B > y0;
@@ -228,63 +373,45 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
       int q0 = __shfl(s0, b, kWave), q1 = __shfl(s1, b, kWave);
       const Job job = P.jobs[jj];
       const PaintDev pd = P.paints[job.paint];
-      if (job.kind == kJobImage) {
-        // PixmapBrush over the filter rect: cover 255 & global alpha.
-        const Canvas layer = P.canvases[pd.layer];
-        uint32_t ga = (uint32_t)pd.ga;
+      const int mode = pd.blend;
+      // covers of the four pixels: plane 0 and plane 1 of a walked job, or
+      // 255 inside a rectangle job's rectangle
+      uint32_t cov0, cov1 = 0;
+      if (job.kind == kJobRect) {
+        cov0 = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           int x = px0 + q;
-          if (!row_ok || x >= cv.width || x < pd.L || x >= pd.R || py < pd.T || py >= pd.B)
-            continue;
-          float u = (x + 0.5f) * pd.isx + (py + 0.5f) * 0.f + pd.itx;
-          float v = (x + 0.5f) * 0.f + (py + 0.5f) * pd.isy + pd.ity;
-          uint32_t color = 0;
-          if (!(u < 0.0f || u >= 1.0f || v < 0.0f || v >= 1.0f)) {
-            uint32_t ix = (uint32_t)(u * (float)(uint32_t)layer.width);
-            uint32_t iy = (uint32_t)(v * (float)(uint32_t)layer.height);
-            ix = ix > (uint32_t)layer.width - 1 ? (uint32_t)layer.width - 1 : ix;
-            iy = iy > (uint32_t)layer.height - 1 ? (uint32_t)layer.height - 1 : iy;
-            uint32_t s = ((const uint32_t*)layer.pixels)[(int64_t)iy * layer.width + ix];
-            // Color4fFromColor -> Color4fToColor round trip per channel.
-            uint32_t o = 0;
-#pragma unroll
-            for (int ch = 0; ch < 4; ch++) {
-              float f = (float)((s >> (8 * ch)) & 0xFF) / 255.f;
-              o |= UnitToByte(f) << (8 * ch);
-            }
-            color = o;
-          }
-          uint32_t alpha = 255u & ga;
-          pix[q] = ApplyCover(pix[q], color, alpha);
-          blends++;
-          touched |= 1u << q;
+          bool in = row_ok && x < cv.width && x >= pd.L && x < pd.R && py >= pd.T && py < pd.B;
+          cov0 |= (in ? 0xFFu : 0u) << (8 * q);
         }
-        continue;
+      } else {
+        // the maps hold absolute plane numbers
+        cov0 = *((const uint32_t*)(P.planes + (int64_t)q0 * kTilePlaneBytes) + lane);
+        if (q1 >= 0) cov1 = *((const uint32_t*)(P.planes + (int64_t)q1 * kTilePlaneBytes) + lane);
       }
-      // the maps hold absolute plane numbers
-      const uint32_t* pl0 = (const uint32_t*)(P.planes + (int64_t)q0 * kTilePlaneBytes) + lane;
-      uint32_t cov0 = *pl0;
-      uint32_t cov1 = 0;
-      if (q1 >= 0) cov1 = *((const uint32_t*)(P.planes + (int64_t)q1 * kTilePlaneBytes) + lane);
-      if (pd.kind == kPaintSolid) {
+      const uint32_t ga = (uint32_t)pd.ga;
+      if (pd.kind == kPaintSolid && ga == 255) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
-          if (c0) pix[q] = ApplyCover(pix[q], pd.color, c0);
-          if (c1) pix[q] = ApplyCover(pix[q], pd.color, c1);
+          if (c0) pix[q] = ApplyCover(pix[q], pd.color, c0, mode, premul);
+          if (c1) pix[q] = ApplyCover(pix[q], pd.color, c1, mode, premul);
           blends += (c0 != 0) + (c1 != 0);
           touched |= (uint32_t)((c0 | c1) != 0) << q;
         }
       } else {
-        const GradDev& g = P.grads[pd.grad];
+        const GradDev* g = pd.kind == kPaintGradient ? &P.grads[pd.grad] : nullptr;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
           if (c0 | c1) {
-            uint32_t color = GradientColor(g, px0 + q, py);
-            if (c0) pix[q] = ApplyCover(pix[q], color, c0);
-            if (c1) pix[q] = ApplyCover(pix[q], color, c1);
+            uint32_t color = pd.kind == kPaintSolid ? pd.color
+                             : g ? GradientColor(*g, px0 + q, py)
+                                 : ImageColor(pd, P.canvases[pd.layer], px0 + q, py);
+            // Brush: cover & global alpha (sw_span_brush.cc:101)
+            if (c0) pix[q] = ApplyCover(pix[q], color, c0 & ga, mode, premul);
+            if (c1) pix[q] = ApplyCover(pix[q], color, c1 & ga, mode, premul);
             blends += (c0 != 0) + (c1 != 0);
             touched |= 1u << q;
           }
@@ -295,7 +422,7 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     int x = px0 + q;
-    if (row_ok && x < cv.width) rowp[x] = pix[q];
+    if (row_ok && x < cv.width) rowp[x] = bgra ? SwapRB(pix[q]) : pix[q];
   }
   if (P.counters) {
     int nb = WaveSum(blends);

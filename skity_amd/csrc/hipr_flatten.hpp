@@ -79,12 +79,11 @@ struct EdgeSink {
 // pts[pi] (the source start point is pts[pi - 1]) and whose quad-path start
 // point is qstart.  Returns the quad-path end point.
 __device__ P2 VisitSegment(int verb, const float* pts, int pi, const float* w,
-                           int wi, P2 qstart, float tx, float ty, EdgeSink* sink,
+                           int wi, P2 qstart, const Affine& M, EdgeSink* sink,
                            bool write, bool bound) {
   auto P = [&](int i) { return P2{pts[2 * i], pts[2 * i + 1]}; };
-  // Canvas translation applied to quad-path points (Path::CopyWithMatrix):
-  // glm mat4 * vec4(x, y, 0, 1) = (1*x + 0*y) + (0*0 + t*1) = x + t.
-  auto T = [&](P2 p) { return P2{p.x + tx, p.y + ty}; };
+  // the canvas matrix on quad-path points (Path::CopyWithMatrix)
+  auto T = [&](P2 p) { return M.Map(p); };
   P2 qs = T(qstart);
   if (verb == HIPR_VERB_LINE) {
     P2 e = P(pi);
@@ -163,7 +162,7 @@ __global__ void __launch_bounds__(64)
   const int j = list ? list[blockIdx.x] : (int)blockIdx.x;
   const Job job = jobs[j];
   JobGeo g;
-  if (job.kind == kJobImage) {
+  if (job.kind == kJobRect) {
     if (Lane() == 0) {
       g.l = g.t = g.r = g.b = 0.f;
       g.edges = 0;
@@ -201,7 +200,7 @@ __global__ void __launch_bounds__(64)
     // Bounds: every move point except a trailing move (Path::ComputePtBounds
     // and Path::Iter's kDone on a final move).
     if (v == HIPR_VERB_MOVE && k != nv - 1) {
-      sink.Bound(P2{pts[2 * pi] + job.tx, pts[2 * pi + 1] + job.ty});
+      sink.Bound(job.m.Map(P2{pts[2 * pi], pts[2 * pi + 1]}));
     }
     bool seg = v >= 0 && IsSegment(v);
     P2 qstart{0.f, 0.f}, mv{0.f, 0.f};
@@ -214,23 +213,21 @@ __global__ void __launch_bounds__(64)
         qstart = SegmentQEnd(pv, pts, pi - VerbPoints(pv));
       }
       last = (k + 1 >= nv) || !IsSegment(verbs[k + 1]);
-      mv = P2{pts[2 * move_pt] + job.tx, pts[2 * move_pt + 1] + job.ty};
+      mv = job.m.Map(P2{pts[2 * move_pt], pts[2 * move_pt + 1]});
     }
     // Pass 1 (both modes): count this lane's edges and extend the bounds.
     sink.pos = 0;
     if (seg) {
-      P2 qend = VisitSegment(v, pts, pi, w, wi, qstart, job.tx, job.ty, &sink,
-                             false, true);
-      if (last) sink.Line(P2{qend.x + job.tx, qend.y + job.ty}, mv, false);
+      P2 qend = VisitSegment(v, pts, pi, w, wi, qstart, job.m, &sink, false, true);
+      if (last) sink.Line(job.m.Map(qend), mv, false);
     }
     int mine = sink.pos;
     int incl = WaveInclusiveSum(mine);
     // Pass 2 (write mode): build the records at the compacted positions.
     if (write && seg) {
       sink.pos = edge_carry + incl - mine;
-      P2 qend = VisitSegment(v, pts, pi, w, wi, qstart, job.tx, job.ty, &sink,
-                             true, false);
-      if (last) sink.Line(P2{qend.x + job.tx, qend.y + job.ty}, mv, true);
+      P2 qend = VisitSegment(v, pts, pi, w, wi, qstart, job.m, &sink, true, false);
+      if (last) sink.Line(job.m.Map(qend), mv, true);
     }
     edge_carry += __builtin_amdgcn_readlane(incl, kWave - 1);
     pt_carry += __builtin_amdgcn_readlane(pi_incl, kWave - 1);

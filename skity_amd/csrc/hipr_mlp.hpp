@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(64) FeaturesKernel(FeatureParams P) {
   const int lane = Lane();
   const Job job = P.jobs[j];
   float* out = P.out + (int64_t)j * kMlpIn;
-  if (job.kind == kJobImage) {
+  if (job.kind == kJobRect) {
     if (lane < kMlpIn) out[lane] = 0.f;
     return;
   }
@@ -383,7 +383,8 @@ __global__ void __launch_bounds__(64) FeaturesKernel(FeatureParams P) {
       }
     }
     for (int q = 0; q < np; q++) {
-      float x = pts[2 * (pi + q)] + job.tx, y = pts[2 * (pi + q) + 1] + job.ty;
+      P2 d = job.m.Map(P2{pts[2 * (pi + q)], pts[2 * (pi + q) + 1]});
+      float x = d.x, y = d.y;
       minx = fminf(minx, x);
       maxx = fmaxf(maxx, x);
       miny = fminf(miny, y);

@@ -91,6 +91,9 @@ struct WalkParams {
 // (the reference's float -> 26.6 conversion overflows there); the clamp keeps
 // every width and height of a refused job within int32.
 constexpr float kCoordLimit = 8388608.f;  // 2^23 px
+// Source coordinates are refused far beyond it (a canvas matrix could bring
+// them back into range, so the device-space check is the one that counts).
+constexpr float kSourceCoordLimit = 1073741824.f;  // 2^30
 struct PixBounds {
   int l, t, r, b;
   bool empty;

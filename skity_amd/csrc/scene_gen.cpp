@@ -372,9 +372,10 @@ void sg_canvas_size(sg_scene s, int32_t* w, int32_t* h) {
   *h = s->height;
 }
 
-void sg_struct_sizes(int32_t* paint, int32_t* gradient) {
+void sg_struct_sizes(int32_t* paint, int32_t* gradient, int32_t* scene) {
   *paint = (int32_t)sizeof(hipr_paint);
   *gradient = (int32_t)sizeof(hipr_gradient);
+  *scene = (int32_t)sizeof(hipr_scene);
 }
 
 sg_scene sg_slice(sg_scene s, int32_t begin, int32_t end) {

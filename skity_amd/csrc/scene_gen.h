@@ -36,10 +36,12 @@ __attribute__((visibility("default"))) const hipr_scene* sg_view(sg_scene s);
 __attribute__((visibility("default"))) void sg_canvas_size(sg_scene s,
                                                            int32_t* w,
                                                            int32_t* h);
-/* sizeof(hipr_paint) and sizeof(hipr_gradient) this library was built
- * with, so that a caller can refuse a library built from another header. */
+/* sizeof(hipr_paint), sizeof(hipr_gradient) and sizeof(hipr_scene) this
+ * library was built with, so that a caller can refuse a library built from
+ * another header. */
 __attribute__((visibility("default"))) void sg_struct_sizes(int32_t* paint,
-                                                            int32_t* gradient);
+                                                            int32_t* gradient,
+                                                            int32_t* scene);
 /* Keep only paths [begin, end) (for bounded samples). */
 __attribute__((visibility("default"))) sg_scene sg_slice(sg_scene s,
                                                          int32_t begin,

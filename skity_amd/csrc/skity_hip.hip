@@ -259,6 +259,7 @@ struct hipr_context_s {
   std::vector<hipGraphExec_t> graph_execs;
   DevBuf stroke_out, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w;
   DevBuf layers, blur_rows, blur_cols, blur_rb, blur_cb, blur_tmp, blur_jobs, blur_lines;
+  DevBuf blur_raw, blur_styles;  // unblurred layers and jobs of the style pass
   // debug/utility uploads
   DevBuf d_verbs, d_pts, d_w, d_jobs, run_jobs, counters;
   // learned pre-sizer
@@ -288,6 +289,7 @@ struct hipr_context_s {
 struct hipr_surface_s {
   hipr_context ctx = nullptr;
   int32_t width = 0, height = 0;
+  int32_t format = 0;  // kFormat* bits
   uint8_t* pixels = nullptr;
   hipr_stats stats{};
   std::vector<int32_t> job_edges, job_planes;  // exact counts of the last draw
@@ -315,6 +317,9 @@ struct hipr_prepared_s {
   std::vector<GradDev> grads;
   std::vector<BlurJob> blurs;
   std::vector<int32_t> layer_of_blur;
+  std::vector<int32_t> blur_style;   // per blur: HIPR_BLUR_*
+  std::vector<uint8_t> x_verbs;      // geometry appended after the scene's:
+  std::vector<float> x_pts;          // the rectangles of walked layer composites
   std::vector<int32_t> walk_jobs;    // job indices that are rasterised
   std::vector<int32_t> stroke_jobs;  // job indices that need an outline
   std::vector<int32_t> stroke_quads; // per stroke job: segments of its quad path
@@ -452,15 +457,28 @@ hipr_result ValidateScene(const hipr_scene* s) {
       // rect.cc:12-49, only tests isinf of 0 * x products, which a NaN passes)
       if (!std::isfinite(s->points[2 * i]) || !std::isfinite(s->points[2 * i + 1]))
         return HIPR_ERROR_INVALID_ARGUMENT;
+      // far beyond any device coordinate the walk can take (checked again in
+      // device space after the flatten)
+      if (std::fabs(s->points[2 * i]) > kSourceCoordLimit ||
+          std::fabs(s->points[2 * i + 1]) > kSourceCoordLimit)
+        return HIPR_ERROR_NOT_SUPPORTED;
+    }
+    const int kind = s->draw_kinds ? s->draw_kinds[p] : HIPR_DRAW_PATH;
+    if (kind != HIPR_DRAW_PATH && kind != HIPR_DRAW_PAINT) return HIPR_ERROR_INVALID_ARGUMENT;
+    if (kind == HIPR_DRAW_PAINT && v1 != v0) return HIPR_ERROR_INVALID_ARGUMENT;
+    if (s->matrices) {
+      for (int k = 0; k < 6; k++)
+        if (!std::isfinite(s->matrices[6 * (size_t)p + k])) return HIPR_ERROR_INVALID_ARGUMENT;
     }
     for (int i = s->weight_offsets[p]; i < s->weight_offsets[p + 1]; i++) {
       float w = s->weights[i];
       if (!std::isfinite(w) || !(w > 0)) return HIPR_ERROR_INVALID_ARGUMENT;
     }
     const hipr_paint& pt = s->paints[p];
-    if (pt.blend != HIPR_BLEND_SRC_OVER) return HIPR_ERROR_NOT_SUPPORTED;
+    if (pt.blend < HIPR_BLEND_CLEAR || pt.blend > HIPR_BLEND_LAST)
+      return HIPR_ERROR_INVALID_ARGUMENT;
     if (pt.style < 0 || pt.style > 3) return HIPR_ERROR_INVALID_ARGUMENT;
-    if (pt.style != HIPR_STYLE_FILL &&
+    if (kind == HIPR_DRAW_PATH && pt.style != HIPR_STYLE_FILL &&
         (!std::isfinite(pt.stroke_width) || pt.stroke_width <= 0.f))
       return HIPR_ERROR_NOT_SUPPORTED;  // hairlines take a different path
     if (pt.cap < 0 || pt.cap > 2 || pt.join < 0 || pt.join > 2)
@@ -473,8 +491,9 @@ hipr_result ValidateScene(const hipr_scene* s) {
       if (g.type < HIPR_GRADIENT_LINEAR || g.type > HIPR_GRADIENT_SWEEP)
         return HIPR_ERROR_NOT_SUPPORTED;
     }
-    if (pt.blur_radius > 0 && pt.blur_style != HIPR_BLUR_NORMAL)
-      return HIPR_ERROR_NOT_SUPPORTED;
+    if (kind == HIPR_DRAW_PATH && pt.blur_radius > 0 &&
+        (pt.blur_style < HIPR_BLUR_NORMAL || pt.blur_style > HIPR_BLUR_INNER))
+      return HIPR_ERROR_INVALID_ARGUMENT;
   }
   return HIPR_SUCCESS;
 }
@@ -503,6 +522,8 @@ PaintDev SolidOrGradient(const hipr_scene* s, const hipr_paint& pt, const Aff& c
                          bool stroke, std::vector<GradDev>* grads) {
   PaintDev pd{};
   pd.ga = 255;
+  pd.blend = pt.blend;
+  pd.layer = -1;
   if (pt.gradient >= 0) {
     pd.kind = kPaintGradient;
     pd.grad = (int32_t)grads->size();
@@ -514,9 +535,26 @@ PaintDev SolidOrGradient(const hipr_scene* s, const hipr_paint& pt, const Aff& c
   return pd;
 }
 
+Affine ToAffine(const Aff& a) { return Affine{a.sx, a.kx, a.tx, a.ky, a.sy, a.ty}; }
+
+// The canvas matrix of draw p (identity when the scene has none).
+Aff DrawMatrix(const hipr_scene* s, int p) {
+  Aff m;
+  if (s->matrices) {
+    const float* f = s->matrices + 6 * (size_t)p;
+    m.sx = f[0];
+    m.kx = f[1];
+    m.tx = f[2];
+    m.ky = f[3];
+    m.sy = f[4];
+    m.ty = f[5];
+  }
+  return m;
+}
+
 // Fill and/or stroke jobs in paint order (DrawFillStrokeInPaintOrder,
-// src/render/paint_order.hpp:12-31).
-void AddShapeJobs(const hipr_scene* s, int p, int canvas, float tx, float ty, const Aff& ctm,
+// src/render/paint_order.hpp:12-31), rasterised through `ctm`.
+void AddShapeJobs(const hipr_scene* s, int p, int canvas, const Aff& ctm,
                   hipr_prepared_s* pl, std::vector<Job>* out) {
   const hipr_paint& pt = s->paints[p];
   bool need_fill = pt.style != HIPR_STYLE_STROKE;
@@ -533,8 +571,7 @@ void AddShapeJobs(const hipr_scene* s, int p, int canvas, float tx, float ty, co
     jb.verb_end = s->verb_offsets[p + 1];
     jb.pt_begin = s->point_offsets[p];
     jb.w_begin = s->weight_offsets[p];
-    jb.tx = tx;
-    jb.ty = ty;
+    jb.m = ToAffine(ctm);
     jb.image = -1;
     jb.paint = (int32_t)pl->paints.size();
     pl->paints.push_back(SolidOrGradient(s, pt, ctm, stroke, &pl->grads));
@@ -596,11 +633,41 @@ void LongestStrokesFirst(hipr_prepared_s* pl) {
   pl->stroke_quads.swap(quads);
 }
 
+// A job of full coverage over an integer rectangle (see kJobRect).
+Job RectJob(int p, int canvas, int paint) {
+  Job jb{};
+  jb.path = p;
+  jb.canvas = canvas;
+  jb.kind = kJobRect;
+  jb.fill = HIPR_FILL_WINDING;
+  jb.m = ToAffine(Aff());
+  jb.image = -1;
+  jb.paint = paint;
+  return jb;
+}
+
 void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
   std::vector<std::vector<Job>> canvas_jobs(1);
   pl->canvases.push_back(Canvas{});
+  const int32_t scene_verbs = s->verb_offsets[s->n_paths];
+  const int32_t scene_pts = s->point_offsets[s->n_paths];
   for (int p = 0; p < s->n_paths; p++) {
     const hipr_paint& pt = s->paints[p];
+    const Aff ctm = DrawMatrix(s, p);
+    const int kind = s->draw_kinds ? s->draw_kinds[p] : HIPR_DRAW_PATH;
+    if (kind == HIPR_DRAW_PAINT) {
+      // OnDrawPaint (sw_canvas.cc:412-438): the target rectangle under the
+      // identity, the paint's fill brush (its shader through the canvas
+      // matrix); no mask filter, style ignored.
+      PaintDev pd = SolidOrGradient(s, pt, ctm, false, &pl->grads);
+      pd.L = 0;
+      pd.T = 0;
+      pd.R = INT32_MAX;
+      pd.B = INT32_MAX;
+      canvas_jobs[0].push_back(RectJob(p, 0, (int32_t)pl->paints.size()));
+      pl->paints.push_back(pd);
+      continue;
+    }
     if (pt.blur_radius > 0) {
       // HandleFilter (sw_canvas.cc:797-826): local bounds, stroke-expanded
       // (floor on all sides, sw_canvas.cc:135-144), +- radius, floored /
@@ -627,42 +694,74 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
       pl->layer_bytes += ((int64_t)W * H * 4 + 255) & ~(int64_t)255;
       pl->canvases.push_back(lc);
       canvas_jobs.emplace_back();
-      AddShapeJobs(s, p, layer, -L, -T, Translate(-L, -T), pl, &canvas_jobs[layer]);
+      // the layer is a fresh canvas translated by -LT: the draw's canvas
+      // matrix does not reach it
+      AddShapeJobs(s, p, layer, Translate(-L, -T), pl, &canvas_jobs[layer]);
       BlurJob bj{};
       bj.width = W;
       bj.height = H;
       bj.radius = std::min((int)std::round(r), 254);
       pl->blurs.push_back(bj);
       pl->layer_of_blur.push_back(layer);
-      // The composite draw: OnDrawImageRect with a decal nearest pixmap
-      // shader over the filter bounds (sw_canvas.cc:641-677); the paint's
-      // alpha is applied as the brush's global alpha (Paint::GetAlphaF).
+      pl->blur_style.push_back(pt.blur_style);
+      // The composite draw: OnDrawImageRect of the layer over the filter
+      // bounds onto this canvas (sw_canvas.cc:641-677): the rectangle through
+      // the canvas matrix, a decal nearest pixmap shader mapping device
+      // points to the layer's unit square (GenerateBrush, sw_canvas.cc:
+      // 765-789: Scale(1/w, 1/h) * inverse(local) * inverse(CTM)), and the
+      // paint with its style set to fill, so that its alpha (Paint::GetAlphaF,
+      // paint.cc:96-98) is the fill colour's.
       PaintDev ip{};
       ip.kind = kPaintImage;
       ip.layer = layer;
-      float alpha_f = pt.style == HIPR_STYLE_STROKE ? pt.stroke_color[3] : pt.color[3];
-      ip.ga = (int32_t)(uint8_t)(255 * alpha_f);
-      ip.L = (int)L;
-      ip.T = (int)T;
-      ip.R = (int)R;
-      ip.B = (int)B;
-      Aff inv;
+      ip.blend = pt.blend;
+      ip.ga = (int32_t)(uint8_t)(255 * pt.color[3]);
+      Aff inv, l2l;
       Invert(Translate(L, T), &inv);
-      Aff m = Mul(Scale(1.f / W, 1.f / H), inv);  // Scale(1/w, 1/h) * inverse(local)
-      ip.isx = m.sx;
-      ip.itx = m.tx;
-      ip.isy = m.sy;
-      ip.ity = m.ty;
-      Job jb{};
-      jb.path = p;
-      jb.canvas = 0;
-      jb.kind = kJobImage;
-      jb.image = layer;
-      jb.paint = (int32_t)pl->paints.size();
+      if (!Invert(ctm, &l2l)) l2l = Aff();
+      Aff m = Mul(Mul(Scale(1.f / W, 1.f / H), inv), l2l);
+      ip.im[0] = m.sx;
+      ip.im[1] = m.kx;
+      ip.im[2] = m.tx;
+      ip.im[3] = m.ky;
+      ip.im[4] = m.sy;
+      ip.im[5] = m.ty;
+      const int32_t paint = (int32_t)pl->paints.size();
+      if (ctm.Identity()) {
+        // an integer rectangle under the identity: full coverage, no walk
+        ip.L = (int)L;
+        ip.T = (int)T;
+        ip.R = (int)R;
+        ip.B = (int)B;
+        Job jb = RectJob(p, 0, paint);
+        jb.image = layer;
+        canvas_jobs[0].push_back(jb);
+      } else {
+        // the rectangle path (Path::AddRect, path.cc:815-835) walked through
+        // the canvas matrix
+        Job jb{};
+        jb.path = p;
+        jb.canvas = 0;
+        jb.kind = kJobFill;
+        jb.fill = HIPR_FILL_WINDING;
+        jb.geom = 0;
+        jb.verb_begin = scene_verbs + (int32_t)pl->x_verbs.size();
+        jb.pt_begin = scene_pts + (int32_t)(pl->x_pts.size() / 2);
+        jb.w_begin = s->weight_offsets[s->n_paths];
+        const uint8_t rv[5] = {HIPR_VERB_MOVE, HIPR_VERB_LINE, HIPR_VERB_LINE, HIPR_VERB_LINE,
+                               HIPR_VERB_CLOSE};
+        const float rp[8] = {L, T, R, T, R, B, L, B};
+        pl->x_verbs.insert(pl->x_verbs.end(), rv, rv + 5);
+        pl->x_pts.insert(pl->x_pts.end(), rp, rp + 8);
+        jb.verb_end = jb.verb_begin + 5;
+        jb.m = ToAffine(ctm);
+        jb.image = layer;
+        jb.paint = paint;
+        canvas_jobs[0].push_back(jb);
+      }
       pl->paints.push_back(ip);
-      canvas_jobs[0].push_back(jb);
     } else {
-      AddShapeJobs(s, p, 0, 0.f, 0.f, Aff(), pl, &canvas_jobs[0]);
+      AddShapeJobs(s, p, 0, ctm, pl, &canvas_jobs[0]);
     }
   }
   // layers first (composited and blurred before the canvas), canvas last
@@ -674,7 +773,7 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
     cv.job_end = (int)pl->jobs.size();
   }
   for (size_t i = 0; i < pl->jobs.size(); i++) {
-    if (pl->jobs[i].kind != kJobImage) pl->walk_jobs.push_back((int32_t)i);
+    if (pl->jobs[i].kind != kJobRect) pl->walk_jobs.push_back((int32_t)i);
     if (pl->jobs[i].kind == kJobStroke) pl->stroke_jobs.push_back((int32_t)i);
   }
 }
@@ -904,7 +1003,7 @@ hipr_result Flatten(Frame& f, int mode, const int32_t* edge_est, std::vector<int
     HIP_TRY(hipStreamSynchronize(st));
     for (int j = 0; j < nj; j++) (*ecap)[j] = ctx->h_geo[j].edges;
   } else {
-    for (int j = 0; j < nj; j++) (*ecap)[j] = f.jobs[j].kind == kJobImage ? 0 : edge_est[j];
+    for (int j = 0; j < nj; j++) (*ecap)[j] = f.jobs[j].kind == kJobRect ? 0 : edge_est[j];
   }
   for (int j = 0; j < nj; j++) (*ebase)[j + 1] = (*ebase)[j] + (*ecap)[j];
   int64_t total = (*ebase)[nj];
@@ -1085,6 +1184,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     f.canvases[t].pixels = targets[t]->pixels;
     f.canvases[t].width = targets[t]->width;
     f.canvases[t].height = targets[t]->height;
+    f.canvases[t].format = targets[t]->format;
   }
   if (pl->layer_bytes > 0) {
     if (ctx->layers.Ensure(pl->layer_bytes) != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
@@ -1136,7 +1236,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   std::vector<int64_t> mbase(nj + 1, 0);
   for (int j = 0; j < nj; j++) {
     int64_t msz = 0;
-    if (f.jobs[j].kind != kJobImage) {
+    if (f.jobs[j].kind != kJobRect) {
       const JobGeo& g = ctx->h_geo[j];
       if (g.l < g.r && g.t < g.b &&
           (std::fabs(g.l) > kCoordLimit || std::fabs(g.r) > kCoordLimit ||
@@ -1564,7 +1664,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   }
   int64_t max_active = 0;
   for (int j = 0; j < nj; j++) {
-    if (f.jobs[j].kind == kJobImage) continue;
+    if (f.jobs[j].kind == kJobRect) continue;
     f.stats.tiles += hw[j].tiles;
     f.stats.tile_planes += hw[j].planes;
     max_active = std::max<int64_t>(max_active, hw[j].max_active);
@@ -1587,7 +1687,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     surf->job_path[j] = f.jobs[j].path;
     surf->job_kind[j] = f.jobs[j].kind;
     surf->job_canvas[j] = f.jobs[j].canvas;
-    if (f.jobs[j].kind == kJobImage) continue;
+    if (f.jobs[j].kind == kJobRect) continue;
     surf->job_ebase[j] = ebase[j];
     surf->job_mbase[j] = mbase[j];
     const Canvas& cv = f.canvases[f.jobs[j].canvas];
@@ -1600,7 +1700,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     t[4] = cv.tiles_x;
   }
   for (int j = 0; j < nj; j++) {
-    if (f.jobs[j].kind == kJobImage) continue;
+    if (f.jobs[j].kind == kJobRect) continue;
     surf->job_edges[j] = ctx->h_geo[j].edges;
     surf->job_planes[j] = hw[j].planes;
     surf->job_steps[j] = hw[j].steps;
@@ -1636,6 +1736,26 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     STAGE_LAUNCH(ctx, HIPR_STAGE_LAYERS,
     hipLaunchKernelGGL(CompositeKernel, dim3(layer_tiles), dim3(64), 0, st, lp));
     HIP_TRY(hipGetLastError());
+    // Layers of the blur styles other than kNormal keep their unblurred
+    // pixels for the style pass.
+    std::vector<BlurStyleJob> styled;
+    int64_t raw_bytes = 0;
+    for (size_t b = 0; b < pl->blurs.size(); b++)
+      if (pl->blur_style[b] != HIPR_BLUR_NORMAL)
+        raw_bytes += (int64_t)pl->blurs[b].width * pl->blurs[b].height * 4;
+    if (raw_bytes > 0) {
+      if (ctx->blur_raw.Ensure(raw_bytes) != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+      int64_t roff = 0;
+      for (size_t b = 0; b < pl->blurs.size(); b++) {
+        if (pl->blur_style[b] == HIPR_BLUR_NORMAL) continue;
+        const int64_t n = (int64_t)pl->blurs[b].width * pl->blurs[b].height;
+        uint8_t* raw = ctx->blur_raw.as<uint8_t>() + roff;
+        uint8_t* layer_px = f.canvases[pl->layer_of_blur[b]].pixels;
+        HIP_TRY(hipMemcpyAsync(raw, layer_px, n * 4, hipMemcpyDeviceToDevice, st));
+        styled.push_back(BlurStyleJob{raw, layer_px, n, pl->blur_style[b], 0});
+        roff += n * 4;
+      }
+    }
     // StackBlur each layer: rows into a temporary, columns back
     std::vector<BlurJob> rows, cols;
     std::vector<int64_t> rb{0}, cb{0};
@@ -1679,6 +1799,15 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     hipLaunchKernelGGL(StackBlurLineKernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st,
                          ctx->blur_cols.as<BlurJob>(), ctx->blur_cb.as<int64_t>(),
                          (int)cols.size(), nc, 1));
+      HIP_TRY(hipGetLastError());
+    }
+    if (!styled.empty()) {
+      if ((r = Upload(&ctx->blur_styles, styled.data(), styled.size(), st)) != HIPR_SUCCESS)
+        return r;
+      dim3 grid(256, (unsigned)std::min<size_t>(styled.size(), 65535));
+      STAGE_LAUNCH(ctx, HIPR_STAGE_BLUR,
+    hipLaunchKernelGGL(BlurStyleKernel, grid, dim3(256), 0, st,
+                         ctx->blur_styles.as<BlurStyleJob>(), (int)styled.size()));
       HIP_TRY(hipGetLastError());
     }
   }
@@ -1765,13 +1894,28 @@ hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s**
     BuildPlan(scene, pl);
     int n = scene->n_paths;
     int nv = scene->verb_offsets[n], np = scene->point_offsets[n], nw = scene->weight_offsets[n];
+    for (int32_t j : pl->stroke_jobs) pl->stroke_quads.push_back(QuadPathSegments(scene, pl->jobs[j].path));
+    LongestStrokesFirst(pl);
+    // the scene's geometry, then the rectangles the plan added
+    std::vector<uint8_t> all_verbs;
+    std::vector<float> all_pts;
+    const uint8_t* vsrc = scene->verbs;
+    const float* psrc = scene->points;
+    if (!pl->x_verbs.empty()) {
+      all_verbs.assign(scene->verbs, scene->verbs + nv);
+      all_verbs.insert(all_verbs.end(), pl->x_verbs.begin(), pl->x_verbs.end());
+      all_pts.assign(scene->points, scene->points + 2 * (size_t)np);
+      all_pts.insert(all_pts.end(), pl->x_pts.begin(), pl->x_pts.end());
+      vsrc = all_verbs.data();
+      psrc = all_pts.data();
+      nv += (int)pl->x_verbs.size();
+      np += (int)(pl->x_pts.size() / 2);
+    }
     pl->n_verbs = nv;
     pl->n_pts = np;
     pl->n_weights = nw;
-    for (int32_t j : pl->stroke_jobs) pl->stroke_quads.push_back(QuadPathSegments(scene, pl->jobs[j].path));
-    LongestStrokesFirst(pl);
-    if ((r = Upload(&pl->verbs, scene->verbs, nv, st)) != HIPR_SUCCESS ||
-        (r = Upload(&pl->pts, scene->points, 2 * (size_t)np, st)) != HIPR_SUCCESS ||
+    if ((r = Upload(&pl->verbs, vsrc, nv, st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->pts, psrc, 2 * (size_t)np, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->weights, scene->weights, nw, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->scene_paints, scene->paints, n, st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->d_jobs, pl->jobs.data(), pl->jobs.size(), st)) != HIPR_SUCCESS ||
@@ -1806,6 +1950,7 @@ hipr_result MergeBatch(hipr_batch b) {
   m.grads.clear();
   m.blurs.clear();
   m.layer_of_blur.clear();
+  m.blur_style.clear();
   m.walk_jobs.clear();
   m.stroke_jobs.clear();
   m.stroke_quads.clear();
@@ -1831,14 +1976,13 @@ hipr_result MergeBatch(hipr_batch b) {
       Job jb = src;
       jb.path += path_off;
       jb.canvas = canv(jb.canvas);
-      if (jb.kind != kJobImage) {
+      if (jb.kind != kJobRect) {
         jb.verb_begin += (int32_t)voff;
         jb.verb_end += (int32_t)voff;
         jb.pt_begin += (int32_t)poff;
         jb.w_begin += (int32_t)woff;
-      } else {
-        jb.image = canv(jb.image);
       }
+      if (jb.image >= 0) jb.image = canv(jb.image);
       jb.paint += paint_off;
       m.jobs.push_back(jb);
     }
@@ -1861,6 +2005,7 @@ hipr_result MergeBatch(hipr_batch b) {
     }
     m.blurs.insert(m.blurs.end(), s->blurs.begin(), s->blurs.end());
     for (int32_t l : s->layer_of_blur) m.layer_of_blur.push_back(canv(l));
+    m.blur_style.insert(m.blur_style.end(), s->blur_style.begin(), s->blur_style.end());
     for (int32_t j : s->walk_jobs) m.walk_jobs.push_back(j + job_off);
     for (int32_t j : s->stroke_jobs) m.stroke_jobs.push_back(j + job_off);
     m.stroke_quads.insert(m.stroke_quads.end(), s->stroke_quads.begin(), s->stroke_quads.end());
@@ -1991,7 +2136,7 @@ static void FreeContext(hipr_context ctx) {
                     &ctx->d_jobs, &ctx->run_jobs, &ctx->counters, &ctx->wcov,
                     &ctx->qstate, &ctx->q_base, &ctx->snaps, &ctx->snap_base,
                     &ctx->band_count, &ctx->queues, &ctx->flatten_redo, &ctx->pool_ctr, &ctx->stroke_redo,
-                    &ctx->stroke_redo_out};
+                    &ctx->stroke_redo_out, &ctx->blur_raw, &ctx->blur_styles};
   ReleaseGraphs(ctx);
   for (DevBuf* b : bufs) b->Free();
   DevBuf* mlp[] = {&ctx->mlp_prm, &ctx->mlp_ops, &ctx->mlp_feats, &ctx->mlp_preds,
@@ -2018,14 +2163,25 @@ static void FreeContext(hipr_context ctx) {
 
 HIPR_API hipr_result hipr_surface_create(hipr_context ctx, int32_t width, int32_t height,
                                          hipr_surface* out) {
+  return hipr_surface_create_ex(ctx, width, height, HIPR_ALPHA_PREMUL, HIPR_COLOR_RGBA, out);
+}
+
+HIPR_API hipr_result hipr_surface_create_ex(hipr_context ctx, int32_t width, int32_t height,
+                                            int32_t alpha_type, int32_t color_type,
+                                            hipr_surface* out) {
   if (!out) return HIPR_ERROR_INVALID_ARGUMENT;
   *out = nullptr;
   if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
   if (width <= 0 || height <= 0) return HIPR_ERROR_INVALID_ARGUMENT;
   if (width > kMaxCanvasSide || height > kMaxCanvasSide) return HIPR_ERROR_NOT_SUPPORTED;
+  if (alpha_type < HIPR_ALPHA_OPAQUE || alpha_type > HIPR_ALPHA_UNPREMUL ||
+      (color_type != HIPR_COLOR_RGBA && color_type != HIPR_COLOR_BGRA))
+    return HIPR_ERROR_INVALID_ARGUMENT;
   hipr_surface s = new (std::nothrow) hipr_surface_s();
   if (!s) return HIPR_ERROR_OUT_OF_HOST_MEMORY;
   s->ctx = ctx;
+  s->format = (alpha_type != HIPR_ALPHA_PREMUL ? kFormatUnpremul : 0) |
+              (color_type == HIPR_COLOR_BGRA ? kFormatBGRA : 0);
   s->width = width;
   s->height = height;
   if (hipSetDevice(ctx->device) != hipSuccess ||
@@ -2184,7 +2340,7 @@ HIPR_API hipr_result hipr_surface_get_job_edges(hipr_surface s, int32_t job, int
   hipr_context ctx = s->ctx;
   if (s->draw_serial == 0 || s->draw_serial != ctx->draw_serial)
     return HIPR_ERROR_INVALID_ARGUMENT;  // another draw reused the buffers
-  const int64_t n = s->job_kind[job] == kJobImage ? 0 : s->job_edges[job];
+  const int64_t n = s->job_kind[job] == kJobRect ? 0 : s->job_edges[job];
   *n_edges = n;
   if (!edges || n == 0) return HIPR_SUCCESS;
   if (cap < n) return HIPR_ERROR_CAPACITY;
@@ -2217,7 +2373,7 @@ HIPR_API hipr_result hipr_surface_get_tile_bins(hipr_surface s, int64_t* offsets
   int64_t k = 0;
   for (int j = 0; j < n_jobs; j++) {
     offsets[j] = k;
-    if (s->job_kind[j] == kJobImage) continue;
+    if (s->job_kind[j] == kJobRect) continue;
     const int32_t* t = &s->job_tiles[(size_t)j * 5];
     for (int r = 0; r < t[3]; r++) {
       for (int c = 0; c < t[2]; c++) {
@@ -2568,6 +2724,7 @@ HIPR_API hipr_result hipr_debug_build_edges(hipr_context ctx, const hipr_scene* 
     jb.verb_end = scene->verb_offsets[p + 1];
     jb.pt_begin = scene->point_offsets[p];
     jb.w_begin = scene->weight_offsets[p];
+    jb.m = ToAffine(DrawMatrix(scene, p));
     jobs[p] = jb;
   }
   int nv = n ? scene->verb_offsets[n] : 0, np = n ? scene->point_offsets[n] : 0;

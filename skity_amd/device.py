@@ -27,6 +27,10 @@ def lib():
         L.hipr_surface_create.argtypes = [c_ctx, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.POINTER(c_surf)]
         L.hipr_surface_create.restype = ctypes.c_int
+        L.hipr_surface_create_ex.argtypes = [c_ctx, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.POINTER(c_surf)]
+        L.hipr_surface_create_ex.restype = ctypes.c_int
         L.hipr_surface_destroy.argtypes = [c_surf]
         L.hipr_surface_destroy.restype = None
         L.hipr_surface_clear.argtypes = [c_surf, ctypes.c_uint32]
@@ -195,12 +199,14 @@ class Prepared:
 
 
 class Surface:
-    def __init__(self, ctx, width, height):
+    def __init__(self, ctx, width, height, alpha_type=abi.ALPHA_PREMUL,
+                 color_type=abi.COLOR_RGBA):
         self.ctx = ctx
         self.width, self.height = width, height
         self._h = ctypes.c_void_p()
-        check(lib().hipr_surface_create(ctx.handle, width, height, ctypes.byref(self._h)),
-              "hipr_surface_create")
+        check(lib().hipr_surface_create_ex(ctx.handle, width, height, alpha_type, color_type,
+                                           ctypes.byref(self._h)),
+              "hipr_surface_create_ex")
 
     def clear(self, rgba=0):
         check(lib().hipr_surface_clear(self._h, rgba), "hipr_surface_clear")

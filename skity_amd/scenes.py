@@ -29,13 +29,15 @@ def _load():
                                        ctypes.POINTER(ctypes.c_int32)]
         lib.sg_slice.restype = ctypes.c_void_p
         lib.sg_slice.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
-        lib.sg_struct_sizes.argtypes = [ctypes.POINTER(ctypes.c_int32)] * 2
-        p, g = ctypes.c_int32(), ctypes.c_int32()
-        lib.sg_struct_sizes(ctypes.byref(p), ctypes.byref(g))
-        if (p.value, g.value) != (ctypes.sizeof(abi.Paint), ctypes.sizeof(abi.Gradient)):
+        lib.sg_struct_sizes.argtypes = [ctypes.POINTER(ctypes.c_int32)] * 3
+        p, g, sc = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        lib.sg_struct_sizes(ctypes.byref(p), ctypes.byref(g), ctypes.byref(sc))
+        if (p.value, g.value, sc.value) != (ctypes.sizeof(abi.Paint),
+                                            ctypes.sizeof(abi.Gradient),
+                                            ctypes.sizeof(abi.Scene)):
             raise RuntimeError(
                 "libskity_scene.so was built from another include/skity_hip.h "
-                f"(paint {p.value} B, gradient {g.value} B); rebuild it")
+                f"(paint {p.value} B, gradient {g.value} B, scene {sc.value} B); rebuild it")
         _lib = lib
     return _lib
 

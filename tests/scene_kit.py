@@ -1,0 +1,131 @@
+"""Small seeded scenes exercising the canvas matrix, DrawPaint, blend modes,
+target formats and blur styles (shared by the oracle pins and the GPU
+parity tests)."""
+import math
+
+import numpy as np
+
+from skity_amd import abi, canvas
+from skity_amd.canvas import Canvas, Paint, Path
+
+BLEND_MODES = list(range(abi.BLEND_SCREEN + 1)) + [abi.BLEND_SOFT_LIGHT, 16, 28]
+
+
+def random_path(rng, cx, cy, size, kind="mixed"):
+    p = Path()
+    p.move_to(cx + rng.uniform(-size, size), cy + rng.uniform(-size, size))
+    for _ in range(int(rng.integers(3, 9))):
+        u = rng.random() if kind == "mixed" else 0.0
+        pt = lambda: (cx + rng.uniform(-size, size), cy + rng.uniform(-size, size))
+        if u < 0.35:
+            p.line_to(*pt())
+        elif u < 0.6:
+            p.quad_to(*pt(), *pt())
+        elif u < 0.8:
+            p.cubic_to(*pt(), *pt(), *pt())
+        else:
+            p.conic_to(*pt(), *pt(), rng.uniform(0.3, 3.0))
+    p.close()
+    if rng.random() < 0.5:
+        p.fill_type = abi.FILL_EVEN_ODD
+    return p
+
+
+def random_gradient(rng, cx, cy, size):
+    g = abi.Gradient()
+    g.type = int(rng.choice([abi.GRADIENT_LINEAR, abi.GRADIENT_RADIAL, abi.GRADIENT_SWEEP]))
+    g.tile_mode = int(rng.integers(0, 3))
+    g.color_count = int(rng.integers(2, 6))
+    g.has_offsets = 1
+    offs = np.sort(rng.random(g.color_count))
+    for i in range(g.color_count):
+        g.offsets[i] = float(offs[i])
+        for c in range(4):
+            g.colors[i][c] = float(rng.random())
+    g.points[0], g.points[1] = cx - size / 2, cy - size / 3
+    g.points[2], g.points[3] = cx + size / 2, cy + size / 3
+    if g.type == abi.GRADIENT_RADIAL:
+        g.radius[0] = size
+    elif g.type == abi.GRADIENT_SWEEP:
+        start, end = 0.0, float(rng.uniform(90, 360))
+        g.radius[0] = -start / 360.0
+        g.radius[1] = 1.0 / ((end - start) / 360.0)
+    g.local_matrix[0] = 1.0
+    g.local_matrix[4] = 1.0
+    return g
+
+
+def random_paint(rng, size, cx, cy, allow_blur=True, blend=None, gradient=0.3):
+    paint = Paint((rng.random(), rng.random(), rng.random(), rng.uniform(0.3, 1.0)))
+    r = rng.random()
+    if r < 0.25:
+        paint.set_style(abi.STYLE_STROKE)
+    elif r < 0.35:
+        paint.set_style(abi.STYLE_STROKE_AND_FILL)
+    paint.set_stroke_width(float(rng.uniform(1, 8)))
+    paint.set_stroke_join(int(rng.integers(0, 3)))
+    paint.set_stroke_cap(int(rng.integers(0, 3)))
+    if rng.random() < gradient:
+        paint.set_gradient(random_gradient(rng, cx, cy, size))
+    if allow_blur and rng.random() < 0.2:
+        paint.set_blur(float(rng.uniform(1.5, 8.0)), int(rng.integers(1, 5)))
+    if blend is not None:
+        paint.set_blend_mode(blend)
+    return paint
+
+
+def matrix_scene(seed, w=320, h=240, n=24):
+    """Draws under rotations, scales, skews and translations (Save/Restore
+    around each), with fills, strokes, gradients and blurred draws."""
+    cv = Canvas(w, h)
+    for path, paint, m, kind in matrix_scene_draws(seed, w, h, n):
+        cv.save()
+        cv.concat(m)
+        cv.draw_path(path, paint)
+        cv.restore()
+    return cv.scene()
+
+
+def matrix_scene_draws(seed, w=320, h=240, n=24):
+    """(path, paint, matrix, kind) of matrix_scene's draws."""
+    rng = np.random.default_rng(seed)
+    cv = Canvas(w, h)
+    for i in range(n):
+        cv.save()
+        cx, cy = rng.uniform(40, w - 40), rng.uniform(40, h - 40)
+        cv.translate(cx, cy)
+        k = i % 4
+        if k == 0:
+            cv.rotate(float(rng.uniform(0, 360)))
+        elif k == 1:
+            cv.scale(float(rng.uniform(0.4, 2.5)), float(rng.uniform(0.4, 2.5)))
+        elif k == 2:
+            cv.skew(float(rng.uniform(-0.6, 0.6)), float(rng.uniform(-0.6, 0.6)))
+        else:
+            cv.rotate(float(rng.uniform(0, 360)))
+            cv.scale(float(rng.uniform(0.5, 2.0)), float(rng.uniform(0.5, 2.0)))
+        size = float(rng.uniform(10, 40))
+        cv.draw_path(random_path(rng, 0.0, 0.0, size), random_paint(rng, size, 0.0, 0.0))
+        cv.restore()
+    return cv.draws
+
+
+def paint_blend_scene(seed, w=96, h=72):
+    """A backdrop, then DrawPaint and DrawPath calls in every blend mode the
+    software backend blends (and modes that fall back to kSrcOver)."""
+    rng = np.random.default_rng(seed)
+    cv = Canvas(w, h)
+    cv.draw_paint(Paint((0.2, 0.4, 0.6, 0.5)))
+    for mode in BLEND_MODES:
+        size = float(rng.uniform(8, 30))
+        cx, cy = rng.uniform(0, w), rng.uniform(0, h)
+        paint = random_paint(rng, size, cx, cy, allow_blur=False, blend=mode)
+        cv.draw_path(random_path(rng, cx, cy, size), paint)
+    cv.save()
+    cv.rotate(20, w / 2, h / 2)
+    g = Paint((1, 1, 1, 1)).set_gradient(random_gradient(rng, w / 2, h / 2, 40))
+    g.set_blend_mode(abi.BLEND_SRC_ATOP)
+    cv.draw_paint(g)
+    cv.restore()
+    cv.draw_paint(Paint((0.9, 0.1, 0.1, 0.25)).set_blend_mode(abi.BLEND_DST_OVER))
+    return cv.scene()
