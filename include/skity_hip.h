@@ -422,6 +422,18 @@ HIPR_API hipr_result hipr_stackblur(hipr_context ctx, const uint8_t* src_host,
                                     uint8_t* dst_host, int32_t width,
                                     int32_t height, int32_t radius);
 
+/* hipr_stackblur_ex with HIPR_BLUR_IMPL_MFMA, on device-resident images, that
+ * allocates nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  src and dst are device pointers
+ * to width*height*4 bytes (premultiplied RGBA8) that do not overlap; the
+ * intermediate of the two passes lives in one static device buffer per GPU,
+ * so width*height is at most 4,194,304 (else HIPR_ERROR_CAPACITY), and calls
+ * that overlap on one GPU race.  Returns when the blur is done. */
+HIPR_API hipr_result hipr_stackblur_device_noalloc(hipr_context ctx, const uint8_t* src,
+                                                   uint8_t* dst, int32_t width,
+                                                   int32_t height, int32_t radius);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

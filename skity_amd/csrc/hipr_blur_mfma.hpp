@@ -38,9 +38,9 @@ struct BlurTileJob {
 
 // pass 0: rows (src -> dst); pass 1: columns.  One wavefront per tile.
 template <int kPass>
-__global__ void __launch_bounds__(64)
-    StackBlurMfmaKernel(const BlurTileJob* __restrict__ jobs, int n_jobs, int64_t total_tiles) {
-  __shared__ _Float16 stage[kBlurLines * 4 * kBlurSpan];
+__device__ __forceinline__ void StackBlurMfmaTiles(const BlurTileJob* __restrict__ jobs,
+                                                   int n_jobs, int64_t total_tiles,
+                                                   _Float16* stage) {
   const int lane = Lane();
   for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
     int lo = 0, hi = n_jobs - 1;
@@ -111,6 +111,20 @@ __global__ void __launch_bounds__(64)
     }
     WaveSync();
   }
+}
+
+template <int kPass>
+__global__ void __launch_bounds__(64)
+    StackBlurMfmaKernel(const BlurTileJob* __restrict__ jobs, int n_jobs, int64_t total_tiles) {
+  __shared__ _Float16 stage[kBlurLines * 4 * kBlurSpan];
+  StackBlurMfmaTiles<kPass>(jobs, n_jobs, total_tiles, stage);
+}
+
+// One image, its job passed by value (no job table in device memory).
+template <int kPass>
+__global__ void __launch_bounds__(64) StackBlurMfmaOneKernel(BlurTileJob job, int64_t total_tiles) {
+  __shared__ _Float16 stage[kBlurLines * 4 * kBlurSpan];
+  StackBlurMfmaTiles<kPass>(&job, 1, total_tiles, stage);
 }
 
 }  // namespace hipr

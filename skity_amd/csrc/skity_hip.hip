@@ -28,6 +28,7 @@
 #include "hipr_composite.hpp"
 #include "hipr_flatten.hpp"
 #include "hipr_mlp.hpp"
+#include "hipr_noalloc.hpp"
 #include "hipr_sort.hpp"
 #include "hipr_stroke.hpp"
 #include "hipr_walk.hpp"
@@ -2816,6 +2817,46 @@ HIPR_API hipr_result hipr_stackblur_ex(hipr_context ctx, const uint8_t* src_host
       return r;
   }
   HIP_TRY(hipMemcpyAsync(dst_host, a, bytes, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return HIPR_SUCCESS;
+}
+
+HIPR_API hipr_result hipr_stackblur_device_noalloc(hipr_context ctx, const uint8_t* src,
+                                                   uint8_t* dst, int32_t width, int32_t height,
+                                                   int32_t radius) {
+  if (!ctx) return HIPR_ERROR_INVALID_HANDLE;
+  if (!src || !dst || width <= 0 || height <= 0) return HIPR_ERROR_INVALID_ARGUMENT;
+  if ((int64_t)width * height > kNoAllocMaxPixels) {
+    std::fprintf(stderr, "skity_hip: hipr_stackblur_device_noalloc takes at most %lld pixels\n",
+                 (long long)kNoAllocMaxPixels);
+    return HIPR_ERROR_CAPACITY;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  uint8_t* tmp = nullptr;
+  HIP_TRY(NoAllocScratch(ctx->device, &tmp));
+  hipStream_t st = ctx->stream;
+  const size_t bytes = (size_t)width * height * 4;
+  radius = std::min(radius, 254);
+  if (radius <= 1) {  // SWStackBlur copies for radius <= 1
+    if (src != dst) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+  } else {
+    // rows: src -> scratch, columns: scratch -> dst
+    BlurTileJob rj{}, cj{};
+    rj.b = BlurJob{src, tmp, width, height, radius};
+    rj.line_tiles = (height + kBlurLines - 1) / kBlurLines;
+    rj.out_tiles = (width + kBlurOut - 1) / kBlurOut;
+    cj.b = BlurJob{tmp, dst, width, height, radius};
+    cj.line_tiles = (width + kBlurLines - 1) / kBlurLines;
+    cj.out_tiles = (height + kBlurOut - 1) / kBlurOut;
+    const int64_t tr = (int64_t)rj.line_tiles * rj.out_tiles;
+    const int64_t tc = (int64_t)cj.line_tiles * cj.out_tiles;
+    hipLaunchKernelGGL(StackBlurMfmaOneKernel<0>, dim3((unsigned)std::min<int64_t>(tr, 1 << 16)),
+                       dim3(64), 0, st, rj, tr);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(StackBlurMfmaOneKernel<1>, dim3((unsigned)std::min<int64_t>(tc, 1 << 16)),
+                       dim3(64), 0, st, cj, tc);
+    HIP_TRY(hipGetLastError());
+  }
   HIP_TRY(hipStreamSynchronize(st));
   return HIPR_SUCCESS;
 }
