@@ -40,6 +40,11 @@ def algorithmic_bytes(stage, st, geom_bytes, canvas_px):
     }.get(stage)
 
 
+def log(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def geometry_bytes(sc):
     """G of the scene: 8 B per point, 1 B per verb, 4 B per conic weight, a
     32-byte draw record per path and 20 B per gradient stop (the scene has
@@ -215,20 +220,37 @@ def blur_leg(ctx, torch, steps):
     return out
 
 
-def batched_leg(ctx, torch, steps, n_canvases=64):
-    """Batched uneven scenes (scaled family): n canvases of side 256-4096 with
-    10-1,000 paths of 4-316 line/quad/cubic segments each, drawn per canvas
+def batched_leg(ctx, torch, steps, n_canvases=64, workload=4, tune_canvases=8):
+    """Batched uneven scenes at the family's stated scale (workload 4): n
+    canvases of side 256-4096, each with 100-50,000 closed outlines of
+    4-10,000 line/quad/cubic segments (30% gradient paints), drawn per canvas
     (plain: one exact pipeline run per canvas) and as one batch (learned
-    pre-sizing, autotuned work split)."""
+    pre-sizing, work split autotuned on the first canvases)."""
     from skity_amd import abi, device, scenes
-    scs = [scenes.generate(5, seed=scenes.SEED_BASE + 500 + i) for i in range(n_canvases)]
-    preps = [device.Prepared(ctx, sc.scene) for sc in scs]
+    t0 = time.perf_counter()
+    scs = []
+    for i in range(n_canvases):
+        scs.append(scenes.generate(workload, seed=scenes.SEED_BASE + 400 + i))
+        if (i + 1) % 8 == 0:
+            log(f"batched leg: generated {i + 1} canvases, {time.perf_counter() - t0:.0f} s")
+    t_gen = time.perf_counter() - t0
+    preps = []
+    for sc in scs:
+        preps.append(device.Prepared(ctx, sc.scene))
+    log(f"batched leg: {len(preps)} scenes prepared")
     surfs = [device.Surface(ctx, sc.width, sc.height) for sc in scs]
     n_paths = sum(sc.n_paths for sc in scs)
+    n_segs = sum(sc.scene.contents.verb_offsets[sc.n_paths] for sc in scs)
+    tune = device.Batch(ctx)
+    for su, pr in list(zip(surfs, preps))[:tune_canvases]:
+        tune.add(su, pr)
+    tuned = tune.autotune(abi.MODE_LEARNED, repeats=1)
+    tune.close()
+    log(f"batched leg: autotuned on {tune_canvases} canvases: {tuned}")
     batch = device.Batch(ctx)
     for su, pr in zip(surfs, preps):
         batch.add(su, pr)
-    tuned = batch.autotune(abi.MODE_LEARNED, repeats=1)
+    batch.set_split(*tuned)
 
     def timed(fn):
         fn()
@@ -242,9 +264,12 @@ def batched_leg(ctx, torch, steps, n_canvases=64):
         return (time.perf_counter() - t0) / steps
 
     def plain():
-        for su, pr in zip(surfs, preps):
+        for i, (su, pr) in enumerate(zip(surfs, preps)):
             su.clear(0)
             su.draw(pr, mode=abi.MODE_EXACT)
+            if (i + 1) % 16 == 0:
+                su.finish()
+                log(f"batched leg: plain, {i + 1} canvases drawn")
 
     def batched():
         for su in surfs:
@@ -252,19 +277,31 @@ def batched_leg(ctx, torch, steps, n_canvases=64):
         batch.draw(mode=abi.MODE_LEARNED)
 
     t_plain = timed(plain)
+    log(f"batched leg: plain {t_plain:.2f} s per frame")
+    plain_frames = [su.read() for su in surfs]
     t_batch = timed(batched)
+    log(f"batched leg: batched + learned {t_batch:.2f} s per frame")
     st = batch.stats()
-    out = {"workload": f"batched uneven (scaled): {n_canvases} canvases 256^2-4096^2, "
-                       "10-1,000 paths of 4-316 segments each",
-           "canvases": n_canvases, "paths": n_paths,
+    same = all(np.array_equal(su.read(), f) for su, f in zip(surfs, plain_frames))
+    if not same:
+        print("bench: the batched frame differs from the per-canvas frames", file=sys.stderr,
+              flush=True)
+        sys.exit(3)
+    out = {"workload": f"batched uneven: {n_canvases} canvases 256^2-4096^2, 100-50,000 "
+                       "outlines of 4-10,000 line/quad/cubic segments each",
+           "canvases": n_canvases, "paths": n_paths, "segments": int(n_segs),
+           "generation_s": round(t_gen, 1),
            "plain_paths_per_s": round(n_paths / t_plain, 1),
            "plain_ms_per_step": round(t_plain * 1e3, 3),
            "batched_learned_paths_per_s": round(n_paths / t_batch, 1),
            "batched_learned_ms_per_step": round(t_batch * 1e3, 3),
            "speedup": round(t_plain / t_batch, 3),
+           "identical_to_plain": same,
            "mpix_per_s": round(st["blends"] / t_batch / 1e6, 1),
            "fallbacks": st["fallbacks"], "underestimates": st["underestimates"],
-           "autotuned": {"band_rows": tuned[0], "band_min_edges": tuned[1], "flags": tuned[2]}}
+           "edges": st["edges"], "tile_planes": st["tile_planes"],
+           "autotuned": {"band_rows": tuned[0], "band_min_edges": tuned[1], "flags": tuned[2],
+                         "on_canvases": tune_canvases}}
     batch.close()
     for su in surfs:
         su.close()
@@ -283,6 +320,8 @@ def main():
                     help="budget of the CPU baseline sample (0 = skip)")
     ap.add_argument("--forms", default="exact,learned",
                     help="comma list of forms to time: exact, learned")
+    ap.add_argument("--batch-steps", type=int, default=1,
+                    help="timed frames of the batched uneven leg (each is ~650M segments)")
     ap.add_argument("--legs", default="batched,blur",
                     help="extra workloads reported beside the headline: batched, blur")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -315,6 +354,7 @@ def main():
     for form in forms:
         mode = abi.MODE_EXACT if form == "exact" else abi.MODE_LEARNED
         results[form] = run_form(surf, prep, mode, args.steps, args.warmup, torch)
+        log(f"{form} form: {results[form][0] * 1e3 / args.steps:.1f} ms per frame")
     geom = geometry_bytes(sc)
     out_forms = {}
     for form, (wall, stage_tot, st) in results.items():
@@ -384,11 +424,13 @@ def main():
     legs = {}
     leg_names = [x for x in args.legs.split(",") if x]
     if "batched" in leg_names and "learned" in forms:
-        legs["batched_uneven"] = batched_leg(ctx, torch, max(2, args.steps // 4))
+        legs["batched_uneven"] = batched_leg(ctx, torch, args.batch_steps)
     if "blur" in leg_names:
         legs["gradients_blur"] = blur_leg(ctx, torch, max(2, args.steps // 4))
+        log("blur leg done")
     out["legs"] = legs
     if args.cpu_seconds > 0:
+        log("cpu baseline")
         cb = cpu_baseline(sc, args.cpu_seconds)
         cb["value"] = round(cb["value"], 2)
         out["cpu_baseline"] = cb

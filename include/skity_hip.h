@@ -196,6 +196,11 @@ typedef struct hipr_options {
                                      (257: the 8- and 16-slot walks), < 0 =
                                      every path, a huge value = none */
   int32_t flags;                  /* HIPR_FLAG_* */
+  int32_t chunk_points;           /* a draw of more source points than this
+                                     (or of more bounding-box tiles than twice
+                                     it) runs as consecutive sub-plans of
+                                     whole draws, in order: 0 = 8,388,608.
+                                     Draws with mask filters run whole. */
 } hipr_options;
 enum {
   HIPR_FLAG_NO_GRAPH = 1,   /* launch the walk passes directly, not as hipGraphs */
@@ -421,7 +426,6 @@ HIPR_API hipr_result hipr_stackblur_ex(hipr_context ctx, const uint8_t* src_host
 HIPR_API hipr_result hipr_stackblur(hipr_context ctx, const uint8_t* src_host,
                                     uint8_t* dst_host, int32_t width,
                                     int32_t height, int32_t radius);
-
 /* hipr_stackblur_ex with HIPR_BLUR_IMPL_MFMA, on device-resident images, that
  * allocates nothing.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving

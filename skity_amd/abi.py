@@ -118,6 +118,7 @@ class Options(ctypes.Structure):
         ("band_rows", ctypes.c_int32),
         ("band_min_edges", ctypes.c_int32),
         ("flags", ctypes.c_int32),
+        ("chunk_points", ctypes.c_int32),
     ]
 
 

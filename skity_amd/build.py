@@ -40,7 +40,7 @@ def build_hip():
 def build_scene():
     src = os.path.join(HERE, "csrc", "scene_gen.cpp")
     out = os.path.join(HERE, "libskity_scene.so")
-    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-pthread",
          "-o", out + ".tmp", src])
     os.replace(out + ".tmp", out)
 

@@ -32,11 +32,15 @@ struct RowSink {
   int32_t* d;
   int32_t left;   // pixel x of delta index 0 (the first visible column)
   int32_t width;  // visible columns; entry `width` is the end-of-row entry
+  // Direct spans of cover 0 blend nothing in src-over, and are dropped; a
+  // job whose blend mode does change pixels under a transparent source keeps
+  // them (they still count as a direct span of the pixel).
+  bool keep_zero;
   __device__ __forceinline__ int Clamp(int i) const {
     return i < 0 ? 0 : (i > width ? width : i);
   }
   __device__ __forceinline__ void Direct(int x, int len, int v) {
-    if (v == 0 || len <= 0) return;
+    if ((v == 0 && !keep_zero) || len <= 0) return;
     int i = Clamp(x - left), e = Clamp(x - left + len);
     if (i == e) return;
     int inc = (kDirectCount + v) << 16;

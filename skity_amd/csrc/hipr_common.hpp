@@ -54,8 +54,11 @@ struct Job {
   Affine m;
   int32_t image;        // layer canvas index of an image paint, else -1
   int32_t paint;        // index into the device paint table
-  int32_t pad;
+  int32_t flags;        // kJob* bits, set per draw
 };
+// The job keeps zero-cover direct spans and three planes per tile (its blend
+// mode changes pixels under a transparent source on this target).
+constexpr int32_t kJobKeepZero = 1;
 
 // Per-job outputs of the flatten stage.
 struct JobGeo {

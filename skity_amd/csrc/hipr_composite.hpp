@@ -376,7 +376,10 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
       const int mode = pd.blend;
       // covers of the four pixels: plane 0 and plane 1 of a walked job, or
       // 255 inside a rectangle job's rectangle
-      uint32_t cov0, cov1 = 0;
+      // three-plane tiles: the first cover is blended where a direct span
+      // lies (cover 0 included); otherwise where it is non-zero
+      const bool three = job.kind != kJobRect && (job.flags & kJobKeepZero) != 0;
+      uint32_t cov0, cov1 = 0, direct = 0;
       if (job.kind == kJobRect) {
         cov0 = 0;
 #pragma unroll
@@ -387,32 +390,42 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
         }
       } else {
         // the maps hold absolute plane numbers
-        cov0 = *((const uint32_t*)(P.planes + (int64_t)q0 * kTilePlaneBytes) + lane);
-        if (q1 >= 0) cov1 = *((const uint32_t*)(P.planes + (int64_t)q1 * kTilePlaneBytes) + lane);
+        const uint32_t* pl0 = (const uint32_t*)(P.planes + (int64_t)q0 * kTilePlaneBytes) + lane;
+        cov0 = pl0[0];
+        if (three) {
+          // direct covers (0 included), accumulated covers, and which
+          // pixels have a direct span
+          cov1 = pl0[kTilePlaneBytes / 4];
+          direct = pl0[2 * kTilePlaneBytes / 4];
+        } else if (q1 >= 0) {
+          cov1 = *((const uint32_t*)(P.planes + (int64_t)q1 * kTilePlaneBytes) + lane);
+        }
       }
       const uint32_t ga = (uint32_t)pd.ga;
       if (pd.kind == kPaintSolid && ga == 255) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
-          if (c0) pix[q] = ApplyCover(pix[q], pd.color, c0, mode, premul);
+          bool d0 = three ? ((direct >> (8 * q)) & 0xFF) != 0 : c0 != 0;
+          if (d0) pix[q] = ApplyCover(pix[q], pd.color, c0, mode, premul);
           if (c1) pix[q] = ApplyCover(pix[q], pd.color, c1, mode, premul);
-          blends += (c0 != 0) + (c1 != 0);
-          touched |= (uint32_t)((c0 | c1) != 0) << q;
+          blends += (int)d0 + (c1 != 0);
+          touched |= (uint32_t)(d0 || c1 != 0) << q;
         }
       } else {
         const GradDev* g = pd.kind == kPaintGradient ? &P.grads[pd.grad] : nullptr;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
-          if (c0 | c1) {
+          bool d0 = three ? ((direct >> (8 * q)) & 0xFF) != 0 : c0 != 0;
+          if (d0 || c1) {
             uint32_t color = pd.kind == kPaintSolid ? pd.color
                              : g ? GradientColor(*g, px0 + q, py)
                                  : ImageColor(pd, P.canvases[pd.layer], px0 + q, py);
             // Brush: cover & global alpha (sw_span_brush.cc:101)
-            if (c0) pix[q] = ApplyCover(pix[q], color, c0 & ga, mode, premul);
+            if (d0) pix[q] = ApplyCover(pix[q], color, c0 & ga, mode, premul);
             if (c1) pix[q] = ApplyCover(pix[q], color, c1 & ga, mode, premul);
-            blends += (c0 != 0) + (c1 != 0);
+            blends += (int)d0 + (c1 != 0);
             touched |= 1u << q;
           }
         }

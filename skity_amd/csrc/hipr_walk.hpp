@@ -293,7 +293,11 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   const int winding_mask = job.fill == HIPR_FILL_EVEN_ODD ? 1 : -1;
   const int cx0 = pb.l > 0 ? pb.l : 0, cx1 = pb.r < cv.width ? pb.r : cv.width;
   const int cy0 = pb.t > 0 ? pb.t : 0, cy1 = pb.b < cv.height ? pb.b : cv.height;
-  RowSink sink{L.d, ox, W};
+  // A job whose blend mode is not a no-op for a transparent source keeps the
+  // reference's zero-cover direct spans: its tiles take three planes (direct
+  // covers, accumulated covers, and a byte per pixel marking a direct span).
+  const bool triple = (job.flags & kJobKeepZero) != 0;
+  RowSink sink{L.d, ox, W, triple};
 
   // Active list registers.
   Fixed X[K], DX[K], DY[K], LY[K];
@@ -337,9 +341,11 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
         const uint32_t written = (uint32_t)L.rows[c];
         const int4 zero = {0, 0, 0, 0};
         if (s0 >= 0 && s0 != kNoPlane) {
-          int4* pl = (int4*)(planes + (int64_t)s0 * kTilePlaneBytes);
-          for (int k = 0; k < kTile; k++)
-            if (!((written >> k) & 1)) pl[k] = zero;
+          for (int t = 0; t < (triple ? 3 : 1); t++) {
+            int4* pl = (int4*)(planes + (int64_t)(s0 + t) * kTilePlaneBytes);
+            for (int k = 0; k < kTile; k++)
+              if (!((written >> k) & 1)) pl[k] = zero;
+          }
         }
         if (s1 >= 0 && s1 != kNoPlane) {
           int4* pl = (int4*)(planes + (int64_t)s1 * kTilePlaneBytes);
@@ -390,7 +396,7 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
       int iS = ScanSum(sS);
       int eS = iS - sS + carry;
       carry += __builtin_amdgcn_readlane(iS, kWave - 1);
-      uint32_t c0 = 0, c1 = 0;
+      uint32_t c0 = 0, c1 = 0, c2 = 0;
       bool stacked = false;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
@@ -400,19 +406,28 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
         int av = a > 255 ? 255 : a;
         int first = cnt ? dv : av;
         int second = (cnt && av) ? av : 0;
+        int third = 0;
+        if (triple) {
+          first = cnt ? dv : 0;
+          second = av;
+          third = cnt ? 1 : 0;
+        }
         if (cnt >= 2) stacked = true;
         int x = px + q;
         if (!visible || x < cx0 || x >= cx1) {
           first = 0;
           second = 0;
+          third = 0;
         }
         c0 |= (uint32_t)(first & 0xFF) << (8 * q);
         c1 |= (uint32_t)(second & 0xFF) << (8 * q);
+        c2 |= (uint32_t)third << (8 * q);
       }
       if (stacked) flags |= 4;
       if (!visible) continue;
       // chunk (4 lanes = 16 px) status
-      uint64_t nz0 = __ballot(c0 != 0), nz1 = __ballot(c1 != 0);
+      uint64_t nz0 = __ballot(triple ? (c0 | c1 | c2) != 0 : c0 != 0);
+      uint64_t nz1 = __ballot(!triple && c1 != 0);
       int g = lane >> 2;
       bool chunk0 = ((nz0 >> (4 * g)) & 0xF) != 0;
       bool chunk1 = ((nz1 >> (4 * g)) & 0xF) != 0;
@@ -426,7 +441,7 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
       bool need1 = in_cols && leader && chunk1 && cur1 < 0;
       // most chunks already hold their planes: allocate only when needed
       if (__ballot(need0 || need1) != 0) {
-        int req = (need0 ? 1 : 0) + (need1 ? 1 : 0);
+        int req = triple ? (need0 ? 3 : 0) : (need0 ? 1 : 0) + (need1 ? 1 : 0);
         int rincl = ScanSum(req);
         int rtot = __builtin_amdgcn_readlane(rincl, kWave - 1);
         int slot_base = next_slot;
@@ -455,7 +470,9 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
             int pi = pool0 + (sl - over_from);
             return pi < P.pool_cap ? P.pool_base + pi : kNoPlane;
           };
-          a0 = to_abs(a0);
+          // a triple must be contiguous: it never comes from the pool (a
+          // job that outgrows its range that way is walked again exactly)
+          a0 = triple ? (a0 + 2 < pcap ? (int)(pbase + a0) : kNoPlane) : to_abs(a0);
           a1 = to_abs(a1);
         }
         if (need0) {
@@ -481,6 +498,10 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
             if (cur0 != kNoPlane) {
               uint32_t* pl = (uint32_t*)(planes + (int64_t)cur0 * kTilePlaneBytes);
               pl[rowk * 4 + sub] = c0;
+              if (triple) {
+                pl[kTilePlaneBytes / 4 + rowk * 4 + sub] = c1;
+                pl[2 * kTilePlaneBytes / 4 + rowk * 4 + sub] = c2;
+              }
             } else {
               flags |= 1;
             }

@@ -2,8 +2,10 @@
 #include "scene_gen.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -271,12 +273,12 @@ void GenGradientBlur(sg_scene_s* s, Pcg32& rng, int n) {
   }
 }
 
-// One canvas of the batched uneven family: side 2^U[8,12], paths
-// round(10^U[lo_p, hi_p]), segments per path round(10^U[0.602, hi_s]) of
-// lines / quads / cubics at 40 / 30 / 30 %, control points uniform in the
-// canvas, 50% even-odd; paints 70% solid, 30% linear or radial gradient
-// (2-16 stops, clamp / repeat / mirror).  Full scale is hi_p = 4.699
-// (50,000 paths) and hi_s = 4 (10,000 segments); the scaled form caps both.
+// A canvas of uneven scribbles: side 2^U[8,12], paths round(10^U[lo_p,
+// hi_p]), segments per path round(10^U[0.602, hi_s]) of lines / quads /
+// cubics at 40 / 30 / 30 %, control points uniform in the canvas, 50%
+// even-odd; paints 70% solid, 30% linear or radial gradient (2-16 stops,
+// clamp / repeat / mirror).  Used scaled (10-1,000 paths of 4-316 segments):
+// every path spans the canvas, a stress case for wide active lists.
 void GenUneven(sg_scene_s* s, Pcg32& rng, int n, float lo_p, float hi_p, float hi_s) {
   int side = 1 << rng.Int(8, 12);
   s->width = side;
@@ -332,6 +334,139 @@ void GenUneven(sg_scene_s* s, Pcg32& rng, int n, float lo_p, float hi_p, float h
   }
 }
 
+// A gradient paint for a path in box (cx, cy, b): linear or radial, 2-16
+// stops, clamp / repeat / mirror.
+void AddGradient(sg_scene_s* s, Pcg32& rng, float cx, float cy, float b, hipr_paint* paint) {
+  hipr_gradient g{};
+  g.type = rng.Int(0, 1) ? HIPR_GRADIENT_RADIAL : HIPR_GRADIENT_LINEAR;
+  g.tile_mode = rng.Int(0, 2);
+  g.color_count = rng.Int(2, 16);
+  g.has_offsets = 1;
+  std::vector<float> offs(g.color_count);
+  for (auto& o : offs) o = rng.Unit();
+  std::sort(offs.begin(), offs.end());
+  for (int c = 0; c < g.color_count; c++) {
+    g.offsets[c] = offs[c];
+    for (int ch = 0; ch < 4; ch++) g.colors[c][ch] = rng.Unit();
+  }
+  g.points[0] = cx + rng.Uniform(-0.5f, 0.5f) * b;
+  g.points[1] = cy + rng.Uniform(-0.5f, 0.5f) * b;
+  if (g.type == HIPR_GRADIENT_LINEAR) {
+    g.points[2] = cx + rng.Uniform(-0.5f, 0.5f) * b;
+    g.points[3] = cy + rng.Uniform(-0.5f, 0.5f) * b;
+  } else {
+    g.radius[0] = rng.Uniform(0.1f, 0.7f) * b;
+  }
+  g.local_matrix[0] = 1.f;
+  g.local_matrix[4] = 1.f;
+  paint->gradient = (int32_t)s->gradients.size();
+  s->gradients.push_back(g);
+}
+
+// One canvas of the batched uneven family at its stated scale: side
+// 2^U{8..12}, round(10^U[2, 4.699]) paths (100-50,000), each of
+// round(10^U[0.602, 4]) segments (4-10,000) of lines / quads / cubics at
+// 40 / 30 / 30 %; paints 70% solid, 30% linear or radial gradient; half
+// even-odd.  Geometry (a choice the family leaves open): each path is a
+// closed outline around a centre uniform in the canvas, inside a box of
+// side (canvas side) * 2^U[-5, -1], through s anchors at equal angles whose
+// radius follows two random harmonics; quads and cubics bend their chord by
+// up to 30% of its length.  Such outlines keep a handful of edges per row
+// however many segments they have, as drawn shapes do.  Every path has its
+// own random stream (seed, path index), so paths are generated in parallel.
+struct OutlinePath {
+  Builder b;
+  int fill;
+  hipr_paint paint;
+  bool gradient;
+  hipr_gradient g;
+};
+
+void GenOutlinePath(uint64_t seed, int index, int side, float box_lo, float box_hi,
+                    OutlinePath* out) {
+  Pcg32 rng(seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(index + 1)));
+  const int segs = (int)std::lround(std::pow(10.f, rng.Uniform(0.602f, 4.f)));
+  const float b = (float)side * std::exp2(rng.Uniform(box_lo, box_hi));
+  const float cx = rng.Uniform(0.f, (float)side), cy = rng.Uniform(0.f, (float)side);
+  const int m1 = rng.Int(2, 7), m2 = rng.Int(5, 13);
+  const float ph1 = rng.Uniform(0.f, 6.2831853f), ph2 = rng.Uniform(0.f, 6.2831853f);
+  const float a1 = rng.Uniform(0.1f, 0.3f), a2 = rng.Uniform(0.02f, 0.12f);
+  const float R = 0.5f * b;
+  auto anchor = [&](int k, float* x, float* y) {
+    float th = 6.2831853f * (float)(k % segs) / (float)segs;
+    float r = R * (0.6f + a1 * std::sin(m1 * th + ph1) + a2 * std::sin(m2 * th + ph2));
+    *x = cx + r * std::cos(th);
+    *y = cy + r * std::sin(th);
+  };
+  Builder& bd = out->b;
+  bd.verbs.reserve(segs + 2);
+  bd.pts.reserve(2 * (2 * segs + 1));
+  float x0, y0;
+  anchor(0, &x0, &y0);
+  bd.Move(x0, y0);
+  for (int k = 0; k < segs; k++) {
+    float x1, y1;
+    anchor(k + 1, &x1, &y1);
+    float dx = x1 - x0, dy = y1 - y0;
+    float nx = -dy, ny = dx;  // chord normal, chord length
+    float u = rng.Unit();
+    if (u < 0.4f) {
+      bd.Line(x1, y1);
+    } else if (u < 0.7f) {
+      float o = rng.Uniform(-0.3f, 0.3f);
+      bd.Quad(x0 + 0.5f * dx + o * nx, y0 + 0.5f * dy + o * ny, x1, y1);
+    } else {
+      float o1 = rng.Uniform(-0.3f, 0.3f), o2 = rng.Uniform(-0.3f, 0.3f);
+      bd.Cubic(x0 + dx / 3.f + o1 * nx, y0 + dy / 3.f + o1 * ny, x0 + 2.f * dx / 3.f + o2 * nx,
+               y0 + 2.f * dy / 3.f + o2 * ny, x1, y1);
+    }
+    x0 = x1;
+    y0 = y1;
+  }
+  bd.Close();
+  out->fill = rng.Int(0, 1) ? HIPR_FILL_EVEN_ODD : HIPR_FILL_WINDING;
+  out->paint = SolidPaint(rng, 77);
+  out->gradient = rng.Unit() < 0.3f;
+  if (out->gradient) {
+    sg_scene_s tmp;
+    AddGradient(&tmp, rng, cx, cy, b, &out->paint);
+    out->g = tmp.gradients[0];
+  }
+}
+
+// side > 0 fixes the canvas side; first_path offsets the path indices (so
+// that a large scene can be generated as consecutive slices).
+void GenOutlines(sg_scene_s* s, Pcg32& rng, uint64_t seed, int n, int side, float box_lo,
+                 float box_hi, int first_path) {
+  if (side <= 0) side = 1 << rng.Int(8, 12);
+  s->width = side;
+  s->height = side;
+  const int paths = n > 0 ? n : (int)std::lround(std::pow(10.f, rng.Uniform(2.f, 4.699f)));
+  const int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int chunk = 1024;
+  std::vector<OutlinePath> buf;
+  for (int p0 = 0; p0 < paths; p0 += chunk) {
+    const int m = std::min(chunk, paths - p0);
+    buf.assign(m, OutlinePath{});
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++)
+      pool.emplace_back([&]() {
+        for (int i; (i = next.fetch_add(1)) < m;)
+          GenOutlinePath(seed, first_path + p0 + i, side, box_lo, box_hi, &buf[i]);
+      });
+    for (auto& th : pool) th.join();
+    for (int i = 0; i < m; i++) {
+      OutlinePath& op = buf[i];
+      if (op.gradient) {
+        op.paint.gradient = (int32_t)s->gradients.size();
+        s->gradients.push_back(op.g);
+      }
+      s->Add(op.b, op.fill, op.paint);
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -349,8 +484,8 @@ sg_scene sg_generate(int32_t workload, uint64_t seed, int32_t n_paths) {
     case 3:
       GenGradientBlur(s, rng, n_paths > 0 ? n_paths : 1024);
       break;
-    case 4:  // one canvas of the batched uneven family, full scale
-      GenUneven(s, rng, n_paths, 2.f, 4.699f, 4.f);
+    case 4:  // one canvas of the batched uneven family at its stated scale
+      GenOutlines(s, rng, seed, n_paths, 0, -5.f, -1.f, 0);
       break;
     case 5:  // the same family scaled: 10-1,000 paths of 4-316 segments
       GenUneven(s, rng, n_paths, 1.f, 3.f, 2.5f);
@@ -395,6 +530,16 @@ sg_scene sg_slice(sg_scene s, int32_t begin, int32_t end) {
   }
   r->Finish();
   return r;
+}
+
+sg_scene sg_generate_outlines(uint64_t seed, int32_t side, int32_t first_path, int32_t n_paths,
+                              float box_lo, float box_hi) {
+  if (side <= 0 || n_paths <= 0 || first_path < 0) return nullptr;
+  sg_scene_s* s = new sg_scene_s();
+  Pcg32 rng(seed);
+  GenOutlines(s, rng, seed, n_paths, side, box_lo, box_hi, first_path);
+  s->Finish();
+  return s;
 }
 
 }  // extern "C"

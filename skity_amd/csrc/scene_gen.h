@@ -25,7 +25,12 @@ typedef struct sg_scene_s* sg_scene;
  *      (width 1-16, miter/round/bevel joins, butt/round caps);
  *   3  gradients + blur: 4-cubic closed paths in boxes of side 32-1024 in
  *      4096^2 with linear/radial/sweep gradients (2-16 stops, clamp/repeat/
- *      mirror, random affine local matrix) and a normal blur, sigma 1-32.
+ *      mirror, random affine local matrix) and a normal blur, sigma 1-32;
+ *   4  one canvas of the batched uneven family at its stated scale: side
+ *      256-4096, 100-50,000 closed outlines of 4-10,000 line/quad/cubic
+ *      segments (log-uniform), 30% gradients;
+ *   5  uneven scribbles, scaled: side 256-4096, 10-1,000 paths of 4-316
+ *      segments with control points anywhere in the canvas.
  * n_paths <= 0 selects the workload's default count. */
 __attribute__((visibility("default"))) sg_scene sg_generate(int32_t workload,
                                                             uint64_t seed,
@@ -42,6 +47,17 @@ __attribute__((visibility("default"))) void sg_canvas_size(sg_scene s,
 __attribute__((visibility("default"))) void sg_struct_sizes(int32_t* paint,
                                                             int32_t* gradient,
                                                             int32_t* scene);
+/* Paths [first_path, first_path + n_paths) of the outline family of
+ * workload 4 on a side x side canvas, with path boxes of side
+ * side * 2^U[box_lo, box_hi]: a large scene (the learned pre-sizer's 1M-path
+ * evaluation on 8192^2) is generated as consecutive slices, each its own
+ * scene, which together are the same draws in the same order. */
+__attribute__((visibility("default"))) sg_scene sg_generate_outlines(uint64_t seed,
+                                                                     int32_t side,
+                                                                     int32_t first_path,
+                                                                     int32_t n_paths,
+                                                                     float box_lo,
+                                                                     float box_hi);
 /* Keep only paths [begin, end) (for bounded samples). */
 __attribute__((visibility("default"))) sg_scene sg_slice(sg_scene s,
                                                          int32_t begin,

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -328,10 +329,23 @@ struct hipr_prepared_s {
   int32_t n_targets = 1;             // canvases [0, n_targets) are draw targets
   int64_t n_verbs = 0, n_pts = 0, n_weights = 0;
   DevBuf verbs, pts, weights, scene_paints, d_jobs, d_paints, d_grads, d_walk, d_stroke;
+  // Per job, the cost that bounds a sub-plan: source points (edges grow
+  // with them) and the tiles of its bounding box (coverage planes).
+  std::vector<int64_t> job_points, job_tiles;
+  // A plan too large for one pipeline run is drawn as these sub-plans of
+  // consecutive jobs, which borrow its geometry and paints (`borrowed`).
+  std::vector<std::unique_ptr<hipr_prepared_s>> chunks;
+  bool chunks_built = false;
+  int64_t chunk_budget = 0;
+  bool borrowed = false;
+  int32_t job_offset = 0;  // a sub-plan: index of its first job in the plan
   ~hipr_prepared_s() {
-    DevBuf* b[] = {&verbs, &pts, &weights, &scene_paints, &d_jobs, &d_paints, &d_grads,
-                   &d_walk, &d_stroke};
-    for (DevBuf* x : b) x->Free();
+    DevBuf* own[] = {&d_jobs, &d_walk, &d_stroke};
+    for (DevBuf* x : own) x->Free();
+    if (!borrowed) {
+      DevBuf* b[] = {&verbs, &pts, &weights, &scene_paints, &d_paints, &d_grads};
+      for (DevBuf* x : b) x->Free();
+    }
   }
 };
 
@@ -423,10 +437,13 @@ hipr_result ValidateScene(const hipr_scene* s) {
   if (!s) return HIPR_ERROR_INVALID_ARGUMENT;
   if (s->n_paths < 0) return HIPR_ERROR_INVALID_ARGUMENT;
   if (s->n_paths == 0) return HIPR_SUCCESS;
-  if (!s->verbs || !s->verb_offsets || !s->points || !s->point_offsets ||
-      !s->weight_offsets || !s->paints)
+  if (!s->verb_offsets || !s->point_offsets || !s->weight_offsets || !s->paints)
     return HIPR_ERROR_INVALID_ARGUMENT;
   if (s->verb_offsets[0] != 0 || s->point_offsets[0] != 0 || s->weight_offsets[0] != 0)
+    return HIPR_ERROR_INVALID_ARGUMENT;
+  // the arrays may be NULL only when empty (a scene of DrawPaint calls)
+  if ((!s->verbs && s->verb_offsets[s->n_paths] != 0) ||
+      (!s->points && s->point_offsets[s->n_paths] != 0))
     return HIPR_ERROR_INVALID_ARGUMENT;
   for (int p = 0; p < s->n_paths; p++) {
     int v0 = s->verb_offsets[p], v1 = s->verb_offsets[p + 1];
@@ -538,6 +555,30 @@ PaintDev SolidOrGradient(const hipr_scene* s, const hipr_paint& pt, const Aff& c
 
 Affine ToAffine(const Aff& a) { return Affine{a.sx, a.kx, a.tx, a.ky, a.sy, a.ty}; }
 
+// Whether SWRenderTarget::BlendPixel changes a pixel for a transparent
+// source (sw_render_target.cc:12-141, blend_mode.cc:125-191): FastBlend
+// returns early for kSrcOver, kDst and kDstOut; kDstOver, kSrcATop, kXor,
+// kPlus, kScreen and the modes falling back to kSrcOver are identities on
+// premultiplied targets but round-trip through premultiplication on the
+// others; the rest clear or rescale the pixel.
+bool TransparentSourceMatters(int mode, bool premul_target) {
+  switch (mode) {
+    case HIPR_BLEND_SRC_OVER:
+    case HIPR_BLEND_DST:
+    case HIPR_BLEND_DST_OUT:
+      return false;
+    case HIPR_BLEND_DST_OVER:
+    case HIPR_BLEND_SRC_ATOP:
+    case HIPR_BLEND_XOR:
+    case HIPR_BLEND_PLUS:
+    case HIPR_BLEND_SCREEN:
+      return !premul_target;
+    default:
+      if (mode > HIPR_BLEND_SCREEN && mode != HIPR_BLEND_SOFT_LIGHT) return !premul_target;
+      return true;
+  }
+}
+
 // The canvas matrix of draw p (identity when the scene has none).
 Aff DrawMatrix(const hipr_scene* s, int p) {
   Aff m;
@@ -632,6 +673,46 @@ void LongestStrokesFirst(hipr_prepared_s* pl) {
   }
   pl->stroke_jobs.swap(jobs);
   pl->stroke_quads.swap(quads);
+}
+
+// Sub-plan budget: a run of at most this many source points (the edges of
+// a run grow with them) and twice as many bounding-box tiles (its coverage
+// planes): about 2-3 GB of edges and 4 GB of planes at most.
+constexpr int64_t kChunkPointsDefault = int64_t(8) << 20;
+
+// Per job: source points (an outline has several per source point) and the
+// tiles of its device-space bounding box (corners of the source bounds
+// through the job's matrix, widened by the stroke width).
+void JobCosts(const hipr_scene* s, hipr_prepared_s* pl) {
+  const size_t nj = pl->jobs.size();
+  pl->job_points.assign(nj, 0);
+  pl->job_tiles.assign(nj, 0);
+  for (size_t i = 0; i < nj; i++) {
+    const Job& jb = pl->jobs[i];
+    if (jb.kind == kJobRect) continue;
+    const int p = jb.path;
+    const bool src_path = jb.pt_begin == s->point_offsets[p];
+    int64_t pts = src_path ? s->point_offsets[p + 1] - s->point_offsets[p] : 4;
+    if (jb.kind == kJobStroke) pts *= 8;
+    pl->job_points[i] = pts;
+    float b[4] = {0, 0, 0, 0};
+    if (src_path) SourceBounds(s, p, b);
+    float w = jb.kind == kJobStroke ? s->paints[p].stroke_width : 0.f;
+    float xs[4] = {b[0] - w, b[2] + w, b[0] - w, b[2] + w};
+    float ys[4] = {b[1] - w, b[1] - w, b[3] + w, b[3] + w};
+    float l = INFINITY, t = INFINITY, r = -INFINITY, bt = -INFINITY;
+    for (int k = 0; k < 4; k++) {
+      P2 q = jb.m.Map(P2{xs[k], ys[k]});
+      l = std::min(l, q.x);
+      r = std::max(r, q.x);
+      t = std::min(t, q.y);
+      bt = std::max(bt, q.y);
+    }
+    const double side = kMaxCanvasSide / kTile;
+    double tw = std::min(std::max((r - l) / kTile + 2.0, 1.0), side);
+    double th = std::min(std::max((bt - t) / kTile + 2.0, 1.0), side);
+    pl->job_tiles[i] = (int64_t)(tw * th);
+  }
 }
 
 // A job of full coverage over an integer rectangle (see kJobRect).
@@ -777,6 +858,7 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
     if (pl->jobs[i].kind != kJobRect) pl->walk_jobs.push_back((int32_t)i);
     if (pl->jobs[i].kind == kJobStroke) pl->stroke_jobs.push_back((int32_t)i);
   }
+  JobCosts(s, pl);
 }
 
 void SetTiles(std::vector<Canvas>* cvs) {
@@ -1187,6 +1269,18 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     f.canvases[t].height = targets[t]->height;
     f.canvases[t].format = targets[t]->format;
   }
+  // Walked draws whose blend changes pixels under a transparent source keep
+  // the reference's zero-cover spans on this target.
+  bool any_keep_zero = false;
+  for (Job& jb : f.jobs) {
+    jb.flags = 0;
+    if (jb.kind == kJobRect) continue;
+    const bool premul = (f.canvases[jb.canvas].format & kFormatUnpremul) == 0;
+    if (TransparentSourceMatters(pl->paints[jb.paint].blend, premul)) {
+      jb.flags |= kJobKeepZero;
+      any_keep_zero = true;
+    }
+  }
   if (pl->layer_bytes > 0) {
     if (ctx->layers.Ensure(pl->layer_bytes) != hipSuccess) return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
     for (size_t c = n_targets; c < f.canvases.size(); c++)
@@ -1210,7 +1304,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   // the prepared job table keeps the source ranges; stroke jobs are redirected
   // to their outlines in a per-draw copy
   f.d_jobs = pl->d_jobs.as<Job>();
-  if (!pl->stroke_jobs.empty()) {
+  if (!pl->stroke_jobs.empty() || any_keep_zero) {
     if ((r = Upload(&ctx->run_jobs, f.jobs.data(), f.jobs.size(), st)) != HIPR_SUCCESS) return r;
     f.d_jobs = ctx->run_jobs.as<Job>();
   }
@@ -1882,6 +1976,149 @@ hipr_result LaunchBlurMfma(hipr_context ctx, const std::vector<BlurJob>& rows,
   return HIPR_SUCCESS;
 }
 
+// Splits a plan into sub-plans of consecutive jobs within the budgets.  Each
+// sub-plan borrows the plan's geometry and paints, keeps the global path
+// indices (per-path estimates and paints apply unchanged) and draws on the
+// same targets; drawn in order, the sub-plans blend every pixel in the
+// plan's order.  Plans with mask-filter layers are not split: a layer is
+// composited, blurred and drawn within one run.
+hipr_result BuildChunks(hipr_prepared_s* pl, int64_t budget_points) {
+  const int64_t kChunkPoints = budget_points;
+  const int64_t kChunkTiles = 2 * budget_points;
+  pl->chunks_built = true;
+  pl->chunk_budget = budget_points;
+  pl->chunks.clear();
+  const int nj = (int)pl->jobs.size();
+  if (pl->layer_bytes > 0 || nj == 0 || (int)pl->job_points.size() != nj) return HIPR_SUCCESS;
+  int64_t tp = 0, tt = 0;
+  for (int j = 0; j < nj; j++) {
+    tp += pl->job_points[j];
+    tt += pl->job_tiles[j];
+  }
+  if (tp <= kChunkPoints && tt <= kChunkTiles) return HIPR_SUCCESS;
+  std::vector<int> cut{0};
+  int64_t cp = 0, ct = 0;
+  for (int j = 0; j < nj; j++) {
+    if (j > cut.back() && (cp + pl->job_points[j] > kChunkPoints ||
+                           ct + pl->job_tiles[j] > kChunkTiles)) {
+      cut.push_back(j);
+      cp = ct = 0;
+    }
+    cp += pl->job_points[j];
+    ct += pl->job_tiles[j];
+  }
+  cut.push_back(nj);
+  hipStream_t st = pl->ctx->stream;
+  for (size_t c = 0; c + 1 < cut.size(); c++) {
+    const int j0 = cut[c], j1 = cut[c + 1];
+    auto sub = std::make_unique<hipr_prepared_s>();
+    sub->ctx = pl->ctx;
+    sub->n_paths = pl->n_paths;
+    sub->n_targets = pl->n_targets;
+    sub->borrowed = true;
+    sub->chunks_built = true;
+    sub->verbs = pl->verbs;
+    sub->pts = pl->pts;
+    sub->weights = pl->weights;
+    sub->scene_paints = pl->scene_paints;
+    sub->d_paints = pl->d_paints;
+    sub->d_grads = pl->d_grads;
+    sub->paints = pl->paints;
+    sub->n_verbs = pl->n_verbs;
+    sub->n_pts = pl->n_pts;
+    sub->n_weights = pl->n_weights;
+    sub->job_offset = j0;
+    sub->jobs.assign(pl->jobs.begin() + j0, pl->jobs.begin() + j1);
+    sub->job_points.assign(pl->job_points.begin() + j0, pl->job_points.begin() + j1);
+    sub->job_tiles.assign(pl->job_tiles.begin() + j0, pl->job_tiles.begin() + j1);
+    sub->canvases.assign(pl->canvases.begin(), pl->canvases.begin() + pl->n_targets);
+    for (Canvas& cv : sub->canvases) {
+      cv.job_begin = std::min(std::max(cv.job_begin, j0), j1) - j0;
+      cv.job_end = std::min(std::max(cv.job_end, j0), j1) - j0;
+    }
+    for (int32_t j : pl->walk_jobs)
+      if (j >= j0 && j < j1) sub->walk_jobs.push_back(j - j0);
+    for (size_t k = 0; k < pl->stroke_jobs.size(); k++) {
+      const int32_t j = pl->stroke_jobs[k];
+      if (j < j0 || j >= j1) continue;
+      sub->stroke_jobs.push_back(j - j0);
+      if (k < pl->stroke_quads.size()) sub->stroke_quads.push_back(pl->stroke_quads[k]);
+    }
+    hipr_result r;
+    if ((r = Upload(&sub->d_jobs, sub->jobs.data(), sub->jobs.size(), st)) != HIPR_SUCCESS ||
+        (r = Upload(&sub->d_walk, sub->walk_jobs.data(), sub->walk_jobs.size(), st)) !=
+            HIPR_SUCCESS ||
+        (r = Upload(&sub->d_stroke, sub->stroke_jobs.data(), sub->stroke_jobs.size(), st)) !=
+            HIPR_SUCCESS)
+      return r;
+    pl->chunks.push_back(std::move(sub));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return HIPR_SUCCESS;
+}
+
+// Draws a plan, in one pipeline run or, when it exceeds the sub-plan
+// budgets, one run per sub-plan; the statistics and stage times are summed
+// over the runs (`pixels` then counts each run's touched pixels).
+hipr_result RunPlan(const std::vector<hipr_surface>& targets, hipr_surface surf,
+                    hipr_prepared_s* pl, const hipr_options* opt) {
+  hipr_result r;
+  const int64_t budget = opt && opt->chunk_points > 0 ? opt->chunk_points : kChunkPointsDefault;
+  if ((!pl->chunks_built || pl->chunk_budget != budget) &&
+      (r = BuildChunks(pl, budget)) != HIPR_SUCCESS)
+    return r;
+  if (pl->chunks.empty()) return RunPipeline(targets, surf, pl, opt);
+  hipr_stats sum{};
+  double ms[HIPR_STAGE_COUNT] = {};
+  int32_t launches[HIPR_STAGE_COUNT] = {};
+  // per-job records of the whole plan, gathered from the sub-plan runs
+  const size_t nj = pl->jobs.size();
+  std::vector<int32_t> edges(nj, -1), planes(nj, -1), steps(nj, 0), tms(nj, 0), act(nj, 0);
+  std::vector<int32_t> phase(nj * 5, 0);
+  for (auto& sub : pl->chunks) {
+    if ((r = RunPipeline(targets, surf, sub.get(), opt)) != HIPR_SUCCESS) return r;
+    const size_t o = (size_t)sub->job_offset, n = sub->jobs.size();
+    if (surf->job_edges.size() == n) {
+      std::copy(surf->job_edges.begin(), surf->job_edges.end(), edges.begin() + o);
+      std::copy(surf->job_planes.begin(), surf->job_planes.end(), planes.begin() + o);
+      std::copy(surf->job_steps.begin(), surf->job_steps.end(), steps.begin() + o);
+      std::copy(surf->job_time.begin(), surf->job_time.end(), tms.begin() + o);
+      std::copy(surf->job_active.begin(), surf->job_active.end(), act.begin() + o);
+      std::copy(surf->job_phase.begin(), surf->job_phase.end(), phase.begin() + 5 * o);
+    }
+    const hipr_stats& st = surf->stats;
+    sum.raster_paths += st.raster_paths;
+    sum.edges += st.edges;
+    sum.tiles += st.tiles;
+    sum.tile_planes += st.tile_planes;
+    sum.fallbacks += st.fallbacks;
+    sum.max_active = std::max(sum.max_active, st.max_active);
+    sum.ms_total += st.ms_total;
+    sum.blends += st.blends;
+    sum.pixels += st.pixels;
+    sum.underestimates += st.underestimates;
+    for (int k = 0; k < HIPR_STAGE_COUNT; k++) {
+      ms[k] += surf->stage_ms[k];
+      launches[k] += surf->stage_launches[k];
+    }
+  }
+  sum.paths = pl->n_paths;
+  surf->stats = sum;
+  for (int k = 0; k < HIPR_STAGE_COUNT; k++) {
+    surf->stage_ms[k] = ms[k];
+    surf->stage_launches[k] = launches[k];
+  }
+  surf->job_edges.swap(edges);
+  surf->job_planes.swap(planes);
+  surf->job_steps.swap(steps);
+  surf->job_time.swap(tms);
+  surf->job_active.swap(act);
+  surf->job_phase.swap(phase);
+  // edge records and tile maps of earlier sub-plans are gone
+  surf->draw_serial = 0;
+  return HIPR_SUCCESS;
+}
+
 hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s** out) {
   hipr_result vr = ValidateScene(scene);
   if (vr != HIPR_SUCCESS) return vr;
@@ -1952,6 +2189,10 @@ hipr_result MergeBatch(hipr_batch b) {
   m.blurs.clear();
   m.layer_of_blur.clear();
   m.blur_style.clear();
+  m.job_points.clear();
+  m.job_tiles.clear();
+  m.chunks.clear();
+  m.chunks_built = false;
   m.walk_jobs.clear();
   m.stroke_jobs.clear();
   m.stroke_quads.clear();
@@ -2007,6 +2248,8 @@ hipr_result MergeBatch(hipr_batch b) {
     m.blurs.insert(m.blurs.end(), s->blurs.begin(), s->blurs.end());
     for (int32_t l : s->layer_of_blur) m.layer_of_blur.push_back(canv(l));
     m.blur_style.insert(m.blur_style.end(), s->blur_style.begin(), s->blur_style.end());
+    m.job_points.insert(m.job_points.end(), s->job_points.begin(), s->job_points.end());
+    m.job_tiles.insert(m.job_tiles.end(), s->job_tiles.begin(), s->job_tiles.end());
     for (int32_t j : s->walk_jobs) m.walk_jobs.push_back(j + job_off);
     for (int32_t j : s->stroke_jobs) m.stroke_jobs.push_back(j + job_off);
     m.stroke_quads.insert(m.stroke_quads.end(), s->stroke_quads.begin(), s->stroke_quads.end());
@@ -2560,7 +2803,7 @@ HIPR_API hipr_result hipr_surface_draw_prepared(hipr_surface surf, hipr_prepared
   if (!surf || !p) return HIPR_ERROR_INVALID_HANDLE;
   if (p->ctx != surf->ctx) return HIPR_ERROR_INVALID_ARGUMENT;
   HIP_TRY(hipSetDevice(surf->ctx->device));
-  return RunPipeline({surf}, surf, p, options);
+  return RunPlan({surf}, surf, p, options);
 }
 
 HIPR_API hipr_result hipr_batch_create(hipr_context ctx, hipr_batch* out) {
@@ -2617,7 +2860,7 @@ HIPR_API hipr_result hipr_batch_draw(hipr_batch b, const hipr_options* options) 
   if (opt.flags == 0) opt.flags = b->tuned_flags;
   std::vector<hipr_surface> targets;
   for (const auto& pr : b->pairs) targets.push_back(pr.first);
-  return RunPipeline(targets, &b->sink, &b->merged, &opt);
+  return RunPlan(targets, &b->sink, &b->merged, &opt);
 }
 
 HIPR_API hipr_result hipr_batch_set_split(hipr_batch b, int32_t band_rows, int32_t band_min_edges,
@@ -2701,7 +2944,7 @@ HIPR_API hipr_result hipr_surface_draw_scene(hipr_surface surf, const hipr_scene
   HIP_TRY(hipSetDevice(surf->ctx->device));
   hipr_result r = Prepare(surf->ctx, scene, &p);
   if (r != HIPR_SUCCESS) return r;
-  r = RunPipeline({surf}, surf, p, options);
+  r = RunPlan({surf}, surf, p, options);
   hipr_prepared_destroy(p);
   return r;
 }

@@ -154,9 +154,10 @@ class Context:
 
 
 def _options(mode, edge_estimates, tile_estimates, blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0,
-             band_min_edges=0, flags=0):
+             band_min_edges=0, flags=0, chunk_points=0):
     opt = abi.Options()
     opt.flags = flags
+    opt.chunk_points = chunk_points
     opt.mode = mode
     opt.blur_impl = blur_impl
     opt.band_rows = band_rows
@@ -212,13 +213,15 @@ class Surface:
         check(lib().hipr_surface_clear(self._h, rgba), "hipr_surface_clear")
 
     def draw(self, scene, mode=0, edge_estimates=None, tile_estimates=None,
-             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0, flags=0):
+             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0, flags=0,
+             chunk_points=0):
         """Draws a scene (a Scene pointer or a Prepared scene) onto the surface.
         band_rows: rows per walk band (0 = the library default, < 0 = none);
         band_min_edges: in the estimate modes, band only paths whose active
-        edge bound reaches this (0 = default, < 0 = all); flags: FLAG_*."""
+        edge bound reaches this (0 = default, < 0 = all); flags: FLAG_*;
+        chunk_points: source points per sub-plan run (0 = default)."""
         opt, keep = _options(mode, edge_estimates, tile_estimates, blur_impl, band_rows,
-                             band_min_edges, flags)
+                             band_min_edges, flags, chunk_points)
         if isinstance(scene, Prepared):
             check(lib().hipr_surface_draw_prepared(self._h, scene.handle, ctypes.byref(opt)),
                   "hipr_surface_draw_prepared")
@@ -336,9 +339,10 @@ class Batch:
         return r.value, m.value, f.value
 
     def draw(self, mode=0, edge_estimates=None, tile_estimates=None,
-             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0, flags=0):
+             blur_impl=abi.BLUR_IMPL_MFMA, band_rows=0, band_min_edges=0, flags=0,
+             chunk_points=0):
         opt, keep = _options(mode, edge_estimates, tile_estimates, blur_impl, band_rows,
-                             band_min_edges, flags)
+                             band_min_edges, flags, chunk_points)
         check(lib().hipr_batch_draw(self._h, ctypes.byref(opt)), "hipr_batch_draw")
         del keep
 

@@ -212,14 +212,84 @@ hipr_paint Paint::Pack(int32_t gradient_index) const {
   p.gradient = gradient_index;
   p.blur_radius = mask_ ? mask_->radius() : 0.f;
   p.blur_style = mask_ ? static_cast<int32_t>(mask_->style()) : 0;
-  p.blend = HIPR_BLEND_SRC_OVER;
+  p.blend = static_cast<int32_t>(blend_);
   return p;
 }
 
+// ---- Matrix -------------------------------------------------------------------
+Matrix Matrix::Translate(float dx, float dy) {
+  Matrix m;
+  m.tx = dx;
+  m.ty = dy;
+  return m;
+}
+Matrix Matrix::Scale(float x, float y) {
+  Matrix m;
+  m.sx = x;
+  m.sy = y;
+  return m;
+}
+Matrix Matrix::Skew(float kx, float ky) {
+  Matrix m;
+  m.kx = kx;
+  m.ky = ky;
+  return m;
+}
+Matrix Matrix::RotateDeg(float degrees, float px, float py) {
+  const float r = degrees * 0.01745329251994329577f;
+  const float c = std::cos(r), s = std::sin(r);
+  Matrix rot;
+  rot.sx = c;
+  rot.kx = -s;
+  rot.ky = s;
+  rot.sy = c;
+  if (px == 0 && py == 0) return rot;
+  return Translate(px, py) * rot * Translate(-px, -py);
+}
+Matrix Matrix::operator*(const Matrix& b) const {
+  const Matrix& a = *this;
+  if (a.IsIdentity()) return b;
+  if (b.IsIdentity()) return a;
+  Matrix r;
+  r.sx = a.sx * b.sx + a.kx * b.ky;
+  r.kx = a.sx * b.kx + a.kx * b.sy;
+  r.tx = (a.sx * b.tx + a.kx * b.ty) + a.tx;
+  r.ky = a.ky * b.sx + a.sy * b.ky;
+  r.sy = a.ky * b.kx + a.sy * b.sy;
+  r.ty = (a.ky * b.tx + a.sy * b.ty) + a.ty;
+  return r;
+}
+
 // ---- Canvas -----------------------------------------------------------------
+int HIPCanvas::Save() {
+  stack_.push_back(stack_.back());
+  return GetSaveCount() - 1;
+}
+
+void HIPCanvas::Restore() {
+  if (stack_.size() > 1) stack_.pop_back();
+}
+
+void HIPCanvas::RestoreToCount(int count) {
+  if (count < 1) count = 1;
+  while ((int)stack_.size() > count) stack_.pop_back();
+}
+
 void HIPCanvas::DrawPath(const Path& path, const Paint& paint) {
-  draws_.push_back(Draw{path, paint});
+  draws_.push_back(Draw{path, paint, stack_.back(), HIPR_DRAW_PATH});
   surface_->readback_valid_ = false;
+}
+
+void HIPCanvas::DrawPaint(const Paint& paint) {
+  draws_.push_back(Draw{Path(), paint, stack_.back(), HIPR_DRAW_PAINT});
+  surface_->readback_valid_ = false;
+}
+
+void HIPCanvas::DrawColor(Color color, BlendMode mode) {
+  Paint paint;
+  paint.SetFillColor(color);
+  paint.SetBlendMode(mode);
+  DrawPaint(paint);
 }
 
 void HIPCanvas::DrawRect(const Rect& rect, const Paint& paint) {
@@ -257,7 +327,16 @@ int HIPCanvas::Flush() {
   std::vector<uint8_t> fills(n);
   std::vector<hipr_paint> paints(n);
   std::vector<hipr_gradient> grads;
+  std::vector<float> matrices(6 * (size_t)n);
+  std::vector<uint8_t> kinds(n);
+  bool any_matrix = false, any_paint = false;
   for (int32_t i = 0; i < n; i++) {
+    const Matrix& m = draws_[i].matrix;
+    const float mf[6] = {m.sx, m.kx, m.tx, m.ky, m.sy, m.ty};
+    std::copy(mf, mf + 6, matrices.begin() + 6 * (size_t)i);
+    any_matrix |= !m.IsIdentity();
+    kinds[i] = draws_[i].kind;
+    any_paint |= kinds[i] != HIPR_DRAW_PATH;
     const Path& p = draws_[i].path;
     verbs.insert(verbs.end(), p.verbs().begin(), p.verbs().end());
     for (const Point& q : p.points()) {
@@ -289,6 +368,8 @@ int HIPCanvas::Flush() {
   scene.paints = paints.data();
   scene.gradients = grads.empty() ? nullptr : grads.data();
   scene.n_gradients = (int32_t)grads.size();
+  scene.matrices = any_matrix ? matrices.data() : nullptr;
+  scene.draw_kinds = any_paint ? kinds.data() : nullptr;
   hipr_options opt{};
   opt.mode = HIPR_MODE_EXACT;
   int r = hipr_surface_draw_scene(surface_->surface_, &scene, &opt);
@@ -298,9 +379,13 @@ int HIPCanvas::Flush() {
 }
 
 // ---- Surface ----------------------------------------------------------------
-std::unique_ptr<HIPSurface> HIPSurface::Make(hipr_context ctx, int width, int height) {
+std::unique_ptr<HIPSurface> HIPSurface::Make(hipr_context ctx, int width, int height,
+                                             int alpha_type, int color_type) {
   std::unique_ptr<HIPSurface> s(new HIPSurface());
-  if (hipr_surface_create(ctx, width, height, &s->surface_) != HIPR_SUCCESS) return nullptr;
+  if (hipr_surface_create_ex(ctx, width, height, alpha_type, color_type, &s->surface_) !=
+      HIPR_SUCCESS)
+    return nullptr;
+  s->color_type_ = color_type;
   s->width_ = width;
   s->height_ = height;
   s->canvas_.reset(new HIPCanvas(s.get()));
@@ -336,7 +421,9 @@ Color HIPSurface::GetPixel(int x, int y) {
     readback_valid_ = true;
   }
   const uint8_t* p = &readback_[((size_t)y * width_ + x) * 4];
-  return ColorSetARGB(p[3], p[0], p[1], p[2]);
+  // Bitmap::GetPixel (bitmap.cc:25-35)
+  return color_type_ == HIPR_COLOR_BGRA ? ColorSetARGB(p[3], p[2], p[1], p[0])
+                                        : ColorSetARGB(p[3], p[0], p[1], p[2]);
 }
 
 }  // namespace skity_hip

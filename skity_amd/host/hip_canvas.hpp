@@ -96,6 +96,31 @@ class Path {
   int64_t last_move_to_index_ = -1;  // ~index after Close
 };
 
+// An affine Matrix (include/skity/geometry/matrix.hpp) restricted to the
+// plane: x' = (scale_x x + skew_x y) + trans_x, y' = (skew_y x + scale_y y)
+// + trans_y.  Products follow src/geometry/matrix.cc:450-458 (identity
+// operands short-cut, otherwise glm's column sums, every operation rounded).
+struct Matrix {
+  float sx = 1, kx = 0, tx = 0, ky = 0, sy = 1, ty = 0;
+  static Matrix Translate(float dx, float dy);
+  static Matrix Scale(float x, float y);
+  static Matrix RotateDeg(float degrees, float px = 0, float py = 0);
+  static Matrix Skew(float kx, float ky);
+  bool IsIdentity() const {
+    return sx == 1 && kx == 0 && tx == 0 && ky == 0 && sy == 1 && ty == 0;
+  }
+  Matrix operator*(const Matrix& b) const;
+};
+
+// BlendMode (include/skity/graphic/blend_mode.hpp:45-80)
+enum class BlendMode {
+  kClear, kSrc, kDst, kSrcOver, kDstOver, kSrcIn, kDstIn, kSrcOut, kDstOut, kSrcATop,
+  kDstATop, kXor, kPlus, kModulate, kScreen, kOverlay, kDarken, kLighten, kColorDodge,
+  kColorBurn, kHardLight, kSoftLight, kDifference, kExclusion, kMultiply, kHue,
+  kSaturation, kColor, kLuminosity,
+  kDefault = kSrcOver
+};
+
 enum class TileMode { kClamp, kRepeat, kMirror, kDecal };
 
 // Gradient shaders (include/skity/effect/shader.hpp:88-146), premultiply
@@ -120,7 +145,12 @@ class Shader {
   hipr_gradient info_{};
 };
 
-enum class BlurStyle { kNormal = HIPR_BLUR_NORMAL };
+enum class BlurStyle {
+  kNormal = HIPR_BLUR_NORMAL,
+  kSolid = HIPR_BLUR_SOLID,
+  kOuter = HIPR_BLUR_OUTER,
+  kInner = HIPR_BLUR_INNER
+};
 
 class MaskFilter {
  public:
@@ -154,6 +184,8 @@ class Paint {
   void SetAntiAlias(bool) {}  // the software path always anti-aliases
   void SetShader(std::shared_ptr<Shader> s) { shader_ = std::move(s); }
   void SetMaskFilter(std::shared_ptr<MaskFilter> m) { mask_ = std::move(m); }
+  void SetBlendMode(BlendMode m) { blend_ = m; }
+  BlendMode GetBlendMode() const { return blend_; }
 
   hipr_paint Pack(int32_t gradient_index) const;
   const std::shared_ptr<Shader>& shader() const { return shader_; }
@@ -166,14 +198,36 @@ class Paint {
   Color4f fill_{0, 0, 0, 1}, stroke_{0, 0, 0, 1};
   std::shared_ptr<Shader> shader_;
   std::shared_ptr<MaskFilter> mask_;
+  BlendMode blend_ = BlendMode::kSrcOver;
 };
 
 class HIPSurface;
 
-// Records DrawPath calls (Canvas::DrawPath, include/skity/render/canvas.hpp)
-// and the shape helpers that the reference dispatches to OnDrawPath.
+// Records DrawPath / DrawPaint calls (include/skity/render/canvas.hpp) with
+// the canvas matrix of each, and the shape helpers that the reference
+// dispatches to OnDrawPath.  The matrix stack follows Canvas::Save/Restore
+// and the pre-concatenation of src/render/canvas_state.cc:18-45.
 class HIPCanvas {
  public:
+  int Save();
+  void Restore();
+  int GetSaveCount() const { return (int)stack_.size(); }
+  void RestoreToCount(int count);
+  void Translate(float dx, float dy) { Concat(Matrix::Translate(dx, dy)); }
+  void Scale(float sx, float sy) { Concat(Matrix::Scale(sx, sy)); }
+  void Rotate(float degrees) { Concat(Matrix::RotateDeg(degrees)); }
+  void Rotate(float degrees, float px, float py) { Concat(Matrix::RotateDeg(degrees, px, py)); }
+  void Skew(float sx, float sy) { Concat(Matrix::Skew(sx, sy)); }
+  void Concat(const Matrix& m) { stack_.back() = stack_.back() * m; }
+  void SetMatrix(const Matrix& m) { stack_.back() = m; }
+  void ResetMatrix() { stack_.back() = Matrix(); }
+  Matrix GetTotalMatrix() const { return stack_.back(); }
+
+  void DrawPaint(const Paint& paint);
+  // Canvas::DrawColor / Clear (canvas.cc:241-251): DrawPaint with a fill
+  // colour and a blend mode (kSrc for Clear).
+  void DrawColor(Color color, BlendMode mode = BlendMode::kSrcOver);
+  void Clear(Color color) { DrawColor(color, BlendMode::kSrc); }
   void DrawPath(const Path& path, const Paint& paint);
   void DrawRect(const Rect& rect, const Paint& paint);
   void DrawOval(const Rect& oval, const Paint& paint);
@@ -189,22 +243,29 @@ class HIPCanvas {
   struct Draw {
     Path path;
     Paint paint;
+    Matrix matrix;
+    uint8_t kind;  // HIPR_DRAW_*
   };
   HIPSurface* surface_;
   std::vector<Draw> draws_;
+  std::vector<Matrix> stack_{Matrix()};
 };
 
-// A device RGBA8 premultiplied target (the GPUSurface pattern of
-// include/skity/gpu/gpu_surface.hpp:95-113).
+// A device RGBA8 target (the GPUSurface pattern of
+// include/skity/gpu/gpu_surface.hpp:95-113), premultiplied RGBA by default;
+// the alpha and colour types of skity's Bitmap are HIPR_ALPHA_* / HIPR_COLOR_*.
 class HIPSurface {
  public:
-  static std::unique_ptr<HIPSurface> Make(hipr_context ctx, int width, int height);
+  static std::unique_ptr<HIPSurface> Make(hipr_context ctx, int width, int height,
+                                          int alpha_type = HIPR_ALPHA_PREMUL,
+                                          int color_type = HIPR_COLOR_RGBA);
   ~HIPSurface();
   HIPCanvas* LockCanvas(bool clear = true);
   int Flush();
   int Width() const { return width_; }
   int Height() const { return height_; }
-  // RGBA8 premultiplied, row-major, width*4 bytes per row.
+  // 4 bytes per pixel in the colour type's order, row-major, width*4 bytes
+  // per row.
   int ReadPixels(void* dst, size_t bytes);
   // skity Color (ARGB) of one pixel, read back from the device.
   Color GetPixel(int x, int y);
@@ -215,6 +276,7 @@ class HIPSurface {
   HIPSurface() = default;
   hipr_surface surface_ = nullptr;
   int width_ = 0, height_ = 0;
+  int color_type_ = HIPR_COLOR_RGBA;
   std::unique_ptr<HIPCanvas> canvas_;
   std::vector<uint8_t> readback_;
   bool readback_valid_ = false;

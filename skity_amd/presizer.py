@@ -175,17 +175,146 @@ def training_scenes(n_scenes, seed0=0x7A11):
     return out
 
 
+def uneven_training_canvases(min_paths, seed0=0xC4A10000):
+    """Canvases of the batched uneven family at its stated scale (workload 4:
+    100-50,000 outlines of 4-10,000 segments on 256^2-4096^2) at fresh
+    seeds, yielded one at a time until min_paths paths were produced."""
+    n, i = 0, 0
+    while n < min_paths:
+        sc = scenes.generate(4, seed=seed0 + i)
+        n += sc.n_paths
+        i += 1
+        yield sc
+
+
+def collect_stream(ctx, scene_iter, log_every=16):
+    """collect() over an iterator, releasing each scene after use."""
+    feats, targets = [], []
+    t0 = time.time()
+    n = 0
+    for k, sc in enumerate(scene_iter):
+        f, t = collect(ctx, [sc])
+        feats.append(f)
+        targets.append(t)
+        n += len(f)
+        if (k + 1) % log_every == 0:
+            print(f"  {k + 1} canvases, {n} labelled jobs, {time.time() - t0:.0f} s", flush=True)
+    return np.concatenate(feats), np.concatenate(targets)
+
+
+def linear_estimates(sc, edges_per_segment=4, edges_const=8, planes_per_tile=2):
+    """A pre-sizer without the network: edges bounded linearly in the segment
+    count (each segment rarely yields more than a few edges after the quad
+    conversion and the y-extremum chop) and planes in the bounding box's
+    tiles.  Per path, as HIPR_MODE_ESTIMATES takes them."""
+    s = sc.scene.contents
+    n = s.n_paths
+    vo = np.ctypeslib.as_array(s.verb_offsets, shape=(n + 1,))
+    po = np.ctypeslib.as_array(s.point_offsets, shape=(n + 1,))
+    pts = np.ctypeslib.as_array(s.points, shape=(2 * int(po[-1]),)).reshape(-1, 2)
+    segs = np.diff(vo).astype(np.int64)
+    # per-path bounding box of the points
+    idx = np.repeat(np.arange(n), np.diff(po))
+    lx = np.full(n, np.inf)
+    hx = np.full(n, -np.inf)
+    ly = np.full(n, np.inf)
+    hy = np.full(n, -np.inf)
+    np.minimum.at(lx, idx, pts[:, 0])
+    np.maximum.at(hx, idx, pts[:, 0])
+    np.minimum.at(ly, idx, pts[:, 1])
+    np.maximum.at(hy, idx, pts[:, 1])
+    tiles = (np.floor((hx - lx) / 16) + 2) * (np.floor((hy - ly) / 16) + 2)
+    e = np.minimum(edges_per_segment * segs + edges_const, 2 ** 30)
+    t = np.minimum(planes_per_tile * tiles, 2 ** 30)
+    return e.astype(np.int32), t.astype(np.int32)
+
+
+def evaluate_large(ctx, n_paths=1 << 20, side=8192, slice_paths=1 << 17, seed=0xC5E7A10,
+                   box=(-8.0, -3.0), log=print):
+    """The pre-sizer's evaluation at its stated scale: n_paths outlines of the
+    uneven family on a side x side canvas, drawn as consecutive scenes of
+    slice_paths paths, by the exact two-pass method, by the learned method
+    and by a linear estimate; the three frames must be identical.  Returns
+    per-form device time, fallbacks, under-estimates and capacity totals."""
+    import torch
+    forms = {"exact": abi.MODE_EXACT, "learned": abi.MODE_LEARNED,
+             "linear_estimate": abi.MODE_ESTIMATES}
+    surfs = {k: device.Surface(ctx, side, side) for k in forms}
+    rep = {k: {"ms": 0.0, "fallbacks": 0, "underestimates": 0, "jobs": 0, "edges": 0,
+               "tile_planes": 0, "blends": 0} for k in forms}
+    t_gen = 0.0
+    done = 0
+    while done < n_paths:
+        m = min(slice_paths, n_paths - done)
+        t0 = time.time()
+        sc = scenes.outlines(seed, side, done, m, box[0], box[1])
+        t_gen += time.time() - t0
+        prep = device.Prepared(ctx, sc.scene)
+        ee, te = linear_estimates(sc)
+        for k, mode in forms.items():
+            kw = {"edge_estimates": ee, "tile_estimates": te} if k == "linear_estimate" else {}
+            torch.cuda.synchronize()
+            surfs[k].draw(prep, mode=mode, **kw)
+            surfs[k].finish()
+            st = surfs[k].stats()
+            r = rep[k]
+            r["ms"] += st["ms_total"]
+            r["fallbacks"] += st["fallbacks"]
+            r["underestimates"] += st["underestimates"]
+            r["jobs"] += st["raster_paths"]
+            r["edges"] += st["edges"]
+            r["tile_planes"] += st["tile_planes"]
+            r["blends"] += st["blends"]
+        prep.close()
+        done += m
+        log(f"  {done} paths: " + ", ".join(f"{k} {rep[k]['ms']:.0f} ms" for k in forms))
+        del sc
+    frames = {k: surfs[k].read() for k in forms}
+    for k in forms:
+        surfs[k].close()
+        r = rep[k]
+        r["paths_per_s"] = n_paths / (r["ms"] * 1e-3)
+        r["fallback_rate"] = r["fallbacks"] / max(r["jobs"], 1)
+        r["underestimate_rate"] = r["underestimates"] / max(r["jobs"], 1)
+        r["identical_to_exact"] = bool(np.array_equal(frames[k], frames["exact"]))
+    rep["scene"] = {"paths": n_paths, "canvas": [side, side], "slices": -(-n_paths // slice_paths),
+                    "box_log2": list(box), "seed": seed, "generation_s": round(t_gen, 1)}
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--scenes", type=int, default=24)
+    ap.add_argument("--uneven-paths", type=int, default=1000000,
+                    help="labelled paths of the batched uneven family (workload 4)")
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--quantile", type=float, default=0.995)
+    ap.add_argument("--evaluate", metavar="JSON",
+                    help="instead of training: evaluate the installed weights on the 1M-path "
+                         "8192^2 scene and write the report here")
+    ap.add_argument("--eval-paths", type=int, default=1 << 20)
     args = ap.parse_args()
     ctx = device.Context(0)
+    if args.evaluate:
+        import json
+        install(ctx, *load())
+        rep = evaluate_large(ctx, n_paths=args.eval_paths,
+                             log=lambda m: print(m, flush=True))
+        with open(args.evaluate, "w") as fh:
+            json.dump(rep, fh, indent=1)
+        print(json.dumps(rep), flush=True)
+        ctx.close()
+        return 0
     t0 = time.time()
     sc = training_scenes(args.scenes)
     feats, targets = collect(ctx, sc)
+    del sc
+    if args.uneven_paths > 0:
+        f4, t4 = collect_stream(ctx, uneven_training_canvases(args.uneven_paths))
+        print(f"uneven family: {len(f4)} labelled jobs", flush=True)
+        feats = np.concatenate([feats, f4])
+        targets = np.concatenate([targets, t4])
     n = len(feats)
     rng = np.random.default_rng(5)
     perm = rng.permutation(n)

@@ -27,6 +27,9 @@ def _load():
         lib.sg_canvas_size.argtypes = [ctypes.c_void_p,
                                        ctypes.POINTER(ctypes.c_int32),
                                        ctypes.POINTER(ctypes.c_int32)]
+        lib.sg_generate_outlines.restype = ctypes.c_void_p
+        lib.sg_generate_outlines.argtypes = [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_float, ctypes.c_float]
         lib.sg_slice.restype = ctypes.c_void_p
         lib.sg_slice.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
         lib.sg_struct_sizes.argtypes = [ctypes.POINTER(ctypes.c_int32)] * 3
@@ -86,4 +89,14 @@ def generate(workload, n_paths=0, seed=None):
     h = _load().sg_generate(workload, seed, n_paths)
     if not h:
         raise ValueError(f"unknown workload {workload}")
+    return SceneHandle(h)
+
+
+def outlines(seed, side, first_path, n_paths, box_lo, box_hi):
+    """Paths [first_path, first_path + n_paths) of the outline family on a
+    side x side canvas, boxes of side side * 2^U[box_lo, box_hi] (see
+    sg_generate_outlines)."""
+    h = _load().sg_generate_outlines(seed, side, first_path, n_paths, box_lo, box_hi)
+    if not h:
+        raise ValueError("bad outline scene parameters")
     return SceneHandle(h)

@@ -107,6 +107,47 @@ def parallelogram_covers(a, f, y0, w, h, direction, rules=RULES):
     return {k: v for k, v in out.items() if v}
 
 
+def parallelogram_span_pixels(a, f, y0, w, h, direction):
+    """The pixels that receive a span (of any cover, 0 included) from the
+    parallelogram of parallelogram_covers: every pixel its edges cross and
+    every pixel between them; for f > 0 the two pixels each edge crosses
+    always receive a span, so a triangle of area below 1/256 px still emits
+    a span of cover 0."""
+    out = set()
+    for k in range(h):
+        A = a + k if direction > 0 else a - k - 1
+        last = A + w if f == 0 else A + w + 1
+        for x in range(A, last + 1):
+            out.add((x, y0 + k))
+    return out
+
+
+def clear_parallelogram_scene(seed=12):
+    """Parallelograms drawn with kClear over an opaque backdrop (DrawPaint):
+    returns (packed scene, set of pixels the reference clears)."""
+    rng = np.random.default_rng(seed)
+    cv = canvas.Canvas(320, 100)
+    cv.draw_paint(canvas.Paint((0.2, 0.5, 0.8, 1.0)))
+    cleared = set()
+    fracs = [0, 1, 5, 7, 200, 233, 240, 250, 255]
+    for i, f in enumerate(fracs):
+        direction = 1 if i % 2 == 0 else -1
+        h = int(rng.integers(3, 10))
+        w = int(rng.integers(2, 6))
+        a = 8 + 34 * i + (0 if direction > 0 else h)
+        y0 = 10 + int(rng.integers(0, 40))
+        x0 = a + f / 256.0
+        p = canvas.Path()
+        p.move_to(x0, y0)
+        p.line_to(x0 + w, y0)
+        p.line_to(x0 + w + direction * h, y0 + h)
+        p.line_to(x0 + direction * h, y0 + h)
+        p.close()
+        cv.draw_path(p, canvas.Paint(WHITE).set_blend_mode(abi.BLEND_CLEAR))
+        cleared |= parallelogram_span_pixels(a, f, y0, w, h, direction)
+    return cv.scene(), cleared
+
+
 def pixels_of(covers, width, height):
     """RGBA8 image of opaque white drawn with the given covers."""
     img = np.zeros((height, width, 4), np.uint8)

@@ -120,6 +120,48 @@ static void LinearGradientEnds(hipr_context ctx) {
   EXPECT_EQ(s->GetPixel(97, 4), Color_BLUE);
 }
 
+// Canvas matrix: the red rect of RedRect moved by Translate and doubled by
+// Scale (Canvas::Translate / Scale pre-concatenate, canvas_state.cc:18-24).
+static void MatrixMovesAndScalesDraws(hipr_context ctx) {
+  auto surface = HIPSurface::Make(ctx, 32, 32);
+  HIPCanvas* canvas = surface->LockCanvas();
+  Paint paint;
+  paint.SetColor(Color_RED);
+  canvas->Save();
+  canvas->Translate(10.f, 6.f);
+  canvas->Scale(2.f, 2.f);
+  canvas->DrawRect(Rect::MakeXYWH(1.f, 1.f, 4.f, 4.f), paint);  // -> (12, 8)-(20, 16)
+  canvas->Restore();
+  canvas->DrawRect(Rect::MakeXYWH(0.f, 24.f, 4.f, 4.f), paint);  // identity again
+  EXPECT_EQ(canvas->GetSaveCount(), 1);
+  EXPECT_EQ(surface->Flush(), HIPR_SUCCESS);
+  EXPECT_EQ(surface->GetPixel(12, 8), Color_RED);
+  EXPECT_EQ(surface->GetPixel(19, 15), Color_RED);
+  EXPECT_EQ(surface->GetPixel(20, 16), Color_TRANSPARENT);
+  EXPECT_EQ(surface->GetPixel(11, 8), Color_TRANSPARENT);
+  EXPECT_EQ(surface->GetPixel(1, 25), Color_RED);
+}
+
+// Canvas::Clear is DrawColor(c, kSrc) and replaces the pixels; DrawColor with
+// kSrcOver blends (canvas.cc:241-251, SWCanvas::OnDrawPaint).
+static void ClearAndDrawColor(hipr_context ctx) {
+  for (int ct : {HIPR_COLOR_RGBA, HIPR_COLOR_BGRA}) {
+    auto surface = HIPSurface::Make(ctx, 8, 8, HIPR_ALPHA_PREMUL, ct);
+    HIPCanvas* canvas = surface->LockCanvas();
+    canvas->Clear(Color_BLUE);
+    EXPECT_EQ(surface->Flush(), HIPR_SUCCESS);
+    EXPECT_EQ(surface->GetPixel(3, 3), Color_BLUE);
+    canvas->Clear(Color_TRANSPARENT);
+    canvas->DrawColor(Color_GREEN, BlendMode::kDstOver);
+    Paint red;
+    red.SetColor(Color_RED);
+    canvas->DrawRect(Rect::MakeXYWH(0.f, 0.f, 4.f, 8.f), red);
+    EXPECT_EQ(surface->Flush(), HIPR_SUCCESS);
+    EXPECT_EQ(surface->GetPixel(1, 1), Color_RED);
+    EXPECT_EQ(surface->GetPixel(6, 1), Color_GREEN);
+  }
+}
+
 // Unsupported input is refused, not approximated.
 static void RefusesHairline(hipr_context ctx) {
   auto s = HIPSurface::Make(ctx, 16, 16);
@@ -144,6 +186,8 @@ int main() {
                {"Path.StrokePathContainsEnd", StrokePathContainsEnd},
                {"Canvas.ShapesDispatchToDrawPath", ShapesDispatchToDrawPath},
                {"Shader.LinearGradientEnds", LinearGradientEnds},
+               {"Canvas.MatrixMovesAndScalesDraws", MatrixMovesAndScalesDraws},
+               {"Canvas.ClearAndDrawColor", ClearAndDrawColor},
                {"HIPCanvas.RefusesHairline", RefusesHairline}};
   for (auto& t : tests) {
     int before = g_failed;

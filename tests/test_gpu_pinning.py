@@ -112,3 +112,22 @@ def test_stroke_outline_edges_match_oracle(ctx, form):
           f"record equal", flush=True)
     assert checked == n_stroke
     surf.close()
+
+
+def test_clear_blend_keeps_zero_cover_spans(ctx):
+    """The device keeps the reference's zero-cover spans for a blend mode
+    that changes pixels under a transparent source: kClear parallelograms
+    clear exactly the pixels the closed form names."""
+    sc, cleared = F.clear_parallelogram_scene(12)
+    surf = device.Surface(ctx, sc.width, sc.height)
+    surf.draw(sc.scene)
+    got = surf.read()
+    surf.close()
+    backdrop = got[0, 0].copy()
+    want = np.empty_like(got)
+    want[...] = backdrop
+    for (x, y) in cleared:
+        want[y, x] = 0
+    bad = np.argwhere((got != want).any(-1))
+    print(f"kClear parallelograms: {len(cleared)} pixels cleared, {len(bad)} differ", flush=True)
+    assert bad.size == 0

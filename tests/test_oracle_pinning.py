@@ -109,3 +109,18 @@ def test_threaded_oracle_equals_serial(wl, n):
     a = O.render(sc.scene, sc.width, sc.height)
     b = O.render_mt(sc.scene, sc.width, sc.height, 4)
     assert np.array_equal(a, b)
+
+
+def test_clear_blend_clears_zero_cover_spans():
+    """kClear clears every pixel that receives a span, including the spans of
+    cover 0 the trapezoid blitters emit for triangles under 1/256 px
+    (blit_trapezoid_row sends both pixels an edge crosses to the span
+    builder, sw_raster.cc:485-497): the pixels a path barely misses are
+    cleared too."""
+    sc, cleared = F.clear_parallelogram_scene(12)
+    got = O.render(sc.scene, sc.width, sc.height)
+    want = np.zeros((sc.height, sc.width, 4), np.uint8)
+    want[...] = O.render(F.clear_parallelogram_scene(12)[0].scene, sc.width, sc.height)[0, 0]
+    for (x, y) in cleared:
+        want[y, x] = 0
+    assert np.array_equal(got, want)
