@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-fvisibility=hidden",
     # every float operation rounds separately, as in the reference build
     "-ffp-contract=off",
@@ -30,10 +30,26 @@ def run(cmd, cwd=None):
     subprocess.run(cmd, cwd=cwd, check=True)
 
 
+# translation units of libskity_hip.so; the wide walks compile beside the
+# main unit
+HIP_UNITS = ["skity_hip.hip", "hipr_walk_wide.hip"]
+
+
 def build_hip():
-    src = os.path.join(HERE, "csrc", "skity_hip.hip")
     out = os.path.join(HERE, "libskity_hip.so")
-    run([HIPCC] + HIP_FLAGS + ["-o", out + ".tmp", src])
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs, objs = [], []
+    for unit in HIP_UNITS:
+        obj = os.path.join(objdir, unit.replace(".hip", ".o"))
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", "-o", obj, os.path.join(HERE, "csrc", unit)]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+        objs.append(obj)
+    failed = [cmd for cmd, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
     os.replace(out + ".tmp", out)
 
 

@@ -228,6 +228,9 @@ __device__ __forceinline__ int ShiftDown(int v, int fill) {
 #endif
 }
 
+// Non-template kernels are defined once, by the library's main translation
+// unit; a second unit (hipr_walk_wide.hip) defines HIPR_TEMPLATES_ONLY.
+#ifndef HIPR_TEMPLATES_ONLY
 // Checks the DPP forms against the shuffle forms: per wave of 64 inputs,
 // out[0..5][64] = DppScanSum, WaveInclusiveSum, DppScanMax, WaveInclusiveMax,
 // DppShiftUp(fill -7), DppShiftDown(fill -9).
@@ -242,6 +245,7 @@ __global__ void __launch_bounds__(64) WaveOpsCheckKernel(const int* in, int* out
   o[4 * kWave + lane] = DppShiftUp(v, -7);
   o[5 * kWave + lane] = DppShiftDown(v, -9);
 }
+#endif  // HIPR_TEMPLATES_ONLY
 
 }  // namespace hipr
 

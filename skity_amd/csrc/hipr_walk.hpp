@@ -45,6 +45,8 @@ enum WalkMode : int { kWalkCount = 0, kWalkWrite = 1, kWalkSchedule = 2 };
 // row.  Restarted from that state, the walk of a band is the same computation
 // as the sequential walk over those rows, and every tile row belongs to one
 // band.
+// Widest walk: 64 slots x 64 lanes, an active list of 4096 edges.
+constexpr int kMaxSlots = 64;
 constexpr int kBandBits = 5;  // work items are job << kBandBits | band
 constexpr int kMaxBands = 1 << kBandBits;
 struct SnapHeader {
@@ -189,6 +191,8 @@ __device__ void BitonicSortSlots(int64_t (&key)[K], int N, int lane) {
         else if (ss == 2) SlotExchange<K, 2>(key, size, lane);
         else if (ss == 4) SlotExchange<K, 4>(key, size, lane);
         else if (ss == 8) SlotExchange<K, 8>(key, size, lane);
+        else if (ss == 16) SlotExchange<K, 16>(key, size, lane);
+        else if (ss == 32) SlotExchange<K, 32>(key, size, lane);
       } else {
         bool lower = (lane & stride) == 0;
 #pragma unroll
@@ -1348,7 +1352,7 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
 // The 4- and 8-slot walks are held to 256 registers (two waves per SIMD):
 // the 4-slot walk fits without spilling; the 8-slot walk spills, which its
 // row bands more than make up for, since 4-slot waves can then share its
-// SIMDs.  The 16-slot walk takes what it needs.
+// SIMDs.  The 16-, 32- and 64-slot walks take what they need.
 template <int K, int kMode>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 2 : 1)))
 WalkKernel(WalkParams P) {
@@ -1371,6 +1375,14 @@ WalkKernel(WalkParams P) {
   }
 }
 
+// The 32- and 64-slot walks (active lists of up to 2048 and 4096 edges) are
+// instantiated in their own translation unit, hipr_walk_wide.hip, which
+// compiles beside the main one; they walk unbanded jobs in the count and
+// write modes only.  Launches WalkKernel<K, mode> for K = 32 or 64.
+hipError_t LaunchWideWalk(int K, int mode, unsigned grid, size_t lds, hipStream_t st,
+                          const WalkParams& P);
+
+#ifndef HIPR_TEMPLATES_ONLY
 // Grid-stride fill of n int32 words.
 __global__ void FillKernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1384,6 +1396,7 @@ __global__ void ResetCovKernel(const int32_t* __restrict__ list, int n, JobCov* 
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) cov[list[i]] = JobCov{};
 }
+#endif  // HIPR_TEMPLATES_ONLY
 
 }  // namespace hipr
 

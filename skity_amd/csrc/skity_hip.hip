@@ -257,6 +257,7 @@ struct hipr_context_s {
   DevBuf stroke_redo, stroke_redo_out;    // stroke jobs written again (estimate forms)
   DevBuf pool_ctr;                        // overflow plane pool counter
   int n_cus = 0;                          // compute units of the device
+  int max_lds = 0;                        // LDS bytes one workgroup may take
   std::vector<hipGraph_t> graphs;         // graphs of the walk passes in flight
   std::vector<hipGraphExec_t> graph_execs;
   DevBuf stroke_out, o_verbs, o_pts, o_w, s_verbs, s_pts, s_w;
@@ -886,14 +887,28 @@ constexpr int kBandRowsEstimates = 64;
 // autotuner tries other thresholds.)
 constexpr int kBandMinEdges = 4 * kWave + 1;
 
+// LDS of a walk wave: row deltas, three per-tile-column rows (two slots,
+// written rows), five scratch arrays of K * 64 entries.
+size_t WalkLdsBytes(int K, int lds_width) {
+  return (size_t)((lds_width + 1) + 3 * (lds_width / kTile + 2) + 5 * K * kWave) *
+         sizeof(int32_t);
+}
+
+hipr_result WalkLdsCheck(hipr_context ctx, int K, size_t lds) {
+  if (ctx->max_lds > 0 && lds > (size_t)ctx->max_lds) {
+    std::fprintf(stderr, "skity_hip: a %d-slot walk needs %zu bytes of LDS (device limit %d)\n",
+                 K, lds, ctx->max_lds);
+    return HIPR_ERROR_CAPACITY;
+  }
+  return HIPR_SUCCESS;
+}
+
 template <int K, int kMode>
 hipr_result LaunchWalk(hipr_context ctx, WalkParams P, hipStream_t st, bool timed,
                        int32_t* queue, double share) {
   if (P.n_walk == 0) return HIPR_SUCCESS;
-  // row deltas, three per-tile-column rows (two slots, written rows), five
-  // scratch arrays of K * 64 entries
-  size_t lds = (size_t)((P.lds_width + 1) + 3 * (P.lds_width / kTile + 2) + 5 * K * kWave) *
-               sizeof(int32_t);
+  size_t lds = WalkLdsBytes(K, P.lds_width);
+  if (hipr_result r = WalkLdsCheck(ctx, K, lds)) return r;
   unsigned grid = (unsigned)P.n_walk;
   P.queue = queue;
   if (queue) {
@@ -911,6 +926,22 @@ hipr_result LaunchWalk(hipr_context ctx, WalkParams P, hipStream_t st, bool time
   hipEvent_t b = timed ? StageBegin(ctx, st) : nullptr;
   hipLaunchKernelGGL((WalkKernel<K, kMode>), dim3(grid), dim3(kWave), lds, st, P);
   HIP_TRY(hipGetLastError());
+  if (timed) StageEnd(ctx, stage, b, st);
+  return HIPR_SUCCESS;
+}
+
+// The 32- and 64-slot walks (hipr_walk_wide.hip): one wave per job, never
+// persistent, count and write modes only.
+hipr_result LaunchWalkWide(hipr_context ctx, int K, int kmode, WalkParams P, hipStream_t st,
+                           bool timed) {
+  if (P.n_walk == 0) return HIPR_SUCCESS;
+  if (kmode == kWalkSchedule) return HIPR_ERROR_INVALID_ARGUMENT;  // wide jobs are unbanded
+  size_t lds = WalkLdsBytes(K, P.lds_width);
+  if (hipr_result r = WalkLdsCheck(ctx, K, lds)) return r;
+  P.queue = nullptr;
+  const int stage = kmode == kWalkCount ? HIPR_STAGE_WALK_COUNT : HIPR_STAGE_WALK;
+  hipEvent_t b = timed ? StageBegin(ctx, st) : nullptr;
+  HIP_TRY(LaunchWideWalk(K, kmode, (unsigned)P.n_walk, lds, st, P));
   if (timed) StageEnd(ctx, stage, b, st);
   return HIPR_SUCCESS;
 }
@@ -1381,7 +1412,9 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   // The walk keeps the active edge list in registers.  The sort bounded each
   // job's list by its largest row overlap: jobs within 256 edges take the
   // 4-slot walk on the main stream, the others the 8-slot (512 edges) or
-  // 16-slot (1024 edges) walk on the side stream, at the same time.
+  // 16-slot (1024 edges) walk on a side stream, at the same time, and the
+  // rare longer lists the unbanded 32- or 64-slot walk (2048 and 4096
+  // edges).  A longer list is refused (HIPR_ERROR_CAPACITY).
   std::vector<int32_t> bound(nj, 0);
   HIP_TRY(hipMemcpyAsync(bound.data(), ctx->bound.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost,
                          st));
@@ -1390,12 +1423,14 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   std::vector<int32_t> small, wide;
   std::vector<int8_t> tier(nj, 4);  // slots of each job's walk
   for (int32_t j : all) {
-    if (bound[j] > 16 * kWave) {
+    if (bound[j] > kMaxSlots * kWave) {
       std::fprintf(stderr, "skity_hip: draw %d keeps up to %d edges active (limit %d)\n",
-                   f.jobs[j].path, bound[j], 16 * kWave);
+                   f.jobs[j].path, bound[j], kMaxSlots * kWave);
       return HIPR_ERROR_CAPACITY;
     }
-    tier[j] = bound[j] <= 4 * kWave ? 4 : bound[j] <= 8 * kWave ? 8 : 16;
+    int k = 4;
+    while (bound[j] > k * kWave) k *= 2;
+    tier[j] = (int8_t)k;
     (tier[j] == 4 ? small : wide).push_back(j);
   }
   // Row bands: each job's rows are split into bands walked concurrently by
@@ -1413,7 +1448,10 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   std::vector<int64_t> qb(nj, 0), sb(nj, 0);
   int64_t q_total = 0, snap_total = 0;
   for (int32_t j : all) {
-    nbands[j] = bound[j] >= band_min ? BandCount(JobPixBounds(ctx->h_geo[j]), band_rows) : 1;
+    // the 32- and 64-slot walks are unbanded
+    nbands[j] = bound[j] >= band_min && tier[j] <= 16
+                    ? BandCount(JobPixBounds(ctx->h_geo[j]), band_rows)
+                    : 1;
     qb[j] = q_total;
     q_total += (int64_t)nbands[j] * ctx->h_geo[j].edges;
   }
@@ -1571,7 +1609,10 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   (l.kmode == kWalkCount   ? LaunchWalk<KK, kWalkCount>(ctx, w, ls, timed, qi, share) \
    : l.kmode == kWalkWrite ? LaunchWalk<KK, kWalkWrite>(ctx, w, ls, timed, qi, share) \
                            : LaunchWalk<KK, kWalkSchedule>(ctx, w, ls, timed, qi, share))
-        lr = l.K == 4 ? HIPR_WALK_K(4) : l.K == 8 ? HIPR_WALK_K(8) : HIPR_WALK_K(16);
+        lr = l.K == 4    ? HIPR_WALK_K(4)
+             : l.K == 8  ? HIPR_WALK_K(8)
+             : l.K == 16 ? HIPR_WALK_K(16)
+                         : LaunchWalkWide(ctx, l.K, l.kmode, w, ls, timed);
 #undef HIPR_WALK_K
         if (lr != HIPR_SUCCESS) return lr;
       }
@@ -1609,9 +1650,14 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   // and 16-slot jobs on a side stream.
   auto walk_pass = [&](int kmode, const std::vector<int32_t>& sm,
                        const std::vector<int32_t>& wd) -> hipr_result {
-    std::vector<int32_t> j8, j16;
-    for (int32_t j : wd) (tier[j] <= 8 ? j8 : j16).push_back(j);
-    return run_plan({{kmode, 8, 1, j8}, {kmode, 16, 1, j16}, {kmode, 4, 0, sm}});
+    std::vector<int32_t> j8, j16, j32, j64;
+    for (int32_t j : wd)
+      (tier[j] <= 8 ? j8 : tier[j] == 16 ? j16 : tier[j] == 32 ? j32 : j64).push_back(j);
+    return run_plan({{kmode, 64, 2, j64},
+                     {kmode, 32, 2, j32},
+                     {kmode, 8, 1, j8},
+                     {kmode, 16, 1, j16},
+                     {kmode, 4, 0, sm}});
   };
   // A 4-slot job whose list still outgrew 256 edges (flag 2) is counted again
   // by the 16-slot walk (recording its snapshots there); the bound makes this
@@ -1698,16 +1744,20 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     // The banded (longest) walks need a schedule pass before their write
     // pass; both run on side streams while the main stream writes the other
     // jobs, so the schedule pass overlaps the bulk of the writing.
-    std::vector<int32_t> b4, u4, b8, u8, b16, u16;
+    std::vector<int32_t> b4, u4, b8, u8, b16, u16, u32, u64;
     for (int32_t j : small) (nbands[j] > 1 ? b4 : u4).push_back(j);
     for (int32_t j : wide) {
       if (tier[j] <= 8) {
         (nbands[j] > 1 ? b8 : u8).push_back(j);
-      } else {
+      } else if (tier[j] == 16) {
         (nbands[j] > 1 ? b16 : u16).push_back(j);
+      } else {
+        (tier[j] == 32 ? u32 : u64).push_back(j);
       }
     }
-    if ((r = run_plan({{kWalkSchedule, 8, 2, b8},
+    if ((r = run_plan({{kWalkWrite, 64, 1, u64},
+                       {kWalkWrite, 32, 1, u32},
+                       {kWalkSchedule, 8, 2, b8},
                        {kWalkWrite, 8, 2, b8},
                        {kWalkSchedule, 16, 2, b16},
                        {kWalkWrite, 16, 2, b16},
@@ -2343,6 +2393,8 @@ HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess ||
       hipDeviceGetAttribute(&ctx->n_cus, hipDeviceAttributeMultiprocessorCount, device) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&ctx->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
           hipSuccess ||
       hipStreamCreate(&ctx->stream) != hipSuccess ||
       hipStreamCreate(&ctx->side) != hipSuccess || hipStreamCreate(&ctx->side2) != hipSuccess ||
