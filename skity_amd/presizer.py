@@ -213,16 +213,19 @@ def linear_estimates(sc, edges_per_segment=4, edges_const=8, planes_per_tile=2):
     po = np.ctypeslib.as_array(s.point_offsets, shape=(n + 1,))
     pts = np.ctypeslib.as_array(s.points, shape=(2 * int(po[-1]),)).reshape(-1, 2)
     segs = np.diff(vo).astype(np.int64)
-    # per-path bounding box of the points
-    idx = np.repeat(np.arange(n), np.diff(po))
-    lx = np.full(n, np.inf)
-    hx = np.full(n, -np.inf)
-    ly = np.full(n, np.inf)
-    hy = np.full(n, -np.inf)
-    np.minimum.at(lx, idx, pts[:, 0])
-    np.maximum.at(hx, idx, pts[:, 0])
-    np.minimum.at(ly, idx, pts[:, 1])
-    np.maximum.at(hy, idx, pts[:, 1])
+    # per-path bounding box of the points (paths own contiguous point ranges;
+    # a path without points gets an empty box)
+    has = np.diff(po) > 0
+    starts = po[:-1][has].astype(np.int64)
+    lx = np.zeros(n)
+    hx = np.zeros(n)
+    ly = np.zeros(n)
+    hy = np.zeros(n)
+    if starts.size:
+        lx[has] = np.minimum.reduceat(pts[:, 0], starts)
+        hx[has] = np.maximum.reduceat(pts[:, 0], starts)
+        ly[has] = np.minimum.reduceat(pts[:, 1], starts)
+        hy[has] = np.maximum.reduceat(pts[:, 1], starts)
     tiles = (np.floor((hx - lx) / 16) + 2) * (np.floor((hy - ly) / 16) + 2)
     e = np.minimum(edges_per_segment * segs + edges_const, 2 ** 30)
     t = np.minimum(planes_per_tile * tiles, 2 ** 30)
@@ -236,7 +239,6 @@ def evaluate_large(ctx, n_paths=1 << 20, side=8192, slice_paths=1 << 17, seed=0x
     slice_paths paths, by the exact two-pass method, by the learned method
     and by a linear estimate; the three frames must be identical.  Returns
     per-form device time, fallbacks, under-estimates and capacity totals."""
-    import torch
     forms = {"exact": abi.MODE_EXACT, "learned": abi.MODE_LEARNED,
              "linear_estimate": abi.MODE_ESTIMATES}
     surfs = {k: device.Surface(ctx, side, side) for k in forms}
@@ -253,7 +255,6 @@ def evaluate_large(ctx, n_paths=1 << 20, side=8192, slice_paths=1 << 17, seed=0x
         ee, te = linear_estimates(sc)
         for k, mode in forms.items():
             kw = {"edge_estimates": ee, "tile_estimates": te} if k == "linear_estimate" else {}
-            torch.cuda.synchronize()
             surfs[k].draw(prep, mode=mode, **kw)
             surfs[k].finish()
             st = surfs[k].stats()
@@ -265,6 +266,10 @@ def evaluate_large(ctx, n_paths=1 << 20, side=8192, slice_paths=1 << 17, seed=0x
             r["edges"] += st["edges"]
             r["tile_planes"] += st["tile_planes"]
             r["blends"] += st["blends"]
+            stages = r.setdefault("stage_ms", {})
+            for name, (ms, _) in surfs[k].stage_times().items():
+                if ms > 0:
+                    stages[name] = stages.get(name, 0.0) + ms
         prep.close()
         done += m
         log(f"  {done} paths: " + ", ".join(f"{k} {rep[k]['ms']:.0f} ms" for k in forms))
