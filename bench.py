@@ -92,14 +92,15 @@ def measured_traffic(stage, form):
     return int(sum(d["fetch_mib"] + d["write_mib"] for d in hits) * 1024 * 1024)
 
 
-def pmc_probe():
+def pmc_probe(workload=WORKLOAD):
     """The program the committed counter summaries profile: one exact frame
-    and one learned frame of the workload (walk passes launched directly, so
-    every kernel is its own dispatch), then a 256 MiB device-to-device copy
-    whose counters calibrate FETCH_SIZE / WRITE_SIZE."""
+    and one learned frame of the workload (curves + strokes by default; 3 =
+    gradients + blur) with the walk passes launched directly, so every
+    kernel is its own dispatch, then a 256 MiB device-to-device copy whose
+    counters calibrate FETCH_SIZE / WRITE_SIZE."""
     import torch
     from skity_amd import abi, device, presizer, scenes
-    sc = scenes.generate(WORKLOAD)
+    sc = scenes.generate(workload)
     ctx = device.Context(0)
     prep = device.Prepared(ctx, sc.scene)
     surf = device.Surface(ctx, sc.width, sc.height)
@@ -326,9 +327,11 @@ def main():
                     help="extra workloads reported beside the headline: batched, blur")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short program the rocprofv3 counter summaries profile")
+    ap.add_argument("--probe-workload", type=int, default=WORKLOAD,
+                    help="workload of --pmc-probe: 2 curves + strokes, 3 gradients + blur")
     args = ap.parse_args()
     if args.pmc_probe:
-        pmc_probe()
+        pmc_probe(args.probe_workload)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -197,9 +197,9 @@ typedef struct hipr_options {
                                      every path, a huge value = none */
   int32_t flags;                  /* HIPR_FLAG_* */
   int32_t chunk_points;           /* a draw of more source points than this
-                                     (or of more bounding-box tiles than twice
-                                     it) runs as consecutive sub-plans of
-                                     whole draws, in order: 0 = 8,388,608.
+                                     (or of more bounding-box tiles than four
+                                     times it) runs as consecutive sub-plans
+                                     of whole draws, in order: 0 = 33,554,432.
                                      Draws with mask filters run whole. */
 } hipr_options;
 enum {

@@ -201,9 +201,17 @@ __device__ __forceinline__ float RemapTile(float t, int tile) {
   return t;
 }
 
+// The stop table of the gradient being drawn, staged in LDS by the wave
+// (16 offsets, 16 unpremultiplied colours).
+struct GradStops {
+  float offsets[16];
+  float4 colors[16];
+};
+
 // GradientColorBrush::LerpColor + CalculateColor for linear / radial / sweep
-// (sw_span_brush.cc:239-383).
-__device__ uint32_t GradientColor(const GradDev& g, int x, int y) {
+// (sw_span_brush.cc:239-383); the stop scan reads the LDS copy `sl` of the
+// gradient's table.
+__device__ uint32_t GradientColor(const GradDev& g, const GradStops& sl, int x, int y) {
   float sx = x + 0.5f, sy = y + 0.5f;
   float mx = sx * g.sx + sy * g.kx + g.tx;
   float my = sx * g.ky + sy * g.sy + g.ty;
@@ -228,8 +236,9 @@ __device__ uint32_t GradientColor(const GradDev& g, int x, int y) {
   int n = g.count;
   bool stops = g.has_offsets != 0;
   float step = 1.f / (n - 1);
-  if (stops && t <= g.offsets[0]) {
-    for (int q = 0; q < 4; q++) c[q] = g.colors[0][q];
+  if (stops && t <= sl.offsets[0]) {
+    const float4 c0 = sl.colors[0];
+    c[0] = c0.x, c[1] = c0.y, c[2] = c0.z, c[3] = c0.w;
     return PremulRGBA(c);
   }
   int i = 0;
@@ -237,8 +246,8 @@ __device__ uint32_t GradientColor(const GradDev& g, int x, int y) {
   int si = 0, ei = 1;
   for (i = 0; i < n - 1; i++) {
     if (stops) {
-      start = g.offsets[i];
-      end = g.offsets[i + 1];
+      start = sl.offsets[i];
+      end = sl.offsets[i + 1];
     } else {
       start = step * i;
       end = step * (i + 1);
@@ -250,13 +259,18 @@ __device__ uint32_t GradientColor(const GradDev& g, int x, int y) {
     }
   }
   if (i == n - 1) {
-    for (int q = 0; q < 4; q++) c[q] = g.colors[n - 1][q];
+    const float4 cl = sl.colors[n - 1];
+    c[0] = cl.x, c[1] = cl.y, c[2] = cl.z, c[3] = cl.w;
     return PremulRGBA(c);
   }
   float total = end - start, value = t - start;
   float mix = 0.5f;
   if (total > 0) mix = value / total;
-  for (int q = 0; q < 4; q++) c[q] = g.colors[si][q] * (1 - mix) + g.colors[ei][q] * mix;
+  const float4 cs = sl.colors[si], ce = sl.colors[ei];
+  c[0] = cs.x * (1 - mix) + ce.x * mix;
+  c[1] = cs.y * (1 - mix) + ce.y * mix;
+  c[2] = cs.z * (1 - mix) + ce.z * mix;
+  c[3] = cs.w * (1 - mix) + ce.w * mix;
   return PremulRGBA(c);
 }
 
@@ -309,6 +323,7 @@ __device__ uint32_t ImageColor(const PaintDev& pd, const Canvas& layer, int x, i
 }
 
 __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
+  __shared__ GradStops stops;
   const int lane = Lane();
   int t = blockIdx.x;
   if (t >= P.total_tiles) return;
@@ -414,13 +429,24 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
         }
       } else {
         const GradDev* g = pd.kind == kPaintGradient ? &P.grads[pd.grad] : nullptr;
+        if (g) {
+          // stage the stop table in LDS: one load per stop instead of a
+          // global load per scan step of every pixel
+          WaveSync();  // the previous gradient's reads are done
+          if (lane < 16) {
+            stops.offsets[lane] = g->offsets[lane];
+            stops.colors[lane] = make_float4(g->colors[lane][0], g->colors[lane][1],
+                                             g->colors[lane][2], g->colors[lane][3]);
+          }
+          WaveSync();
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
           bool d0 = three ? ((direct >> (8 * q)) & 0xFF) != 0 : c0 != 0;
           if (d0 || c1) {
             uint32_t color = pd.kind == kPaintSolid ? pd.color
-                             : g ? GradientColor(*g, px0 + q, py)
+                             : g ? GradientColor(*g, stops, px0 + q, py)
                                  : ImageColor(pd, P.canvases[pd.layer], px0 + q, py);
             // Brush: cover & global alpha (sw_span_brush.cc:101)
             if (d0) pix[q] = ApplyCover(pix[q], color, c0 & ga, mode, premul);

@@ -497,9 +497,12 @@ hipr_result ValidateScene(const hipr_scene* s) {
     if (pt.blend < HIPR_BLEND_CLEAR || pt.blend > HIPR_BLEND_LAST)
       return HIPR_ERROR_INVALID_ARGUMENT;
     if (pt.style < 0 || pt.style > 3) return HIPR_ERROR_INVALID_ARGUMENT;
+    // Width 0 is stroked like any width: SWCanvas::OnDrawPath runs the
+    // PathStroker with radius 0 (sw_canvas.cc:388-405, stroke.cc:865-866),
+    // which leaves near-empty outlines.  A negative width is refused.
     if (kind == HIPR_DRAW_PATH && pt.style != HIPR_STYLE_FILL &&
-        (!std::isfinite(pt.stroke_width) || pt.stroke_width <= 0.f))
-      return HIPR_ERROR_NOT_SUPPORTED;  // hairlines take a different path
+        (!std::isfinite(pt.stroke_width) || pt.stroke_width < 0.f))
+      return HIPR_ERROR_NOT_SUPPORTED;
     if (pt.cap < 0 || pt.cap > 2 || pt.join < 0 || pt.join > 2)
       return HIPR_ERROR_INVALID_ARGUMENT;
     if (pt.gradient >= 0) {
@@ -677,9 +680,14 @@ void LongestStrokesFirst(hipr_prepared_s* pl) {
 }
 
 // Sub-plan budget: a run of at most this many source points (the edges of
-// a run grow with them) and twice as many bounding-box tiles (its coverage
-// planes): about 2-3 GB of edges and 4 GB of planes at most.
-constexpr int64_t kChunkPointsDefault = int64_t(8) << 20;
+// a run grow with them: about 0.5 KB of edge records, sorted copies and quad
+// state per point) and four times as many bounding-box tiles (8 B of tile
+// map each, and at most three 256-byte planes per touched tile): about 16 GB
+// of edges and, for bounding boxes that are mostly covered, tens of GB of
+// planes, a fraction of the 288 GB of HBM.  A 2048^2 frame of 4,096 large
+// paths (about 70M bounding-box tiles) is one run.
+constexpr int64_t kChunkPointsDefault = int64_t(32) << 20;
+constexpr int64_t kChunkTilesPerPoint = 4;
 
 // Per job: source points (an outline has several per source point) and the
 // tiles of its device-space bounding box (corners of the source bounds
@@ -2034,7 +2042,7 @@ hipr_result LaunchBlurMfma(hipr_context ctx, const std::vector<BlurJob>& rows,
 // composited, blurred and drawn within one run.
 hipr_result BuildChunks(hipr_prepared_s* pl, int64_t budget_points) {
   const int64_t kChunkPoints = budget_points;
-  const int64_t kChunkTiles = 2 * budget_points;
+  const int64_t kChunkTiles = kChunkTilesPerPoint * budget_points;
   pl->chunks_built = true;
   pl->chunk_budget = budget_points;
   pl->chunks.clear();

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_py as O
-from skity_amd import abi, device
+from skity_amd import abi, device, presizer
 from skity_amd.canvas import Canvas, Paint, Path
 
 from helpers import diff_report, workload
@@ -273,3 +273,29 @@ def test_stroke_degenerate_segments(ctx, cap, mode):
     want = O.render(sc.scene, 200, 160)
     mx, cnt = diff_report(got, want)
     assert mx <= 1 and cnt == 0, f"max diff {mx}, {cnt} pixels"
+
+
+def test_width_zero_strokes_match_oracle(ctx):
+    """Width 0 is stroked as any width (PathStroker at radius 0,
+    sw_canvas.cc:388-405, stroke.cc:865-866): open paths draw nothing, closed
+    ones leave the slivers of their near-coincident outer and inner
+    contours; the device draws what the oracle draws."""
+    c = Canvas(200, 160)
+    for join in (abi.JOIN_MITER, abi.JOIN_ROUND, abi.JOIN_BEVEL):
+        for cap in (abi.CAP_BUTT, abi.CAP_ROUND, abi.CAP_SQUARE):
+            p = Paint((0.9, 0.2, 0.1, 1.0)).set_style(abi.STYLE_STROKE).set_stroke_width(0.0)
+            p.set_stroke_join(join).set_stroke_cap(cap)
+            c.draw_path(Path().move_to(12, 100).line_to(40, 20).line_to(70, 95)
+                        .quad_to(100, 10, 140, 60), p)
+            c.draw_path(Path().move_to(20, 20).cubic_to(60, -10, 120, 50, 60, 70)
+                        .line_to(25, 60).close(), p)
+            c.draw_path(Path().add_circle(150, 110, 30), p)
+    sc = c.scene()
+    presizer.install(ctx, *presizer.load())
+    for mode in (abi.MODE_EXACT, abi.MODE_LEARNED):
+        got, st = render_gpu(ctx, sc.scene, sc.width, sc.height, mode=mode)
+        want = O.render(sc.scene, sc.width, sc.height)
+        mx, cnt = diff_report(got, want)
+        print(f"width 0, mode {mode}: {int((want[..., 3] > 0).sum())} pixels drawn, max diff "
+              f"{mx}, {cnt} differ", flush=True)
+        assert mx == 0 and cnt == 0

@@ -147,3 +147,37 @@ def test_recursion_limit_carries_over(host):
     sc = c.scene()
     bad = [i for i in range(sc.n_paths) if not _same(host, sc.scene, i)]
     assert not bad, bad
+
+
+def _hairline_scene(closed):
+    c = Canvas(200, 160)
+    for join in (abi.JOIN_MITER, abi.JOIN_ROUND, abi.JOIN_BEVEL):
+        for cap in (abi.CAP_BUTT, abi.CAP_ROUND, abi.CAP_SQUARE):
+            p = Paint().set_style(abi.STYLE_STROKE).set_stroke_width(0.0)
+            p.set_stroke_join(join).set_stroke_cap(cap)
+            if closed:
+                c.draw_path(Path().move_to(20, 20).cubic_to(60, -10, 120, 50, 60, 70)
+                            .line_to(25, 60).close(), p)
+            else:
+                c.draw_path(Path().move_to(12, 100).line_to(40, 20).line_to(70, 95)
+                            .quad_to(100, 10, 140, 60), p)
+    return c.scene()
+
+
+@pytest.mark.parametrize("closed", [False, True])
+def test_width_zero_strokes(host, closed):
+    """Width 0: SWCanvas strokes it like any width, with PathStroker at radius
+    0 (sw_canvas.cc:388-405, stroke.cc:865-866).  The host build of the
+    device stroker gives the oracle's outlines bit for bit."""
+    sc = _hairline_scene(closed)
+    for i in range(sc.n_paths):
+        assert _same(host, sc.scene, i), i
+
+
+def test_width_zero_open_strokes_draw_nothing():
+    """Closed form: an open path stroked at radius 0 has an outline whose
+    forward and backward sides coincide, so every interval of the walk has
+    winding 0 and no pixel is covered."""
+    sc = _hairline_scene(False)
+    img = O.render(sc.scene, sc.width, sc.height)
+    assert not img.any()
