@@ -39,26 +39,40 @@ struct RowSink {
   __device__ __forceinline__ int Clamp(int i) const {
     return i < 0 ? 0 : (i > width ? width : i);
   }
+  // kClamp = false: the caller guarantees left <= x and x + len <= left +
+  // width (the per-pixel loop of BlitAAARow, which runs over visible
+  // columns only), so the two entries need no clamping.
+  template <bool kClamp = true>
   __device__ __forceinline__ void Direct(int x, int len, int v) {
     if ((v == 0 && !keep_zero) || len <= 0) return;
-    int i = Clamp(x - left), e = Clamp(x - left + len);
-    if (i == e) return;
+    int i = x - left, e = x - left + len;
+    if (kClamp) {
+      i = Clamp(i);
+      e = Clamp(e);
+      if (i == e) return;
+    }
     int inc = (kDirectCount + v) << 16;
     atomicAdd(&d[i], inc);
     atomicAdd(&d[e], -inc);
   }
+  template <bool kClamp = true>
   __device__ __forceinline__ void Acc(int x, int len, int v) {
     if (v == 0 || len <= 0) return;
-    int i = Clamp(x - left), e = Clamp(x - left + len);
-    if (i == e) return;
+    int i = x - left, e = x - left + len;
+    if (kClamp) {
+      i = Clamp(i);
+      e = Clamp(e);
+      if (i == e) return;
+    }
     atomicAdd(&d[i], v);
     atomicAdd(&d[e], -v);
   }
+  template <bool kClamp = true>
   __device__ __forceinline__ void Put(bool direct, int x, int len, int v) {
     if (direct) {
-      Direct(x, len, v);
+      Direct<kClamp>(x, len, v);
     } else {
-      Acc(x, len, v);
+      Acc<kClamp>(x, len, v);
     }
   }
 };
@@ -195,14 +209,14 @@ __device__ void BlitAAARow(RowSink* s, Fixed ul, Fixed ur, Fixed ll, Fixed lr, F
       run_len++;
     } else {
       if (run_len > 0) {
-        s->Put(direct, run_begin, run_len, full);
+        s->Put<false>(direct, run_begin, run_len, full);
         run_begin = -1;
         run_len = 0;
       }
-      s->Put(direct, x, 1, a);
+      s->Put<false>(direct, x, 1, a);
     }
   }
-  if (run_len > 0) s->Put(direct, run_begin, run_len, full);
+  if (run_len > 0) s->Put<false>(direct, run_begin, run_len, full);
 }
 
 // blit_trapezoid_row (sw_raster.cc:457-544).

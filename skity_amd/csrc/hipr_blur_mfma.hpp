@@ -55,8 +55,12 @@ __device__ __forceinline__ void StackBlurMfmaTiles(const BlurTileJob* __restrict
     const BlurTileJob tj = jobs[lo];
     const BlurJob b = tj.b;
     const int64_t lt = t - tj.tile_begin;
-    const int line0 = (int)(lt / tj.out_tiles) * kBlurLines;
-    const int a_tile = (int)(lt % tj.out_tiles) * kBlurOut;
+    // Row pass: consecutive tiles walk along a row band.  Column pass:
+    // consecutive tiles take neighbouring column groups at the same rows, so
+    // the workgroups running at one time read (and write) whole memory lines
+    // of each row between them and the L2 keeps them.
+    const int line0 = (int)(kPass == 0 ? lt / tj.out_tiles : lt % tj.line_tiles) * kBlurLines;
+    const int a_tile = (int)(kPass == 0 ? lt % tj.out_tiles : lt / tj.line_tiles) * kBlurOut;
     const int n_lines = kPass == 0 ? b.height : b.width;
     const int len = kPass == 0 ? b.width : b.height;
     const int r = b.radius;
@@ -66,7 +70,11 @@ __device__ __forceinline__ void StackBlurMfmaTiles(const BlurTileJob* __restrict
     StackBlurMulShr(r, &mul, &shr);
     // stage [line][channel][position], positions a_tile - 16 J ..
     for (int e = lane; e < kBlurLines * span; e += kWave) {
-      int ln = e / span, pos = e % span;
+      // row pass: lanes take consecutive positions of one row; column pass:
+      // four lanes take the tile's four columns of one row (16 contiguous
+      // bytes), the wave 16 rows
+      int ln = kPass == 0 ? e / span : e % kBlurLines;
+      int pos = kPass == 0 ? e % span : e / kBlurLines;
       int line = line0 + ln < n_lines ? line0 + ln : n_lines - 1;
       int a = a_tile - 16 * J + pos;
       a = a < 0 ? 0 : (a >= len ? len - 1 : a);
@@ -81,20 +89,27 @@ __device__ __forceinline__ void StackBlurMfmaTiles(const BlurTileJob* __restrict
     const int ak = 4 * (lane >> 4);    // A / B k base
     const int bn = lane & 15;          // B column n
     const _Float16* arow = stage + am * span;
+    // The band block B_j is the same for the four 16-output blocks of the
+    // tile: build it once per j and feed it to the four accumulators.
+    float4_t acc[kBlurOut / 16];
+#pragma unroll
+    for (int blk = 0; blk < kBlurOut / 16; blk++) acc[blk] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j <= 2 * J; j++) {
+      half4_t w;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int d = 16 * j - 16 * J + ak + i - bn;
+        d = d < 0 ? -d : d;
+        w[i] = (_Float16)(float)(d <= r ? r + 1 - d : 0);
+      }
+#pragma unroll
+      for (int blk = 0; blk < kBlurOut / 16; blk++) {
+        half4_t a = *(const half4_t*)(arow + 16 * blk + 16 * j + ak);
+        acc[blk] = Mfma16(a, w, acc[blk]);
+      }
+    }
 #pragma unroll
     for (int blk = 0; blk < kBlurOut / 16; blk++) {
-      float4_t acc = {0.f, 0.f, 0.f, 0.f};
-      for (int j = 0; j <= 2 * J; j++) {
-        half4_t a = *(const half4_t*)(arow + 16 * blk + 16 * j + ak);
-        half4_t w;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          int d = 16 * j - 16 * J + ak + i - bn;
-          d = d < 0 ? -d : d;
-          w[i] = (_Float16)(float)(d <= r ? r + 1 - d : 0);
-        }
-        acc = Mfma16(a, w, acc);
-      }
       // lane: line = lane / 16, output a_tile + 16 blk + lane % 16, channels i
       const int line = line0 + (lane >> 4);
       const int a = a_tile + 16 * blk + (lane & 15);
@@ -102,7 +117,7 @@ __device__ __forceinline__ void StackBlurMfmaTiles(const BlurTileJob* __restrict
         uint32_t o = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          uint64_t S = (uint64_t)acc[i];  // an exact integer below 2^24
+          uint64_t S = (uint64_t)acc[blk][i];  // an exact integer below 2^24
           o |= (uint32_t)(((S * mul) >> shr) & 0xFF) << (8 * i);
         }
         int64_t px = kPass == 0 ? (int64_t)line * b.width + a : (int64_t)a * b.width + line;

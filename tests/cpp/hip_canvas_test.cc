@@ -162,12 +162,19 @@ static void ClearAndDrawColor(hipr_context ctx) {
   }
 }
 
-// Unsupported input is refused, not approximated.
-static void RefusesHairline(hipr_context ctx) {
+// Width 0 is stroked as SWCanvas strokes it (PathStroker at radius 0): an
+// open line's outline has no area, so nothing is drawn.  A negative width is
+// refused, not approximated.
+static void WidthZeroAndNegativeStrokes(hipr_context ctx) {
   auto s = HIPSurface::Make(ctx, 16, 16);
   Paint paint;
   paint.SetStyle(Paint::kStroke_Style);
   paint.SetStrokeWidth(0);
+  s->LockCanvas()->DrawLine(1, 1, 10, 10, paint);
+  EXPECT_EQ(s->Flush(), HIPR_SUCCESS);
+  for (int y = 0; y < 16; y++)
+    for (int x = 0; x < 16; x++) EXPECT_EQ(s->GetPixel(x, y), Color_TRANSPARENT);
+  paint.SetStrokeWidth(-1);
   s->LockCanvas()->DrawLine(1, 1, 10, 10, paint);
   EXPECT_EQ(s->Flush(), HIPR_ERROR_NOT_SUPPORTED);
 }
@@ -188,7 +195,7 @@ int main() {
                {"Shader.LinearGradientEnds", LinearGradientEnds},
                {"Canvas.MatrixMovesAndScalesDraws", MatrixMovesAndScalesDraws},
                {"Canvas.ClearAndDrawColor", ClearAndDrawColor},
-               {"HIPCanvas.RefusesHairline", RefusesHairline}};
+               {"HIPCanvas.WidthZeroAndNegativeStrokes", WidthZeroAndNegativeStrokes}};
   for (auto& t : tests) {
     int before = g_failed;
     t.fn(ctx);
