@@ -59,7 +59,13 @@ enum { HIPR_STYLE_FILL = 0, HIPR_STYLE_STROKE = 1, HIPR_STYLE_STROKE_AND_FILL = 
 enum { HIPR_CAP_BUTT = 0, HIPR_CAP_ROUND = 1, HIPR_CAP_SQUARE = 2 };
 enum { HIPR_JOIN_MITER = 0, HIPR_JOIN_ROUND = 1, HIPR_JOIN_BEVEL = 2 };
 /* Shader::GradientType (include/skity/effect/shader.hpp:40-47). */
-enum { HIPR_GRADIENT_LINEAR = 2, HIPR_GRADIENT_RADIAL = 3, HIPR_GRADIENT_SWEEP = 4 };
+enum {
+  HIPR_GRADIENT_LINEAR = 2,
+  HIPR_GRADIENT_RADIAL = 3,
+  HIPR_GRADIENT_SWEEP = 4,
+  HIPR_GRADIENT_CONICAL = 5 /* two-point conical: points = start, end centres;
+                               radius = start, end radii */
+};
 /* TileMode (include/skity/graphic/tile_mode.hpp:13-30). */
 enum { HIPR_TILE_CLAMP = 0, HIPR_TILE_REPEAT = 1, HIPR_TILE_MIRROR = 2, HIPR_TILE_DECAL = 3 };
 /* BlurStyle (include/skity/effect/mask_filter.hpp:15-20). */

@@ -292,10 +292,122 @@ float RemapFloatTile(float t, int tile) {
   return t;
 }
 
+// PointsToUnit(p0, p1) of sw_span_brush.cc:200-219: rotate . scale .
+// translate taking p0 to the origin and p1 to (1, 0).
+Aff TwoPointsToUnit(float x0, float y0, float x1, float y1) {
+  float ssx = x1 - x0, ssy = y1 - y0;
+  float length = std::sqrt(ssx * ssx + ssy * ssy);
+  float scale = length > 0 ? 1.0f / length : 0;
+  float ux = ssx * scale, uy = ssy * scale;
+  float sine = -uy, cosine = ux;
+  Aff rot;
+  rot.sx = cosine;
+  rot.kx = -sine;
+  rot.ky = sine;
+  rot.sy = cosine;
+  return Mul(Mul(rot, Scale(scale, scale)), Translate(-x0, -y0));
+}
+
+// ConicalGradientColorBrush (sw_span_brush.cc:385-506): the per-gradient
+// set-up of OnPreBrush and the three cases of CalculateConical.
+struct Conical {
+  enum Kind { kTransparent, kRadial, kStrip, kFocal } kind = kTransparent;
+  float c0x = 0, c0y = 0, scale = 0, sign = 1, bias = 0;  // kRadial
+  float r2 = 0;                                          // kStrip
+  Aff to_unit;                                           // kStrip, kFocal
+  float f = 0, r1 = 0, r1sq = 0;                         // kFocal
+  bool swap = false;
+
+  void Setup(const hipr_gradient& g) {
+    const float kNear = 1.f / 4096;
+    float c0[2] = {g.points[0], g.points[1]}, c1[2] = {g.points[2], g.points[3]};
+    float r0 = g.radius[0], r_1 = g.radius[1];
+    float dcx = c1[0] - c0[0], dcy = c1[1] - c0[1];
+    float delta_center = std::sqrt(dcx * dcx + dcy * dcy);
+    float delta_radius = std::abs(r_1 - r0);
+    kind = kTransparent;
+    if (r0 < 0 || r_1 < 0) return;
+    bool radial = delta_center < kNear, strip = delta_radius < kNear;
+    if (radial) {
+      if (strip) return;
+      kind = kRadial;
+      scale = (float)(1.0 / delta_radius);
+      sign = delta_radius < 0 ? -1.f : 1.f;
+      bias = r0 / delta_radius;
+      c0x = c0[0];
+      c0y = c0[1];
+    } else if (strip) {
+      kind = kStrip;
+      float r = r0 / delta_center;
+      r2 = r * r;
+      to_unit = TwoPointsToUnit(c0[0], c0[1], c1[0], c1[1]);
+    } else {
+      kind = kFocal;
+      swap = r_1 < kNear;
+      if (swap) {
+        std::swap(c0[0], c1[0]);
+        std::swap(c0[1], c1[1]);
+        std::swap(r0, r_1);
+      }
+      f = r0 / (r0 - r_1);
+      // points are (x, y, 0, 1): the focal point's w is (1 - f) + f
+      float cfx = c0[0] * (1.f - f) + c1[0] * f;
+      float cfy = c0[1] * (1.f - f) + c1[1] * f;
+      float cfw = 1.f * (1.f - f) + 1.f * f;
+      float ex = c1[0] - cfx, ey = c1[1] - cfy, ew = 1.f - cfw;
+      float len = std::sqrt(ex * ex + ey * ey + 0.f * 0.f + ew * ew);
+      r1 = r_1 / len;
+      r1sq = r1 * r1;
+      to_unit = TwoPointsToUnit(cfx, cfy, c1[0], c1[1]);
+    }
+  }
+
+  // false: the pixel is transparent
+  bool T(float px, float py, float* t) const {
+    const float kNear = 1.f / 4096;
+    if (kind == kTransparent) return false;
+    if (kind == kRadial) {
+      float ax = (px - c0x) * scale, ay = (py - c0y) * scale;
+      *t = std::sqrt(ax * ax + ay * ay) * sign - bias;
+      return true;
+    }
+    float qx, qy;
+    MapPoint(to_unit, px, py, &qx, &qy);
+    if (kind == kStrip) {
+      float v = r2 - qy * qy;
+      if (v < 0.0) return false;
+      *t = qx + std::sqrt(v);
+      return true;
+    }
+    float xt = -1.f;
+    if (std::abs(r1 - 1.f) < kNear) {
+      xt = (qx * qx + qy * qy) / 2;
+    } else if (r1 > 1.f) {
+      float m = r1sq - 1.f;
+      float delta = m * qy * qy + r1sq * qx * qx;
+      xt = (std::sqrt(delta) - qx) / m;
+    } else {
+      float m = r1sq - 1.f;
+      float delta = m * qy * qy + r1sq * qx * qx;
+      if (delta > 0) {
+        float xt1 = (std::sqrt(delta) - qx) / m;
+        float xt2 = (-std::sqrt(delta) - qx) / m;
+        xt = 1.f - f < 0 ? std::min(xt1, xt2) : std::max(xt1, xt2);
+      }
+    }
+    if (xt < 0) return false;
+    float tt = f + (1.f - f) * xt;
+    if (swap) tt = 1.0 - tt;
+    *t = tt;
+    return true;
+  }
+};
+
 struct GradientBrush {
   hipr_gradient g;
   std::vector<float> offsets;  // after EnsureColorOffsetsNonDescending
   Aff points_to_unit;
+  Conical conical;
 
   void LerpColor(float current, float out[4]) const {
     if (std::abs(current) <= 1.f / 4096) {
@@ -352,6 +464,8 @@ struct GradientBrush {
       t = mx;
     } else if (g.type == HIPR_GRADIENT_RADIAL) {
       t = std::sqrt(mx * mx + my * my);
+    } else if (g.type == HIPR_GRADIENT_CONICAL) {
+      if (!conical.T(mx, my, &t)) return 0;
     } else {
       float angle = std::atan2(-my, -mx);
       float bias = g.radius[0], scale = g.radius[1];
@@ -369,21 +483,13 @@ struct GradientBrush {
 Aff GradientPointsToUnit(const hipr_gradient& g, const Aff& ctm) {
   Aff ptu;
   if (g.type == HIPR_GRADIENT_LINEAR) {
-    float ssx = g.points[2] - g.points[0], ssy = g.points[3] - g.points[1];
-    float length = std::sqrt(ssx * ssx + ssy * ssy);
-    float scale = length > 0 ? 1.0f / length : 0;
-    float ux = ssx * scale, uy = ssy * scale;
-    float sine = -uy, cosine = ux;
-    Aff rot;
-    rot.sx = cosine;
-    rot.kx = -sine;
-    rot.ky = sine;
-    rot.sy = cosine;
-    ptu = Mul(Mul(rot, Scale(scale, scale)), Translate(-g.points[0], -g.points[1]));
+    ptu = TwoPointsToUnit(g.points[0], g.points[1], g.points[2], g.points[3]);
   } else if (g.type == HIPR_GRADIENT_RADIAL) {
     float r = g.radius[0];
     float scale = r > 0 ? 1.0f / r : 0;
     ptu = Mul(Scale(scale, scale), Translate(-g.points[0], -g.points[1]));
+  } else if (g.type == HIPR_GRADIENT_CONICAL) {
+    ptu = Aff();  // the conical brush maps device -> local itself
   } else {
     ptu = Translate(-g.points[0], -g.points[1]);
   }
@@ -408,6 +514,7 @@ void PrepareGradient(const hipr_gradient& g, const Aff& ctm, GradientBrush* b) {
       if (b->offsets[i] > b->offsets[i + 1]) b->offsets[i + 1] = b->offsets[i];
   }
   b->points_to_unit = GradientPointsToUnit(g, ctm);
+  if (g.type == HIPR_GRADIENT_CONICAL) b->conical.Setup(g);
 }
 
 // A brush: solid colour, gradient, or the premultiplied image of a blurred

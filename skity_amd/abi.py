@@ -16,7 +16,7 @@ FILL_WINDING, FILL_EVEN_ODD = 0, 1
 STYLE_FILL, STYLE_STROKE, STYLE_STROKE_AND_FILL, STYLE_STROKE_THEN_FILL = range(4)
 CAP_BUTT, CAP_ROUND, CAP_SQUARE = range(3)
 JOIN_MITER, JOIN_ROUND, JOIN_BEVEL = range(3)
-GRADIENT_LINEAR, GRADIENT_RADIAL, GRADIENT_SWEEP = 2, 3, 4
+GRADIENT_LINEAR, GRADIENT_RADIAL, GRADIENT_SWEEP, GRADIENT_CONICAL = 2, 3, 4, 5
 TILE_CLAMP, TILE_REPEAT, TILE_MIRROR, TILE_DECAL = range(4)
 BLUR_NORMAL, BLUR_SOLID, BLUR_OUTER, BLUR_INNER = 1, 2, 3, 4
 (BLEND_CLEAR, BLEND_SRC, BLEND_DST, BLEND_SRC_OVER, BLEND_DST_OVER, BLEND_SRC_IN,

@@ -20,13 +20,67 @@ namespace hipr {
 
 enum PaintKind : int32_t { kPaintSolid = 0, kPaintGradient = 1, kPaintImage = 2 };
 
+// Two-point conical gradient set-up (ConicalGradientColorBrush::OnPreBrush,
+// sw_span_brush.cc:407-440), computed on the host.
+enum ConicalKind : int32_t { kConicalNone = 0, kConicalRadial, kConicalStrip, kConicalFocal };
+struct ConicalDev {
+  int32_t kind, swap;
+  float c0x, c0y, scale, sign, bias;  // concentric circles
+  float r2;                           // equal radii: (r0 / |c1 - c0|)^2
+  float ux, ukx, utx, uky, uy, uty;   // strip / focal: points -> unit
+  float f, r1, r1sq;                  // focal
+};
+
 struct GradDev {
   int32_t type, tile, count, has_offsets;
-  float sx, kx, tx, ky, sy, ty;  // device -> unit space
+  float sx, kx, tx, ky, sy, ty;  // device -> unit space (conical: device -> local)
   float bias, scale;             // sweep parameters
   float offsets[16];
   float colors[16][4];
+  ConicalDev con;
 };
+
+// CalculateConical (sw_span_brush.cc:443-506): t at the local point (px, py),
+// false where the pixel is transparent.
+__device__ __forceinline__ bool ConicalT(const ConicalDev& c, float px, float py, float* t) {
+  const float kNear = 1.f / 4096;
+  if (c.kind == kConicalNone) return false;
+  if (c.kind == kConicalRadial) {
+    float ax = (px - c.c0x) * c.scale, ay = (py - c.c0y) * c.scale;
+    *t = sqrtf(ax * ax + ay * ay) * c.sign - c.bias;
+    return true;
+  }
+  float qx = px * c.ux + py * c.ukx + c.utx;
+  float qy = px * c.uky + py * c.uy + c.uty;
+  if (c.kind == kConicalStrip) {
+    float v = c.r2 - qy * qy;
+    if (v < 0.0f) return false;
+    *t = qx + sqrtf(v);
+    return true;
+  }
+  float xt = -1.f;
+  if (fabsf(c.r1 - 1.f) < kNear) {
+    xt = (qx * qx + qy * qy) / 2;
+  } else if (c.r1 > 1.f) {
+    float m = c.r1sq - 1.f;
+    float delta = m * qy * qy + c.r1sq * qx * qx;
+    xt = (sqrtf(delta) - qx) / m;
+  } else {
+    float m = c.r1sq - 1.f;
+    float delta = m * qy * qy + c.r1sq * qx * qx;
+    if (delta > 0) {
+      float xt1 = (sqrtf(delta) - qx) / m;
+      float xt2 = (-sqrtf(delta) - qx) / m;
+      // std::min / std::max
+      xt = 1.f - c.f < 0 ? (xt2 < xt1 ? xt2 : xt1) : (xt1 < xt2 ? xt2 : xt1);
+    }
+  }
+  if (xt < 0) return false;
+  float tt = c.f + (1.f - c.f) * xt;
+  if (c.swap) tt = 1.f - tt;
+  *t = tt;
+  return true;
+}
 
 struct PaintDev {
   int32_t kind;
@@ -220,6 +274,8 @@ __device__ uint32_t GradientColor(const GradDev& g, const GradStops& sl, int x, 
     t = mx;
   } else if (g.type == HIPR_GRADIENT_RADIAL) {
     t = sqrtf(mx * mx + my * my);
+  } else if (g.type == HIPR_GRADIENT_CONICAL) {
+    if (!ConicalT(g.con, mx, my, &t)) return 0;
   } else {
     float angle = atan2f(-my, -mx);
     const float k1Over2Pi = 0.1591549430918f;

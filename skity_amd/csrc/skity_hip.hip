@@ -182,6 +182,79 @@ uint32_t PremulRGBAHost(const float c[4]) {
 }
 
 // PointsToUnit (sw_span_brush.cc:164-198) * inverse(local) * inverse(CTM).
+// PointsToUnit(p0, p1) (sw_span_brush.cc:200-219): rotation . scale .
+// translation taking p0 to the origin and p1 to (1, 0).
+Aff TwoPointsToUnit(float x0, float y0, float x1, float y1) {
+  float ssx = x1 - x0, ssy = y1 - y0;
+  float length = std::sqrt(ssx * ssx + ssy * ssy);
+  float scale = length > 0 ? 1.0f / length : 0;
+  float ux = ssx * scale, uy = ssy * scale;
+  Aff rot;
+  rot.sx = ux;
+  rot.kx = uy;  // -sine with sine = -uy
+  rot.ky = -uy;
+  rot.sy = ux;
+  return Mul(Mul(rot, Scale(scale, scale)), Translate(-x0, -y0));
+}
+
+// ConicalGradientColorBrush::OnPreBrush (sw_span_brush.cc:407-440), in the
+// same float operations.  Points are (x, y, 0, 1), so the focal point's w is
+// (1 - f) + f and enters |c1 - cf|.
+ConicalDev ConicalSetup(const hipr_gradient& g) {
+  ConicalDev c{};
+  const float kNear = 1.f / 4096;
+  float c0x = g.points[0], c0y = g.points[1], c1x = g.points[2], c1y = g.points[3];
+  float r0 = g.radius[0], r1 = g.radius[1];
+  float dcx = c1x - c0x, dcy = c1y - c0y;
+  float delta_center = std::sqrt(dcx * dcx + dcy * dcy);
+  float delta_radius = std::fabs(r1 - r0);
+  c.kind = kConicalNone;
+  c.sign = 1.f;
+  if (r0 < 0 || r1 < 0) return c;
+  bool radial = delta_center < kNear, strip = delta_radius < kNear;
+  Aff u;
+  if (radial) {
+    if (strip) return c;
+    c.kind = kConicalRadial;
+    c.scale = (float)(1.0 / delta_radius);
+    c.sign = delta_radius < 0 ? -1.f : 1.f;
+    c.bias = r0 / delta_radius;
+    c.c0x = c0x;
+    c.c0y = c0y;
+    return c;
+  }
+  if (strip) {
+    c.kind = kConicalStrip;
+    float r = r0 / delta_center;
+    c.r2 = r * r;
+    u = TwoPointsToUnit(c0x, c0y, c1x, c1y);
+  } else {
+    c.kind = kConicalFocal;
+    c.swap = r1 < kNear;
+    if (c.swap) {
+      std::swap(c0x, c1x);
+      std::swap(c0y, c1y);
+      std::swap(r0, r1);
+    }
+    c.f = r0 / (r0 - r1);
+    float cfx = c0x * (1.f - c.f) + c1x * c.f;
+    float cfy = c0y * (1.f - c.f) + c1y * c.f;
+    float cfw = 1.f * (1.f - c.f) + 1.f * c.f;
+    float ex = c1x - cfx, ey = c1y - cfy, ew = 1.f - cfw;
+    float len = std::sqrt(ex * ex + ey * ey + 0.f * 0.f + ew * ew);
+    c.r1 = r1 / len;
+    c.r1sq = c.r1 * c.r1;
+    u = TwoPointsToUnit(cfx, cfy, c1x, c1y);
+  }
+  c.ux = u.sx;
+  c.ukx = u.kx;
+  c.utx = u.tx;
+  c.uky = u.ky;
+  c.uy = u.sy;
+  c.uty = u.ty;
+  return c;
+}
+
 GradDev MakeGradDev(const hipr_gradient& g, const Aff& ctm) {
   GradDev d{};
   d.type = g.type;
@@ -190,16 +263,9 @@ GradDev MakeGradDev(const hipr_gradient& g, const Aff& ctm) {
   d.has_offsets = g.has_offsets;
   Aff ptu;
   if (g.type == HIPR_GRADIENT_LINEAR) {
-    float ssx = g.points[2] - g.points[0], ssy = g.points[3] - g.points[1];
-    float length = std::sqrt(ssx * ssx + ssy * ssy);
-    float scale = length > 0 ? 1.0f / length : 0;
-    float ux = ssx * scale, uy = ssy * scale;
-    Aff rot;
-    rot.sx = ux;
-    rot.kx = uy;  // -sine with sine = -uy
-    rot.ky = -uy;
-    rot.sy = ux;
-    ptu = Mul(Mul(rot, Scale(scale, scale)), Translate(-g.points[0], -g.points[1]));
+    ptu = TwoPointsToUnit(g.points[0], g.points[1], g.points[2], g.points[3]);
+  } else if (g.type == HIPR_GRADIENT_CONICAL) {
+    d.con = ConicalSetup(g);  // the brush maps device -> local itself
   } else if (g.type == HIPR_GRADIENT_RADIAL) {
     float r = g.radius[0];
     float scale = r > 0 ? 1.0f / r : 0;
@@ -510,7 +576,7 @@ hipr_result ValidateScene(const hipr_scene* s) {
       const hipr_gradient& g = s->gradients[pt.gradient];
       if (g.color_count < 2 || g.color_count > HIPR_MAX_GRADIENT_STOPS)
         return HIPR_ERROR_INVALID_ARGUMENT;
-      if (g.type < HIPR_GRADIENT_LINEAR || g.type > HIPR_GRADIENT_SWEEP)
+      if (g.type < HIPR_GRADIENT_LINEAR || g.type > HIPR_GRADIENT_CONICAL)
         return HIPR_ERROR_NOT_SUPPORTED;
     }
     if (kind == HIPR_DRAW_PATH && pt.blur_radius > 0 &&

@@ -176,6 +176,41 @@ std::shared_ptr<Shader> Shader::MakeSweep(float cx, float cy, float start_angle,
   return s;
 }
 
+// Shader::MakeTwoPointConical (src/effect/shader.cc:50-85).
+std::shared_ptr<Shader> Shader::MakeTwoPointConical(const Point& start, float start_radius,
+                                                    const Point& end, float end_radius,
+                                                    const Color4f colors[], const float pos[],
+                                                    int count, TileMode mode) {
+  if (count < 2 || count > HIPR_MAX_GRADIENT_STOPS) return nullptr;
+  const float kNear = 1.f / 4096;
+  auto near0 = [&](float v) { return std::fabs(v) <= kNear; };
+  if (near0(start.x - end.x) && near0(start.y - end.y)) {
+    if (near0(start_radius - end_radius)) return nullptr;
+    if (near0(start_radius) || near0(end_radius)) {
+      if (near0(end_radius)) {
+        // the start circle's radial gradient, stops reversed and mirrored
+        Color4f rc[HIPR_MAX_GRADIENT_STOPS];
+        float rp[HIPR_MAX_GRADIENT_STOPS];
+        for (int i = 0; i < count; i++) {
+          rc[i] = colors[count - 1 - i];
+          if (pos) rp[i] = 1.f - pos[count - 1 - i];
+        }
+        return MakeRadial(start, start_radius, rc, pos ? rp : nullptr, count, mode);
+      }
+      return MakeRadial(start, end_radius, colors, pos, count, mode);
+    }
+  }
+  auto s = std::make_shared<Shader>();
+  MakeGradient(HIPR_GRADIENT_CONICAL, colors, pos, count, mode, &s->info_);
+  s->info_.points[0] = start.x;
+  s->info_.points[1] = start.y;
+  s->info_.points[2] = end.x;
+  s->info_.points[3] = end.y;
+  s->info_.radius[0] = start_radius;
+  s->info_.radius[1] = end_radius;
+  return s;
+}
+
 void Shader::SetLocalMatrix(const float m[6]) { std::memcpy(info_.local_matrix, m, 6 * sizeof(float)); }
 
 std::shared_ptr<MaskFilter> MaskFilter::MakeBlur(BlurStyle style, float radius) {

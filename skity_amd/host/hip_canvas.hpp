@@ -137,6 +137,13 @@ class Shader {
                                            float end_angle, const Color4f colors[],
                                            const float pos[], int count,
                                            TileMode tile_mode = TileMode::kClamp);
+  // Concentric circles with one radius 0 become a radial gradient (colours
+  // reversed when the end radius is 0); equal circles give no shader.
+  static std::shared_ptr<Shader> MakeTwoPointConical(const Point& start, float start_radius,
+                                                     const Point& end, float end_radius,
+                                                     const Color4f colors[], const float pos[],
+                                                     int count,
+                                                     TileMode tile_mode = TileMode::kClamp);
   // {scale_x, skew_x, trans_x, skew_y, scale_y, trans_y}
   void SetLocalMatrix(const float m[6]);
   const hipr_gradient& info() const { return info_; }

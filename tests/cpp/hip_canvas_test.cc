@@ -179,6 +179,27 @@ static void WidthZeroAndNegativeStrokes(hipr_context ctx) {
   EXPECT_EQ(s->Flush(), HIPR_ERROR_NOT_SUPPORTED);
 }
 
+// Shader::MakeTwoPointConical's routing (src/effect/shader.cc:50-85).
+static void TwoPointConicalFactory(hipr_context) {
+  Color4f c[2] = {{1, 0, 0, 1}, {0, 0, 1, 1}};
+  float pos[2] = {0.f, 0.25f};
+  Point p{10, 20}, q{30, 25};
+  EXPECT_EQ(Shader::MakeTwoPointConical(p, 5, p, 5, c, pos, 2) == nullptr, true);
+  auto a = Shader::MakeTwoPointConical(p, 0, p, 40, c, pos, 2);
+  EXPECT_EQ(a->info().type, HIPR_GRADIENT_RADIAL);
+  EXPECT_EQ(a->info().radius[0], 40.f);
+  auto b = Shader::MakeTwoPointConical(p, 30, p, 0, c, pos, 2);
+  EXPECT_EQ(b->info().type, HIPR_GRADIENT_RADIAL);
+  EXPECT_EQ(b->info().radius[0], 30.f);
+  EXPECT_EQ(b->info().colors[0][2], 1.f);     // colours reversed
+  EXPECT_EQ(b->info().offsets[0], 0.75f);     // 1 - 0.25
+  EXPECT_EQ(b->info().offsets[1], 1.f);       // 1 - 0
+  auto g = Shader::MakeTwoPointConical(p, 5, q, 20, c, pos, 2);
+  EXPECT_EQ(g->info().type, HIPR_GRADIENT_CONICAL);
+  EXPECT_EQ(g->info().points[2], 30.f);
+  EXPECT_EQ(g->info().radius[1], 20.f);
+}
+
 int main() {
   hipr_context ctx = nullptr;
   if (hipr_context_create(0, &ctx) != HIPR_SUCCESS) {
@@ -195,7 +216,8 @@ int main() {
                {"Shader.LinearGradientEnds", LinearGradientEnds},
                {"Canvas.MatrixMovesAndScalesDraws", MatrixMovesAndScalesDraws},
                {"Canvas.ClearAndDrawColor", ClearAndDrawColor},
-               {"HIPCanvas.WidthZeroAndNegativeStrokes", WidthZeroAndNegativeStrokes}};
+               {"HIPCanvas.WidthZeroAndNegativeStrokes", WidthZeroAndNegativeStrokes},
+               {"Shader.TwoPointConicalFactory", TwoPointConicalFactory}};
   for (auto& t : tests) {
     int before = g_failed;
     t.fn(ctx);

@@ -129,3 +129,66 @@ def paint_blend_scene(seed, w=96, h=72):
     cv.restore()
     cv.draw_paint(Paint((0.9, 0.1, 0.1, 0.25)).set_blend_mode(abi.BLEND_DST_OVER))
     return cv.scene()
+
+
+def conical_gradient(rng, cx, cy, size, case):
+    """A two-point conical gradient around (cx, cy): nested circles, a cone of
+    separate circles, an end radius of 0, equal radii (strip) or concentric
+    circles, with random stops, tiling and local rotation."""
+    g = abi.Gradient()
+    g.type = abi.GRADIENT_CONICAL
+    g.tile_mode = int(rng.integers(0, 4))
+    g.color_count = int(rng.integers(2, 6))
+    g.has_offsets = 1
+    offs = np.sort(rng.random(g.color_count))
+    for i in range(g.color_count):
+        g.offsets[i] = float(offs[i])
+        for c in range(4):
+            g.colors[i][c] = float(rng.uniform(0.2, 1.0))
+    s = size
+    c0 = (cx - 0.3 * s, cy - 0.1 * s)
+    if case == "nested":
+        c1, r0, r1 = (cx + 0.2 * s, cy + 0.1 * s), 0.2 * s, 0.9 * s
+    elif case == "cone":
+        c0, c1, r0, r1 = (cx - 0.6 * s, cy), (cx + 0.6 * s, cy + 0.2 * s), 0.1 * s, 0.35 * s
+    elif case == "to_point":
+        c1, r0, r1 = (cx + 0.25 * s, cy + 0.15 * s), 0.8 * s, 0.0
+    elif case == "strip":
+        c1, r0, r1 = (cx + 0.5 * s, cy + 0.2 * s), 0.3 * s, 0.3 * s
+    else:
+        c1, r0, r1 = c0, 0.15 * s, 0.8 * s
+    g.points[0], g.points[1], g.points[2], g.points[3] = c0[0], c0[1], c1[0], c1[1]
+    g.radius[0], g.radius[1] = r0, r1
+    a = float(rng.uniform(-0.4, 0.4))
+    ca, sa = float(np.cos(a)), float(np.sin(a))
+    # rotation about (cx, cy) as the shader's local matrix
+    m = (ca, -sa, cx - ca * cx + sa * cy, sa, ca, cy - sa * cx - ca * cy)
+    for i, v in enumerate(m):
+        g.local_matrix[i] = v
+    return g
+
+
+CONICAL_CASES = ("nested", "cone", "to_point", "strip", "concentric")
+
+
+def conical_scene(seed, w=256, h=192, n=30):
+    """Fills and strokes with two-point conical gradients of every case,
+    some under a canvas matrix."""
+    rng = np.random.default_rng(seed)
+    cv = Canvas(w, h)
+    for i in range(n):
+        cx, cy = float(rng.uniform(30, w - 30)), float(rng.uniform(30, h - 30))
+        size = float(rng.uniform(20, 70))
+        paint = Paint((1, 1, 1, float(rng.uniform(0.5, 1.0))))
+        if i % 5 == 4:
+            paint.set_style(abi.STYLE_STROKE)
+            paint.set_stroke_width(float(rng.uniform(2, 9)))
+        paint.set_gradient(conical_gradient(rng, cx, cy, size, CONICAL_CASES[i % 5]))
+        cv.save()
+        if i % 3 == 2:
+            cv.translate(cx, cy)
+            cv.rotate(float(rng.uniform(-30, 30)))
+            cv.translate(-cx, -cy)
+        cv.draw_path(random_path(rng, cx, cy, size), paint)
+        cv.restore()
+    return cv.scene()
