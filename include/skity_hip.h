@@ -140,6 +140,27 @@ typedef struct hipr_gradient {
  * the quad conversion (Path::CopyWithMatrix, src/graphic/path.cc:1274-1295).
  * The fields after n_gradients may be left zero (identity matrices, all
  * draws DrawPath). */
+/* A paint's ColorFilter (include/skity/effect/color_filter.hpp:26-37), applied
+ * to every source pixel after its cover, before the blend
+ * (SWSpanBrush::BrushH, src/render/sw/sw_span_brush.cc:106-137). */
+enum {
+  HIPR_COLOR_FILTER_NONE = 0,
+  HIPR_COLOR_FILTER_BLEND = 1,           /* ColorFilters::Blend(color, blend)   */
+  HIPR_COLOR_FILTER_MATRIX = 2,          /* ColorFilters::Matrix(matrix)        */
+  HIPR_COLOR_FILTER_LINEAR_TO_SRGB = 3,  /* ColorFilters::LinearToSRGBGamma()   */
+  HIPR_COLOR_FILTER_SRGB_TO_LINEAR = 4,  /* ColorFilters::SRGBToLinearGamma()   */
+  HIPR_COLOR_FILTER_COMPOSE = 5          /* ColorFilters::Compose(outer, inner):
+                                            the software backend passes colours
+                                            through it unchanged
+                                            (color_filter.cc:182-185)          */
+};
+typedef struct hipr_color_filter {
+  int32_t type;       /* HIPR_COLOR_FILTER_*                                  */
+  uint32_t color;     /* BLEND: the Color, 0xAARRGGBB, unpremultiplied        */
+  int32_t blend;      /* BLEND: HIPR_BLEND_*                                  */
+  float matrix[20];   /* MATRIX: row-major 4x5, each entry with |255 m| < 2^15 */
+} hipr_color_filter;
+
 typedef struct hipr_scene {
   int32_t n_paths;
   const uint8_t* verbs;
@@ -157,6 +178,7 @@ typedef struct hipr_scene {
    * y' = (skew_y * x + scale_y * y) + trans_y; NULL = identity. */
   const float* matrices;
   const uint8_t* draw_kinds;     /* n_paths, HIPR_DRAW_*; NULL = all paths */
+  const hipr_color_filter* color_filters; /* n_paths; NULL = none            */
 } hipr_scene;
 
 typedef struct hipr_context_s* hipr_context;

@@ -528,6 +528,13 @@ struct Brush {
   int img_w = 0, img_h = 0;
   Aff img_ptu;
   uint8_t global_alpha = 255;
+  // the paint's ColorFilter (color_filter.cc:113-185)
+  int cf_type = HIPR_COLOR_FILTER_NONE;
+  uint32_t cf_pm = 0;  // BLEND: ColorToPMColor of the filter colour
+  int cf_mode = 0;
+  int32_t cf_m[4][5] = {};  // MATRIX: (int16_t)(m * 255)
+
+  uint32_t Filter(uint32_t c) const;
 
   uint32_t Color(int x, int y) const {
     if (kind == 0) return solid;
@@ -563,6 +570,7 @@ void ApplySpans(It begin, It end, const Brush& br, int blend, Target* t) {
     for (int l = 0; l < len; l++) {
       uint32_t c = br.Color(x + l, y);
       if (alpha != 255) c = AlphaMulQ(c, alpha);
+      if (br.cf_type != HIPR_COLOR_FILTER_NONE) c = br.Filter(c);
       t->Blend(x + l, y, c, blend);
     }
   }
@@ -628,6 +636,64 @@ void StackBlur(const uint8_t* src, uint8_t* dst, int w, int h, int radius) {
 }
 
 namespace {
+
+// The sRGB transfer tables of SRGBGammaColorFilter (color_filter.cc:128-166),
+// from their definition: entry i = floor(255 f(i / 255)) with f the sRGB
+// encoding (linear -> sRGB) or decoding, in double precision (a 1e-6 nudge
+// keeps f(1) = 1 from rounding below 255).
+const uint8_t* SrgbTable(bool linear_to_srgb) {
+  static uint8_t tab[2][256];
+  static bool init = [] {
+    for (int i = 0; i < 256; i++) {
+      double x = i / 255.0;
+      double e = x <= 0.0031308 ? 12.92 * x : 1.055 * std::pow(x, 1 / 2.4) - 0.055;
+      double d = x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4);
+      tab[0][i] = (uint8_t)std::floor(255 * e + 1e-6);
+      tab[1][i] = (uint8_t)std::floor(255 * d + 1e-6);
+    }
+    return true;
+  }();
+  (void)init;
+  return tab[linear_to_srgb ? 0 : 1];
+}
+
+// ColorFilter::FilterColor of a premultiplied ARGB colour.
+uint32_t FilterPM(const Brush& br, uint32_t c) {
+  switch (br.cf_type) {
+    case HIPR_COLOR_FILTER_BLEND:
+      return PorterDuffBlend(br.cf_pm, c, br.cf_mode);
+    case HIPR_COLOR_FILTER_MATRIX: {
+      uint32_t u = PMColorToColor(c);
+      int32_t v[4] = {(int32_t)GetR(u), (int32_t)GetG(u), (int32_t)GetB(u), (int32_t)GetA(u)};
+      uint32_t o[4];
+      for (int i = 0; i < 4; i++) {
+        int32_t mul = v[0] * br.cf_m[i][0] + v[1] * br.cf_m[i][1] + v[2] * br.cf_m[i][2] +
+                      v[3] * br.cf_m[i][3];
+        o[i] = (uint32_t)std::clamp(mul / 255 + br.cf_m[i][4], 0, 255);
+      }
+      return ColorToPMColor(SetARGB(o[3], o[0], o[1], o[2]));
+    }
+    case HIPR_COLOR_FILTER_LINEAR_TO_SRGB:
+    case HIPR_COLOR_FILTER_SRGB_TO_LINEAR: {
+      const uint8_t* t = SrgbTable(br.cf_type == HIPR_COLOR_FILTER_LINEAR_TO_SRGB);
+      uint32_t u = PMColorToColor(c);
+      return ColorToPMColor(SetARGB(GetA(u), t[GetR(u)], t[GetG(u)], t[GetB(u)]));
+    }
+    default:  // NONE; COMPOSE passes colours through (color_filter.cc:182-185)
+      return c;
+  }
+}
+
+uint32_t Brush::Filter(uint32_t c) const { return FilterPM(*this, c); }
+
+void SetColorFilter(const hipr_scene* s, int p, Brush* br) {
+  if (!s->color_filters) return;
+  const hipr_color_filter& f = s->color_filters[p];
+  br->cf_type = f.type;
+  br->cf_mode = f.blend;
+  br->cf_pm = ColorToPMColor(f.color);
+  for (int i = 0; i < 20; i++) br->cf_m[i / 5][i % 5] = (int16_t)(f.matrix[i] * 255);
+}
 
 // GenerateBrush (sw_canvas.cc:727-795): gradient shader, else the stroke or
 // fill colour.
@@ -815,6 +881,9 @@ void BlurredLayer(const hipr_scene* s, int p, std::vector<uint8_t>* blurred,
   work.blur_radius = 0;
   Ops ops;
   ShapeOps(s, path, work, Translate(-l, -tp), &ops);
+  // the layer is drawn with the paint's colour filter (HandleFilter keeps it
+  // in work_paint, sw_canvas.cc:797-826)
+  for (auto& op : ops) SetColorFilter(s, p, &op->br);
   DrawOps(ops, &lt);
   blurred->assign((size_t)W * H * 4, 0);
   int radius = (int)std::round(r);
@@ -825,7 +894,18 @@ void BlurredLayer(const hipr_scene* s, int p, std::vector<uint8_t>* blurred,
 namespace {
 
 // The rasterisations of draw p onto a tw x th target.
+void DrawEntryOpsUnfiltered(const hipr_scene* s, int p, int tw, int th, Ops* ops);
+
+// Every brush of draw p carries the paint's colour filter: fills, strokes,
+// DrawPaint and the composite of a blurred layer (DrawImage with the work
+// paint, image_filter.cc:184-194).
 void DrawEntryOps(const hipr_scene* s, int p, int tw, int th, Ops* ops) {
+  const size_t first = ops->size();
+  DrawEntryOpsUnfiltered(s, p, tw, th, ops);
+  for (size_t i = first; i < ops->size(); i++) SetColorFilter(s, p, &(*ops)[i]->br);
+}
+
+void DrawEntryOpsUnfiltered(const hipr_scene* s, int p, int tw, int th, Ops* ops) {
   const hipr_paint& paint = s->paints[p];
   const Aff ctm = DrawMatrix(s, p);
   const int kind = s->draw_kinds ? s->draw_kinds[p] : HIPR_DRAW_PATH;

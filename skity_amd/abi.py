@@ -71,6 +71,18 @@ class Gradient(ctypes.Structure):
     ]
 
 
+CF_NONE, CF_BLEND, CF_MATRIX, CF_LINEAR_TO_SRGB, CF_SRGB_TO_LINEAR, CF_COMPOSE = range(6)
+
+
+class ColorFilter(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("color", ctypes.c_uint32),
+        ("blend", ctypes.c_int32),
+        ("matrix", ctypes.c_float * 20),
+    ]
+
+
 class Scene(ctypes.Structure):
     _fields_ = [
         ("n_paths", ctypes.c_int32),
@@ -86,6 +98,7 @@ class Scene(ctypes.Structure):
         ("n_gradients", ctypes.c_int32),
         ("matrices", ctypes.POINTER(ctypes.c_float)),
         ("draw_kinds", ctypes.POINTER(ctypes.c_uint8)),
+        ("color_filters", ctypes.POINTER(ColorFilter)),
     ]
 
 

@@ -161,6 +161,7 @@ class Paint:
         self.p.gradient = -1
         self.p.blend = abi.BLEND_SRC_OVER
         self.gradient = None
+        self.color_filter = None
 
     def set_color(self, rgba):
         # Paint::SetColor sets both the fill and the stroke colour
@@ -210,9 +211,56 @@ class Paint:
         return self
 
     def set_gradient(self, g):
-        """A Shader::MakeLinear/Radial/Sweep gradient (abi.Gradient)."""
+        """A Shader::MakeLinear/Radial/Sweep/TwoPointConical gradient
+        (abi.Gradient)."""
         self.gradient = g
         return self
+
+    def set_color_filter(self, cf):
+        """Paint::SetColorFilter: an abi.ColorFilter from the ColorFilters
+        factories below, or None."""
+        self.color_filter = cf
+        return self
+
+
+# ColorFilters (include/skity/effect/color_filter.hpp:26-37,
+# src/effect/color_filter.cc:64-106): the factories return None where the
+# reference returns no filter.
+def blend_color_filter(argb, mode):
+    a = (argb >> 24) & 255
+    if (mode == abi.BLEND_DST or (mode == abi.BLEND_DST_IN and a == 255)
+            or (mode == abi.BLEND_DST_OUT and a == 0)):
+        return None
+    cf = abi.ColorFilter()
+    cf.type, cf.color, cf.blend = abi.CF_BLEND, argb, mode
+    return cf
+
+
+def matrix_color_filter(row_major):
+    ident = (1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0)
+    if row_major is None or tuple(float(v) for v in row_major) == ident:
+        return None
+    cf = abi.ColorFilter()
+    cf.type = abi.CF_MATRIX
+    for i, v in enumerate(row_major):
+        cf.matrix[i] = v
+    return cf
+
+
+def gamma_color_filter(linear_to_srgb):
+    cf = abi.ColorFilter()
+    cf.type = abi.CF_LINEAR_TO_SRGB if linear_to_srgb else abi.CF_SRGB_TO_LINEAR
+    return cf
+
+
+def compose_color_filter(outer, inner):
+    if inner is None:
+        return outer
+    if outer is None:
+        return inner
+    cf = abi.ColorFilter()
+    cf.type = abi.CF_COMPOSE
+    return cf
 
 
 class Canvas:
@@ -286,7 +334,13 @@ class PackedScene:
         draws = [d if len(d) == 4 else (d[0], d[1], IDENTITY, abi.DRAW_PATH) for d in draws]
         mats = np.array([d[2] for d in draws] or [IDENTITY], np.float32).reshape(-1)
         kinds = np.array([d[3] for d in draws] or [0], np.uint8)
+        self.cfs = (abi.ColorFilter * max(n, 1))()
+        any_cf = False
         for i, (path, paint, _, _) in enumerate(draws):
+            if getattr(paint, "color_filter", None) is not None:
+                ctypes.memmove(ctypes.byref(self.cfs[i]), ctypes.byref(paint.color_filter),
+                               ctypes.sizeof(abi.ColorFilter))
+                any_cf = True
             verbs += path.verbs
             pts += path.points
             ws += path.weights
@@ -324,6 +378,8 @@ class PackedScene:
             s.matrices = mats.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         if any(d[3] != abi.DRAW_PATH for d in draws):
             s.draw_kinds = kinds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        if any_cf:
+            s.color_filters = ctypes.cast(self.cfs, ctypes.POINTER(abi.ColorFilter))
         self._s = s
         self.scene = ctypes.pointer(s)
 

@@ -91,6 +91,11 @@ struct PaintDev {
   int32_t blend;    // HIPR_BLEND_* of the paint
   int32_t L, T, R, B;  // kJobRect: the rectangle of full coverage
   float im[6];      // image: device point -> unit square of the layer
+  // the paint's colour filter (HIPR_COLOR_FILTER_*)
+  int32_t cf_type;
+  uint32_t cf_pm;   // BLEND: the filter colour, premultiplied, RGBA byte order
+  int32_t cf_mode;  // BLEND: HIPR_BLEND_*
+  int32_t cf_m[20]; // MATRIX: (int16_t)(m * 255), row-major 4 x 5
 };
 
 __device__ __forceinline__ uint32_t AlphaMulQ(uint32_t c, uint32_t scale) {
@@ -330,6 +335,41 @@ __device__ uint32_t GradientColor(const GradDev& g, const GradStops& sl, int x, 
   return PremulRGBA(c);
 }
 
+// ColorFilter::FilterColor (color_filter.cc:113-185) of a premultiplied
+// word: the blend filter's Porter-Duff blend of its colour over the source;
+// the matrix filter on the unpremultiplied channels in integers,
+// clamp(sum / 255 + offset); the sRGB filters through their tables
+// (`srgb`: 256 entries linear -> sRGB, then 256 sRGB -> linear); compose
+// passes the colour through.
+__device__ uint32_t FilterWord(uint32_t c, const PaintDev& pd, const uint8_t* __restrict__ srgb) {
+  switch (pd.cf_type) {
+    case HIPR_COLOR_FILTER_BLEND:
+      return PorterDuffWord(pd.cf_pm, c, pd.cf_mode);
+    case HIPR_COLOR_FILTER_MATRIX: {
+      const uint32_t u = UnpremulWord(c);
+      const int32_t v[4] = {(int32_t)Chan(u, 0), (int32_t)Chan(u, 1), (int32_t)Chan(u, 2),
+                            (int32_t)(u >> 24)};
+      uint32_t o[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int32_t* m = pd.cf_m + 5 * i;
+        int32_t mul = v[0] * m[0] + v[1] * m[1] + v[2] * m[2] + v[3] * m[3];
+        int32_t r = mul / 255 + m[4];
+        o[i] = (uint32_t)(r < 0 ? 0 : (r > 255 ? 255 : r));
+      }
+      return PremulWord(Pack(o[0], o[1], o[2], o[3]));
+    }
+    case HIPR_COLOR_FILTER_LINEAR_TO_SRGB:
+    case HIPR_COLOR_FILTER_SRGB_TO_LINEAR: {
+      const uint8_t* t = srgb + (pd.cf_type == HIPR_COLOR_FILTER_SRGB_TO_LINEAR ? 256 : 0);
+      const uint32_t u = UnpremulWord(c);
+      return PremulWord(Pack(t[Chan(u, 0)], t[Chan(u, 1)], t[Chan(u, 2)], u >> 24));
+    }
+    default:
+      return c;
+  }
+}
+
 struct CompositeParams {
   const Canvas* canvases;
   int32_t n_canvases;
@@ -342,6 +382,7 @@ struct CompositeParams {
   const uint8_t* planes;
   const PaintDev* paints;
   const GradDev* grads;
+  const uint8_t* srgb;  // the sRGB filter tables (512 entries)
   // optional: [0] += pixel blends performed, [1] += target pixels touched
   unsigned long long* counters;
 };
@@ -349,8 +390,10 @@ struct CompositeParams {
 // SWSpanBrush::BrushH for one pixel and cover (sw_span_brush.cc:108-138):
 // AlphaMulQ by the cover unless it is 255, then the target blend.
 __device__ __forceinline__ uint32_t ApplyCover(uint32_t dst, uint32_t color, uint32_t cover,
-                                               int mode, bool premul) {
+                                               int mode, bool premul, const PaintDev& pd,
+                                               const uint8_t* srgb) {
   uint32_t src = cover != 255 ? AlphaMulQ(color, cover) : color;
+  if (pd.cf_type != HIPR_COLOR_FILTER_NONE) src = FilterWord(src, pd, srgb);
   if (mode == HIPR_BLEND_SRC_OVER && premul) return BlendSrcOver(src, dst);
   return BlendStored(src, dst, mode, premul);
 }
@@ -478,8 +521,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
         for (int q = 0; q < 4; q++) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
           bool d0 = three ? ((direct >> (8 * q)) & 0xFF) != 0 : c0 != 0;
-          if (d0) pix[q] = ApplyCover(pix[q], pd.color, c0, mode, premul);
-          if (c1) pix[q] = ApplyCover(pix[q], pd.color, c1, mode, premul);
+          if (d0) pix[q] = ApplyCover(pix[q], pd.color, c0, mode, premul, pd, P.srgb);
+          if (c1) pix[q] = ApplyCover(pix[q], pd.color, c1, mode, premul, pd, P.srgb);
           blends += (int)d0 + (c1 != 0);
           touched |= (uint32_t)(d0 || c1 != 0) << q;
         }
@@ -505,8 +548,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
                              : g ? GradientColor(*g, stops, px0 + q, py)
                                  : ImageColor(pd, P.canvases[pd.layer], px0 + q, py);
             // Brush: cover & global alpha (sw_span_brush.cc:101)
-            if (d0) pix[q] = ApplyCover(pix[q], color, c0 & ga, mode, premul);
-            if (c1) pix[q] = ApplyCover(pix[q], color, c1 & ga, mode, premul);
+            if (d0) pix[q] = ApplyCover(pix[q], color, c0 & ga, mode, premul, pd, P.srgb);
+            if (c1) pix[q] = ApplyCover(pix[q], color, c1 & ga, mode, premul, pd, P.srgb);
             blends += (int)d0 + (c1 != 0);
             touched |= 1u << q;
           }

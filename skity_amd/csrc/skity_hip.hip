@@ -319,6 +319,7 @@ struct hipr_context_s {
       gather_list;
   DevBuf wcov, qstate, q_base, snaps, snap_base, band_count;  // write-pass coverage, row bands
   DevBuf queues;                          // persistent walk: item counters
+  DevBuf srgb;                            // sRGB colour-filter tables (512 bytes)
   DevBuf flatten_redo;                    // jobs whose edges outgrew their estimate
   DevBuf stroke_redo, stroke_redo_out;    // stroke jobs written again (estimate forms)
   DevBuf pool_ctr;                        // overflow plane pool counter
@@ -582,6 +583,18 @@ hipr_result ValidateScene(const hipr_scene* s) {
     if (kind == HIPR_DRAW_PATH && pt.blur_radius > 0 &&
         (pt.blur_style < HIPR_BLUR_NORMAL || pt.blur_style > HIPR_BLUR_INNER))
       return HIPR_ERROR_INVALID_ARGUMENT;
+    if (s->color_filters) {
+      const hipr_color_filter& f = s->color_filters[p];
+      if (f.type < HIPR_COLOR_FILTER_NONE || f.type > HIPR_COLOR_FILTER_COMPOSE)
+        return HIPR_ERROR_INVALID_ARGUMENT;
+      if (f.type == HIPR_COLOR_FILTER_BLEND &&
+          (f.blend < HIPR_BLEND_CLEAR || f.blend > HIPR_BLEND_LAST))
+        return HIPR_ERROR_INVALID_ARGUMENT;
+      if (f.type == HIPR_COLOR_FILTER_MATRIX)
+        for (int k = 0; k < 20; k++)  // the reference keeps (int16_t)(m * 255)
+          if (!std::isfinite(f.matrix[k]) || !(std::fabs(f.matrix[k] * 255) < 32767.f))
+            return HIPR_ERROR_NOT_SUPPORTED;
+    }
   }
   return HIPR_SUCCESS;
 }
@@ -621,6 +634,34 @@ PaintDev SolidOrGradient(const hipr_scene* s, const hipr_paint& pt, const Aff& c
     pd.color = PremulRGBAHost(stroke ? pt.stroke_color : pt.color);
   }
   return pd;
+}
+
+// The paint's colour filter on a device paint record (every brush of the
+// draw carries it: fills, strokes, DrawPaint, the blurred layer's draw and
+// its composite).
+void ApplyColorFilter(const hipr_scene* s, int p, PaintDev* pd) {
+  pd->cf_type = HIPR_COLOR_FILTER_NONE;
+  if (!s->color_filters) return;
+  const hipr_color_filter& f = s->color_filters[p];
+  pd->cf_type = f.type;
+  pd->cf_mode = f.blend;
+  // ColorToPMColor of the filter colour (0xAARRGGBB), RGBA byte order
+  uint32_t a = f.color >> 24, r = (f.color >> 16) & 255, g = (f.color >> 8) & 255,
+           b = f.color & 255;
+  if (a != 255) {
+    r = MulDiv255(r, a);
+    g = MulDiv255(g, a);
+    b = MulDiv255(b, a);
+  }
+  pd->cf_pm = r | (g << 8) | (b << 16) | (a << 24);
+  for (int i = 0; i < 20; i++) pd->cf_m[i] = (int16_t)(f.matrix[i] * 255);
+}
+
+// A filter that can give a transparent source a colour makes the draw's
+// zero-cover spans visible (blend and matrix filters; the sRGB tables map 0
+// to 0 and compose passes colours through).
+bool FilterColoursTransparent(const PaintDev& pd) {
+  return pd.cf_type == HIPR_COLOR_FILTER_BLEND || pd.cf_type == HIPR_COLOR_FILTER_MATRIX;
 }
 
 Affine ToAffine(const Aff& a) { return Affine{a.sx, a.kx, a.tx, a.ky, a.sy, a.ty}; }
@@ -687,6 +728,7 @@ void AddShapeJobs(const hipr_scene* s, int p, int canvas, const Aff& ctm,
     jb.image = -1;
     jb.paint = (int32_t)pl->paints.size();
     pl->paints.push_back(SolidOrGradient(s, pt, ctm, stroke, &pl->grads));
+    ApplyColorFilter(s, p, &pl->paints.back());
     out->push_back(jb);
   };
   if (pt.style == HIPR_STYLE_STROKE_THEN_FILL) {
@@ -821,6 +863,7 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
       pd.T = 0;
       pd.R = INT32_MAX;
       pd.B = INT32_MAX;
+      ApplyColorFilter(s, p, &pd);
       canvas_jobs[0].push_back(RectJob(p, 0, (int32_t)pl->paints.size()));
       pl->paints.push_back(pd);
       continue;
@@ -869,6 +912,7 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
       // paint with its style set to fill, so that its alpha (Paint::GetAlphaF,
       // paint.cc:96-98) is the fill colour's.
       PaintDev ip{};
+      ApplyColorFilter(s, p, &ip);
       ip.kind = kPaintImage;
       ip.layer = layer;
       ip.blend = pt.blend;
@@ -1381,7 +1425,8 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     jb.flags = 0;
     if (jb.kind == kJobRect) continue;
     const bool premul = (f.canvases[jb.canvas].format & kFormatUnpremul) == 0;
-    if (TransparentSourceMatters(pl->paints[jb.paint].blend, premul)) {
+    const PaintDev& jp = pl->paints[jb.paint];
+    if (TransparentSourceMatters(jp.blend, premul) || FilterColoursTransparent(jp)) {
       jb.flags |= kJobKeepZero;
       any_keep_zero = true;
     }
@@ -1938,6 +1983,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   cp.planes = ctx->planes.as<uint8_t>();
   cp.paints = pl->d_paints.as<PaintDev>();
   cp.grads = pl->d_grads.as<GradDev>();
+  cp.srgb = ctx->srgb.as<uint8_t>();
   int main_tiles = 0;
   for (int t = 0; t < n_targets; t++) main_tiles += f.canvases[t].tiles_x * f.canvases[t].tiles_y;
   if ((int)f.canvases.size() > n_targets) {
@@ -2476,6 +2522,23 @@ HIPR_API hipr_result hipr_context_create(int32_t device, hipr_context* out) {
       hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_join2, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    delete ctx;
+    return HIPR_ERROR_INITIALIZATION_FAILED;
+  }
+  // The sRGB tables of SRGBGammaColorFilter (color_filter.cc:128-166) from
+  // their definition: floor(255 f(i / 255)) with f the sRGB encoding
+  // (linear -> sRGB) or decoding, in double (a 1e-6 nudge keeps f(1) = 1 at
+  // 255).
+  uint8_t tab[512];
+  for (int i = 0; i < 256; i++) {
+    double x = i / 255.0;
+    double e = x <= 0.0031308 ? 12.92 * x : 1.055 * std::pow(x, 1 / 2.4) - 0.055;
+    double d = x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4);
+    tab[i] = (uint8_t)std::floor(255 * e + 1e-6);
+    tab[256 + i] = (uint8_t)std::floor(255 * d + 1e-6);
+  }
+  if (ctx->srgb.Ensure(sizeof(tab)) != hipSuccess ||
+      hipMemcpy(ctx->srgb.p, tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess) {
     delete ctx;
     return HIPR_ERROR_INITIALIZATION_FAILED;
   }

@@ -213,6 +213,49 @@ std::shared_ptr<Shader> Shader::MakeTwoPointConical(const Point& start, float st
 
 void Shader::SetLocalMatrix(const float m[6]) { std::memcpy(info_.local_matrix, m, 6 * sizeof(float)); }
 
+std::shared_ptr<ColorFilter> ColorFilters::Blend(Color color, BlendMode mode) {
+  const uint32_t a = color >> 24;
+  if (mode == BlendMode::kDst || (mode == BlendMode::kDstIn && a == 255) ||
+      (mode == BlendMode::kDstOut && a == 0))
+    return nullptr;
+  auto f = std::make_shared<ColorFilter>();
+  f->info_.type = HIPR_COLOR_FILTER_BLEND;
+  f->info_.color = color;
+  f->info_.blend = static_cast<int32_t>(mode);
+  return f;
+}
+
+std::shared_ptr<ColorFilter> ColorFilters::Matrix(const float row_major[20]) {
+  static const float kIdentity[20] = {1, 0, 0, 0, 0, 0, 1, 0, 0, 0,
+                                      0, 0, 1, 0, 0, 0, 0, 0, 1, 0};
+  if (!row_major || std::memcmp(row_major, kIdentity, sizeof(kIdentity)) == 0) return nullptr;
+  auto f = std::make_shared<ColorFilter>();
+  f->info_.type = HIPR_COLOR_FILTER_MATRIX;
+  std::memcpy(f->info_.matrix, row_major, sizeof(f->info_.matrix));
+  return f;
+}
+
+std::shared_ptr<ColorFilter> ColorFilters::LinearToSRGBGamma() {
+  auto f = std::make_shared<ColorFilter>();
+  f->info_.type = HIPR_COLOR_FILTER_LINEAR_TO_SRGB;
+  return f;
+}
+
+std::shared_ptr<ColorFilter> ColorFilters::SRGBToLinearGamma() {
+  auto f = std::make_shared<ColorFilter>();
+  f->info_.type = HIPR_COLOR_FILTER_SRGB_TO_LINEAR;
+  return f;
+}
+
+std::shared_ptr<ColorFilter> ColorFilters::Compose(std::shared_ptr<ColorFilter> outer,
+                                                   std::shared_ptr<ColorFilter> inner) {
+  if (!inner) return outer;
+  if (!outer) return inner;
+  auto f = std::make_shared<ColorFilter>();
+  f->info_.type = HIPR_COLOR_FILTER_COMPOSE;
+  return f;
+}
+
 std::shared_ptr<MaskFilter> MaskFilter::MakeBlur(BlurStyle style, float radius) {
   auto m = std::make_shared<MaskFilter>();
   m->style_ = style;
@@ -364,7 +407,8 @@ int HIPCanvas::Flush() {
   std::vector<hipr_gradient> grads;
   std::vector<float> matrices(6 * (size_t)n);
   std::vector<uint8_t> kinds(n);
-  bool any_matrix = false, any_paint = false;
+  std::vector<hipr_color_filter> filters(n);
+  bool any_matrix = false, any_paint = false, any_filter = false;
   for (int32_t i = 0; i < n; i++) {
     const Matrix& m = draws_[i].matrix;
     const float mf[6] = {m.sx, m.kx, m.tx, m.ky, m.sy, m.ty};
@@ -390,6 +434,10 @@ int HIPCanvas::Flush() {
       grads.push_back(draws_[i].paint.shader()->info());
     }
     paints[i] = draws_[i].paint.Pack(gi);
+    if (const auto& cf = draws_[i].paint.GetColorFilter()) {
+      filters[i] = cf->info();
+      any_filter = true;
+    }
   }
   hipr_scene scene{};
   scene.n_paths = n;
@@ -405,6 +453,7 @@ int HIPCanvas::Flush() {
   scene.n_gradients = (int32_t)grads.size();
   scene.matrices = any_matrix ? matrices.data() : nullptr;
   scene.draw_kinds = any_paint ? kinds.data() : nullptr;
+  scene.color_filters = any_filter ? filters.data() : nullptr;
   hipr_options opt{};
   opt.mode = HIPR_MODE_EXACT;
   int r = hipr_surface_draw_scene(surface_->surface_, &scene, &opt);

@@ -170,6 +170,27 @@ class MaskFilter {
   float radius_ = 0;
 };
 
+// ColorFilters (include/skity/effect/color_filter.hpp:15-37): the factories
+// return nullptr where the reference does (src/effect/color_filter.cc:64-106).
+class ColorFilter {
+ public:
+  const hipr_color_filter& info() const { return info_; }
+
+ private:
+  friend class ColorFilters;
+  hipr_color_filter info_{};
+};
+
+class ColorFilters {
+ public:
+  static std::shared_ptr<ColorFilter> Compose(std::shared_ptr<ColorFilter> outer,
+                                              std::shared_ptr<ColorFilter> inner);
+  static std::shared_ptr<ColorFilter> Blend(Color color, BlendMode mode);
+  static std::shared_ptr<ColorFilter> Matrix(const float row_major[20]);
+  static std::shared_ptr<ColorFilter> LinearToSRGBGamma();
+  static std::shared_ptr<ColorFilter> SRGBToLinearGamma();
+};
+
 class Paint {
  public:
   enum Style { kFill_Style, kStroke_Style, kStrokeAndFill_Style, kStrokeThenFill_Style };
@@ -193,6 +214,8 @@ class Paint {
   void SetMaskFilter(std::shared_ptr<MaskFilter> m) { mask_ = std::move(m); }
   void SetBlendMode(BlendMode m) { blend_ = m; }
   BlendMode GetBlendMode() const { return blend_; }
+  void SetColorFilter(std::shared_ptr<ColorFilter> f) { color_filter_ = std::move(f); }
+  const std::shared_ptr<ColorFilter>& GetColorFilter() const { return color_filter_; }
 
   hipr_paint Pack(int32_t gradient_index) const;
   const std::shared_ptr<Shader>& shader() const { return shader_; }
@@ -205,6 +228,7 @@ class Paint {
   Color4f fill_{0, 0, 0, 1}, stroke_{0, 0, 0, 1};
   std::shared_ptr<Shader> shader_;
   std::shared_ptr<MaskFilter> mask_;
+  std::shared_ptr<ColorFilter> color_filter_;
   BlendMode blend_ = BlendMode::kSrcOver;
 };
 

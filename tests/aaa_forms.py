@@ -125,6 +125,15 @@ def parallelogram_span_pixels(a, f, y0, w, h, direction):
 def clear_parallelogram_scene(seed=12):
     """Parallelograms drawn with kClear over an opaque backdrop (DrawPaint):
     returns (packed scene, set of pixels the reference clears)."""
+    cv, cleared = parallelogram_canvas(
+        seed, lambda: canvas.Paint(WHITE).set_blend_mode(abi.BLEND_CLEAR))
+    return cv.scene(), cleared
+
+
+def parallelogram_canvas(seed, make_paint):
+    """A 320 x 100 canvas: an opaque DrawPaint backdrop, then parallelograms
+    at 1/256-px offsets drawn with make_paint(); returns (canvas, set of the
+    pixels that receive a span of some parallelogram, cover 0 included)."""
     rng = np.random.default_rng(seed)
     cv = canvas.Canvas(320, 100)
     cv.draw_paint(canvas.Paint((0.2, 0.5, 0.8, 1.0)))
@@ -143,9 +152,9 @@ def clear_parallelogram_scene(seed=12):
         p.line_to(x0 + w + direction * h, y0 + h)
         p.line_to(x0 + direction * h, y0 + h)
         p.close()
-        cv.draw_path(p, canvas.Paint(WHITE).set_blend_mode(abi.BLEND_CLEAR))
+        cv.draw_path(p, make_paint())
         cleared |= parallelogram_span_pixels(a, f, y0, w, h, direction)
-    return cv.scene(), cleared
+    return cv, cleared
 
 
 def pixels_of(covers, width, height):

@@ -192,3 +192,45 @@ def conical_scene(seed, w=256, h=192, n=30):
         cv.draw_path(random_path(rng, cx, cy, size), paint)
         cv.restore()
     return cv.scene()
+
+
+def random_color_filter(rng):
+    """One of the ColorFilters factories with random arguments (never None)."""
+    k = int(rng.integers(0, 5))
+    if k == 0:
+        argb = (int(rng.integers(40, 256)) << 24) | int(rng.integers(0, 1 << 24))
+        mode = int(rng.choice([abi.BLEND_SRC, abi.BLEND_SRC_OVER, abi.BLEND_SRC_IN,
+                               abi.BLEND_DST_OVER, abi.BLEND_MODULATE, abi.BLEND_SCREEN,
+                               abi.BLEND_XOR, abi.BLEND_PLUS, abi.BLEND_SRC_ATOP]))
+        return canvas.blend_color_filter(argb, mode)
+    if k == 1:
+        m = rng.uniform(-0.5, 1.2, 20)
+        m[[4, 9, 14, 19]] = rng.uniform(-0.3, 0.3, 4)
+        m[18] = rng.uniform(0.6, 1.0)
+        return canvas.matrix_color_filter([float(v) for v in m])
+    if k == 2:
+        return canvas.gamma_color_filter(True)
+    if k == 3:
+        return canvas.gamma_color_filter(False)
+    return canvas.compose_color_filter(canvas.gamma_color_filter(True),
+                                       canvas.gamma_color_filter(False))
+
+
+def filter_scene(seed, w=200, h=150, n=36):
+    """Fills, strokes, gradients, blurred draws and DrawPaint, each with a
+    random colour filter and some with other blend modes."""
+    rng = np.random.default_rng(seed)
+    cv = Canvas(w, h)
+    cv.draw_paint(Paint((0.3, 0.6, 0.9, 0.8)))
+    for i in range(n):
+        cx, cy = float(rng.uniform(20, w - 20)), float(rng.uniform(20, h - 20))
+        size = float(rng.uniform(8, 40))
+        blend = None
+        if i % 4 == 3:
+            blend = int(rng.choice([abi.BLEND_SRC, abi.BLEND_DST_IN, abi.BLEND_XOR,
+                                    24, abi.BLEND_SOFT_LIGHT]))
+        paint = random_paint(rng, size, cx, cy, allow_blur=True, blend=blend)
+        if i % 6 != 5:
+            paint.set_color_filter(random_color_filter(rng))
+        cv.draw_path(random_path(rng, cx, cy, size), paint)
+    return cv.scene()
