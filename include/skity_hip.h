@@ -161,6 +161,29 @@ typedef struct hipr_color_filter {
   float matrix[20];   /* MATRIX: row-major 4x5, each entry with |255 m| < 2^15 */
 } hipr_color_filter;
 
+/* An Image's pixels (Image::MakeImage of a Pixmap, kRGBA_8888): rows of
+ * width * 4 bytes, R, G, B, A; premultiplied (HIPR_ALPHA_PREMUL or
+ * HIPR_ALPHA_OPAQUE) or not (HIPR_ALPHA_UNPREMUL). */
+typedef struct hipr_image {
+  int32_t width, height;
+  int32_t alpha_type;     /* HIPR_ALPHA_*                                      */
+  const uint8_t* pixels;
+} hipr_image;
+
+enum { HIPR_FILTER_NEAREST = 0, HIPR_FILTER_LINEAR = 1 };
+
+/* A pixmap shader (Shader::MakeShader(image, sampling, tile_x, tile_y,
+ * local_matrix), sampled by PixmapBrush / BitmapSampler,
+ * src/render/sw/sw_span_brush.cc:555-579, src/graphic/bitmap_sampler.cc:
+ * 26-108).  Cubic sampling is drawn as linear, as the software backend
+ * does. */
+typedef struct hipr_image_shader {
+  int32_t image;          /* index into hipr_scene.images; -1 = no shader      */
+  int32_t filter;         /* HIPR_FILTER_*                                     */
+  int32_t tile_x, tile_y; /* HIPR_TILE_*                                       */
+  float local_matrix[6];  /* {scale_x, skew_x, trans_x, skew_y, scale_y, trans_y} */
+} hipr_image_shader;
+
 typedef struct hipr_scene {
   int32_t n_paths;
   const uint8_t* verbs;
@@ -179,6 +202,10 @@ typedef struct hipr_scene {
   const float* matrices;
   const uint8_t* draw_kinds;     /* n_paths, HIPR_DRAW_*; NULL = all paths */
   const hipr_color_filter* color_filters; /* n_paths; NULL = none            */
+  const hipr_image* images;
+  int32_t n_images;
+  /* n_paths; NULL = none.  A draw with an image shader has no gradient. */
+  const hipr_image_shader* image_shaders;
 } hipr_scene;
 
 typedef struct hipr_context_s* hipr_context;

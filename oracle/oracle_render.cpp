@@ -520,8 +520,59 @@ void PrepareGradient(const hipr_gradient& g, const Aff& ctm, GradientBrush* b) {
 // A brush: solid colour, gradient, or the premultiplied image of a blurred
 // draw sampled nearest through a decal pixmap shader (PixmapBrush,
 // sw_span_brush.cc:555-579; BitmapSampler, bitmap_sampler.cc:26-40,85-108).
+// BitmapSampler (bitmap_sampler.cc:26-108) over an RGBA8 image.  Float ->
+// uint32 conversions are x86-64's (through a 64-bit integer), so the -1
+// column of a bilinear footprint at the left edge wraps to 2^32 - 1 and is
+// clamped to the last column, as the reference's x86-64 build does.
+struct Sampler {
+  const uint8_t* px = nullptr;
+  uint32_t w = 0, h = 0;
+  int filter = HIPR_FILTER_NEAREST, tx = HIPR_TILE_DECAL, ty = HIPR_TILE_DECAL;
+
+  static uint32_t ToU32(float f) { return (uint32_t)(int64_t)f; }
+  static float Mod(float x, float y) { return x - y * std::floor(x / y); }
+  static float Fract(float x) { return x - std::floor(x); }
+  void SampleXY(float fx, float fy, float out[4]) const {
+    uint32_t ix = std::min(ToU32(fx), w - 1), iy = std::min(ToU32(fy), h - 1);
+    const uint8_t* p = px + ((size_t)iy * w + ix) * 4;
+    for (int c = 0; c < 4; c++) out[c] = p[c] / 255.f;  // Color4fFromColor
+  }
+  uint32_t Color(float u, float v) const {  // BitmapSampler::GetColor, ARGB
+    if ((tx == HIPR_TILE_DECAL && (u < 0.0 || u >= 1.0)) ||
+        (ty == HIPR_TILE_DECAL && (v < 0.0 || v >= 1.0)))
+      return 0;
+    u = RemapFloatTile(u, tx);
+    v = RemapFloatTile(v, ty);
+    float c[4];
+    if (filter == HIPR_FILTER_NEAREST) {
+      SampleXY(u * w, v * h, c);
+      return Color4fToColor(c);
+    }
+    float x = u * w, y = v * h;
+    float i0 = std::floor(x - 0.5f), j0 = std::floor(y - 0.5f);
+    if (tx == HIPR_TILE_REPEAT) i0 = Mod(i0, (float)w);
+    if (ty == HIPR_TILE_REPEAT) j0 = Mod(j0, (float)h);
+    float i1 = i0 + 1.0f, j1 = j0 + 1.0f;
+    if (tx == HIPR_TILE_REPEAT) i1 = Mod(i1, (float)w);
+    if (ty == HIPR_TILE_REPEAT) j1 = Mod(j1, (float)h);
+    float a = Fract(x - 0.5f), b = Fract(y - 0.5f);
+    float t00[4], t10[4], t01[4], t11[4];
+    SampleXY(i0, j0, t00);
+    SampleXY(i1, j0, t10);
+    SampleXY(i0, j1, t01);
+    SampleXY(i1, j1, t11);
+    float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
+    for (int k = 0; k < 4; k++)
+      c[k] = ((w00 * t00[k] + w10 * t10[k]) + w01 * t01[k]) + w11 * t11[k];
+    return Color4fToColor(c);
+  }
+};
+
 struct Brush {
-  int kind = 0;  // 0 solid, 1 gradient, 2 image
+  int kind = 0;  // 0 solid, 1 gradient, 2 image (blurred layer), 3 pixmap shader
+  Sampler pix;           // kind 3
+  bool pix_unpremul = false;
+  Aff pix_ptu;
   uint32_t solid = 0;
   GradientBrush grad;
   const uint8_t* img = nullptr;
@@ -539,6 +590,12 @@ struct Brush {
   uint32_t Color(int x, int y) const {
     if (kind == 0) return solid;
     if (kind == 1) return grad.Color(x, y);
+    if (kind == 3) {  // PixmapBrush::CalculateColor (sw_span_brush.cc:569-579)
+      float u, v;
+      MapPoint(pix_ptu, x + 0.5f, y + 0.5f, &u, &v);
+      uint32_t c = pix.Color(u, v);
+      return pix_unpremul ? ColorToPMColor(c) : c;
+    }
     float u, v;
     MapPoint(img_ptu, x + 0.5f, y + 0.5f, &u, &v);
     if (u < 0.0 || u >= 1.0 || v < 0.0 || v >= 1.0) return 0;
@@ -685,6 +742,38 @@ uint32_t FilterPM(const Brush& br, uint32_t c) {
 }
 
 uint32_t Brush::Filter(uint32_t c) const { return FilterPM(*this, c); }
+
+// GenerateBrush's pixmap branch (sw_canvas.cc:755-788): the image shader of
+// draw p replaces the brush colour; points to unit = Scale(1/w, 1/h) .
+// inverse(local) . inverse(CTM); global alpha Paint::GetAlphaF (the stroke
+// colour's alpha for kStroke_Style, else the fill colour's, paint.cc:96-98).
+void SetImageShader(const hipr_scene* s, int p, const Aff& ctm, Brush* br) {
+  if (!s->image_shaders || s->image_shaders[p].image < 0) return;
+  const hipr_image_shader& sh = s->image_shaders[p];
+  const hipr_image& im = s->images[sh.image];
+  br->kind = 3;
+  br->pix.px = im.pixels;
+  br->pix.w = (uint32_t)im.width;
+  br->pix.h = (uint32_t)im.height;
+  br->pix.filter = sh.filter;
+  br->pix.tx = sh.tile_x;
+  br->pix.ty = sh.tile_y;
+  br->pix_unpremul = im.alpha_type == HIPR_ALPHA_UNPREMUL;
+  Aff local;
+  local.sx = sh.local_matrix[0];
+  local.kx = sh.local_matrix[1];
+  local.tx = sh.local_matrix[2];
+  local.ky = sh.local_matrix[3];
+  local.sy = sh.local_matrix[4];
+  local.ty = sh.local_matrix[5];
+  Aff inv, l2l;
+  if (!Invert(local, &inv)) inv = Aff();
+  if (!Invert(ctm, &l2l)) l2l = Aff();
+  br->pix_ptu = Mul(Mul(Scale(1.f / (uint32_t)im.width, 1.f / (uint32_t)im.height), inv), l2l);
+  const hipr_paint& pt = s->paints[p];
+  float alpha = pt.style == HIPR_STYLE_STROKE ? pt.stroke_color[3] : pt.color[3];
+  br->global_alpha = (uint8_t)(255 * alpha);
+}
 
 void SetColorFilter(const hipr_scene* s, int p, Brush* br) {
   if (!s->color_filters) return;
@@ -881,9 +970,12 @@ void BlurredLayer(const hipr_scene* s, int p, std::vector<uint8_t>* blurred,
   work.blur_radius = 0;
   Ops ops;
   ShapeOps(s, path, work, Translate(-l, -tp), &ops);
-  // the layer is drawn with the paint's colour filter (HandleFilter keeps it
-  // in work_paint, sw_canvas.cc:797-826)
-  for (auto& op : ops) SetColorFilter(s, p, &op->br);
+  // the layer is drawn with the paint's colour filter and shader
+  // (HandleFilter keeps them in work_paint, sw_canvas.cc:797-826)
+  for (auto& op : ops) {
+    SetColorFilter(s, p, &op->br);
+    SetImageShader(s, p, Translate(-l, -tp), &op->br);
+  }
   DrawOps(ops, &lt);
   blurred->assign((size_t)W * H * 4, 0);
   int radius = (int)std::round(r);
@@ -902,7 +994,12 @@ void DrawEntryOpsUnfiltered(const hipr_scene* s, int p, int tw, int th, Ops* ops
 void DrawEntryOps(const hipr_scene* s, int p, int tw, int th, Ops* ops) {
   const size_t first = ops->size();
   DrawEntryOpsUnfiltered(s, p, tw, th, ops);
-  for (size_t i = first; i < ops->size(); i++) SetColorFilter(s, p, &(*ops)[i]->br);
+  for (size_t i = first; i < ops->size(); i++) {
+    Brush* br = &(*ops)[i]->br;
+    SetColorFilter(s, p, br);
+    // the blurred layer's composite samples the layer instead of the shader
+    if (br->kind != 2) SetImageShader(s, p, DrawMatrix(s, p), br);
+  }
 }
 
 void DrawEntryOpsUnfiltered(const hipr_scene* s, int p, int tw, int th, Ops* ops) {

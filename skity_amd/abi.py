@@ -83,6 +83,28 @@ class ColorFilter(ctypes.Structure):
     ]
 
 
+FILTER_NEAREST, FILTER_LINEAR = 0, 1
+
+
+class Image(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("alpha_type", ctypes.c_int32),
+        ("pixels", ctypes.POINTER(ctypes.c_uint8)),
+    ]
+
+
+class ImageShader(ctypes.Structure):
+    _fields_ = [
+        ("image", ctypes.c_int32),
+        ("filter", ctypes.c_int32),
+        ("tile_x", ctypes.c_int32),
+        ("tile_y", ctypes.c_int32),
+        ("local_matrix", ctypes.c_float * 6),
+    ]
+
+
 class Scene(ctypes.Structure):
     _fields_ = [
         ("n_paths", ctypes.c_int32),
@@ -99,6 +121,9 @@ class Scene(ctypes.Structure):
         ("matrices", ctypes.POINTER(ctypes.c_float)),
         ("draw_kinds", ctypes.POINTER(ctypes.c_uint8)),
         ("color_filters", ctypes.POINTER(ColorFilter)),
+        ("images", ctypes.POINTER(Image)),
+        ("n_images", ctypes.c_int32),
+        ("image_shaders", ctypes.POINTER(ImageShader)),
     ]
 
 

@@ -162,6 +162,7 @@ class Paint:
         self.p.blend = abi.BLEND_SRC_OVER
         self.gradient = None
         self.color_filter = None
+        self.image_shader = None
 
     def set_color(self, rgba):
         # Paint::SetColor sets both the fill and the stroke colour
@@ -214,6 +215,15 @@ class Paint:
         """A Shader::MakeLinear/Radial/Sweep/TwoPointConical gradient
         (abi.Gradient)."""
         self.gradient = g
+        return self
+
+    def set_image_shader(self, pixels, alpha_type=abi.ALPHA_PREMUL, filter=abi.FILTER_NEAREST,
+                         tile_x=abi.TILE_CLAMP, tile_y=abi.TILE_CLAMP,
+                        local_matrix=(1, 0, 0, 0, 1, 0)):
+        """Shader::MakeShader(Image::MakeImage(pixmap), sampling, tile_x,
+        tile_y, local_matrix): pixels is an (H, W, 4) uint8 RGBA array."""
+        self.image_shader = (np.ascontiguousarray(pixels, np.uint8), alpha_type, filter,
+                             tile_x, tile_y, tuple(float(v) for v in local_matrix))
         return self
 
     def set_color_filter(self, cf):
@@ -336,6 +346,28 @@ class PackedScene:
         kinds = np.array([d[3] for d in draws] or [0], np.uint8)
         self.cfs = (abi.ColorFilter * max(n, 1))()
         any_cf = False
+        self.shaders = (abi.ImageShader * max(n, 1))()
+        images, image_index = [], {}
+        for i, (_, paint, _, _) in enumerate(draws):
+            self.shaders[i].image = -1
+            ish = getattr(paint, "image_shader", None)
+            if ish is None:
+                continue
+            px, alpha_type, filt, tx, ty, local = ish
+            key = (id(px), alpha_type)
+            if key not in image_index:
+                image_index[key] = len(images)
+                images.append((px, alpha_type))
+            sh = self.shaders[i]
+            sh.image, sh.filter, sh.tile_x, sh.tile_y = image_index[key], filt, tx, ty
+            for k, v in enumerate(local):
+                sh.local_matrix[k] = v
+        self.image_arrays = [px for px, _ in images]
+        self.images = (abi.Image * max(len(images), 1))()
+        for k, (px, alpha_type) in enumerate(images):
+            self.images[k].height, self.images[k].width = px.shape[0], px.shape[1]
+            self.images[k].alpha_type = alpha_type
+            self.images[k].pixels = px.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         for i, (path, paint, _, _) in enumerate(draws):
             if getattr(paint, "color_filter", None) is not None:
                 ctypes.memmove(ctypes.byref(self.cfs[i]), ctypes.byref(paint.color_filter),
@@ -380,6 +412,10 @@ class PackedScene:
             s.draw_kinds = kinds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         if any_cf:
             s.color_filters = ctypes.cast(self.cfs, ctypes.POINTER(abi.ColorFilter))
+        if images:
+            s.images = ctypes.cast(self.images, ctypes.POINTER(abi.Image))
+            s.n_images = len(images)
+            s.image_shaders = ctypes.cast(self.shaders, ctypes.POINTER(abi.ImageShader))
         self._s = s
         self.scene = ctypes.pointer(s)
 

@@ -18,7 +18,19 @@
 
 namespace hipr {
 
-enum PaintKind : int32_t { kPaintSolid = 0, kPaintGradient = 1, kPaintImage = 2 };
+enum PaintKind : int32_t {
+  kPaintSolid = 0,
+  kPaintGradient = 1,
+  kPaintImage = 2,   // a blurred layer's composite (nearest, decal)
+  kPaintPixmap = 3   // a pixmap shader
+};
+
+// An image of a pixmap shader: RGBA8 rows at `offset` in the scene's image
+// buffer.
+struct ImageDev {
+  int32_t width, height;
+  int64_t offset;
+};
 
 // Two-point conical gradient set-up (ConicalGradientColorBrush::OnPreBrush,
 // sw_span_brush.cc:407-440), computed on the host.
@@ -91,6 +103,9 @@ struct PaintDev {
   int32_t blend;    // HIPR_BLEND_* of the paint
   int32_t L, T, R, B;  // kJobRect: the rectangle of full coverage
   float im[6];      // image: device point -> unit square of the layer
+  // pixmap shader: image, HIPR_FILTER_*, HIPR_TILE_* per axis, whether the
+  // image is unpremultiplied; im[] maps device points to its unit square
+  int32_t pix_img, pix_filter, pix_tx, pix_ty, pix_unpremul;
   // the paint's colour filter (HIPR_COLOR_FILTER_*)
   int32_t cf_type;
   uint32_t cf_pm;   // BLEND: the filter colour, premultiplied, RGBA byte order
@@ -383,6 +398,8 @@ struct CompositeParams {
   const PaintDev* paints;
   const GradDev* grads;
   const uint8_t* srgb;  // the sRGB filter tables (512 entries)
+  const ImageDev* images;       // pixmap shader images
+  const uint8_t* image_pixels;
   // optional: [0] += pixel blends performed, [1] += target pixels touched
   unsigned long long* counters;
 };
@@ -419,6 +436,59 @@ __device__ uint32_t ImageColor(const PaintDev& pd, const Canvas& layer, int x, i
     o |= UnitToByte(f) << (8 * ch);
   }
   return o;
+}
+
+// BitmapSampler::GetColor (bitmap_sampler.cc:26-108) then
+// PixmapBrush::CalculateColor (sw_span_brush.cc:569-579).  Float -> uint32
+// conversions go through a 64-bit integer as on x86-64, so texel -1 of a
+// bilinear footprint becomes 2^32 - 1 and is clamped to the last column.
+__device__ __forceinline__ void SampleTexel(const uint32_t* __restrict__ px, uint32_t w,
+                                            uint32_t h, float fx, float fy, float out[4]) {
+  uint32_t ix = (uint32_t)(int64_t)fx, iy = (uint32_t)(int64_t)fy;
+  ix = ix > w - 1 ? w - 1 : ix;
+  iy = iy > h - 1 ? h - 1 : iy;
+  const uint32_t c = px[(int64_t)iy * w + ix];
+#pragma unroll
+  for (int k = 0; k < 4; k++) out[k] = (float)((c >> (8 * k)) & 0xFF) / 255.f;
+}
+
+__device__ uint32_t PixmapColor(const PaintDev& pd, const ImageDev& im,
+                                const uint8_t* __restrict__ pixels, int x, int y) {
+  const float cx = x + 0.5f, cy = y + 0.5f;
+  float u = cx * pd.im[0] + cy * pd.im[1] + pd.im[2];
+  float v = cx * pd.im[3] + cy * pd.im[4] + pd.im[5];
+  if ((pd.pix_tx == HIPR_TILE_DECAL && (u < 0.0f || u >= 1.0f)) ||
+      (pd.pix_ty == HIPR_TILE_DECAL && (v < 0.0f || v >= 1.0f)))
+    return 0;
+  u = RemapTile(u, pd.pix_tx);
+  v = RemapTile(v, pd.pix_ty);
+  const uint32_t w = (uint32_t)im.width, h = (uint32_t)im.height;
+  const uint32_t* px = (const uint32_t*)(pixels + im.offset);
+  float c[4];
+  if (pd.pix_filter == HIPR_FILTER_NEAREST) {
+    SampleTexel(px, w, h, u * w, v * h, c);
+  } else {
+    const float fx = u * w, fy = v * h;
+    float i0 = floorf(fx - 0.5f), j0 = floorf(fy - 0.5f);
+    auto mod = [](float a, float b) { return a - b * floorf(a / b); };
+    if (pd.pix_tx == HIPR_TILE_REPEAT) i0 = mod(i0, (float)w);
+    if (pd.pix_ty == HIPR_TILE_REPEAT) j0 = mod(j0, (float)h);
+    float i1 = i0 + 1.0f, j1 = j0 + 1.0f;
+    if (pd.pix_tx == HIPR_TILE_REPEAT) i1 = mod(i1, (float)w);
+    if (pd.pix_ty == HIPR_TILE_REPEAT) j1 = mod(j1, (float)h);
+    const float a = (fx - 0.5f) - floorf(fx - 0.5f), b = (fy - 0.5f) - floorf(fy - 0.5f);
+    float t00[4], t10[4], t01[4], t11[4];
+    SampleTexel(px, w, h, i0, j0, t00);
+    SampleTexel(px, w, h, i1, j0, t10);
+    SampleTexel(px, w, h, i0, j1, t01);
+    SampleTexel(px, w, h, i1, j1, t11);
+    const float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      c[k] = ((w00 * t00[k] + w10 * t10[k]) + w01 * t01[k]) + w11 * t11[k];
+  }
+  uint32_t o = Pack(UnitToByte(c[0]), UnitToByte(c[1]), UnitToByte(c[2]), UnitToByte(c[3]));
+  return pd.pix_unpremul ? PremulWord(o) : o;
 }
 
 __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
@@ -544,9 +614,12 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
           bool d0 = three ? ((direct >> (8 * q)) & 0xFF) != 0 : c0 != 0;
           if (d0 || c1) {
-            uint32_t color = pd.kind == kPaintSolid ? pd.color
-                             : g ? GradientColor(*g, stops, px0 + q, py)
-                                 : ImageColor(pd, P.canvases[pd.layer], px0 + q, py);
+            uint32_t color =
+                pd.kind == kPaintSolid    ? pd.color
+                : g                       ? GradientColor(*g, stops, px0 + q, py)
+                : pd.kind == kPaintPixmap ? PixmapColor(pd, P.images[pd.pix_img],
+                                                        P.image_pixels, px0 + q, py)
+                                          : ImageColor(pd, P.canvases[pd.layer], px0 + q, py);
             // Brush: cover & global alpha (sw_span_brush.cc:101)
             if (d0) pix[q] = ApplyCover(pix[q], color, c0 & ga, mode, premul, pd, P.srgb);
             if (c1) pix[q] = ApplyCover(pix[q], color, c1 & ga, mode, premul, pd, P.srgb);

@@ -394,9 +394,12 @@ struct hipr_prepared_s {
   std::vector<int32_t> stroke_jobs;  // job indices that need an outline
   std::vector<int32_t> stroke_quads; // per stroke job: segments of its quad path
   int64_t layer_bytes = 0;
+  std::vector<ImageDev> images;      // pixmap shader images (offsets into image_pixels)
+  int64_t image_bytes = 0;
   int32_t n_targets = 1;             // canvases [0, n_targets) are draw targets
   int64_t n_verbs = 0, n_pts = 0, n_weights = 0;
   DevBuf verbs, pts, weights, scene_paints, d_jobs, d_paints, d_grads, d_walk, d_stroke;
+  DevBuf d_images, image_pixels;
   // Per job, the cost that bounds a sub-plan: source points (edges grow
   // with them) and the tiles of its bounding box (coverage planes).
   std::vector<int64_t> job_points, job_tiles;
@@ -411,7 +414,8 @@ struct hipr_prepared_s {
     DevBuf* own[] = {&d_jobs, &d_walk, &d_stroke};
     for (DevBuf* x : own) x->Free();
     if (!borrowed) {
-      DevBuf* b[] = {&verbs, &pts, &weights, &scene_paints, &d_paints, &d_grads};
+      DevBuf* b[] = {&verbs, &pts, &weights, &scene_paints, &d_paints, &d_grads,
+                     &d_images, &image_pixels};
       for (DevBuf* x : b) x->Free();
     }
   }
@@ -513,6 +517,14 @@ hipr_result ValidateScene(const hipr_scene* s) {
   if ((!s->verbs && s->verb_offsets[s->n_paths] != 0) ||
       (!s->points && s->point_offsets[s->n_paths] != 0))
     return HIPR_ERROR_INVALID_ARGUMENT;
+  if (s->n_images < 0 || (s->n_images > 0 && !s->images)) return HIPR_ERROR_INVALID_ARGUMENT;
+  for (int i = 0; s->images && i < s->n_images; i++) {
+    const hipr_image& im = s->images[i];
+    if (im.width <= 0 || im.height <= 0 || !im.pixels ||
+        (int64_t)im.width * im.height > (int64_t(1) << 28) ||
+        im.alpha_type < HIPR_ALPHA_OPAQUE || im.alpha_type > HIPR_ALPHA_UNPREMUL)
+      return HIPR_ERROR_INVALID_ARGUMENT;
+  }
   for (int p = 0; p < s->n_paths; p++) {
     int v0 = s->verb_offsets[p], v1 = s->verb_offsets[p + 1];
     if (v1 < v0) return HIPR_ERROR_INVALID_ARGUMENT;
@@ -583,6 +595,17 @@ hipr_result ValidateScene(const hipr_scene* s) {
     if (kind == HIPR_DRAW_PATH && pt.blur_radius > 0 &&
         (pt.blur_style < HIPR_BLUR_NORMAL || pt.blur_style > HIPR_BLUR_INNER))
       return HIPR_ERROR_INVALID_ARGUMENT;
+    if (s->image_shaders && s->image_shaders[p].image >= 0) {
+      const hipr_image_shader& sh = s->image_shaders[p];
+      if (!s->images || sh.image >= s->n_images || pt.gradient >= 0)
+        return HIPR_ERROR_INVALID_ARGUMENT;
+      if (sh.filter < HIPR_FILTER_NEAREST || sh.filter > HIPR_FILTER_LINEAR ||
+          sh.tile_x < HIPR_TILE_CLAMP || sh.tile_x > HIPR_TILE_DECAL ||
+          sh.tile_y < HIPR_TILE_CLAMP || sh.tile_y > HIPR_TILE_DECAL)
+        return HIPR_ERROR_INVALID_ARGUMENT;
+      for (int k = 0; k < 6; k++)
+        if (!std::isfinite(sh.local_matrix[k])) return HIPR_ERROR_INVALID_ARGUMENT;
+    }
     if (s->color_filters) {
       const hipr_color_filter& f = s->color_filters[p];
       if (f.type < HIPR_COLOR_FILTER_NONE || f.type > HIPR_COLOR_FILTER_COMPOSE)
@@ -657,6 +680,42 @@ void ApplyColorFilter(const hipr_scene* s, int p, PaintDev* pd) {
   for (int i = 0; i < 20; i++) pd->cf_m[i] = (int16_t)(f.matrix[i] * 255);
 }
 
+// The draw's pixmap shader on a device paint record (GenerateBrush,
+// sw_canvas.cc:755-788): points to unit = Scale(1/w, 1/h) . inverse(local)
+// . inverse(CTM); global alpha Paint::GetAlphaF (the stroke colour's alpha
+// for kStroke_Style, else the fill colour's, paint.cc:96-98).
+void ApplyImageShader(const hipr_scene* s, int p, const Aff& ctm, PaintDev* pd) {
+  if (!s->image_shaders || s->image_shaders[p].image < 0) return;
+  const hipr_image_shader& sh = s->image_shaders[p];
+  const hipr_image& im = s->images[sh.image];
+  const hipr_paint& pt = s->paints[p];
+  pd->kind = kPaintPixmap;
+  pd->pix_img = sh.image;
+  pd->pix_filter = sh.filter;
+  pd->pix_tx = sh.tile_x;
+  pd->pix_ty = sh.tile_y;
+  pd->pix_unpremul = im.alpha_type == HIPR_ALPHA_UNPREMUL;
+  Aff local;
+  local.sx = sh.local_matrix[0];
+  local.kx = sh.local_matrix[1];
+  local.tx = sh.local_matrix[2];
+  local.ky = sh.local_matrix[3];
+  local.sy = sh.local_matrix[4];
+  local.ty = sh.local_matrix[5];
+  Aff inv, l2l;
+  if (!Invert(local, &inv)) inv = Aff();
+  if (!Invert(ctm, &l2l)) l2l = Aff();
+  Aff m = Mul(Mul(Scale(1.f / (uint32_t)im.width, 1.f / (uint32_t)im.height), inv), l2l);
+  pd->im[0] = m.sx;
+  pd->im[1] = m.kx;
+  pd->im[2] = m.tx;
+  pd->im[3] = m.ky;
+  pd->im[4] = m.sy;
+  pd->im[5] = m.ty;
+  const float alpha = pt.style == HIPR_STYLE_STROKE ? pt.stroke_color[3] : pt.color[3];
+  pd->ga = (int32_t)(uint8_t)(255 * alpha);
+}
+
 // A filter that can give a transparent source a colour makes the draw's
 // zero-cover spans visible (blend and matrix filters; the sRGB tables map 0
 // to 0 and compose passes colours through).
@@ -729,6 +788,7 @@ void AddShapeJobs(const hipr_scene* s, int p, int canvas, const Aff& ctm,
     jb.paint = (int32_t)pl->paints.size();
     pl->paints.push_back(SolidOrGradient(s, pt, ctm, stroke, &pl->grads));
     ApplyColorFilter(s, p, &pl->paints.back());
+    ApplyImageShader(s, p, ctm, &pl->paints.back());
     out->push_back(jb);
   };
   if (pt.style == HIPR_STYLE_STROKE_THEN_FILL) {
@@ -864,6 +924,7 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
       pd.R = INT32_MAX;
       pd.B = INT32_MAX;
       ApplyColorFilter(s, p, &pd);
+      ApplyImageShader(s, p, ctm, &pd);
       canvas_jobs[0].push_back(RectJob(p, 0, (int32_t)pl->paints.size()));
       pl->paints.push_back(pd);
       continue;
@@ -978,6 +1039,17 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
     if (pl->jobs[i].kind == kJobStroke) pl->stroke_jobs.push_back((int32_t)i);
   }
   JobCosts(s, pl);
+  // pixmap shader images, each at a 256-byte aligned offset
+  pl->images.clear();
+  pl->image_bytes = 0;
+  for (int i = 0; s->images && i < s->n_images; i++) {
+    ImageDev d{};
+    d.width = s->images[i].width;
+    d.height = s->images[i].height;
+    d.offset = pl->image_bytes;
+    pl->image_bytes += ((int64_t)d.width * d.height * 4 + 255) & ~(int64_t)255;
+    pl->images.push_back(d);
+  }
 }
 
 void SetTiles(std::vector<Canvas>* cvs) {
@@ -1984,6 +2056,8 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   cp.paints = pl->d_paints.as<PaintDev>();
   cp.grads = pl->d_grads.as<GradDev>();
   cp.srgb = ctx->srgb.as<uint8_t>();
+  cp.images = pl->d_images.as<ImageDev>();
+  cp.image_pixels = pl->image_pixels.as<uint8_t>();
   int main_tiles = 0;
   for (int t = 0; t < n_targets; t++) main_tiles += f.canvases[t].tiles_x * f.canvases[t].tiles_y;
   if ((int)f.canvases.size() > n_targets) {
@@ -2193,6 +2267,10 @@ hipr_result BuildChunks(hipr_prepared_s* pl, int64_t budget_points) {
     sub->scene_paints = pl->scene_paints;
     sub->d_paints = pl->d_paints;
     sub->d_grads = pl->d_grads;
+    sub->d_images = pl->d_images;
+    sub->image_pixels = pl->image_pixels;
+    sub->images = pl->images;
+    sub->image_bytes = pl->image_bytes;
     sub->paints = pl->paints;
     sub->n_verbs = pl->n_verbs;
     sub->n_pts = pl->n_pts;
@@ -2330,9 +2408,25 @@ hipr_result Prepare(hipr_context ctx, const hipr_scene* scene, hipr_prepared_s**
         (r = Upload(&pl->d_paints, pl->paints.data(), pl->paints.size(), st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->d_grads, pl->grads.data(), pl->grads.size(), st)) != HIPR_SUCCESS ||
         (r = Upload(&pl->d_walk, pl->walk_jobs.data(), pl->walk_jobs.size(), st)) != HIPR_SUCCESS ||
-        (r = Upload(&pl->d_stroke, pl->stroke_jobs.data(), pl->stroke_jobs.size(), st)) != HIPR_SUCCESS) {
+        (r = Upload(&pl->d_stroke, pl->stroke_jobs.data(), pl->stroke_jobs.size(), st)) != HIPR_SUCCESS ||
+        (r = Upload(&pl->d_images, pl->images.data(), pl->images.size(), st)) != HIPR_SUCCESS) {
       delete pl;
       return r;
+    }
+    if (pl->image_bytes > 0) {
+      if (pl->image_pixels.Ensure(pl->image_bytes) != hipSuccess) {
+        delete pl;
+        return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+      }
+      for (size_t i = 0; i < pl->images.size(); i++) {
+        const ImageDev& d = pl->images[i];
+        if (hipMemcpyAsync(pl->image_pixels.as<uint8_t>() + d.offset, scene->images[i].pixels,
+                           (size_t)d.width * d.height * 4, hipMemcpyHostToDevice,
+                           st) != hipSuccess) {
+          delete pl;
+          return HIPR_ERROR_DEVICE;
+        }
+      }
     }
     if (hipStreamSynchronize(st) != hipSuccess) {
       delete pl;
@@ -2377,7 +2471,9 @@ hipr_result MergeBatch(hipr_batch b) {
   }
   m.canvases.assign(T + n_layers, Canvas{});
   int64_t voff = 0, poff = 0, woff = 0, lbytes = 0;
-  int path_off = 0, paint_off = 0, grad_off = 0, job_off = 0;
+  int path_off = 0, paint_off = 0, grad_off = 0, job_off = 0, image_off = 0;
+  int64_t image_byte_off = 0;
+  m.images.clear();
   for (int p = 0; p < T; p++) {
     const hipr_prepared_s* s = b->pairs[p].second;
     auto canv = [&](int c) { return c == 0 ? p : layer_base[p] + c - 1; };
@@ -2402,7 +2498,12 @@ hipr_result MergeBatch(hipr_batch b) {
       PaintDev q = src;
       if (q.kind == kPaintGradient) q.grad += grad_off;
       if (q.kind == kPaintImage) q.layer = canv(q.layer);
+      if (q.kind == kPaintPixmap) q.pix_img += image_off;
       m.paints.push_back(q);
+    }
+    for (ImageDev d : s->images) {
+      d.offset += image_byte_off;
+      m.images.push_back(d);
     }
     m.grads.insert(m.grads.end(), s->grads.begin(), s->grads.end());
     for (size_t c = 0; c < s->canvases.size(); c++) {
@@ -2431,6 +2532,21 @@ hipr_result MergeBatch(hipr_batch b) {
     paint_off += (int)s->paints.size();
     grad_off += (int)s->grads.size();
     job_off += (int)s->jobs.size();
+    image_off += (int)s->images.size();
+    image_byte_off += s->image_bytes;
+  }
+  m.image_bytes = image_byte_off;
+  if (image_byte_off > 0) {
+    if (m.image_pixels.Ensure(image_byte_off) != hipSuccess)
+      return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+    int64_t o = 0;
+    for (int p = 0; p < T; p++) {
+      const hipr_prepared_s* s = b->pairs[p].second;
+      if (s->image_bytes > 0)
+        HIP_TRY(hipMemcpyAsync(m.image_pixels.as<uint8_t>() + o, s->image_pixels.p, s->image_bytes,
+                               hipMemcpyDeviceToDevice, st));
+      o += s->image_bytes;
+    }
   }
   m.n_paths = path_off;
   m.layer_bytes = lbytes;
@@ -2467,6 +2583,7 @@ hipr_result MergeBatch(hipr_batch b) {
   if ((r = Upload(&m.d_jobs, m.jobs.data(), m.jobs.size(), st)) != HIPR_SUCCESS ||
       (r = Upload(&m.d_paints, m.paints.data(), m.paints.size(), st)) != HIPR_SUCCESS ||
       (r = Upload(&m.d_grads, m.grads.data(), m.grads.size(), st)) != HIPR_SUCCESS ||
+      (r = Upload(&m.d_images, m.images.data(), m.images.size(), st)) != HIPR_SUCCESS ||
       (r = Upload(&m.d_walk, m.walk_jobs.data(), m.walk_jobs.size(), st)) != HIPR_SUCCESS ||
       (r = Upload(&m.d_stroke, m.stroke_jobs.data(), m.stroke_jobs.size(), st)) != HIPR_SUCCESS)
     return r;

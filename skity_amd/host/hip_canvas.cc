@@ -176,6 +176,33 @@ std::shared_ptr<Shader> Shader::MakeSweep(float cx, float cy, float start_angle,
   return s;
 }
 
+std::shared_ptr<Image> Image::MakeImage(const uint8_t* rgba, int width, int height,
+                                        int alpha_type) {
+  if (!rgba || width <= 0 || height <= 0) return nullptr;
+  auto im = std::make_shared<Image>();
+  im->width_ = width;
+  im->height_ = height;
+  im->alpha_type_ = alpha_type;
+  im->pixels_.assign(rgba, rgba + (size_t)width * height * 4);
+  return im;
+}
+
+std::shared_ptr<Shader> Shader::MakeShader(std::shared_ptr<Image> image,
+                                           const SamplingOptions& sampling, TileMode tile_x,
+                                           TileMode tile_y, const Matrix* local) {
+  if (!image) return nullptr;
+  auto s = std::make_shared<Shader>();
+  s->image_ = std::move(image);
+  s->image_info_.image = -1;
+  s->image_info_.filter = static_cast<int32_t>(sampling.filter);
+  s->image_info_.tile_x = static_cast<int32_t>(tile_x);
+  s->image_info_.tile_y = static_cast<int32_t>(tile_y);
+  const Matrix m = local ? *local : Matrix{};
+  const float lm[6] = {m.sx, m.kx, m.tx, m.ky, m.sy, m.ty};
+  std::memcpy(s->image_info_.local_matrix, lm, sizeof(lm));
+  return s;
+}
+
 // Shader::MakeTwoPointConical (src/effect/shader.cc:50-85).
 std::shared_ptr<Shader> Shader::MakeTwoPointConical(const Point& start, float start_radius,
                                                     const Point& end, float end_radius,
@@ -408,6 +435,9 @@ int HIPCanvas::Flush() {
   std::vector<float> matrices(6 * (size_t)n);
   std::vector<uint8_t> kinds(n);
   std::vector<hipr_color_filter> filters(n);
+  std::vector<hipr_image_shader> shaders(n);
+  std::vector<hipr_image> images;
+  std::vector<const Image*> image_of;
   bool any_matrix = false, any_paint = false, any_filter = false;
   for (int32_t i = 0; i < n; i++) {
     const Matrix& m = draws_[i].matrix;
@@ -429,9 +459,23 @@ int HIPCanvas::Flush() {
     fills[i] = p.GetFillType() == Path::PathFillType::kEvenOdd ? HIPR_FILL_EVEN_ODD
                                                                : HIPR_FILL_WINDING;
     int32_t gi = -1;
-    if (draws_[i].paint.shader()) {
-      gi = (int32_t)grads.size();
-      grads.push_back(draws_[i].paint.shader()->info());
+    shaders[i].image = -1;
+    if (const auto& sh = draws_[i].paint.shader()) {
+      if (sh->image()) {
+        // a pixmap shader: one hipr_image per distinct Image
+        const Image* im = sh->image().get();
+        int32_t k = 0;
+        while (k < (int32_t)image_of.size() && image_of[k] != im) k++;
+        if (k == (int32_t)image_of.size()) {
+          image_of.push_back(im);
+          images.push_back(hipr_image{im->Width(), im->Height(), im->AlphaType(), im->Pixels()});
+        }
+        shaders[i] = sh->image_info();
+        shaders[i].image = k;
+      } else {
+        gi = (int32_t)grads.size();
+        grads.push_back(sh->info());
+      }
     }
     paints[i] = draws_[i].paint.Pack(gi);
     if (const auto& cf = draws_[i].paint.GetColorFilter()) {
@@ -454,6 +498,11 @@ int HIPCanvas::Flush() {
   scene.matrices = any_matrix ? matrices.data() : nullptr;
   scene.draw_kinds = any_paint ? kinds.data() : nullptr;
   scene.color_filters = any_filter ? filters.data() : nullptr;
+  if (!images.empty()) {
+    scene.images = images.data();
+    scene.n_images = (int32_t)images.size();
+    scene.image_shaders = shaders.data();
+  }
   hipr_options opt{};
   opt.mode = HIPR_MODE_EXACT;
   int r = hipr_surface_draw_scene(surface_->surface_, &scene, &opt);

@@ -125,8 +125,35 @@ enum class TileMode { kClamp, kRepeat, kMirror, kDecal };
 
 // Gradient shaders (include/skity/effect/shader.hpp:88-146), premultiply
 // flag 0 (interpolation of unpremultiplied colours).
+// An immutable RGBA8 image (Image::MakeImage of a Pixmap): the pixels are
+// copied; alpha_type is HIPR_ALPHA_PREMUL, HIPR_ALPHA_OPAQUE or
+// HIPR_ALPHA_UNPREMUL.
+class Image {
+ public:
+  static std::shared_ptr<Image> MakeImage(const uint8_t* rgba, int width, int height,
+                                          int alpha_type = HIPR_ALPHA_PREMUL);
+  int Width() const { return width_; }
+  int Height() const { return height_; }
+  int AlphaType() const { return alpha_type_; }
+  const uint8_t* Pixels() const { return pixels_.data(); }
+
+ private:
+  int width_ = 0, height_ = 0, alpha_type_ = HIPR_ALPHA_PREMUL;
+  std::vector<uint8_t> pixels_;
+};
+
+enum class FilterMode { kNearest = HIPR_FILTER_NEAREST, kLinear = HIPR_FILTER_LINEAR };
+struct SamplingOptions {
+  FilterMode filter = FilterMode::kNearest;
+};
+
 class Shader {
  public:
+  // Shader::MakeShader(image, sampling, tile_x, tile_y, local_matrix): a
+  // pixmap shader (include/skity/effect/shader.hpp).
+  static std::shared_ptr<Shader> MakeShader(std::shared_ptr<Image> image,
+                                            const SamplingOptions& sampling, TileMode tile_x,
+                                            TileMode tile_y, const Matrix* local = nullptr);
   static std::shared_ptr<Shader> MakeLinear(const Point pts[2], const Color4f colors[],
                                             const float pos[], int count,
                                             TileMode tile_mode = TileMode::kClamp);
@@ -147,9 +174,15 @@ class Shader {
   // {scale_x, skew_x, trans_x, skew_y, scale_y, trans_y}
   void SetLocalMatrix(const float m[6]);
   const hipr_gradient& info() const { return info_; }
+  // pixmap shaders: the image and the shader record (its image index is set
+  // when a canvas packs it)
+  const std::shared_ptr<Image>& image() const { return image_; }
+  const hipr_image_shader& image_info() const { return image_info_; }
 
  private:
   hipr_gradient info_{};
+  std::shared_ptr<Image> image_;
+  hipr_image_shader image_info_{};
 };
 
 enum class BlurStyle {

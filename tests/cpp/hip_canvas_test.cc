@@ -200,6 +200,35 @@ static void TwoPointConicalFactory(hipr_context) {
   EXPECT_EQ(g->info().radius[1], 20.f);
 }
 
+// A pixmap shader through Shader::MakeShader: an image drawn over a rect of
+// its own size at an integer offset, nearest sampling, reproduces its texels
+// (through the Color4f round trip, exact for 0 and 255).
+static void PixmapShaderDrawsTheImage(hipr_context ctx) {
+  const int w = 4, h = 3;
+  uint8_t px[w * h * 4];
+  for (int i = 0; i < w * h; i++) {
+    px[4 * i + 0] = (i & 1) ? 255 : 0;
+    px[4 * i + 1] = (i & 2) ? 255 : 0;
+    px[4 * i + 2] = (i & 4) ? 255 : 0;
+    px[4 * i + 3] = 255;
+  }
+  auto s = HIPSurface::Make(ctx, 16, 16);
+  Matrix local = Matrix::Translate(5, 6);
+  Paint paint;
+  paint.SetShader(Shader::MakeShader(Image::MakeImage(px, w, h), SamplingOptions{},
+                                     TileMode::kDecal, TileMode::kDecal, &local));
+  s->LockCanvas()->DrawRect(Rect::MakeXYWH(0.f, 0.f, 16.f, 16.f), paint);
+  EXPECT_EQ(s->Flush(), HIPR_SUCCESS);
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      const uint8_t* q = px + 4 * (y * w + x);
+      Color want = (0xFFu << 24) | (q[0] << 16) | (q[1] << 8) | q[2];
+      EXPECT_EQ(s->GetPixel(5 + x, 6 + y), want);
+    }
+  EXPECT_EQ(s->GetPixel(4, 6), Color_TRANSPARENT);
+  EXPECT_EQ(s->GetPixel(5 + w, 6), Color_TRANSPARENT);
+}
+
 int main() {
   hipr_context ctx = nullptr;
   if (hipr_context_create(0, &ctx) != HIPR_SUCCESS) {
@@ -217,7 +246,8 @@ int main() {
                {"Canvas.MatrixMovesAndScalesDraws", MatrixMovesAndScalesDraws},
                {"Canvas.ClearAndDrawColor", ClearAndDrawColor},
                {"HIPCanvas.WidthZeroAndNegativeStrokes", WidthZeroAndNegativeStrokes},
-               {"Shader.TwoPointConicalFactory", TwoPointConicalFactory}};
+               {"Shader.TwoPointConicalFactory", TwoPointConicalFactory},
+               {"Shader.PixmapShaderDrawsTheImage", PixmapShaderDrawsTheImage}};
   for (auto& t : tests) {
     int before = g_failed;
     t.fn(ctx);

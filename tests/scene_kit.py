@@ -234,3 +234,68 @@ def filter_scene(seed, w=200, h=150, n=36):
             paint.set_color_filter(random_color_filter(rng))
         cv.draw_path(random_path(rng, cx, cy, size), paint)
     return cv.scene()
+
+
+def random_image(rng, w, h, alpha_type):
+    """An RGBA8 image: premultiplied (each channel at most alpha) or
+    unpremultiplied, with some opaque and some clear texels."""
+    img = rng.integers(0, 256, (h, w, 4)).astype(np.uint8)
+    a = img[..., 3]
+    a[rng.random((h, w)) < 0.3] = 255
+    a[rng.random((h, w)) < 0.1] = 0
+    if alpha_type == abi.ALPHA_PREMUL:
+        img[..., :3] = (img[..., :3].astype(np.uint32) * a[..., None] // 255).astype(np.uint8)
+    return img
+
+
+def random_local_matrix(rng, cx, cy):
+    """A local matrix that scales (0.25-4x), rotates and translates the
+    image to near (cx, cy)."""
+    s = float(rng.uniform(0.25, 4.0))
+    th = float(rng.uniform(-np.pi, np.pi)) if rng.random() < 0.5 else 0.0
+    c, sn = s * np.cos(th), s * np.sin(th)
+    return (c, -sn, cx + float(rng.uniform(-20, 20)), sn, c, cy + float(rng.uniform(-20, 20)))
+
+
+def pixmap_scene(seed, w=240, h=180, n=30):
+    """Fills, strokes, blurred draws and a DrawPaint with pixmap shaders:
+    nearest and linear sampling, every pair of tile modes, local matrices
+    that scale and rotate, some draws under a canvas matrix, premultiplied
+    and unpremultiplied images (a few shared between draws), some with a
+    colour filter or another blend mode."""
+    rng = np.random.default_rng(seed)
+    cv = Canvas(w, h)
+    pool = []
+    for k in range(4):
+        at = abi.ALPHA_PREMUL if k % 2 == 0 else abi.ALPHA_UNPREMUL
+        iw, ih = int(rng.integers(1, 40)), int(rng.integers(1, 40))
+        pool.append((random_image(rng, iw, ih, at), at))
+    bg = Paint((1, 1, 1, 0.7)).set_image_shader(
+        pool[0][0], pool[0][1], abi.FILTER_LINEAR, abi.TILE_REPEAT, abi.TILE_MIRROR,
+        (2.0, 0.3, 5.0, -0.2, 1.5, 3.0))
+    cv.draw_paint(bg)
+    for i in range(n):
+        cx, cy = float(rng.uniform(20, w - 20)), float(rng.uniform(20, h - 20))
+        size = float(rng.uniform(10, 50))
+        blend = None
+        if i % 7 == 6:
+            blend = int(rng.choice([abi.BLEND_SRC, abi.BLEND_DST_IN, abi.BLEND_XOR, 24]))
+        paint = random_paint(rng, size, cx, cy, allow_blur=True, blend=blend, gradient=0.0)
+        if i % 3 == 0:
+            img, at = pool[int(rng.integers(0, len(pool)))]
+        else:
+            at = abi.ALPHA_PREMUL if rng.random() < 0.5 else abi.ALPHA_UNPREMUL
+            img = random_image(rng, int(rng.integers(1, 48)), int(rng.integers(1, 48)), at)
+        paint.set_image_shader(img, at, int(i % 2), int(rng.integers(0, 4)),
+                               int(rng.integers(0, 4)), random_local_matrix(rng, cx, cy))
+        if i % 5 == 4:
+            paint.set_color_filter(random_color_filter(rng))
+        cv.save()
+        if i % 4 == 1:
+            cv.translate(cx, cy)
+            cv.rotate(float(rng.uniform(-40, 40)))
+            cv.scale(float(rng.uniform(0.6, 1.4)), float(rng.uniform(0.6, 1.4)))
+            cv.translate(-cx, -cy)
+        cv.draw_path(random_path(rng, cx, cy, size), paint)
+        cv.restore()
+    return cv.scene()
