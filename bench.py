@@ -221,6 +221,44 @@ def blur_leg(ctx, torch, steps):
     return out
 
 
+TIGER_SKP = os.path.join(HERE, "tests", "golden", "skp", "tiger.skp")
+
+
+def tiger_leg(ctx, torch, steps, side=4096, scale=4.1):
+    """A real scene: the reference's tiger.skp (305 DrawPath fills and
+    strokes of cubic outlines under two CONCAT44 matrices) read, played back
+    into a display list under Scale(4.1) and drawn on a 4096^2 surface;
+    exact and learned forms.  The timed region is the device frame of the
+    recorded scene (reading and recording the SKP are host work done once)."""
+    from skity_amd import abi, device, picture
+    t0 = time.perf_counter()
+    rec = picture.Recording(open(TIGER_SKP, "rb").read(), side, side,
+                            (scale, 0, -560, 0, scale, 60))
+    if rec.scene is None:
+        raise RuntimeError(f"tiger.skp not drawable: {rec.rc} {rec.error}")
+    record_ms = (time.perf_counter() - t0) * 1e3
+    n = rec.scene.contents.n_paths
+    prep = device.Prepared(ctx, rec.scene)
+    surf = device.Surface(ctx, side, side)
+    out = {"workload": f"tiger.skp (305 paths) played back through a display list, "
+                       f"Scale({scale}) on {side}x{side} RGBA8",
+           "paths": n, "read_and_record_ms": round(record_ms, 2)}
+    imgs = {}
+    for name, mode in (("exact", abi.MODE_EXACT), ("learned", abi.MODE_LEARNED)):
+        wall, _, st = run_form(surf, prep, mode, steps, 2, torch)
+        imgs[name] = surf.read()
+        out[name] = {"paths_per_s": round(n / (wall / steps), 1),
+                     "ms_per_step": round(wall * 1e3 / steps, 3),
+                     "mpix_per_s": round(st["blends"] / (wall / steps) / 1e6, 1),
+                     "fallbacks_per_frame": st["fallbacks_per_frame"]}
+    out["learned_equals_exact"] = bool(np.array_equal(imgs["exact"], imgs["learned"]))
+    out["covered_pixels"] = int((imgs["exact"][..., 3] > 0).sum())
+    surf.close()
+    prep.close()
+    rec.close()
+    return out
+
+
 def batched_leg(ctx, torch, steps, n_canvases=64, workload=4, tune_canvases=8):
     """Batched uneven scenes at the family's stated scale (workload 4): n
     canvases of side 256-4096, each with 100-50,000 closed outlines of
@@ -323,8 +361,8 @@ def main():
                     help="comma list of forms to time: exact, learned")
     ap.add_argument("--batch-steps", type=int, default=1,
                     help="timed frames of the batched uneven leg (each is ~650M segments)")
-    ap.add_argument("--legs", default="batched,blur",
-                    help="extra workloads reported beside the headline: batched, blur")
+    ap.add_argument("--legs", default="batched,blur,tiger",
+                    help="extra workloads reported beside the headline: batched, blur, tiger")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short program the rocprofv3 counter summaries profile")
     ap.add_argument("--probe-workload", type=int, default=WORKLOAD,
@@ -431,6 +469,9 @@ def main():
     if "blur" in leg_names:
         legs["gradients_blur"] = blur_leg(ctx, torch, max(2, args.steps // 4))
         log("blur leg done")
+    if "tiger" in leg_names:
+        legs["tiger_skp"] = tiger_leg(ctx, torch, max(3, args.steps // 2))
+        log("tiger leg done")
     out["legs"] = legs
     if args.cpu_seconds > 0:
         log("cpu baseline")

@@ -23,6 +23,7 @@ BLUR_NORMAL, BLUR_SOLID, BLUR_OUTER, BLUR_INNER = 1, 2, 3, 4
  BLEND_DST_IN, BLEND_SRC_OUT, BLEND_DST_OUT, BLEND_SRC_ATOP, BLEND_DST_ATOP, BLEND_XOR,
  BLEND_PLUS, BLEND_MODULATE, BLEND_SCREEN) = range(15)
 BLEND_SOFT_LIGHT = 21
+BLEND_MULTIPLY = 24
 BLEND_LAST = 28
 ALPHA_OPAQUE, ALPHA_PREMUL, ALPHA_UNPREMUL = 1, 2, 3
 COLOR_RGBA, COLOR_BGRA = 1, 2

@@ -2,7 +2,8 @@
 
   skity_amd/libskity_hip.so    gfx950 kernels + C ABI (hipcc)
   skity_amd/libskity_scene.so  seeded scene generator (g++)
-  skity_amd/libskity_hip_canvas.so  C++ Path/Paint/Canvas front end (g++)
+  skity_amd/libskity_hip_canvas.so  C++ Path/Paint/Canvas front end, SKP reader,
+                                    display lists (g++)
   tests/cpp/hip_canvas_test    C++ known-answer tests of the front end (g++)
   oracle/liboracle.so          CPU oracle, test infrastructure (make)
 """
@@ -62,10 +63,10 @@ def build_scene():
 
 
 def build_host():
-    src = os.path.join(HERE, "host", "hip_canvas.cc")
+    srcs = [os.path.join(HERE, "host", f) for f in ("hip_canvas.cc", "hip_picture.cc")]
     out = os.path.join(HERE, "libskity_hip_canvas.so")
-    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", out + ".tmp", src,
-         "-L" + HERE, "-lskity_hip", "-Wl,-rpath,$ORIGIN"])
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", out + ".tmp"] + srcs +
+        ["-L" + HERE, "-lskity_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)
     test_src = os.path.join(ROOT, "tests", "cpp", "hip_canvas_test.cc")
     test_out = os.path.join(ROOT, "tests", "cpp", "hip_canvas_test")

@@ -368,26 +368,57 @@ Matrix Matrix::operator*(const Matrix& b) const {
 // ---- Canvas -----------------------------------------------------------------
 int HIPCanvas::Save() {
   stack_.push_back(stack_.back());
+  if (recording_) recording_->ops_.push_back({DisplayList::OpKind::kSave, Matrix(), 0});
   return GetSaveCount() - 1;
 }
 
 void HIPCanvas::Restore() {
-  if (stack_.size() > 1) stack_.pop_back();
+  if (stack_.size() > 1) {
+    stack_.pop_back();
+    if (recording_) recording_->ops_.push_back({DisplayList::OpKind::kRestore, Matrix(), 0});
+  }
 }
 
 void HIPCanvas::RestoreToCount(int count) {
   if (count < 1) count = 1;
-  while ((int)stack_.size() > count) stack_.pop_back();
+  while ((int)stack_.size() > count) Restore();
+}
+
+void HIPCanvas::Concat(const Matrix& m) {
+  stack_.back() = stack_.back() * m;
+  if (recording_) recording_->ops_.push_back({DisplayList::OpKind::kConcat, m, 0});
+}
+
+void HIPCanvas::SetMatrix(const Matrix& m) {
+  stack_.back() = m;
+  if (recording_) recording_->ops_.push_back({DisplayList::OpKind::kSetMatrix, m, 0});
+}
+
+void HIPCanvas::MarkUnsupported() {
+  if (recording_) {
+    recording_->ops_.push_back({DisplayList::OpKind::kUnsupported, Matrix(), 0});
+  } else {
+    unsupported_ = true;
+  }
+}
+
+void HIPCanvas::AddDraw(Draw d) {
+  if (recording_) {
+    recording_->ops_.push_back(
+        {DisplayList::OpKind::kDraw, Matrix(), (uint32_t)recording_->draws_.size()});
+    recording_->draws_.push_back(std::move(d));
+    return;
+  }
+  draws_.push_back(std::move(d));
+  if (surface_) surface_->readback_valid_ = false;
 }
 
 void HIPCanvas::DrawPath(const Path& path, const Paint& paint) {
-  draws_.push_back(Draw{path, paint, stack_.back(), HIPR_DRAW_PATH});
-  surface_->readback_valid_ = false;
+  AddDraw(Draw{path, paint, stack_.back(), HIPR_DRAW_PATH});
 }
 
 void HIPCanvas::DrawPaint(const Paint& paint) {
-  draws_.push_back(Draw{Path(), paint, stack_.back(), HIPR_DRAW_PAINT});
-  surface_->readback_valid_ = false;
+  AddDraw(Draw{Path(), paint, stack_.back(), HIPR_DRAW_PAINT});
 }
 
 void HIPCanvas::DrawColor(Color color, BlendMode mode) {
@@ -423,44 +454,41 @@ void HIPCanvas::DrawLine(float x0, float y0, float x1, float y1, const Paint& pa
   DrawPath(p, paint);
 }
 
-int HIPCanvas::Flush() {
-  if (draws_.empty()) return HIPR_SUCCESS;
-  const int32_t n = (int32_t)draws_.size();
-  std::vector<uint8_t> verbs;
-  std::vector<float> pts, ws;
-  std::vector<int32_t> vo(n + 1, 0), po(n + 1, 0), wo(n + 1, 0);
-  std::vector<uint8_t> fills(n);
-  std::vector<hipr_paint> paints(n);
-  std::vector<hipr_gradient> grads;
-  std::vector<float> matrices(6 * (size_t)n);
-  std::vector<uint8_t> kinds(n);
-  std::vector<hipr_color_filter> filters(n);
-  std::vector<hipr_image_shader> shaders(n);
-  std::vector<hipr_image> images;
+PackedScene::PackedScene(const std::vector<HIPCanvas::Draw>& draws) {
+  const int32_t n = (int32_t)draws.size();
+  vo_.assign(n + 1, 0);
+  po_.assign(n + 1, 0);
+  wo_.assign(n + 1, 0);
+  fills_.resize(n);
+  paints_.resize(n);
+  matrices_.resize(6 * (size_t)n);
+  kinds_.resize(n);
+  filters_.resize(n);
+  shaders_.resize(n);
   std::vector<const Image*> image_of;
   bool any_matrix = false, any_paint = false, any_filter = false;
   for (int32_t i = 0; i < n; i++) {
-    const Matrix& m = draws_[i].matrix;
+    const Matrix& m = draws[i].matrix;
     const float mf[6] = {m.sx, m.kx, m.tx, m.ky, m.sy, m.ty};
-    std::copy(mf, mf + 6, matrices.begin() + 6 * (size_t)i);
+    std::copy(mf, mf + 6, matrices_.begin() + 6 * (size_t)i);
     any_matrix |= !m.IsIdentity();
-    kinds[i] = draws_[i].kind;
-    any_paint |= kinds[i] != HIPR_DRAW_PATH;
-    const Path& p = draws_[i].path;
-    verbs.insert(verbs.end(), p.verbs().begin(), p.verbs().end());
+    kinds_[i] = draws[i].kind;
+    any_paint |= kinds_[i] != HIPR_DRAW_PATH;
+    const Path& p = draws[i].path;
+    verbs_.insert(verbs_.end(), p.verbs().begin(), p.verbs().end());
     for (const Point& q : p.points()) {
-      pts.push_back(q.x);
-      pts.push_back(q.y);
+      pts_.push_back(q.x);
+      pts_.push_back(q.y);
     }
-    ws.insert(ws.end(), p.weights().begin(), p.weights().end());
-    vo[i + 1] = (int32_t)verbs.size();
-    po[i + 1] = (int32_t)(pts.size() / 2);
-    wo[i + 1] = (int32_t)ws.size();
-    fills[i] = p.GetFillType() == Path::PathFillType::kEvenOdd ? HIPR_FILL_EVEN_ODD
-                                                               : HIPR_FILL_WINDING;
+    ws_.insert(ws_.end(), p.weights().begin(), p.weights().end());
+    vo_[i + 1] = (int32_t)verbs_.size();
+    po_[i + 1] = (int32_t)(pts_.size() / 2);
+    wo_[i + 1] = (int32_t)ws_.size();
+    fills_[i] = p.GetFillType() == Path::PathFillType::kEvenOdd ? HIPR_FILL_EVEN_ODD
+                                                                : HIPR_FILL_WINDING;
     int32_t gi = -1;
-    shaders[i].image = -1;
-    if (const auto& sh = draws_[i].paint.shader()) {
+    shaders_[i].image = -1;
+    if (const auto& sh = draws[i].paint.shader()) {
       if (sh->image()) {
         // a pixmap shader: one hipr_image per distinct Image
         const Image* im = sh->image().get();
@@ -468,47 +496,111 @@ int HIPCanvas::Flush() {
         while (k < (int32_t)image_of.size() && image_of[k] != im) k++;
         if (k == (int32_t)image_of.size()) {
           image_of.push_back(im);
-          images.push_back(hipr_image{im->Width(), im->Height(), im->AlphaType(), im->Pixels()});
+          images_.push_back(hipr_image{im->Width(), im->Height(), im->AlphaType(), im->Pixels()});
         }
-        shaders[i] = sh->image_info();
-        shaders[i].image = k;
+        shaders_[i] = sh->image_info();
+        shaders_[i].image = k;
       } else {
-        gi = (int32_t)grads.size();
-        grads.push_back(sh->info());
+        gi = (int32_t)grads_.size();
+        grads_.push_back(sh->info());
       }
     }
-    paints[i] = draws_[i].paint.Pack(gi);
-    if (const auto& cf = draws_[i].paint.GetColorFilter()) {
-      filters[i] = cf->info();
+    paints_[i] = draws[i].paint.Pack(gi);
+    if (const auto& cf = draws[i].paint.GetColorFilter()) {
+      filters_[i] = cf->info();
       any_filter = true;
     }
   }
-  hipr_scene scene{};
-  scene.n_paths = n;
-  scene.verbs = verbs.data();
-  scene.verb_offsets = vo.data();
-  scene.points = pts.data();
-  scene.point_offsets = po.data();
-  scene.weights = ws.data();
-  scene.weight_offsets = wo.data();
-  scene.fill_types = fills.data();
-  scene.paints = paints.data();
-  scene.gradients = grads.empty() ? nullptr : grads.data();
-  scene.n_gradients = (int32_t)grads.size();
-  scene.matrices = any_matrix ? matrices.data() : nullptr;
-  scene.draw_kinds = any_paint ? kinds.data() : nullptr;
-  scene.color_filters = any_filter ? filters.data() : nullptr;
-  if (!images.empty()) {
-    scene.images = images.data();
-    scene.n_images = (int32_t)images.size();
-    scene.image_shaders = shaders.data();
+  scene_.n_paths = n;
+  scene_.verbs = verbs_.data();
+  scene_.verb_offsets = vo_.data();
+  scene_.points = pts_.data();
+  scene_.point_offsets = po_.data();
+  scene_.weights = ws_.data();
+  scene_.weight_offsets = wo_.data();
+  scene_.fill_types = fills_.data();
+  scene_.paints = paints_.data();
+  scene_.gradients = grads_.empty() ? nullptr : grads_.data();
+  scene_.n_gradients = (int32_t)grads_.size();
+  scene_.matrices = any_matrix ? matrices_.data() : nullptr;
+  scene_.draw_kinds = any_paint ? kinds_.data() : nullptr;
+  scene_.color_filters = any_filter ? filters_.data() : nullptr;
+  if (!images_.empty()) {
+    scene_.images = images_.data();
+    scene_.n_images = (int32_t)images_.size();
+    scene_.image_shaders = shaders_.data();
   }
-  hipr_options opt{};
-  opt.mode = HIPR_MODE_EXACT;
-  int r = hipr_surface_draw_scene(surface_->surface_, &scene, &opt);
+}
+
+int HIPCanvas::Flush() {
+  if (recording_) return HIPR_SUCCESS;  // RecordingCanvas::OnFlush records nothing
+  int r = HIPR_SUCCESS;
+  if (unsupported_) {
+    r = HIPR_ERROR_NOT_SUPPORTED;
+  } else if (!draws_.empty()) {
+    PackedScene packed(draws_);
+    hipr_options opt{};
+    opt.mode = HIPR_MODE_EXACT;
+    r = hipr_surface_draw_scene(surface_->surface_, packed.scene(), &opt);
+  } else {
+    return HIPR_SUCCESS;
+  }
   draws_.clear();
+  unsupported_ = false;
   surface_->last_result_ = r;
   return r;
+}
+
+// ---- DisplayList / PictureRecorder --------------------------------------------
+void DisplayList::Draw(HIPCanvas* canvas) const {
+  if (!canvas) return;
+  for (const Op& op : ops_) {
+    switch (op.kind) {
+      case OpKind::kSave:
+        canvas->Save();
+        break;
+      case OpKind::kRestore:
+        canvas->Restore();
+        break;
+      case OpKind::kConcat:
+        canvas->Concat(op.matrix);
+        break;
+      case OpKind::kSetMatrix:
+        canvas->SetMatrix(op.matrix);
+        break;
+      case OpKind::kDraw: {
+        HIPCanvas::Draw d = draws_[op.draw];
+        d.matrix = canvas->GetTotalMatrix();
+        canvas->AddDraw(std::move(d));
+        break;
+      }
+      case OpKind::kUnsupported:
+        canvas->MarkUnsupported();
+        break;
+    }
+  }
+}
+
+bool ReplayDraws(const DisplayList& list, std::vector<HIPCanvas::Draw>* draws) {
+  HIPCanvas canvas(nullptr);
+  list.Draw(&canvas);
+  *draws = std::move(canvas.draws_);
+  return !canvas.unsupported_;
+}
+
+PictureRecorder::PictureRecorder() = default;
+PictureRecorder::~PictureRecorder() = default;
+
+void PictureRecorder::BeginRecording(const Rect& bounds) {
+  list_.reset(new DisplayList());
+  list_->bounds_ = bounds;
+  canvas_.reset(new HIPCanvas(nullptr));
+  canvas_->recording_ = list_.get();
+}
+
+std::unique_ptr<DisplayList> PictureRecorder::FinishRecording() {
+  canvas_.reset();
+  return std::move(list_);
 }
 
 // ---- Surface ----------------------------------------------------------------

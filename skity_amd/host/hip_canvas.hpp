@@ -266,11 +266,17 @@ class Paint {
 };
 
 class HIPSurface;
+class DisplayList;
+class PictureRecorder;
 
 // Records DrawPath / DrawPaint calls (include/skity/render/canvas.hpp) with
 // the canvas matrix of each, and the shape helpers that the reference
 // dispatches to OnDrawPath.  The matrix stack follows Canvas::Save/Restore
 // and the pre-concatenation of src/render/canvas_state.cc:18-45.
+//
+// A canvas is either a surface's canvas (Flush draws on the device) or a
+// PictureRecorder's recording canvas (every call is appended to a
+// DisplayList, as RecordingCanvas does, src/recorder/recording_canvas.cc).
 class HIPCanvas {
  public:
   int Save();
@@ -282,10 +288,23 @@ class HIPCanvas {
   void Rotate(float degrees) { Concat(Matrix::RotateDeg(degrees)); }
   void Rotate(float degrees, float px, float py) { Concat(Matrix::RotateDeg(degrees, px, py)); }
   void Skew(float sx, float sy) { Concat(Matrix::Skew(sx, sy)); }
-  void Concat(const Matrix& m) { stack_.back() = stack_.back() * m; }
-  void SetMatrix(const Matrix& m) { stack_.back() = m; }
-  void ResetMatrix() { stack_.back() = Matrix(); }
+  void Concat(const Matrix& m);
+  void SetMatrix(const Matrix& m);
+  void ResetMatrix() { SetMatrix(Matrix()); }
   Matrix GetTotalMatrix() const { return stack_.back(); }
+
+  // Clips and save layers are not drawn by this backend: the next Flush
+  // returns HIPR_ERROR_NOT_SUPPORTED (and draws nothing) instead of drawing
+  // something other than the software canvas would.  The same holds for
+  // any other call the backend cannot draw exactly (MarkUnsupported).
+  enum class ClipOp { kDifference, kIntersect };
+  void ClipRect(const Rect&, ClipOp = ClipOp::kIntersect) { MarkUnsupported(); }
+  void ClipPath(const Path&, ClipOp = ClipOp::kIntersect) { MarkUnsupported(); }
+  void SaveLayer(const Rect&, const Paint&) {
+    Save();
+    MarkUnsupported();
+  }
+  void MarkUnsupported();
 
   void DrawPaint(const Paint& paint);
   // Canvas::DrawColor / Clear (canvas.cc:241-251): DrawPaint with a fill
@@ -301,18 +320,90 @@ class HIPCanvas {
   int Flush();
   size_t PendingDraws() const { return draws_.size(); }
 
- private:
-  friend class HIPSurface;
-  explicit HIPCanvas(HIPSurface* s) : surface_(s) {}
+  // One draw with the canvas matrix it was made under.
   struct Draw {
     Path path;
     Paint paint;
     Matrix matrix;
     uint8_t kind;  // HIPR_DRAW_*
   };
+
+ private:
+  friend class HIPSurface;
+  friend class PictureRecorder;
+  friend class DisplayList;
+  friend bool ReplayDraws(const DisplayList& list, std::vector<Draw>* draws);
+  explicit HIPCanvas(HIPSurface* s) : surface_(s) {}
+  void AddDraw(Draw d);
   HIPSurface* surface_;
+  DisplayList* recording_ = nullptr;  // set on a recording canvas
+  bool unsupported_ = false;
   std::vector<Draw> draws_;
   std::vector<Matrix> stack_{Matrix()};
+};
+
+// The flat arrays of a hipr_scene built from a list of draws (what a flush
+// hands to hipr_surface_draw_scene); scene() points into them.
+class PackedScene {
+ public:
+  explicit PackedScene(const std::vector<HIPCanvas::Draw>& draws);
+  const hipr_scene* scene() const { return &scene_; }
+
+ private:
+  std::vector<uint8_t> verbs_, fills_, kinds_;
+  std::vector<float> pts_, ws_, matrices_;
+  std::vector<int32_t> vo_, po_, wo_;
+  std::vector<hipr_paint> paints_;
+  std::vector<hipr_gradient> grads_;
+  std::vector<hipr_color_filter> filters_;
+  std::vector<hipr_image_shader> shaders_;
+  std::vector<hipr_image> images_;
+  hipr_scene scene_{};
+};
+
+// A recorded list of canvas calls (include/skity/recorder/display_list.hpp:
+// 56-77): the matrix and save/restore calls, the draws (shape helpers as the
+// paths they build) and the calls this backend cannot draw.  Draw replays
+// them onto a canvas in order (display_list.cc:330-343).
+class DisplayList {
+ public:
+  void Draw(HIPCanvas* canvas) const;
+  uint32_t OpCount() const { return (uint32_t)ops_.size(); }
+  bool Empty() const { return ops_.empty(); }
+  const Rect& GetBounds() const { return bounds_; }
+
+ private:
+  friend class HIPCanvas;
+  friend class PictureRecorder;
+  enum class OpKind : uint8_t { kSave, kRestore, kConcat, kSetMatrix, kDraw, kUnsupported };
+  struct Op {
+    OpKind kind;
+    Matrix matrix;
+    uint32_t draw;  // index into draws_ for kDraw
+  };
+  std::vector<Op> ops_;
+  std::vector<HIPCanvas::Draw> draws_;
+  Rect bounds_{0, 0, 0, 0};
+};
+
+// The draws a display list makes when replayed onto a fresh canvas (identity
+// matrix), with the matrices they are made under; false when it makes a call
+// this backend does not draw.
+bool ReplayDraws(const DisplayList& list, std::vector<HIPCanvas::Draw>* draws);
+
+// PictureRecorder (include/skity/recorder/picture_recorder.hpp): a recording
+// canvas whose calls become a DisplayList.
+class PictureRecorder {
+ public:
+  PictureRecorder();
+  ~PictureRecorder();
+  void BeginRecording(const Rect& bounds);
+  HIPCanvas* GetRecordingCanvas() { return canvas_.get(); }
+  std::unique_ptr<DisplayList> FinishRecording();
+
+ private:
+  std::unique_ptr<HIPCanvas> canvas_;
+  std::unique_ptr<DisplayList> list_;
 };
 
 // A device RGBA8 target (the GPUSurface pattern of

@@ -229,6 +229,60 @@ static void PixmapShaderDrawsTheImage(hipr_context ctx) {
   EXPECT_EQ(s->GetPixel(5 + w, 6), Color_TRANSPARENT);
 }
 
+// PictureRecorder + DisplayList::Draw (display_list.cc:330-343): replaying
+// the recording onto a canvas under a matrix draws what the same calls made
+// directly under that matrix draw; a recorded clip makes the flush
+// NOT_SUPPORTED and draws nothing.
+static void DisplayListReplayEqualsDirectDraws(hipr_context ctx) {
+  auto calls = [](HIPCanvas* c) {
+    Paint red;
+    red.SetColor(Color_RED);
+    Paint blue;
+    blue.SetColor(0x800000FF);
+    blue.SetStyle(Paint::kStroke_Style);
+    blue.SetStrokeWidth(3.f);
+    c->Save();
+    c->Translate(10.f, 5.f);
+    c->Rotate(20.f);
+    c->DrawRect(Rect::MakeXYWH(4.f, 4.f, 30.f, 20.f), red);
+    c->Restore();
+    c->Scale(1.5f, 1.25f);
+    c->DrawCircle(30.f, 30.f, 12.f, blue);
+    c->DrawPaint(blue);
+  };
+  PictureRecorder rec;
+  rec.BeginRecording(Rect::MakeWH(64.f, 64.f));
+  calls(rec.GetRecordingCanvas());
+  std::unique_ptr<DisplayList> dl = rec.FinishRecording();
+  EXPECT_EQ(dl->OpCount(), 8u);
+
+  auto a = HIPSurface::Make(ctx, 64, 64);
+  auto b = HIPSurface::Make(ctx, 64, 64);
+  HIPCanvas* ca = a->LockCanvas();
+  ca->Translate(3.f, -2.f);
+  dl->Draw(ca);
+  EXPECT_EQ(a->Flush(), HIPR_SUCCESS);
+  HIPCanvas* cb = b->LockCanvas();
+  cb->Translate(3.f, -2.f);
+  calls(cb);
+  EXPECT_EQ(b->Flush(), HIPR_SUCCESS);
+  std::vector<uint8_t> pa(64 * 64 * 4), pb(64 * 64 * 4);
+  EXPECT_EQ(a->ReadPixels(pa.data(), pa.size()), HIPR_SUCCESS);
+  EXPECT_EQ(b->ReadPixels(pb.data(), pb.size()), HIPR_SUCCESS);
+  EXPECT_TRUE(pa == pb);
+  EXPECT_TRUE(pa[4 * (20 * 64 + 20) + 3] != 0);
+
+  PictureRecorder rec2;
+  rec2.BeginRecording(Rect::MakeWH(64.f, 64.f));
+  calls(rec2.GetRecordingCanvas());
+  rec2.GetRecordingCanvas()->ClipRect(Rect::MakeWH(8.f, 8.f));
+  auto dl2 = rec2.FinishRecording();
+  HIPCanvas* cc = a->LockCanvas();
+  dl2->Draw(cc);
+  EXPECT_EQ(a->Flush(), HIPR_ERROR_NOT_SUPPORTED);
+  EXPECT_EQ(a->GetPixel(20, 20), Color_TRANSPARENT);
+}
+
 int main() {
   hipr_context ctx = nullptr;
   if (hipr_context_create(0, &ctx) != HIPR_SUCCESS) {
@@ -247,7 +301,8 @@ int main() {
                {"Canvas.ClearAndDrawColor", ClearAndDrawColor},
                {"HIPCanvas.WidthZeroAndNegativeStrokes", WidthZeroAndNegativeStrokes},
                {"Shader.TwoPointConicalFactory", TwoPointConicalFactory},
-               {"Shader.PixmapShaderDrawsTheImage", PixmapShaderDrawsTheImage}};
+               {"Shader.PixmapShaderDrawsTheImage", PixmapShaderDrawsTheImage},
+               {"DisplayList.ReplayEqualsDirectDraws", DisplayListReplayEqualsDirectDraws}};
   for (auto& t : tests) {
     int before = g_failed;
     t.fn(ctx);

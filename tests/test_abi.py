@@ -1,5 +1,6 @@
-"""CPU checks of the C-ABI library: it loads and exports every entry point
-declared in include/skity_hip.h (no device calls)."""
+"""CPU checks of the C-ABI libraries: they load and export every entry point
+declared in include/skity_hip.h and include/skity_hip_picture.h (no device
+calls)."""
 import ctypes
 import os
 import re
@@ -23,6 +24,15 @@ def test_header_declares_entry_points():
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(os.path.join(abi.HERE, "libskity_hip.so"))
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, f"missing exports: {missing}"
+
+
+def test_picture_binding_exports_every_declared_symbol():
+    text = open(os.path.join(abi.ROOT, "include", "skity_hip_picture.h")).read()
+    syms = sorted(set(re.findall(r"\b(skity_hip_\w+)\s*\(", text)))
+    assert len(syms) == 8
+    lib = ctypes.CDLL(os.path.join(abi.HERE, "libskity_hip_canvas.so"))
+    missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing, f"missing exports: {missing}"
 
 
