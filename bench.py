@@ -259,6 +259,36 @@ def tiger_leg(ctx, torch, steps, side=4096, scale=4.1):
     return out
 
 
+def text_leg(ctx, torch, steps):
+    """Text as paths: a 2048^2 page of 14 px DejaVu Sans glyph runs (about
+    17,000 glyphs, headings larger, rotated and stroked) drawn as glyph runs
+    (DrawGlyphs); exact and learned forms.  Glyph outlines come from the
+    TrueType file installed with the system; skipped without one."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import text_kit
+    from skity_amd import abi, device
+    made = text_kit.text_page(3)
+    if made is None:
+        return {"skipped": "no TrueType font installed"}
+    sc, glyphs = made
+    prep = device.Prepared(ctx, sc.scene)
+    surf = device.Surface(ctx, 2048, 2048)
+    out = {"workload": "text page: 14 px DejaVu Sans glyph runs with larger, rotated and "
+                       "stroked headings, 2048x2048 RGBA8",
+           "glyphs": glyphs, "draws": sc.scene.contents.n_paths}
+    imgs = {}
+    for name, mode in (("exact", abi.MODE_EXACT), ("learned", abi.MODE_LEARNED)):
+        wall, _, st = run_form(surf, prep, mode, steps, 2, torch)
+        imgs[name] = surf.read()
+        out[name] = {"glyphs_per_s": round(glyphs / (wall / steps), 1),
+                     "ms_per_step": round(wall * 1e3 / steps, 3),
+                     "fallbacks_per_frame": st["fallbacks_per_frame"]}
+    out["learned_equals_exact"] = bool(np.array_equal(imgs["exact"], imgs["learned"]))
+    surf.close()
+    prep.close()
+    return out
+
+
 def batched_leg(ctx, torch, steps, n_canvases=64, workload=4, tune_canvases=8):
     """Batched uneven scenes at the family's stated scale (workload 4): n
     canvases of side 256-4096, each with 100-50,000 closed outlines of
@@ -361,8 +391,9 @@ def main():
                     help="comma list of forms to time: exact, learned")
     ap.add_argument("--batch-steps", type=int, default=1,
                     help="timed frames of the batched uneven leg (each is ~650M segments)")
-    ap.add_argument("--legs", default="batched,blur,tiger",
-                    help="extra workloads reported beside the headline: batched, blur, tiger")
+    ap.add_argument("--legs", default="batched,blur,tiger,text",
+                    help="extra workloads reported beside the headline: batched, blur, tiger, "
+                         "text")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short program the rocprofv3 counter summaries profile")
     ap.add_argument("--probe-workload", type=int, default=WORKLOAD,
@@ -472,6 +503,9 @@ def main():
     if "tiger" in leg_names:
         legs["tiger_skp"] = tiger_leg(ctx, torch, max(3, args.steps // 2))
         log("tiger leg done")
+    if "text" in leg_names:
+        legs["text_page"] = text_leg(ctx, torch, max(3, args.steps // 2))
+        log("text leg done")
     out["legs"] = legs
     if args.cpu_seconds > 0:
         log("cpu baseline")

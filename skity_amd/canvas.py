@@ -9,6 +9,7 @@ CubicTo with InjectMoveToIfNeed, :837-859 Close, :815-835 AddRect,
 skity would iterate.
 """
 import ctypes
+import copy
 import math
 
 import numpy as np
@@ -324,6 +325,33 @@ class Canvas:
 
     def draw_paint(self, paint):
         self.draws.append((Path(), paint, self._stack[-1], abi.DRAW_PAINT))
+
+    def draw_glyphs(self, glyph_paths, xs, ys, paint):
+        """Canvas::DrawGlyphs as the software canvas draws a glyph run
+        (sw_canvas.cc:542-639): glyph k's outline under the canvas matrix
+        times Translate(xs[k], ys[k]), all fills then all strokes (the
+        reverse for kStrokeThenFill).  Runs with a mask filter or a shader
+        are not drawn by the backend."""
+        if paint.p.blur_radius > 0 or paint.gradient is not None or paint.image_shader:
+            raise NotImplementedError("glyph runs with a mask filter or a shader")
+        style = paint.p.style
+        need_fill = style != abi.STYLE_STROKE
+        need_stroke = style != abi.STYLE_FILL
+
+        def run(st):
+            p = copy.copy(paint)
+            p.p = abi.Paint.from_buffer_copy(paint.p)
+            p.p.style = st
+            for path, x, y in zip(glyph_paths, xs, ys):
+                m = mat_mul(self._stack[-1], (1.0, 0.0, float(F32(x)), 0.0, 1.0, float(F32(y))))
+                self.draws.append((path, p, m, abi.DRAW_PATH))
+
+        order = [abi.STYLE_FILL, abi.STYLE_STROKE]
+        if style == abi.STYLE_STROKE_THEN_FILL:
+            order.reverse()
+        for st in order:
+            if (st == abi.STYLE_FILL and need_fill) or (st == abi.STYLE_STROKE and need_stroke):
+                run(st)
 
     def scene(self):
         return PackedScene(self.draws, self.width, self.height)

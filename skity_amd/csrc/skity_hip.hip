@@ -1437,7 +1437,11 @@ hipr_result RunPresize(Frame& f, std::vector<int32_t>* eest, std::vector<int32_t
     return (int32_t)std::min(std::max(v, 1.0), 1.0e9);
   };
   for (int j = 0; j < nj; j++) {
-    (*eest)[j] = cap(pred[2 * j], ctx->margin_edges);
+    // A prediction far outside the training distribution (glyph-sized
+    // outlines, say) can be huge; capacities beyond 64 edges per verb only
+    // waste memory (a job that needs more falls back to its exact count).
+    const int64_t verbs = std::max<int64_t>(f.jobs[j].verb_end - f.jobs[j].verb_begin, 0);
+    (*eest)[j] = (int32_t)std::min<int64_t>(cap(pred[2 * j], ctx->margin_edges), 64 * verbs + 64);
     (*test)[j] = cap(pred[2 * j + 1], ctx->margin_planes);
   }
   return HIPR_SUCCESS;
@@ -1907,8 +1911,17 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     if ((r = move_overflowed(ctx->h_cov, nullptr)) != HIPR_SUCCESS) return r;
     for (int32_t j : all) pcap[j] = ctx->h_cov[j].planes;
   } else {
-    // planes from the estimates
-    for (int32_t j : all) pcap[j] = test[j];
+    // planes from the estimates, at most what the job can use: three planes
+    // (a keep-zero triple) per visible tile, and per band that meets the
+    // tile
+    for (int32_t j : all) {
+      const Canvas& cv = f.canvases[f.jobs[j].canvas];
+      const TileRange tr = JobTiles(JobPixBounds(ctx->h_geo[j]), cv.width, cv.height);
+      const int64_t bands_per_tile =
+          nbands[j] > 1 ? 1 + (kTile + std::max(band_rows, 1) - 1) / std::max(band_rows, 1) : 1;
+      const int64_t most = 3 * (int64_t)tr.ntc * tr.ntr * bands_per_tile;
+      pcap[j] = (int32_t)std::max<int64_t>(std::min<int64_t>(test[j], most), 1);
+    }
   }
   assign_planes(all);
   // Estimate modes: a shared pool after the estimated ranges serves the

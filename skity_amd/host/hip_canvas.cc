@@ -402,10 +402,10 @@ void HIPCanvas::MarkUnsupported() {
   }
 }
 
-void HIPCanvas::AddDraw(Draw d) {
+void HIPCanvas::AddDraw(Draw d, const Matrix& local) {
   if (recording_) {
     recording_->ops_.push_back(
-        {DisplayList::OpKind::kDraw, Matrix(), (uint32_t)recording_->draws_.size()});
+        {DisplayList::OpKind::kDraw, local, (uint32_t)recording_->draws_.size()});
     recording_->draws_.push_back(std::move(d));
     return;
   }
@@ -452,6 +452,34 @@ void HIPCanvas::DrawLine(float x0, float y0, float x1, float y1, const Paint& pa
   p.MoveTo(x0, y0);
   p.LineTo(x1, y1);
   DrawPath(p, paint);
+}
+
+void HIPCanvas::DrawGlyphs(uint32_t count, const Path* const glyph_paths[],
+                           const float position_x[], const float position_y[],
+                           const Paint& paint) {
+  if (count == 0) return;
+  if (paint.GetMaskFilter() || paint.shader()) {
+    MarkUnsupported();
+    return;
+  }
+  const bool need_fill = paint.GetStyle() != Paint::kStroke_Style;
+  const bool need_stroke = paint.GetStyle() != Paint::kFill_Style;
+  auto run = [&](Paint::Style style) {
+    Paint p = paint;
+    p.SetStyle(style);
+    for (uint32_t k = 0; k < count; k++) {
+      const Matrix local = Matrix::Translate(position_x[k], position_y[k]);
+      AddDraw(Draw{*glyph_paths[k], p, stack_.back() * local, HIPR_DRAW_PATH}, local);
+    }
+  };
+  // DrawFillStrokeInPaintOrder (src/render/paint_order.hpp:12-31)
+  if (paint.GetStyle() == Paint::kStrokeThenFill_Style) {
+    if (need_stroke) run(Paint::kStroke_Style);
+    if (need_fill) run(Paint::kFill_Style);
+  } else {
+    if (need_fill) run(Paint::kFill_Style);
+    if (need_stroke) run(Paint::kStroke_Style);
+  }
 }
 
 PackedScene::PackedScene(const std::vector<HIPCanvas::Draw>& draws) {
@@ -570,8 +598,8 @@ void DisplayList::Draw(HIPCanvas* canvas) const {
         break;
       case OpKind::kDraw: {
         HIPCanvas::Draw d = draws_[op.draw];
-        d.matrix = canvas->GetTotalMatrix();
-        canvas->AddDraw(std::move(d));
+        d.matrix = canvas->GetTotalMatrix() * op.matrix;
+        canvas->AddDraw(std::move(d), op.matrix);
         break;
       }
       case OpKind::kUnsupported:

@@ -252,6 +252,7 @@ class Paint {
 
   hipr_paint Pack(int32_t gradient_index) const;
   const std::shared_ptr<Shader>& shader() const { return shader_; }
+  const std::shared_ptr<MaskFilter>& GetMaskFilter() const { return mask_; }
 
  private:
   Style style_ = kFill_Style;
@@ -316,6 +317,17 @@ class HIPCanvas {
   void DrawOval(const Rect& oval, const Paint& paint);
   void DrawCircle(float cx, float cy, float radius, const Paint& paint);
   void DrawLine(float x0, float y0, float x1, float y1, const Paint& paint);
+  // Canvas::DrawGlyphs as the software canvas draws a glyph run
+  // (SWCanvas::DrawGlyphsInternal / FillGlyphs / StrokeGlyphs,
+  // sw_canvas.cc:542-639): glyph_paths[k] is the outline Font::LoadGlyphPath
+  // gives for glyph k; each is drawn under the canvas matrix times
+  // Translate(position_x[k], position_y[k]), all fills then all strokes (or
+  // the reverse for kStrokeThenFill).  Runs whose paint has a mask filter
+  // (one layer for the whole run) or a shader (mapped by the canvas matrix
+  // alone) are not drawn by this backend: the next flush returns
+  // HIPR_ERROR_NOT_SUPPORTED.
+  void DrawGlyphs(uint32_t count, const Path* const glyph_paths[], const float position_x[],
+                  const float position_y[], const Paint& paint);
   // Draws everything recorded since the last flush; returns the hipr_result.
   int Flush();
   size_t PendingDraws() const { return draws_.size(); }
@@ -334,7 +346,9 @@ class HIPCanvas {
   friend class DisplayList;
   friend bool ReplayDraws(const DisplayList& list, std::vector<Draw>* draws);
   explicit HIPCanvas(HIPSurface* s) : surface_(s) {}
-  void AddDraw(Draw d);
+  // d.matrix is the canvas matrix times `local`; a recording keeps `local`
+  // and replays it under the replaying canvas's matrix.
+  void AddDraw(Draw d, const Matrix& local = Matrix());
   HIPSurface* surface_;
   DisplayList* recording_ = nullptr;  // set on a recording canvas
   bool unsupported_ = false;

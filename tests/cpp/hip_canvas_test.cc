@@ -283,6 +283,52 @@ static void DisplayListReplayEqualsDirectDraws(hipr_context ctx) {
   EXPECT_EQ(a->GetPixel(20, 20), Color_TRANSPARENT);
 }
 
+// DrawGlyphs (sw_canvas.cc:542-639): a glyph run equals its glyphs drawn as
+// paths under Translate(x, y), all fills before all strokes; a run with a
+// shader is refused.
+static void GlyphRunEqualsTranslatedPaths(hipr_context ctx) {
+  Path a, b;
+  a.AddRect(Rect::MakeLTRB(0.f, -10.f, 8.f, 0.f));
+  b.MoveTo(0.f, 0.f).LineTo(6.f, -12.f).LineTo(12.f, 0.f).Close();
+  const Path* glyphs[2] = {&a, &b};
+  const float xs[2] = {5.f, 9.f}, ys[2] = {20.f, 21.5f};
+  Paint paint;
+  paint.SetColor(0xC0204080);
+  paint.SetStyle(Paint::kStrokeAndFill_Style);
+  paint.SetStrokeWidth(2.f);
+  paint.SetStrokeColor(0xFFFFA000);
+  auto s1 = HIPSurface::Make(ctx, 40, 40);
+  auto s2 = HIPSurface::Make(ctx, 40, 40);
+  HIPCanvas* c1 = s1->LockCanvas();
+  c1->Rotate(5.f);
+  c1->DrawGlyphs(2, glyphs, xs, ys, paint);
+  EXPECT_EQ(s1->Flush(), HIPR_SUCCESS);
+  HIPCanvas* c2 = s2->LockCanvas();
+  c2->Rotate(5.f);
+  Paint fill = paint, stroke = paint;
+  fill.SetStyle(Paint::kFill_Style);
+  stroke.SetStyle(Paint::kStroke_Style);
+  for (const Paint* p : {&fill, &stroke})
+    for (int k = 0; k < 2; k++) {
+      c2->Save();
+      c2->Translate(xs[k], ys[k]);
+      c2->DrawPath(*glyphs[k], *p);
+      c2->Restore();
+    }
+  EXPECT_EQ(s2->Flush(), HIPR_SUCCESS);
+  std::vector<uint8_t> p1(40 * 40 * 4), p2(40 * 40 * 4);
+  EXPECT_EQ(s1->ReadPixels(p1.data(), p1.size()), HIPR_SUCCESS);
+  EXPECT_EQ(s2->ReadPixels(p2.data(), p2.size()), HIPR_SUCCESS);
+  EXPECT_TRUE(p1 == p2);
+  EXPECT_TRUE(s1->GetPixel(9, 15) != Color_TRANSPARENT);
+  Point pts[2] = {{0.f, 0.f}, {10.f, 0.f}};
+  Color4f cols[2] = {{1, 0, 0, 1}, {0, 0, 1, 1}};
+  Paint shaded;
+  shaded.SetShader(Shader::MakeLinear(pts, cols, nullptr, 2));
+  s1->LockCanvas()->DrawGlyphs(2, glyphs, xs, ys, shaded);
+  EXPECT_EQ(s1->Flush(), HIPR_ERROR_NOT_SUPPORTED);
+}
+
 int main() {
   hipr_context ctx = nullptr;
   if (hipr_context_create(0, &ctx) != HIPR_SUCCESS) {
@@ -302,7 +348,8 @@ int main() {
                {"HIPCanvas.WidthZeroAndNegativeStrokes", WidthZeroAndNegativeStrokes},
                {"Shader.TwoPointConicalFactory", TwoPointConicalFactory},
                {"Shader.PixmapShaderDrawsTheImage", PixmapShaderDrawsTheImage},
-               {"DisplayList.ReplayEqualsDirectDraws", DisplayListReplayEqualsDirectDraws}};
+               {"DisplayList.ReplayEqualsDirectDraws", DisplayListReplayEqualsDirectDraws},
+               {"Canvas.GlyphRunEqualsTranslatedPaths", GlyphRunEqualsTranslatedPaths}};
   for (auto& t : tests) {
     int before = g_failed;
     t.fn(ctx);
