@@ -206,6 +206,13 @@ typedef struct hipr_scene {
   int32_t n_images;
   /* n_paths; NULL = none.  A draw with an image shader has no gradient. */
   const hipr_image_shader* image_shaders;
+  /* n_paths x 4 floats {left, top, right, bottom}: the canvas's global clip
+   * bounds at each draw in device space (Canvas::ClipRect intersections,
+   * canvas.cc:130-135,463-476), the scan clip of the draw's path, stroke and
+   * blurred-layer rasters (SWRaster::RastePath, sw_raster.cc:731-786;
+   * DrawPaint rasters without it, sw_canvas.cc:413-439).  NULL = none; a
+   * draw without a clip holds kMaxCullRect {-1e9, -1e9, 1e9, 1e9}. */
+  const float* clip_rects;
 } hipr_scene;
 
 typedef struct hipr_context_s* hipr_context;

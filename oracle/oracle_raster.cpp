@@ -1123,7 +1123,8 @@ void BuildEdges(const PathData& q, float top, float bottom,
 // (sw_raster.cc:679-729).  std::sort is not stable; equal (upper_y, x, dx)
 // keys keep their emission order here.
 void RastePath(const PathData& path, std::vector<Span>* spans,
-               float out_bounds[4], std::vector<Edge>* edges_out, const float* m) {
+               float out_bounds[4], std::vector<Edge>* edges_out, const float* m,
+               const float* clip) {
   spans->clear();
   PathData q = QuadPath(path);
   if (m) {
@@ -1146,9 +1147,12 @@ void RastePath(const PathData& path, std::vector<Span>* spans,
     out_bounds[2] = br;
     out_bounds[3] = bb;
   }
-  // Intersect with the default cull rect (+-1e9): empty unless l<r && t<b.
-  float il = std::max(sb[0], -1e9f), ir = std::min(sb[2], 1e9f);
-  float it = std::max(sb[1], -1e9f), ib = std::min(sb[3], 1e9f);
+  // Rect::Intersect with the clip bounds (rect.cc:160-172; by default the
+  // cull rect +-1e9): on no overlap the scan bounds are SetEmpty.
+  const float cull[4] = {-1e9f, -1e9f, 1e9f, 1e9f};
+  const float* c = clip ? clip : cull;
+  float il = std::max(sb[0], c[0]), ir = std::min(sb[2], c[2]);
+  float it = std::max(sb[1], c[1]), ib = std::min(sb[3], c[3]);
   if (!(il < ir && it < ib)) il = it = ir = ib = 0;
   float sl = std::floor(il), st = std::floor(it), sr = std::ceil(ir),
         sbm = std::ceil(ib);

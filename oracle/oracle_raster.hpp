@@ -81,10 +81,11 @@ void BuildEdges(const PathData& qpath, float top, float bottom,
 // after the quad conversion as Path::CopyWithMatrix does
 // (src/graphic/path.cc:1274-1295, src/geometry/matrix.cc:462-465).
 // Returns the spans in emission order; *bounds receives the floored/ceiled
-// path bounds.
+// path bounds.  clip = {l, t, r, b}: the canvas's global clip bounds the scan
+// bounds are intersected with (nullptr: kMaxCullRect, +-1e9).
 void RastePath(const PathData& path, std::vector<Span>* spans,
                float bounds_ltrb[4], std::vector<Edge>* edges_out,
-               const float* m = nullptr);
+               const float* m = nullptr, const float* clip = nullptr);
 
 // Path bounds (Path::ComputePtBounds, src/graphic/path.cc:1347-1360).
 bool PathBounds(const PathData& p, float ltrb[4]);

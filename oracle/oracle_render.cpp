@@ -861,7 +861,7 @@ void ApplyOp(const SpanOp& op, Target* t, int y0, int y1) {
 // The fill and/or stroke rasterisations of a path under canvas matrix `ctm`,
 // in paint order (sw_canvas.cc:357-411, paint_order.hpp:12-31).
 void ShapeOps(const hipr_scene* s, const PathData& path, const hipr_paint& paint,
-              const Aff& ctm, Ops* ops) {
+              const Aff& ctm, Ops* ops, const float* clip = nullptr) {
   float m6[6];
   AffToArray(ctm, m6);
   auto shape = [&](bool stroke) {
@@ -870,10 +870,10 @@ void ShapeOps(const hipr_scene* s, const PathData& path, const hipr_paint& paint
     op->blend = paint.blend;
     float bounds[4];
     if (!stroke) {
-      RastePath(path, &op->spans, bounds, nullptr, m6);
+      RastePath(path, &op->spans, bounds, nullptr, m6, clip);
     } else {
       PathData outline = StrokeOutline(QuadPath(path), paint);
-      RastePath(outline, &op->spans, bounds, nullptr, m6);
+      RastePath(outline, &op->spans, bounds, nullptr, m6, clip);
     }
     FinishOp(op.get());
     ops->push_back(std::move(op));
@@ -1005,6 +1005,10 @@ void DrawEntryOps(const hipr_scene* s, int p, int tw, int th, Ops* ops) {
 void DrawEntryOpsUnfiltered(const hipr_scene* s, int p, int tw, int th, Ops* ops) {
   const hipr_paint& paint = s->paints[p];
   const Aff ctm = DrawMatrix(s, p);
+  // the canvas's global clip bounds at this draw (GetScanClipBounds,
+  // sw_canvas.hpp:157-161): the scan clip of its path and stroke rasters
+  // and of a blurred layer's image draw; DrawPaint rasters without it
+  const float* clip = s->clip_rects ? s->clip_rects + 4 * (size_t)p : nullptr;
   const int kind = s->draw_kinds ? s->draw_kinds[p] : HIPR_DRAW_PATH;
   if (kind == HIPR_DRAW_PAINT) {
     // OnDrawPaint: the target's rectangle, identity raster matrix, the
@@ -1032,7 +1036,7 @@ void DrawEntryOpsUnfiltered(const hipr_scene* s, int p, int tw, int th, Ops* ops
     float m6[6];
     AffToArray(ctm, m6);
     float bounds[4];
-    RastePath(RectPath(L, T, R, B), &op->spans, bounds, nullptr, m6);
+    RastePath(RectPath(L, T, R, B), &op->spans, bounds, nullptr, m6, clip);
     op->br.kind = 2;
     op->br.img = op->layer.data();
     op->br.img_w = W;
@@ -1047,7 +1051,7 @@ void DrawEntryOpsUnfiltered(const hipr_scene* s, int p, int tw, int th, Ops* ops
     ops->push_back(std::move(op));
     return;
   }
-  ShapeOps(s, ScenePath(s, p), paint, ctm, ops);
+  ShapeOps(s, ScenePath(s, p), paint, ctm, ops, clip);
 }
 
 template <class F>

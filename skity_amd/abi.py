@@ -28,6 +28,7 @@ BLEND_LAST = 28
 ALPHA_OPAQUE, ALPHA_PREMUL, ALPHA_UNPREMUL = 1, 2, 3
 COLOR_RGBA, COLOR_BGRA = 1, 2
 DRAW_PATH, DRAW_PAINT = 0, 1
+CULL_RECT = (-1e9, -1e9, 1e9, 1e9)  # kMaxCullRect: a draw without a clip
 
 # hipr_stage, in enum order
 STAGES = ("stroke", "flatten_count", "flatten", "sort", "walk_count", "walk", "layers",
@@ -125,6 +126,7 @@ class Scene(ctypes.Structure):
         ("images", ctypes.POINTER(Image)),
         ("n_images", ctypes.c_int32),
         ("image_shaders", ctypes.POINTER(ImageShader)),
+        ("clip_rects", ctypes.POINTER(ctypes.c_float)),
     ]
 
 

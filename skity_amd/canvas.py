@@ -138,6 +138,19 @@ def mat_mul(a, b):
             float(aky * bkx + asy * bsy), float((aky * btx + asy * bty) + aty))
 
 
+def map_rect(m, rect):
+    """Matrix::MapRect (matrix.cc:306-337) for an affine m: the corners
+    mapped as glm's mat4 * vec4 sums them, (sx x + kx y) + (0 + tx), and
+    their min / max."""
+    sx, kx, tx, ky, sy, ty = (F32(v) for v in m)
+    l, t, r, b = (F32(v) for v in rect)
+    xs, ys = [], []
+    for x, y in ((l, t), (r, t), (r, b), (l, b)):
+        xs.append((sx * x + kx * y) + (F32(0) + tx))
+        ys.append((ky * x + sy * y) + (F32(0) + ty))
+    return min(xs), min(ys), max(xs), max(ys)
+
+
 def mat_rotate(degrees, px=0.0, py=0.0):
     """Rotation about (px, py) (Matrix::RotateDeg)."""
     r = math.radians(degrees)
@@ -285,15 +298,39 @@ class Canvas:
         self.width, self.height = width, height
         self.draws = []
         self._stack = [IDENTITY]
+        self._clips = [abi.CULL_RECT]  # global clip bounds per save level
 
-    # -- matrix state --
+    # -- matrix and clip state --
     def save(self):
         self._stack.append(self._stack[-1])
+        self._clips.append(self._clips[-1])
         return len(self._stack) - 1
 
     def restore(self):
         if len(self._stack) > 1:
             self._stack.pop()
+            self._clips.pop()
+
+    def clip_rect(self, l, t, r, b, difference=False):
+        """Canvas::ClipRect (canvas.cc:130-135,463-476, sw_canvas.cc:297-313):
+        an intersect clip under a scale + translate matrix narrows the global
+        clip bounds, the scan clip of later path draws; a difference clip or
+        one under a rotating / skewing matrix clips span lists, which the
+        backend does not do."""
+        m = self._stack[-1]
+        if difference or m[1] != 0 or m[3] != 0:
+            raise NotImplementedError("span-list clips (difference, or under skew/rotation)")
+        ml, mt, mr, mb = map_rect(m, (l, t, r, b))
+        cl, ct, cr, cb = self._clips[-1]
+        il, ir = max(F32(cl), F32(ml)), min(F32(cr), F32(mr))
+        it, ib = max(F32(ct), F32(mt)), min(F32(cb), F32(mb))
+        if il < ir and it < ib:
+            self._clips[-1] = (float(il), float(it), float(ir), float(ib))
+        else:
+            self._clips[-1] = (0.0, 0.0, 0.0, 0.0)  # Rect::SetEmpty
+
+    def get_clip_bounds(self):
+        return self._clips[-1]
 
     def get_total_matrix(self):
         return self._stack[-1]
@@ -321,7 +358,10 @@ class Canvas:
 
     # -- draws --
     def draw_path(self, path, paint):
-        self.draws.append((path, paint, self._stack[-1], abi.DRAW_PATH))
+        # (path, paint, matrix, kind[, clip bounds when something clips])
+        clip = self._clips[-1]
+        d = (path, paint, self._stack[-1], abi.DRAW_PATH)
+        self.draws.append(d if clip == abi.CULL_RECT else d + (clip,))
 
     def draw_paint(self, paint):
         self.draws.append((Path(), paint, self._stack[-1], abi.DRAW_PAINT))
@@ -369,14 +409,17 @@ class PackedScene:
         verbs, pts, ws, fills = [], [], [], []
         grads = []
         self.paints = (abi.Paint * max(n, 1))()
-        draws = [d if len(d) == 4 else (d[0], d[1], IDENTITY, abi.DRAW_PATH) for d in draws]
+        draws = [tuple(d) + (None,) if len(d) == 4 else
+                 (tuple(d) if len(d) == 5 else (d[0], d[1], IDENTITY, abi.DRAW_PATH, None))
+                 for d in draws]
         mats = np.array([d[2] for d in draws] or [IDENTITY], np.float32).reshape(-1)
         kinds = np.array([d[3] for d in draws] or [0], np.uint8)
         self.cfs = (abi.ColorFilter * max(n, 1))()
         any_cf = False
         self.shaders = (abi.ImageShader * max(n, 1))()
         images, image_index = [], {}
-        for i, (_, paint, _, _) in enumerate(draws):
+        for i, d in enumerate(draws):
+            paint = d[1]
             self.shaders[i].image = -1
             ish = getattr(paint, "image_shader", None)
             if ish is None:
@@ -396,7 +439,8 @@ class PackedScene:
             self.images[k].height, self.images[k].width = px.shape[0], px.shape[1]
             self.images[k].alpha_type = alpha_type
             self.images[k].pixels = px.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
-        for i, (path, paint, _, _) in enumerate(draws):
+        for i, d in enumerate(draws):
+            path, paint = d[0], d[1]
             if getattr(paint, "color_filter", None) is not None:
                 ctypes.memmove(ctypes.byref(self.cfs[i]), ctypes.byref(paint.color_filter),
                                ctypes.sizeof(abi.ColorFilter))
@@ -440,6 +484,10 @@ class PackedScene:
             s.draw_kinds = kinds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         if any_cf:
             s.color_filters = ctypes.cast(self.cfs, ctypes.POINTER(abi.ColorFilter))
+        self.clips = np.array([d[4] if d[4] is not None else abi.CULL_RECT for d in draws]
+                              or [abi.CULL_RECT], np.float32).reshape(-1)
+        if any(d[4] is not None for d in draws):
+            s.clip_rects = self.clips.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         if images:
             s.images = ctypes.cast(self.images, ctypes.POINTER(abi.Image))
             s.n_images = len(images)

@@ -55,6 +55,9 @@ struct Job {
   int32_t image;        // layer canvas index of an image paint, else -1
   int32_t paint;        // index into the device paint table
   int32_t flags;        // kJob* bits, set per draw
+  // scan clip {l, t, r, b}: the canvas's global clip bounds (kMaxCullRect
+  // when the draw has none)
+  float clip[4] = {-1e9f, -1e9f, 1e9f, 1e9f};
 };
 // The job keeps zero-cover direct spans and three planes per tile (its blend
 // mode changes pixels under a transparent source on this target).
@@ -65,7 +68,13 @@ struct JobGeo {
   float l, t, r, b;     // bounds of the quad path (after translation)
   int32_t edges;        // edges built
   int32_t overflow;     // 1 if more edges than capacity
+  float cl, ct, cr, cb; // the job's scan clip (Job::clip)
 };
+constexpr float kCullExtent = 1e9f;  // kMaxCullRect = {-1e9, -1e9, 1e9, 1e9}
+__host__ __device__ inline bool ClipIsCull(const float c[4]) {
+  return c[0] <= -kCullExtent && c[1] <= -kCullExtent && c[2] >= kCullExtent &&
+         c[3] >= kCullExtent;
+}
 
 // Per-job outputs of the walk stage.
 struct JobCov {

@@ -11,8 +11,12 @@
 //   * every contour is closed by one line from its last quad-path point to
 //     its move point (Path::Iter::AutoClose, path.cc:372-389, and
 //     PathEdgeIter's close line, path_priv.hpp:121-166, give the same edge);
-//   * SWEdge::CanBeIgnored never rejects an edge here: the scan bounds are
-//     the floored/ceiled bounds of the same points, so every edge lies inside.
+//   * SWEdge::CanBeIgnored (sw_edge.cc:72-88) rejects no edge of a job
+//     without a clip: its scan bounds are the floored/ceiled bounds of the
+//     same points, so every edge lies inside.  A job with a clip first
+//     takes its quad path's bounds in a pass of its own, then drops the
+//     edges whose rows lie wholly above or below its scan bounds (full
+//     SetLine / SetQuad in both passes, so the counts agree).
 // Edges are compacted with a wave exclusive scan of per-lane edge counts.
 #ifndef SKITY_HIP_FLATTEN_HPP_
 #define SKITY_HIP_FLATTEN_HPP_
@@ -36,6 +40,11 @@ struct EdgeSink {
   int32_t cap;
   int32_t pos;       // next edge index within the job
   float minx, miny, maxx, maxy;
+  bool clipped;      // CanBeIgnored against [scan_y0, scan_y1)
+  Fixed scan_y0, scan_y1;
+  __device__ bool Keep(Fixed y0, Fixed y1) const {
+    return !clipped || !(y0 >= scan_y1 || y1 <= scan_y0);
+  }
   __device__ void Bound(P2 p) {
     minx = fminf(minx, p.x);
     maxx = fmaxf(maxx, p.x);
@@ -43,26 +52,27 @@ struct EdgeSink {
     maxy = fmaxf(maxy, p.y);
   }
   __device__ void Line(P2 a, P2 b, bool write) {
-    if (!write) {
+    if (!write && !clipped) {
       Fixed y0 = SnapY(ScalarToFixedAA(a.y)), y1 = SnapY(ScalarToFixedAA(b.y));
       if (FixedToFDot6(y1 - y0) != 0) pos++;
       return;
     }
     EdgeRec e;
-    if (SetLine(&e, a, b)) {
-      if (pos < cap) out[pos] = e;
+    if (SetLine(&e, a, b) && Keep(e.upper_y, e.lower_y)) {
+      if (write && pos < cap) out[pos] = e;
       pos++;
     }
   }
   __device__ void MonoQuad(const P2 q[3], bool write) {
-    if (!write) {
+    if (!write && !clipped) {
       int y0 = (int)(q[0].y * 256.f), y2 = (int)(q[2].y * 256.f);
       if (((y0 + 32) >> 6) != ((y2 + 32) >> 6)) pos++;
       return;
     }
     EdgeRec e;
-    if (SetQuad(&e, q)) {
-      if (pos < cap) out[pos] = e;
+    Fixed first_y = 0;
+    if (SetQuad(&e, q, &first_y) && Keep(first_y, e.q_last_y)) {
+      if (write && pos < cap) out[pos] = e;
       pos++;
     }
   }
@@ -162,6 +172,10 @@ __global__ void __launch_bounds__(64)
   const int j = list ? list[blockIdx.x] : (int)blockIdx.x;
   const Job job = jobs[j];
   JobGeo g;
+  g.cl = job.clip[0];
+  g.ct = job.clip[1];
+  g.cr = job.clip[2];
+  g.cb = job.clip[3];
   if (job.kind == kJobRect) {
     if (Lane() == 0) {
       g.l = g.t = g.r = g.b = 0.f;
@@ -182,7 +196,31 @@ __global__ void __launch_bounds__(64)
   sink.cap = write ? edge_cap[j] : 0;
   sink.minx = sink.miny = INFINITY;
   sink.maxx = sink.maxy = -INFINITY;
+  sink.clipped = !ClipIsCull(job.clip);
+  sink.scan_y0 = kMinS32;
+  sink.scan_y1 = kMaxS32;
   int lane = Lane();
+  // pass 0 (clipped jobs only): the quad path's bounds; pass 1: edges
+  for (int pass = sink.clipped ? 0 : 1; pass < 2; pass++) {
+  if (pass == 1 && sink.clipped) {
+    // RastePath's scan bounds: the path bounds intersected with the clip
+    // (Rect::Intersect), floored / ceiled; CanBeIgnored snaps their top
+    // and bottom as SetLine snaps y
+    float l = WaveMinF(sink.minx), t = WaveMinF(sink.miny);
+    float r = WaveMaxF(sink.maxx), b = WaveMaxF(sink.maxy);
+    float il = fmaxf(l, job.clip[0]), ir = fminf(r, job.clip[2]);
+    float it = fmaxf(t, job.clip[1]), ib = fminf(b, job.clip[3]);
+    if (il < ir && it < ib) {
+      sink.scan_y0 = SnapY(ScalarToFixedAA(floorf(it)));
+      sink.scan_y1 = SnapY(ScalarToFixedAA(ceilf(ib)));
+    } else {
+      sink.scan_y0 = kMaxS32;  // no scan rows: every edge is dropped
+      sink.scan_y1 = kMinS32;
+    }
+    sink.minx = sink.miny = INFINITY;
+    sink.maxx = sink.maxy = -INFINITY;
+  }
+  const bool bounds_only = pass == 0;
   int pt_carry = 0, w_carry = 0, edge_carry = 0;
   int move_carry = -1;  // point index of the last move before this chunk
   for (int base = 0; base < nv; base += kWave) {
@@ -217,14 +255,17 @@ __global__ void __launch_bounds__(64)
     }
     // Pass 1 (both modes): count this lane's edges and extend the bounds.
     sink.pos = 0;
+    const bool clip_now = sink.clipped;
+    if (bounds_only) sink.clipped = false;  // the cheap count; nothing kept
     if (seg) {
       P2 qend = VisitSegment(v, pts, pi, w, wi, qstart, job.m, &sink, false, true);
-      if (last) sink.Line(job.m.Map(qend), mv, false);
+      if (last && !bounds_only) sink.Line(job.m.Map(qend), mv, false);
     }
-    int mine = sink.pos;
+    sink.clipped = clip_now;
+    int mine = bounds_only ? 0 : sink.pos;
     int incl = WaveInclusiveSum(mine);
     // Pass 2 (write mode): build the records at the compacted positions.
-    if (write && seg) {
+    if (write && seg && !bounds_only) {
       sink.pos = edge_carry + incl - mine;
       P2 qend = VisitSegment(v, pts, pi, w, wi, qstart, job.m, &sink, true, false);
       if (last) sink.Line(job.m.Map(qend), mv, true);
@@ -234,6 +275,7 @@ __global__ void __launch_bounds__(64)
     w_carry += __builtin_amdgcn_readlane(wi_incl, kWave - 1);
     move_carry = __builtin_amdgcn_readlane(move_pt, kWave - 1);
   }
+  if (pass == 1) {
   float l = WaveMinF(sink.minx), t = WaveMinF(sink.miny);
   float r = WaveMaxF(sink.maxx), b = WaveMaxF(sink.maxy);
   if (lane == 0) {
@@ -244,6 +286,8 @@ __global__ void __launch_bounds__(64)
     g.edges = edge_carry;
     g.overflow = write && edge_carry > edge_cap[j] ? 1 : 0;
     geo[j] = g;
+  }
+  }
   }
 }
 

@@ -197,7 +197,7 @@ HD int DiffToShift(FDot6 dx, FDot6 dy) {
 
 // SWQuadEdge::SetQuad (sw_edge.cc:121-231).  Returns false for a quad whose
 // ends round to the same quarter row.
-HD bool SetQuad(EdgeRec* e, const P2 pts[3]) {
+HD bool SetQuad(EdgeRec* e, const P2 pts[3], Fixed* first_y_out = nullptr) {
   float scale = (float)(1 << (kAcc + 6));
   FDot6 x0 = (int)(pts[0].x * scale), y0 = (int)(pts[0].y * scale);
   FDot6 x1 = (int)(pts[1].x * scale), y1 = (int)(pts[1].y * scale);
@@ -240,6 +240,7 @@ HD bool SetQuad(EdgeRec* e, const P2 pts[3]) {
   e->x = e->dx = e->dy = e->upper_x = e->upper_y = e->lower_y = 0;
   e->packed = PackEdge(w, 1 << shift, shift - 1, true);
   Fixed first_y = e->qy;
+  if (first_y_out) *first_y_out = first_y;  // SWQuadEdge::q_first_y
   UpdateQuad(e, first_y);
   return true;
 }

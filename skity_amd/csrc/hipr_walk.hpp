@@ -115,6 +115,25 @@ __host__ __device__ inline PixBounds JobPixBounds(const JobGeo& g) {
   pb.b = (int)b;
   return pb;
 }
+// Scan bounds of a job (sw_raster.cc:741-754): the quad path's bounds
+// intersected with the job's clip bounds (Rect::Intersect, rect.cc:160-172:
+// empty unless l < r and t < b), floored / ceiled.  Without a clip they are
+// the path's own pixel bounds.  The walk starts at the path's top row
+// (JobPixBounds) and clips everything else to these.
+__host__ __device__ inline PixBounds JobScanBounds(const JobGeo& g) {
+  float l = g.l > g.cl ? g.l : g.cl, r = g.r < g.cr ? g.r : g.cr;
+  float t = g.t > g.ct ? g.t : g.ct, b = g.b < g.cb ? g.b : g.cb;
+  if (!(l < r && t < b)) {
+    PixBounds e{0, 0, 0, 0, true};
+    return e;
+  }
+  JobGeo h = g;
+  h.l = l;
+  h.t = t;
+  h.r = r;
+  h.b = b;
+  return JobPixBounds(h);
+}
 // Visible tile range of a job on its canvas (inclusive); ntc = 0 if none.
 struct TileRange {
   int tc0, tr0, ntc, ntr;
@@ -227,7 +246,8 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   const Job job = P.jobs[j];
   const JobGeo geo = P.geo[j];
   const Canvas cv = P.canvases[job.canvas];
-  PixBounds pb = JobPixBounds(geo);
+  const int path_top = JobPixBounds(geo).t;  // RastePath's start_y
+  PixBounds pb = JobScanBounds(geo);
   TileRange tr = JobTiles(pb, cv.width, cv.height);
   // The LDS row holds the visible columns [ox, ox + W) of the canvas
   // (RowSink folds the deltas of off-canvas pixels), so its width is bounded
@@ -583,7 +603,7 @@ __device__ __forceinline__ void WalkItem(const WalkParams& P, int wi, int32_t* l
   };
 
   // ---- initial activation (sw_raster.cc:554-563) -------------------------------
-  Fixed y = Max32(E[0].upper_y, IntToFixed(pb.t));
+  Fixed y = Max32(E[0].upper_y, IntToFixed(path_top));
   Fixed nny = kMaxS32;
   if (rband > 0) {
     // resume from the band's snapshot

@@ -606,6 +606,10 @@ hipr_result ValidateScene(const hipr_scene* s) {
       for (int k = 0; k < 6; k++)
         if (!std::isfinite(sh.local_matrix[k])) return HIPR_ERROR_INVALID_ARGUMENT;
     }
+    if (s->clip_rects) {
+      for (int k = 0; k < 4; k++)
+        if (std::isnan(s->clip_rects[4 * (size_t)p + k])) return HIPR_ERROR_INVALID_ARGUMENT;
+    }
     if (s->color_filters) {
       const hipr_color_filter& f = s->color_filters[p];
       if (f.type < HIPR_COLOR_FILTER_NONE || f.type > HIPR_COLOR_FILTER_COMPOSE)
@@ -767,7 +771,7 @@ Aff DrawMatrix(const hipr_scene* s, int p) {
 // Fill and/or stroke jobs in paint order (DrawFillStrokeInPaintOrder,
 // src/render/paint_order.hpp:12-31), rasterised through `ctm`.
 void AddShapeJobs(const hipr_scene* s, int p, int canvas, const Aff& ctm,
-                  hipr_prepared_s* pl, std::vector<Job>* out) {
+                  hipr_prepared_s* pl, std::vector<Job>* out, const float* clip = nullptr) {
   const hipr_paint& pt = s->paints[p];
   bool need_fill = pt.style != HIPR_STYLE_STROKE;
   bool need_stroke = pt.style != HIPR_STYLE_FILL;
@@ -785,6 +789,7 @@ void AddShapeJobs(const hipr_scene* s, int p, int canvas, const Aff& ctm,
     jb.w_begin = s->weight_offsets[p];
     jb.m = ToAffine(ctm);
     jb.image = -1;
+    if (clip) std::copy(clip, clip + 4, jb.clip);
     jb.paint = (int32_t)pl->paints.size();
     pl->paints.push_back(SolidOrGradient(s, pt, ctm, stroke, &pl->grads));
     ApplyColorFilter(s, p, &pl->paints.back());
@@ -914,6 +919,11 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
     const hipr_paint& pt = s->paints[p];
     const Aff ctm = DrawMatrix(s, p);
     const int kind = s->draw_kinds ? s->draw_kinds[p] : HIPR_DRAW_PATH;
+    // the canvas's global clip bounds: the scan clip of the path, stroke and
+    // blurred-layer rasters of this draw (DrawPaint rasters without it)
+    const float* clip = s->clip_rects && !ClipIsCull(s->clip_rects + 4 * (size_t)p)
+                            ? s->clip_rects + 4 * (size_t)p
+                            : nullptr;
     if (kind == HIPR_DRAW_PAINT) {
       // OnDrawPaint (sw_canvas.cc:412-438): the target rectangle under the
       // identity, the paint's fill brush (its shader through the canvas
@@ -990,11 +1000,25 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
       ip.im[5] = m.ty;
       const int32_t paint = (int32_t)pl->paints.size();
       if (ctm.Identity()) {
-        // an integer rectangle under the identity: full coverage, no walk
-        ip.L = (int)L;
-        ip.T = (int)T;
-        ip.R = (int)R;
-        ip.B = (int)B;
+        // an integer rectangle under the identity: full coverage over its
+        // scan bounds (the rectangle intersected with the clip bounds,
+        // floored / ceiled), no walk
+        float sl = L, st = T, sr = R, sb = B;
+        if (clip) {
+          sl = std::max(L, clip[0]);
+          st = std::max(T, clip[1]);
+          sr = std::min(R, clip[2]);
+          sb = std::min(B, clip[3]);
+          if (!(sl < sr && st < sb)) sl = st = sr = sb = 0.f;
+          sl = std::floor(sl);
+          st = std::floor(st);
+          sr = std::ceil(sr);
+          sb = std::ceil(sb);
+        }
+        ip.L = (int)sl;
+        ip.T = (int)st;
+        ip.R = (int)sr;
+        ip.B = (int)sb;
         Job jb = RectJob(p, 0, paint);
         jb.image = layer;
         canvas_jobs[0].push_back(jb);
@@ -1019,11 +1043,12 @@ void BuildPlan(const hipr_scene* s, hipr_prepared_s* pl) {
         jb.m = ToAffine(ctm);
         jb.image = layer;
         jb.paint = paint;
+        if (clip) std::copy(clip, clip + 4, jb.clip);
         canvas_jobs[0].push_back(jb);
       }
       pl->paints.push_back(ip);
     } else {
-      AddShapeJobs(s, p, 0, ctm, pl, &canvas_jobs[0]);
+      AddShapeJobs(s, p, 0, ctm, pl, &canvas_jobs[0], clip);
     }
   }
   // layers first (composited and blurred before the canvas), canvas last
@@ -1568,7 +1593,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
                      f.jobs[j].path);
         return HIPR_ERROR_NOT_SUPPORTED;
       }
-      PixBounds pb = JobPixBounds(g);
+      PixBounds pb = JobScanBounds(g);
       const Canvas& cv = f.canvases[f.jobs[j].canvas];
       TileRange tr = JobTiles(pb, cv.width, cv.height);
       // the walk's row holds the job's visible columns only
@@ -1645,7 +1670,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   for (int32_t j : all) {
     // the 32- and 64-slot walks are unbanded
     nbands[j] = bound[j] >= band_min && tier[j] <= 16
-                    ? BandCount(JobPixBounds(ctx->h_geo[j]), band_rows)
+                    ? BandCount(JobScanBounds(ctx->h_geo[j]), band_rows)
                     : 1;
     qb[j] = q_total;
     q_total += (int64_t)nbands[j] * ctx->h_geo[j].edges;
@@ -1721,7 +1746,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     }
     std::vector<std::pair<int64_t, int32_t>> order;
     for (int32_t j : js) {
-      PixBounds pb = JobPixBounds(ctx->h_geo[j]);
+      PixBounds pb = JobScanBounds(ctx->h_geo[j]);
       int64_t work = (int64_t)std::max(bound[j], 1) * std::max(pb.b - pb.t, 1) / nbands[j];
       order.push_back({-work, j});
     }
@@ -1764,7 +1789,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
       double w = 0;
       if (persistent) {
         for (int32_t j : l.jobs) {
-          PixBounds pb = JobPixBounds(ctx->h_geo[j]);
+          PixBounds pb = JobScanBounds(ctx->h_geo[j]);
           w += (double)std::max(bound[j], 1) * std::max(pb.b - pb.t, 1);
         }
       }
@@ -1916,7 +1941,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     // tile
     for (int32_t j : all) {
       const Canvas& cv = f.canvases[f.jobs[j].canvas];
-      const TileRange tr = JobTiles(JobPixBounds(ctx->h_geo[j]), cv.width, cv.height);
+      const TileRange tr = JobTiles(JobScanBounds(ctx->h_geo[j]), cv.width, cv.height);
       const int64_t bands_per_tile =
           nbands[j] > 1 ? 1 + (kTile + std::max(band_rows, 1) - 1) / std::max(band_rows, 1) : 1;
       const int64_t most = 3 * (int64_t)tr.ntc * tr.ntr * bands_per_tile;
@@ -2040,7 +2065,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     surf->job_ebase[j] = ebase[j];
     surf->job_mbase[j] = mbase[j];
     const Canvas& cv = f.canvases[f.jobs[j].canvas];
-    TileRange tr = JobTiles(JobPixBounds(ctx->h_geo[j]), cv.width, cv.height);
+    TileRange tr = JobTiles(JobScanBounds(ctx->h_geo[j]), cv.width, cv.height);
     int32_t* t = &surf->job_tiles[(size_t)j * 5];
     t[0] = tr.tc0;
     t[1] = tr.tr0;

@@ -368,6 +368,7 @@ Matrix Matrix::operator*(const Matrix& b) const {
 // ---- Canvas -----------------------------------------------------------------
 int HIPCanvas::Save() {
   stack_.push_back(stack_.back());
+  clips_.push_back(clips_.back());
   if (recording_) recording_->ops_.push_back({DisplayList::OpKind::kSave, Matrix(), 0});
   return GetSaveCount() - 1;
 }
@@ -375,6 +376,7 @@ int HIPCanvas::Save() {
 void HIPCanvas::Restore() {
   if (stack_.size() > 1) {
     stack_.pop_back();
+    clips_.pop_back();
     if (recording_) recording_->ops_.push_back({DisplayList::OpKind::kRestore, Matrix(), 0});
   }
 }
@@ -392,6 +394,44 @@ void HIPCanvas::Concat(const Matrix& m) {
 void HIPCanvas::SetMatrix(const Matrix& m) {
   stack_.back() = m;
   if (recording_) recording_->ops_.push_back({DisplayList::OpKind::kSetMatrix, m, 0});
+}
+
+void HIPCanvas::ClipRect(const Rect& rect, ClipOp op) {
+  if (recording_) {
+    DisplayList::Op o{DisplayList::OpKind::kClipRect, Matrix(),
+                      op == ClipOp::kDifference ? 1u : 0u, rect};
+    recording_->ops_.push_back(o);
+  }
+  const Matrix& m = stack_.back();
+  if (op == ClipOp::kIntersect) {
+    // CalculateGlobalClipBounds: Matrix::MapRect of the rectangle (corners
+    // as glm's mat4 * vec4 sums them, then their min / max), intersected
+    // with the bounds (Rect::Intersect; SetEmpty when they do not overlap)
+    const float xs[4] = {rect.left, rect.right, rect.right, rect.left};
+    const float ys[4] = {rect.top, rect.top, rect.bottom, rect.bottom};
+    float l = 0, t = 0, r = 0, b = 0;
+    for (int i = 0; i < 4; i++) {
+      float x = (m.sx * xs[i] + m.kx * ys[i]) + (0.f + m.tx);
+      float y = (m.ky * xs[i] + m.sy * ys[i]) + (0.f + m.ty);
+      if (i == 0) {
+        l = r = x;
+        t = b = y;
+      } else {
+        l = std::min(x, l);
+        r = std::max(x, r);
+        t = std::min(y, t);
+        b = std::max(y, b);
+      }
+    }
+    Rect& c = clips_.back();
+    float il = std::max(c.left, l), ir = std::min(c.right, r);
+    float it = std::max(c.top, t), ib = std::min(c.bottom, b);
+    c = il < ir && it < ib ? Rect{il, it, ir, ib} : Rect{0, 0, 0, 0};
+  }
+  // a difference clip, or one under skew / rotation, also clips span lists
+  // (sw_canvas.cc:297-313, OnClipPath)
+  if (!recording_ && (op == ClipOp::kDifference || m.kx != 0.f || m.ky != 0.f))
+    unsupported_ = true;
 }
 
 void HIPCanvas::MarkUnsupported() {
@@ -414,7 +454,7 @@ void HIPCanvas::AddDraw(Draw d, const Matrix& local) {
 }
 
 void HIPCanvas::DrawPath(const Path& path, const Paint& paint) {
-  AddDraw(Draw{path, paint, stack_.back(), HIPR_DRAW_PATH});
+  AddDraw(Draw{path, paint, stack_.back(), HIPR_DRAW_PATH, clips_.back()});
 }
 
 void HIPCanvas::DrawPaint(const Paint& paint) {
@@ -469,7 +509,10 @@ void HIPCanvas::DrawGlyphs(uint32_t count, const Path* const glyph_paths[],
     p.SetStyle(style);
     for (uint32_t k = 0; k < count; k++) {
       const Matrix local = Matrix::Translate(position_x[k], position_y[k]);
-      AddDraw(Draw{*glyph_paths[k], p, stack_.back() * local, HIPR_DRAW_PATH}, local);
+      // FillGlyphs / StrokeGlyphs raster without the clip bounds
+      Draw d{*glyph_paths[k], p, stack_.back() * local, HIPR_DRAW_PATH};
+      d.unclipped = true;
+      AddDraw(std::move(d), local);
     }
   };
   // DrawFillStrokeInPaintOrder (src/render/paint_order.hpp:12-31)
@@ -493,8 +536,9 @@ PackedScene::PackedScene(const std::vector<HIPCanvas::Draw>& draws) {
   kinds_.resize(n);
   filters_.resize(n);
   shaders_.resize(n);
+  clips_.resize(4 * (size_t)n);
   std::vector<const Image*> image_of;
-  bool any_matrix = false, any_paint = false, any_filter = false;
+  bool any_matrix = false, any_paint = false, any_filter = false, any_clip = false;
   for (int32_t i = 0; i < n; i++) {
     const Matrix& m = draws[i].matrix;
     const float mf[6] = {m.sx, m.kx, m.tx, m.ky, m.sy, m.ty};
@@ -502,6 +546,10 @@ PackedScene::PackedScene(const std::vector<HIPCanvas::Draw>& draws) {
     any_matrix |= !m.IsIdentity();
     kinds_[i] = draws[i].kind;
     any_paint |= kinds_[i] != HIPR_DRAW_PATH;
+    const Rect& c = draws[i].clip;
+    const float cf[4] = {c.left, c.top, c.right, c.bottom};
+    std::copy(cf, cf + 4, clips_.begin() + 4 * (size_t)i);
+    any_clip |= !(c.left <= -1e9f && c.top <= -1e9f && c.right >= 1e9f && c.bottom >= 1e9f);
     const Path& p = draws[i].path;
     verbs_.insert(verbs_.end(), p.verbs().begin(), p.verbs().end());
     for (const Point& q : p.points()) {
@@ -553,6 +601,7 @@ PackedScene::PackedScene(const std::vector<HIPCanvas::Draw>& draws) {
   scene_.matrices = any_matrix ? matrices_.data() : nullptr;
   scene_.draw_kinds = any_paint ? kinds_.data() : nullptr;
   scene_.color_filters = any_filter ? filters_.data() : nullptr;
+  scene_.clip_rects = any_clip ? clips_.data() : nullptr;
   if (!images_.empty()) {
     scene_.images = images_.data();
     scene_.n_images = (int32_t)images_.size();
@@ -599,9 +648,14 @@ void DisplayList::Draw(HIPCanvas* canvas) const {
       case OpKind::kDraw: {
         HIPCanvas::Draw d = draws_[op.draw];
         d.matrix = canvas->GetTotalMatrix() * op.matrix;
+        if (d.kind == HIPR_DRAW_PATH && !d.unclipped) d.clip = canvas->GetGlobalClipBounds();
         canvas->AddDraw(std::move(d), op.matrix);
         break;
       }
+      case OpKind::kClipRect:
+        canvas->ClipRect(op.rect, op.draw ? HIPCanvas::ClipOp::kDifference
+                                          : HIPCanvas::ClipOp::kIntersect);
+        break;
       case OpKind::kUnsupported:
         canvas->MarkUnsupported();
         break;

@@ -294,13 +294,20 @@ class HIPCanvas {
   void ResetMatrix() { SetMatrix(Matrix()); }
   Matrix GetTotalMatrix() const { return stack_.back(); }
 
-  // Clips and save layers are not drawn by this backend: the next Flush
+  // Canvas::ClipRect (canvas.cc:130-135,463-476, sw_canvas.cc:297-313): an
+  // intersect clip narrows the canvas's global clip bounds (kept per save
+  // level), the scan clip of later path draws.  Under a matrix with skew or
+  // rotation, or as a difference, the software canvas also clips span lists
+  // (SWCanvas::State::FindSpan, sw_canvas.cc:158-265), and so do path clips:
+  // those, and save layers, are not drawn by this backend: the next Flush
   // returns HIPR_ERROR_NOT_SUPPORTED (and draws nothing) instead of drawing
-  // something other than the software canvas would.  The same holds for
-  // any other call the backend cannot draw exactly (MarkUnsupported).
+  // something other than the software canvas would.  The same holds for any
+  // other call the backend cannot draw exactly (MarkUnsupported).
   enum class ClipOp { kDifference, kIntersect };
-  void ClipRect(const Rect&, ClipOp = ClipOp::kIntersect) { MarkUnsupported(); }
+  void ClipRect(const Rect& rect, ClipOp op = ClipOp::kIntersect);
   void ClipPath(const Path&, ClipOp = ClipOp::kIntersect) { MarkUnsupported(); }
+  // the global clip bounds ({-1e9, -1e9, 1e9, 1e9} when nothing clips)
+  Rect GetGlobalClipBounds() const { return clips_.back(); }
   void SaveLayer(const Rect&, const Paint&) {
     Save();
     MarkUnsupported();
@@ -338,6 +345,8 @@ class HIPCanvas {
     Paint paint;
     Matrix matrix;
     uint8_t kind;  // HIPR_DRAW_*
+    Rect clip{-1e9f, -1e9f, 1e9f, 1e9f};  // scan clip (kMaxCullRect: none)
+    bool unclipped = false;  // glyph rasters ignore the clip bounds
   };
 
  private:
@@ -354,6 +363,7 @@ class HIPCanvas {
   bool unsupported_ = false;
   std::vector<Draw> draws_;
   std::vector<Matrix> stack_{Matrix()};
+  std::vector<Rect> clips_{Rect{-1e9f, -1e9f, 1e9f, 1e9f}};
 };
 
 // The flat arrays of a hipr_scene built from a list of draws (what a flush
@@ -365,7 +375,7 @@ class PackedScene {
 
  private:
   std::vector<uint8_t> verbs_, fills_, kinds_;
-  std::vector<float> pts_, ws_, matrices_;
+  std::vector<float> pts_, ws_, matrices_, clips_;
   std::vector<int32_t> vo_, po_, wo_;
   std::vector<hipr_paint> paints_;
   std::vector<hipr_gradient> grads_;
@@ -389,11 +399,14 @@ class DisplayList {
  private:
   friend class HIPCanvas;
   friend class PictureRecorder;
-  enum class OpKind : uint8_t { kSave, kRestore, kConcat, kSetMatrix, kDraw, kUnsupported };
+  enum class OpKind : uint8_t {
+    kSave, kRestore, kConcat, kSetMatrix, kDraw, kClipRect, kUnsupported
+  };
   struct Op {
     OpKind kind;
     Matrix matrix;
-    uint32_t draw;  // index into draws_ for kDraw
+    uint32_t draw;  // index into draws_ for kDraw; for kClipRect 1 = difference
+    Rect rect{0, 0, 0, 0};
   };
   std::vector<Op> ops_;
   std::vector<HIPCanvas::Draw> draws_;

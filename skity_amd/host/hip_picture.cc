@@ -580,9 +580,21 @@ void Picture::PlayBack(HIPCanvas* canvas) const {
       case DRAW_VERTICES_OBJECT:
         c.Validate(false);
         break;
+      case CLIP_RECT: {
+        // rect, packed op (low 4 bits: 0 = difference), restore offset
+        float r[4];
+        if (!c.ReadPad32(r, sizeof(r))) break;
+        uint32_t packed = c.U32();
+        int32_t offset = c.I32();
+        if (!c.Validate(offset == 0 || ((offset & 3) == 0 && (size_t)offset >= c.Offset())))
+          break;
+        canvas->ClipRect(Rect::MakeLTRB(r[0], r[1], r[2], r[3]),
+                         (packed & 0xF) == 0 ? HIPCanvas::ClipOp::kDifference
+                                             : HIPCanvas::ClipOp::kIntersect);
+        break;
+      }
       // canvas calls this backend does not draw
       case CLIP_PATH:
-      case CLIP_RECT:
       case CLIP_RRECT:
       case DRAW_RRECT:
       case DRAW_DRRECT:

@@ -231,8 +231,8 @@ static void PixmapShaderDrawsTheImage(hipr_context ctx) {
 
 // PictureRecorder + DisplayList::Draw (display_list.cc:330-343): replaying
 // the recording onto a canvas under a matrix draws what the same calls made
-// directly under that matrix draw; a recorded clip makes the flush
-// NOT_SUPPORTED and draws nothing.
+// directly under that matrix draw; a recorded difference clip makes the
+// flush NOT_SUPPORTED and draws nothing.
 static void DisplayListReplayEqualsDirectDraws(hipr_context ctx) {
   auto calls = [](HIPCanvas* c) {
     Paint red;
@@ -275,7 +275,7 @@ static void DisplayListReplayEqualsDirectDraws(hipr_context ctx) {
   PictureRecorder rec2;
   rec2.BeginRecording(Rect::MakeWH(64.f, 64.f));
   calls(rec2.GetRecordingCanvas());
-  rec2.GetRecordingCanvas()->ClipRect(Rect::MakeWH(8.f, 8.f));
+  rec2.GetRecordingCanvas()->ClipRect(Rect::MakeWH(8.f, 8.f), HIPCanvas::ClipOp::kDifference);
   auto dl2 = rec2.FinishRecording();
   HIPCanvas* cc = a->LockCanvas();
   dl2->Draw(cc);
@@ -329,6 +329,39 @@ static void GlyphRunEqualsTranslatedPaths(hipr_context ctx) {
   EXPECT_EQ(s1->Flush(), HIPR_ERROR_NOT_SUPPORTED);
 }
 
+// Canvas::ClipRect (canvas.cc:130-135, sw_canvas.cc:297-313): an intersect
+// clip under scale + translate limits later path draws to the floored /
+// ceiled clip bounds, Restore lifts it, DrawPaint ignores it; a difference
+// clip makes the flush NOT_SUPPORTED.
+static void ClipRectLimitsDraws(hipr_context ctx) {
+  auto s = HIPSurface::Make(ctx, 40, 40);
+  HIPCanvas* c = s->LockCanvas();
+  Paint red;
+  red.SetColor(Color_RED);
+  c->Save();
+  c->Translate(2.f, 1.f);
+  c->Scale(2.f, 2.f);
+  c->ClipRect(Rect::MakeLTRB(2.2f, 3.f, 8.f, 9.6f));  // device [6.4, 7) - (18, 20.2)
+  c->DrawRect(Rect::MakeWH(40.f, 40.f), red);
+  c->Restore();
+  EXPECT_EQ(s->Flush(), HIPR_SUCCESS);
+  EXPECT_EQ(s->GetPixel(6, 7), Color_RED);
+  EXPECT_EQ(s->GetPixel(17, 20), Color_RED);
+  EXPECT_EQ(s->GetPixel(5, 10), Color_TRANSPARENT);
+  EXPECT_EQ(s->GetPixel(18, 10), Color_TRANSPARENT);
+  EXPECT_EQ(s->GetPixel(10, 6), Color_TRANSPARENT);
+  EXPECT_EQ(s->GetPixel(10, 21), Color_TRANSPARENT);
+  c = s->LockCanvas();
+  c->ClipRect(Rect::MakeLTRB(0.f, 0.f, 4.f, 4.f));
+  c->DrawColor(Color_BLUE);  // DrawPaint: not clipped
+  EXPECT_EQ(s->Flush(), HIPR_SUCCESS);
+  EXPECT_EQ(s->GetPixel(30, 30), Color_BLUE);
+  c = s->LockCanvas();
+  c->ClipRect(Rect::MakeLTRB(0.f, 0.f, 4.f, 4.f), HIPCanvas::ClipOp::kDifference);
+  c->DrawRect(Rect::MakeWH(40.f, 40.f), red);
+  EXPECT_EQ(s->Flush(), HIPR_ERROR_NOT_SUPPORTED);
+}
+
 int main() {
   hipr_context ctx = nullptr;
   if (hipr_context_create(0, &ctx) != HIPR_SUCCESS) {
@@ -349,7 +382,8 @@ int main() {
                {"Shader.TwoPointConicalFactory", TwoPointConicalFactory},
                {"Shader.PixmapShaderDrawsTheImage", PixmapShaderDrawsTheImage},
                {"DisplayList.ReplayEqualsDirectDraws", DisplayListReplayEqualsDirectDraws},
-               {"Canvas.GlyphRunEqualsTranslatedPaths", GlyphRunEqualsTranslatedPaths}};
+               {"Canvas.GlyphRunEqualsTranslatedPaths", GlyphRunEqualsTranslatedPaths},
+               {"Canvas.ClipRectLimitsDraws", ClipRectLimitsDraws}};
   for (auto& t : tests) {
     int before = g_failed;
     t.fn(ctx);

@@ -299,3 +299,45 @@ def pixmap_scene(seed, w=240, h=180, n=30):
         cv.draw_path(random_path(rng, cx, cy, size), paint)
         cv.restore()
     return cv.scene()
+
+
+def clip_scene(seed, w=256, h=192, n=40):
+    """Draws under nested Save / ClipRect / Restore: integer, fractional,
+    partly off-canvas and empty clips, under scale + translate matrices;
+    fills, strokes, gradients, blurred draws, pixmap shaders, DrawPaint
+    (which ignores the clip) and other blend modes."""
+    rng = np.random.default_rng(seed)
+    cv = Canvas(w, h)
+    depth = 0
+    img = random_image(rng, 17, 11, abi.ALPHA_PREMUL)
+    for i in range(n):
+        r = rng.random()
+        if r < 0.2 and depth < 4:
+            cv.save()
+            depth += 1
+            if rng.random() < 0.5:
+                cv.translate(float(rng.uniform(-20, 20)), float(rng.uniform(-20, 20)))
+                cv.scale(float(rng.uniform(0.6, 1.6)), float(rng.uniform(0.6, 1.6)))
+            l, t = float(rng.uniform(-30, w - 20)), float(rng.uniform(-30, h - 20))
+            if rng.random() < 0.5:
+                l, t = float(round(l)), float(round(t))
+            cv.clip_rect(l, t, l + float(rng.uniform(0, 160)), t + float(rng.uniform(0, 120)))
+            continue
+        if r < 0.3 and depth > 0:
+            cv.restore()
+            depth -= 1
+            continue
+        cx, cy = float(rng.uniform(0, w)), float(rng.uniform(0, h))
+        size = float(rng.uniform(10, 60))
+        blend = None
+        if i % 9 == 8:
+            blend = int(rng.choice([abi.BLEND_SRC, abi.BLEND_XOR, abi.BLEND_MULTIPLY]))
+        paint = random_paint(rng, size, cx, cy, allow_blur=True, blend=blend)
+        if i % 7 == 3 and paint.gradient is None:
+            paint.set_image_shader(img, abi.ALPHA_PREMUL, abi.FILTER_LINEAR, abi.TILE_REPEAT,
+                                   abi.TILE_MIRROR, (1.5, 0.0, cx, 0.0, 1.5, cy))
+        if i % 13 == 12:
+            cv.draw_paint(Paint((rng.random(), rng.random(), rng.random(), 0.3)))
+        else:
+            cv.draw_path(random_path(rng, cx, cy, size), paint)
+    return cv.scene()

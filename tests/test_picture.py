@@ -202,7 +202,8 @@ def test_calls_not_drawn_make_the_recording_unsupported():
     paths = [K.path_bytes([0, 1, 1, 5], [(0, 0), (50, 0), (0, 50)])]
     draw = K.op_bytes("DRAW_PATH", ("i", 1), ("i", 1))
     cases = {
-        "clip rect": K.op_bytes("CLIP_RECT", struct.pack("<4f", 0, 0, 10, 10), ("I", 1), ("i", 0)),
+        "difference clip rect": K.op_bytes("CLIP_RECT", struct.pack("<4f", 0, 0, 10, 10),
+                                           ("I", 0), ("i", 0)),
         "clip path": K.op_bytes("CLIP_PATH", ("i", 1), ("I", 1), ("i", 0)),
         "save layer": K.op_bytes("SAVE_LAYER_SAVELAYERREC", ("I", 0)),
         "rrect": K.op_bytes("DRAW_RRECT", ("i", 1), struct.pack("<12f", *range(12))),
@@ -231,6 +232,36 @@ def test_calls_not_drawn_make_the_recording_unsupported():
     rec = K.Recording(K.skp_bytes([draw], paints, inv), 64, 64)
     assert rec.rc == abi.ERROR_NOT_SUPPORTED
     rec.close()
+
+
+def test_clip_rect_narrows_later_draws():
+    """CLIP_RECT (intersect) under a scale + translate matrix: the same calls
+    on the canvas mirror draw identical pixels; Restore pops the clip."""
+    paints = [K.paint_bytes((0.2, 0.6, 0.3, 1.0))]
+    paths = [K.path_bytes([0, 1, 1, 1, 5], [(0, 0), (60, 0), (60, 50), (0, 50)])]
+    draw = K.op_bytes("DRAW_PATH", ("i", 1), ("i", 1))
+    ops = [K.op_bytes("SAVE"), K.op_bytes("SCALE", ("f", 1.5), ("f", 1.25)),
+           K.op_bytes("CLIP_RECT", struct.pack("<4f", 5.5, 4.0, 30.25, 22.0), ("I", 1), ("i", 0)),
+           draw, K.op_bytes("RESTORE"), K.op_bytes("TRANSLATE", ("f", 50.0), ("f", 40.0)), draw]
+    rec = K.Recording(K.skp_bytes(ops, paints, paths), 128, 128)
+    assert rec.rc == 0, rec.error
+    got = _oracle(rec)
+    rec.close()
+    cv = Canvas(128, 128)
+    p = Paint(_q((0.2, 0.6, 0.3, 1.0)))
+    path = Path().move_to(0, 0).line_to(60, 0).line_to(60, 50).line_to(0, 50).close()
+    cv.save()
+    cv.save()
+    cv.scale(1.5, 1.25)
+    cv.clip_rect(5.5, 4.0, 30.25, 22.0)
+    cv.draw_path(path, p)
+    cv.restore()
+    cv.translate(50.0, 40.0)
+    cv.draw_path(path, p)
+    sc = cv.scene()
+    want = O.render_ex(sc.scene, 128, 128, abi.ALPHA_PREMUL, abi.COLOR_RGBA)
+    assert np.array_equal(got, want)
+    assert got[20, 20, 3] == 255 and got[10, 60, 3] == 0  # inside / outside the clip
 
 
 def test_ops_that_end_playback_end_it():
