@@ -1101,6 +1101,16 @@ constexpr int kBandRowsEstimates = 64;
 // (Banding every path costs more schedule work than it saves; the
 // autotuner tries other thresholds.)
 constexpr int kBandMinEdges = 4 * kWave + 1;
+// ... and so are paths whose scan bounds span this many rows: a large
+// outline with a short active list (the tiger at 4096^2: 2,000-4,000 steps
+// per path) is as long a sequential walk, and its schedule pass costs a
+// fraction of the write it splits (tiger.skp at 4096^2, learned: walk
+// 34.5 -> 10 ms when every path is banded at 64 rows).  In a frame of many
+// paths they are not the critical path, and banding all of them costs more
+// than it saves (curves + strokes at 2048^2, learned: 175 -> 231 ms), hence
+// the frame-relative threshold below.
+constexpr int kBandMinRows = 512;
+constexpr int kBandSlots = 512;
 
 // LDS of a walk wave: row deltas, three per-tile-column rows (two slots,
 // written rows), five scratch arrays of K * 64 entries.
@@ -1664,14 +1674,30 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
       mode == 0 ? 0
       : (opt && opt->band_min_edges != 0) ? std::max(opt->band_min_edges, 0)
                                           : kBandMinEdges;
+  // (estimate modes, default split only) paths this tall are banded too:
+  // at least kBandMinRows, and taller than the frame's rows spread over
+  // kBandSlots concurrent walks, so that only paths that would outlast the
+  // rest of the walk are split (a frame of many paths keeps its unbanded
+  // walks; one of a few large outlines bands them)
+  int band_min_rows = 0;
+  if (mode != 0 && !(opt && opt->band_min_edges != 0)) {
+    int64_t total_rows = 0;
+    for (int32_t j : all) {
+      const PixBounds b = JobScanBounds(ctx->h_geo[j]);
+      if (!b.empty) total_rows += b.b - b.t;
+    }
+    band_min_rows = (int)std::min<int64_t>(
+        std::max<int64_t>(kBandMinRows, total_rows / kBandSlots), INT32_MAX);
+  }
   std::vector<int32_t> nbands(nj, 1);
   std::vector<int64_t> qb(nj, 0), sb(nj, 0);
   int64_t q_total = 0, snap_total = 0;
   for (int32_t j : all) {
     // the 32- and 64-slot walks are unbanded
-    nbands[j] = bound[j] >= band_min && tier[j] <= 16
-                    ? BandCount(JobScanBounds(ctx->h_geo[j]), band_rows)
-                    : 1;
+    const PixBounds sb_j = JobScanBounds(ctx->h_geo[j]);
+    // a tall path is a long sequential walk too, whatever its list width
+    const bool tall = band_min_rows > 0 && !sb_j.empty && sb_j.b - sb_j.t >= band_min_rows;
+    nbands[j] = (bound[j] >= band_min || tall) && tier[j] <= 16 ? BandCount(sb_j, band_rows) : 1;
     qb[j] = q_total;
     q_total += (int64_t)nbands[j] * ctx->h_geo[j].edges;
   }
