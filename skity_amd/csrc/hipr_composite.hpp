@@ -406,11 +406,12 @@ struct CompositeParams {
 
 // SWSpanBrush::BrushH for one pixel and cover (sw_span_brush.cc:108-138):
 // AlphaMulQ by the cover unless it is 255, then the target blend.
+template <bool kFilters>
 __device__ __forceinline__ uint32_t ApplyCover(uint32_t dst, uint32_t color, uint32_t cover,
                                                int mode, bool premul, const PaintDev& pd,
                                                const uint8_t* srgb) {
   uint32_t src = cover != 255 ? AlphaMulQ(color, cover) : color;
-  if (pd.cf_type != HIPR_COLOR_FILTER_NONE) src = FilterWord(src, pd, srgb);
+  if (kFilters && pd.cf_type != HIPR_COLOR_FILTER_NONE) src = FilterWord(src, pd, srgb);
   if (mode == HIPR_BLEND_SRC_OVER && premul) return BlendSrcOver(src, dst);
   return BlendStored(src, dst, mode, premul);
 }
@@ -491,6 +492,9 @@ __device__ uint32_t PixmapColor(const PaintDev& pd, const ImageDev& im,
   return pd.pix_unpremul ? PremulWord(o) : o;
 }
 
+// kRich: some draw of the frame has a colour filter or a pixmap shader; the
+// frames without them run a variant that carries neither.
+template <bool kRich>
 __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
   __shared__ GradStops stops;
   const int lane = Lane();
@@ -556,7 +560,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
       int jj = jb + b;
       int q0 = __shfl(s0, b, kWave), q1 = __shfl(s1, b, kWave);
       const Job job = P.jobs[jj];
-      const PaintDev pd = P.paints[job.paint];
+      // fields are read where used (the paint index is wave-uniform)
+      const PaintDev& pd = P.paints[job.paint];
       const int mode = pd.blend;
       // covers of the four pixels: plane 0 and plane 1 of a walked job, or
       // 255 inside a rectangle job's rectangle
@@ -591,8 +596,8 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
         for (int q = 0; q < 4; q++) {
           uint32_t c0 = (cov0 >> (8 * q)) & 0xFF, c1 = (cov1 >> (8 * q)) & 0xFF;
           bool d0 = three ? ((direct >> (8 * q)) & 0xFF) != 0 : c0 != 0;
-          if (d0) pix[q] = ApplyCover(pix[q], pd.color, c0, mode, premul, pd, P.srgb);
-          if (c1) pix[q] = ApplyCover(pix[q], pd.color, c1, mode, premul, pd, P.srgb);
+          if (d0) pix[q] = ApplyCover<kRich>(pix[q], pd.color, c0, mode, premul, pd, P.srgb);
+          if (c1) pix[q] = ApplyCover<kRich>(pix[q], pd.color, c1, mode, premul, pd, P.srgb);
           blends += (int)d0 + (c1 != 0);
           touched |= (uint32_t)(d0 || c1 != 0) << q;
         }
@@ -617,12 +622,12 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
             uint32_t color =
                 pd.kind == kPaintSolid    ? pd.color
                 : g                       ? GradientColor(*g, stops, px0 + q, py)
-                : pd.kind == kPaintPixmap ? PixmapColor(pd, P.images[pd.pix_img],
-                                                        P.image_pixels, px0 + q, py)
+                : kRich && pd.kind == kPaintPixmap
+                    ? PixmapColor(pd, P.images[pd.pix_img], P.image_pixels, px0 + q, py)
                                           : ImageColor(pd, P.canvases[pd.layer], px0 + q, py);
             // Brush: cover & global alpha (sw_span_brush.cc:101)
-            if (d0) pix[q] = ApplyCover(pix[q], color, c0 & ga, mode, premul, pd, P.srgb);
-            if (c1) pix[q] = ApplyCover(pix[q], color, c1 & ga, mode, premul, pd, P.srgb);
+            if (d0) pix[q] = ApplyCover<kRich>(pix[q], color, c0 & ga, mode, premul, pd, P.srgb);
+            if (c1) pix[q] = ApplyCover<kRich>(pix[q], color, c1 & ga, mode, premul, pd, P.srgb);
             blends += (int)d0 + (c1 != 0);
             touched |= 1u << q;
           }

@@ -2122,6 +2122,18 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   cp.srgb = ctx->srgb.as<uint8_t>();
   cp.images = pl->d_images.as<ImageDev>();
   cp.image_pixels = pl->image_pixels.as<uint8_t>();
+  // the composite variant with colour filters and pixmap sampling only when
+  // a paint of the frame needs them
+  bool rich = false;
+  for (const PaintDev& pd : pl->paints)
+    rich |= pd.cf_type != HIPR_COLOR_FILTER_NONE || pd.kind == kPaintPixmap;
+  auto composite = [&](const CompositeParams& p, int tiles) {
+    if (rich) {
+      hipLaunchKernelGGL(CompositeKernel<true>, dim3(tiles), dim3(64), 0, st, p);
+    } else {
+      hipLaunchKernelGGL(CompositeKernel<false>, dim3(tiles), dim3(64), 0, st, p);
+    }
+  };
   int main_tiles = 0;
   for (int t = 0; t < n_targets; t++) main_tiles += f.canvases[t].tiles_x * f.canvases[t].tiles_y;
   if ((int)f.canvases.size() > n_targets) {
@@ -2136,8 +2148,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     lp.canvases = ctx->layer_canvases.as<Canvas>();
     lp.n_canvases = (int)lc.size();
     lp.total_tiles = layer_tiles;
-    STAGE_LAUNCH(ctx, HIPR_STAGE_LAYERS,
-    hipLaunchKernelGGL(CompositeKernel, dim3(layer_tiles), dim3(64), 0, st, lp));
+    STAGE_LAUNCH(ctx, HIPR_STAGE_LAYERS, composite(lp, layer_tiles));
     HIP_TRY(hipGetLastError());
     // Layers of the blur styles other than kNormal keep their unblurred
     // pixels for the style pass.
@@ -2222,8 +2233,7 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
     return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
   HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, 2 * sizeof(unsigned long long), st));
   mp.counters = ctx->counters.as<unsigned long long>();
-  STAGE_LAUNCH(ctx, HIPR_STAGE_COMPOSITE,
-    hipLaunchKernelGGL(CompositeKernel, dim3(main_tiles), dim3(64), 0, st, mp));
+  STAGE_LAUNCH(ctx, HIPR_STAGE_COMPOSITE, composite(mp, main_tiles));
   HIP_TRY(hipGetLastError());
   unsigned long long counts[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(counts, ctx->counters.p, sizeof(counts), hipMemcpyDeviceToHost, st));
