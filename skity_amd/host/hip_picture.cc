@@ -141,7 +141,7 @@ struct Cursor {
   // byte-exact read for the container stream
   bool Read(void* dst, size_t n) {
     if (!Validate(ok && Left() >= n)) return false;
-    std::memcpy(dst, p, n);
+    if (n) std::memcpy(dst, p, n);
     p += n;
     return true;
   }
@@ -164,7 +164,7 @@ struct Cursor {
   bool ReadPad32(void* dst, size_t n) {
     const uint8_t* a = Skip(n);
     if (!a) return false;
-    std::memcpy(dst, a, n);
+    if (n) std::memcpy(dst, a, n);  // dst may be an empty vector's null data()
     return true;
   }
 };

@@ -5,7 +5,8 @@ the benchmark workloads (tests/sanitize/sanitize_check.cc, built by the
 Makefile beside it with clang's -fsanitize=address,undefined on host code
 only).  The run aborts on the first sanitizer report; it also checks that
 the threaded oracle render equals the single-threaded one and that the host
-stroker's outlines equal the oracle's bit for bit."""
+stroker's outlines equal the oracle's bit for bit.  A second build runs the
+C++ front end's SKP reader over a real SKP and corrupted copies of it."""
 import os
 import shutil
 import subprocess
@@ -29,3 +30,23 @@ def test_sanitized_cpu_code():
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "sanitized run clean" in r.stdout
+
+
+def test_sanitized_picture_reader():
+    """The SKP reader, playback, display-list replay and scene packing of
+    the C++ front end under the sanitizers, over tiger.skp and 400 seeded
+    corruptions and truncations of it (tests/sanitize/picture_check.cc)."""
+    if not os.path.exists(CLANG) or shutil.which("make") is None:
+        pytest.skip("clang++/make not available")
+    lib = os.path.join(HERE, "..", "skity_amd", "libskity_hip.so")
+    if not os.path.exists(lib):
+        pytest.skip("libskity_hip.so is not built")
+    subprocess.run(["make", "-s", "-j8", "picture_check"], cwd=DIR, check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
+               UBSAN_OPTIONS="print_stacktrace=1")
+    skp = os.path.join(HERE, "golden", "skp", "tiger.skp")
+    r = subprocess.run([os.path.join(DIR, "picture_check"), skp], cwd=DIR, env=env,
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "picture run clean" in r.stdout
