@@ -67,3 +67,22 @@ def test_clipped_batch_matches_single(ctx):
     b.close()
     for p in preps:
         p.close()
+
+
+def test_clipped_scene_in_sub_plans(ctx):
+    """A clipped frame split into sub-plans (a small chunk budget) equals the
+    frame drawn in one run: the jobs carry their clips into every chunk."""
+    sc = clip_scene(9, w=320, h=240, n=80)
+    imgs = []
+    for chunk in (0, 2048):
+        surf = device.Surface(ctx, sc.width, sc.height)
+        surf.clear(0)
+        surf.draw(sc.scene, mode=abi.MODE_LEARNED, chunk_points=chunk)
+        imgs.append(surf.read())
+        surf.close()
+    want = O.render(sc.scene, sc.width, sc.height)
+    for k, got in enumerate(imgs):
+        mx, _ = diff_report(got, want)
+        print(f"clipped scene, chunk budget {(0, 2048)[k]}: max diff {mx}", flush=True)
+        assert mx <= 1
+    assert np.array_equal(imgs[0], imgs[1])
