@@ -16,8 +16,8 @@
  * Results are hipr_result codes.  HIPR_ERROR_NOT_SUPPORTED from
  * skity_hip_picture_make names SKP contents the reader does not parse
  * (skity_hip_picture_last_error says which); from the display-list calls it
- * means the recording holds a call the backend does not draw (clips, save
- * layers, text, images, shadows, arcs, rounded rects, paint effects other
+ * means the recording holds a call the backend does not draw (span-list clips, save
+ * layers, text, images, shadows, arcs, paint effects other
  * than a blend mode), and nothing is drawn. */
 #ifndef SKITY_HIP_PICTURE_H_
 #define SKITY_HIP_PICTURE_H_

@@ -116,6 +116,235 @@ Path& Path::AddCircle(float x, float y, float radius, Direction dir) {
   return *this;
 }
 
+// ---- RRect (src/geometry/rrect.cc) -------------------------------------------
+namespace {
+bool RectFinite(const Rect& r) {
+  return std::isfinite(r.left) && std::isfinite(r.top) && std::isfinite(r.right) &&
+         std::isfinite(r.bottom);
+}
+bool RectEmpty(const Rect& r) { return !(r.left < r.right && r.top < r.bottom); }
+// ScaleToSides::AdjustRadii (rrect.cc:17-53)
+void AdjustRadii(double limit, double scale, float* a, float* b) {
+  *a = (float)((double)*a * scale);
+  *b = (float)((double)*b * scale);
+  if (*a + *b > limit) {
+    float* min_r = a;
+    float* max_r = b;
+    if (*min_r > *max_r) std::swap(min_r, max_r);
+    float new_min = *min_r;
+    float new_max = (float)(limit - new_min);
+    while (new_max + new_min > limit) new_max = std::nextafter(new_max, 0.0f);
+    *max_r = new_max;
+  }
+}
+void FlushToZero(float& a, float& b) {
+  if (a + b == a) {
+    b = 0;
+  } else if (a + b == b) {
+    a = 0;
+  }
+}
+bool ClampToZero(Point radii[4]) {
+  bool all_square = true;
+  for (int i = 0; i < 4; i++) {
+    if (radii[i].x <= 0 || radii[i].y <= 0) {
+      radii[i] = {0, 0};
+    } else {
+      all_square = false;
+    }
+  }
+  return all_square;
+}
+double MinScale(double r1, double r2, double limit, double cur) {
+  return r1 + r2 > limit ? std::min(cur, limit / (r1 + r2)) : cur;
+}
+bool RadiusPredicates(float rad, float mn, float mx) {
+  return (mn <= mx) && (rad <= mx - mn) && (mn + rad <= mx) && (mx - rad >= mn) && rad >= 0;
+}
+}  // namespace
+
+bool RRect::InitializeRect(const Rect& rect) {
+  if (!RectFinite(rect)) {
+    *this = RRect();
+    return false;
+  }
+  // Rect::MakeSorted
+  rect_ = Rect{std::min(rect.left, rect.right), std::min(rect.top, rect.bottom),
+               std::max(rect.left, rect.right), std::max(rect.top, rect.bottom)};
+  if (RectEmpty(rect_)) {
+    type_ = kEmpty;
+    return false;
+  }
+  return true;
+}
+
+void RRect::SetRect(const Rect& rect) {
+  if (!InitializeRect(rect)) return;
+  for (auto& r : radii_) r = {0, 0};
+  type_ = kRect;
+}
+
+void RRect::SetOval(const Rect& oval) {
+  if (!InitializeRect(oval)) return;
+  // Rect::HalfWidth / HalfHeight
+  float x_rad = (rect_.right - rect_.left) * 0.5f, y_rad = (rect_.bottom - rect_.top) * 0.5f;
+  if (x_rad == 0.f || y_rad == 0.f) {
+    type_ = kRect;
+  } else {
+    for (auto& r : radii_) r = {x_rad, y_rad};
+    type_ = kOval;
+  }
+}
+
+void RRect::SetRectXY(const Rect& rect, float x_rad, float y_rad) {
+  if (!InitializeRect(rect)) return;
+  if (!std::isfinite(x_rad) && !std::isfinite(y_rad)) x_rad = y_rad = 0;
+  const float w = rect_.right - rect_.left, h = rect_.bottom - rect_.top;
+  if (w < x_rad + x_rad || h < y_rad + y_rad) {
+    // SkityIEEEFloatDivided: IEEE division, inf for a zero divisor
+    float scale = std::min(w / (x_rad + x_rad), h / (y_rad + y_rad));
+    x_rad *= scale;
+    y_rad *= scale;
+  }
+  if (x_rad <= 0 || y_rad <= 0) {
+    SetRect(rect);
+    return;
+  }
+  for (auto& r : radii_) r = {x_rad, y_rad};
+  type_ = kSimple;
+  if (x_rad >= 0.5f * w && y_rad >= 0.5f * h) type_ = kOval;
+}
+
+bool RRect::ScaleRadii() {
+  float scale = 1.0f;
+  const double width = (double)rect_.right - (double)rect_.left;
+  const double height = (double)rect_.bottom - (double)rect_.top;
+  scale = (float)MinScale(radii_[0].x, radii_[1].x, width, scale);
+  scale = (float)MinScale(radii_[1].y, radii_[2].y, height, scale);
+  scale = (float)MinScale(radii_[2].x, radii_[3].x, width, scale);
+  scale = (float)MinScale(radii_[3].y, radii_[0].y, height, scale);
+  FlushToZero(radii_[0].x, radii_[1].x);
+  FlushToZero(radii_[1].y, radii_[2].y);
+  FlushToZero(radii_[2].x, radii_[3].x);
+  FlushToZero(radii_[3].y, radii_[0].y);
+  if (scale < 1.0f) {
+    AdjustRadii(width, scale, &radii_[0].x, &radii_[1].x);
+    AdjustRadii(height, scale, &radii_[1].y, &radii_[2].y);
+    AdjustRadii(width, scale, &radii_[2].x, &radii_[3].x);
+    AdjustRadii(height, scale, &radii_[3].y, &radii_[0].y);
+  }
+  ClampToZero(radii_);
+  ComputeType();
+  return scale < 1.0f;
+}
+
+// AreRectAndRadiiValid (rrect.cc:309-325): the checks ComputeType's complex
+// case falls back on
+bool RRect::RadiiValid() const {
+  if (!RectFinite(rect_) || !(rect_.left <= rect_.right && rect_.top <= rect_.bottom)) return false;
+  for (int i = 0; i < 4; i++)
+    if (!RadiusPredicates(radii_[i].x, rect_.left, rect_.right) ||
+        !RadiusPredicates(radii_[i].y, rect_.top, rect_.bottom))
+      return false;
+  return true;
+}
+
+void RRect::ComputeType() {
+  if (RectEmpty(rect_)) {
+    type_ = kEmpty;
+    return;
+  }
+  bool all_equal = true;
+  bool all_square = 0 == radii_[0].x || 0 == radii_[0].y;
+  for (int i = 1; i < 4; i++) {
+    if (0 != radii_[i].x && 0 != radii_[i].y) all_square = false;
+    if (radii_[i].x != radii_[i - 1].x || radii_[i].y != radii_[i - 1].y) all_equal = false;
+  }
+  if (all_square) {
+    type_ = kRect;
+    return;
+  }
+  if (all_equal) {
+    type_ = radii_[0].x >= 0.5f * (rect_.right - rect_.left) &&
+                    radii_[0].y >= 0.5f * (rect_.bottom - rect_.top)
+                ? kOval
+                : kSimple;
+    return;
+  }
+  type_ = kComplex;
+  if (!RadiiValid()) SetRect(rect_);
+}
+
+void RRect::SetRectRadii(const Rect& rect, const Point radii[4]) {
+  if (!InitializeRect(rect)) return;
+  for (int i = 0; i < 4; i++) radii_[i] = radii[i];
+  if (ClampToZero(radii_)) {
+    SetRect(rect);
+    return;
+  }
+  ScaleRadii();
+  if (type_ == kComplex && !RadiiValid()) SetRect(rect);
+}
+
+Path& Path::AddRRect(const RRect& rrect, Direction dir) {
+  // legacy start indices: 6 (CW) and 7 (CCW)
+  return AddRRect(rrect, dir, dir == Direction::kCW ? 6 : 7);
+}
+
+Path& Path::AddRRect(const RRect& rrect, Direction dir, uint32_t start) {
+  const Rect& b = rrect.GetBounds();
+  if (rrect.IsRect() || rrect.IsEmpty()) return AddRect(b, dir, (start + 1) / 2);
+  if (rrect.IsOval()) return AddOval(b, dir, start / 2);
+  const bool cw = dir == Direction::kCW;
+  const bool starts_with_conic = ((start & 1) == (cw ? 1u : 0u));
+  const float weight = 0.707106781f;  // FloatRoot2Over2
+  // Path_RRectPointIterator / Path_RectPointIterator (path.cc:131-178)
+  const float L = b.left, T = b.top, R = b.right, B = b.bottom;
+  using C = RRect::Corner;
+  const Point rp[8] = {{L + rrect.Radii(C::kUpperLeft).x, T},
+                       {R - rrect.Radii(C::kUpperRight).x, T},
+                       {R, T + rrect.Radii(C::kUpperRight).y},
+                       {R, B - rrect.Radii(C::kLowerRight).y},
+                       {R - rrect.Radii(C::kLowerRight).x, B},
+                       {L + rrect.Radii(C::kLowerLeft).x, B},
+                       {L, B - rrect.Radii(C::kLowerLeft).y},
+                       {L, T + rrect.Radii(C::kUpperLeft).y}};
+  const Point cp[4] = {{L, T}, {R, T}, {R, B}, {L, B}};
+  uint32_t ri = start % 8, ci = (start / 2 + (cw ? 0 : 1)) % 4;
+  auto rnext = [&]() -> Point {
+    ri = (ri + (cw ? 1 : 7)) % 8;
+    return rp[ri];
+  };
+  auto cnext = [&]() -> Point {
+    ci = (ci + (cw ? 1 : 3)) % 4;
+    return cp[ci];
+  };
+  MoveTo(rp[ri].x, rp[ri].y);
+  if (starts_with_conic) {
+    for (int i = 0; i < 3; i++) {
+      Point c = cnext(), e = rnext();
+      ConicTo(c.x, c.y, e.x, e.y, weight);
+      Point l = rnext();
+      LineTo(l.x, l.y);
+    }
+    Point c = cnext(), e = rnext();
+    ConicTo(c.x, c.y, e.x, e.y, weight);
+  } else {
+    for (int i = 0; i < 4; i++) {
+      Point l = rnext();
+      LineTo(l.x, l.y);
+      Point c = cnext(), e = rnext();
+      ConicTo(c.x, c.y, e.x, e.y, weight);
+    }
+  }
+  return Close();
+}
+
+Path& Path::AddRoundRect(const Rect& rect, float rx, float ry, Direction dir) {
+  if (rx < 0 || ry < 0) return *this;
+  return AddRRect(RRect::MakeRectXY(rect, rx, ry), dir);
+}
+
 // ---- Shader (src/effect/shader.cc) --------------------------------------------
 namespace {
 std::shared_ptr<Shader> MakeGradient(int type, const Color4f colors[], const float pos[],
@@ -485,6 +714,37 @@ void HIPCanvas::DrawOval(const Rect& oval, const Paint& paint) {
 void HIPCanvas::DrawCircle(float cx, float cy, float radius, const Paint& paint) {
   if (radius < 0) radius = 0;
   DrawOval(Rect::MakeLTRB(cx - radius, cy - radius, cx + radius, cy + radius), paint);
+}
+
+void HIPCanvas::DrawRRect(const RRect& rrect, const Paint& paint) {
+  Path p;
+  p.AddRRect(rrect);
+  DrawPath(p, paint);
+}
+
+void HIPCanvas::DrawRoundRect(const Rect& rect, float rx, float ry, const Paint& paint) {
+  if (rx > 0 && ry > 0) {
+    DrawRRect(RRect::MakeRectXY(rect, rx, ry), paint);
+  } else {
+    DrawRect(rect, paint);
+  }
+}
+
+void HIPCanvas::DrawDRRect(const RRect& outer, const RRect& inner, const Paint& paint) {
+  const Rect& o = outer.GetBounds();
+  const Rect& i = inner.GetBounds();
+  if (!(o.left < o.right && o.top < o.bottom)) return;
+  if (!(i.left < i.right && i.top < i.bottom)) {
+    DrawRRect(outer, paint);
+    return;
+  }
+  // Rect::Contains: the inner bounds inside the outer ones
+  if (!(o.left <= i.left && o.top <= i.top && o.right >= i.right && o.bottom >= i.bottom)) return;
+  Path p;
+  p.AddRRect(outer);
+  p.AddRRect(inner);
+  p.SetFillType(Path::PathFillType::kEvenOdd);
+  DrawPath(p, paint);
 }
 
 void HIPCanvas::DrawLine(float x0, float y0, float x1, float y1, const Paint& paint) {

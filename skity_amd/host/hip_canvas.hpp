@@ -63,6 +63,44 @@ struct Rect {
   float CenterY() const { return 0.5f * (top + bottom); }
 };
 
+// RRect (include/skity/geometry/rrect.hpp, src/geometry/rrect.cc): a sorted
+// rectangle and four corner radii {upper-left, upper-right, lower-right,
+// lower-left}, normalised as the reference normalises them.
+class RRect {
+ public:
+  enum Type { kEmpty, kRect, kOval, kSimple, kNinePatch, kComplex };
+  enum Corner { kUpperLeft, kUpperRight, kLowerRight, kLowerLeft };
+  void SetRect(const Rect& rect);
+  void SetOval(const Rect& oval);
+  void SetRectXY(const Rect& rect, float x_rad, float y_rad);
+  void SetRectRadii(const Rect& rect, const Point radii[4]);
+  static RRect MakeRectXY(const Rect& rect, float x_rad, float y_rad) {
+    RRect r;
+    r.SetRectXY(rect, x_rad, y_rad);
+    return r;
+  }
+  static RRect MakeOval(const Rect& oval) {
+    RRect r;
+    r.SetOval(oval);
+    return r;
+  }
+  Type GetType() const { return type_; }
+  bool IsEmpty() const { return type_ == kEmpty; }
+  bool IsRect() const { return type_ == kRect; }
+  bool IsOval() const { return type_ == kOval; }
+  const Rect& GetBounds() const { return rect_; }
+  Point Radii(Corner c) const { return radii_[c]; }
+
+ private:
+  bool InitializeRect(const Rect& rect);
+  bool ScaleRadii();
+  void ComputeType();
+  bool RadiiValid() const;
+  Rect rect_{0, 0, 0, 0};
+  Point radii_[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  Type type_ = kEmpty;
+};
+
 // Path storage and builder rules of src/graphic/path.cc.
 class Path {
  public:
@@ -79,6 +117,11 @@ class Path {
   Path& AddRect(const Rect& rect, Direction dir = Direction::kCW, uint32_t start = 0);
   Path& AddOval(const Rect& oval, Direction dir = Direction::kCW, uint32_t start = 1);
   Path& AddCircle(float x, float y, float radius, Direction dir = Direction::kCW);
+  // Path::AddRRect (path.cc:881-934): a rect or an oval when the radii
+  // collapse, otherwise lines and conics around the eight radii points
+  Path& AddRRect(const RRect& rrect, Direction dir = Direction::kCW);
+  Path& AddRRect(const RRect& rrect, Direction dir, uint32_t start_index);
+  Path& AddRoundRect(const Rect& rect, float rx, float ry, Direction dir = Direction::kCW);
   void SetFillType(PathFillType t) { fill_type_ = t; }
   PathFillType GetFillType() const { return fill_type_; }
   bool IsEmpty() const { return verbs_.empty(); }
@@ -323,6 +366,11 @@ class HIPCanvas {
   void DrawRect(const Rect& rect, const Paint& paint);
   void DrawOval(const Rect& oval, const Paint& paint);
   void DrawCircle(float cx, float cy, float radius, const Paint& paint);
+  // Canvas::DrawRRect / DrawRoundRect / DrawDRRect (canvas.cc:210-235,
+  // 424-457): the paths AddRRect builds (the pair even-odd for DRRect)
+  void DrawRRect(const RRect& rrect, const Paint& paint);
+  void DrawRoundRect(const Rect& rect, float rx, float ry, const Paint& paint);
+  void DrawDRRect(const RRect& outer, const RRect& inner, const Paint& paint);
   void DrawLine(float x0, float y0, float x1, float y1, const Paint& paint);
   // Canvas::DrawGlyphs as the software canvas draws a glyph run
   // (SWCanvas::DrawGlyphsInternal / FillGlyphs / StrokeGlyphs,

@@ -207,6 +207,16 @@ bool AffineFrom4x4(const float c[16], Matrix* out) {
   return true;
 }
 
+// ReadFromMemory<RRect> (module/io/src/io/flat/rrect_flat.cc:23-39): a
+// rectangle and four radii, normalised by SetRectRadii
+bool ReadRRect(Cursor& c, RRect* out) {
+  float v[12];
+  if (!c.ReadPad32(v, sizeof(v))) return false;
+  const Point radii[4] = {{v[4], v[5]}, {v[6], v[7]}, {v[8], v[9]}, {v[10], v[11]}};
+  out->SetRectRadii(Rect::MakeLTRB(v[0], v[1], v[2], v[3]), radii);
+  return true;
+}
+
 }  // namespace
 
 struct PictureParser {
@@ -295,12 +305,20 @@ struct PictureParser {
     Picture::PathEntry e;
     if (type == 1) {
       // a rounded-rect path: direction, 12 floats, start index
+      // ReadRRectPath (path_flat.cc:104-145)
       uint32_t dir = (packed >> 26) & 0x3;
       if (!c.Validate(dir <= 1)) return false;
-      if (!c.Validate(c.Skip(12 * sizeof(float)) != nullptr)) return false;
+      RRect rr;
+      if (!ReadRRect(c, &rr)) return false;
       int32_t start = c.I32();
       if (!c.Validate(start >= 0 && start <= 7)) return false;
-      e.supported = false;  // Path::AddRRect is not part of this front end
+      e.path.AddRRect(rr, dir == 0 ? Path::Direction::kCW : Path::Direction::kCCW,
+                      (uint32_t)start);
+      if (fill == 1) {
+        e.path.SetFillType(Path::PathFillType::kEvenOdd);
+      } else if (fill >= 2) {
+        e.supported = false;  // inverse fill types
+      }
       pic->paths_.push_back(std::move(e));
       return true;
     }
@@ -593,11 +611,36 @@ void Picture::PlayBack(HIPCanvas* canvas) const {
                                              : HIPCanvas::ClipOp::kIntersect);
         break;
       }
+      case DRAW_RRECT: {
+        const PaintEntry* pe = paint_at(c.I32());
+        RRect rr;
+        if (!pe || !ReadRRect(c, &rr)) break;
+        if (pe->supported) {
+          canvas->DrawRRect(rr, pe->paint);
+        } else {
+          canvas->MarkUnsupported();
+        }
+        break;
+      }
+      case DRAW_DRRECT: {
+        // the inner rrect first, then the outer, even-odd (picture.cc:565-581)
+        const PaintEntry* pe = paint_at(c.I32());
+        RRect outer, inner;
+        if (!pe || !ReadRRect(c, &outer) || !ReadRRect(c, &inner)) break;
+        if (!pe->supported) {
+          canvas->MarkUnsupported();
+          break;
+        }
+        Path path;
+        path.AddRRect(inner);
+        path.AddRRect(outer);
+        path.SetFillType(Path::PathFillType::kEvenOdd);
+        canvas->DrawPath(path, pe->paint);
+        break;
+      }
       // canvas calls this backend does not draw
       case CLIP_PATH:
       case CLIP_RRECT:
-      case DRAW_RRECT:
-      case DRAW_DRRECT:
       case DRAW_ARC:
       case DRAW_TEXT_BLOB:
       case DRAW_IMAGE:

@@ -6,7 +6,7 @@
 // chunks, the paint and path tables of the flattened buffer, the op stream)
 // is read here, and playback issues the same canvas calls in the same order
 // as the reference's HandleOp.  Calls this backend cannot draw exactly
-// (clips, save layers, text, images, shadows, arcs, rounded rects, paints
+// (span-list clips, save layers, text, images, shadows, arcs, paints
 // with effects other than a blend mode, inverse fill types) mark the canvas
 // unsupported, so that its next Flush returns HIPR_ERROR_NOT_SUPPORTED; ops
 // on which the reference ends playback end it here too.

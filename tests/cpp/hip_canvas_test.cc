@@ -1,6 +1,7 @@
 // Known-answer tests of the reference's SWCanvas / Path suites, written
 // against the C++ front end (skity_amd/host/hip_canvas.hpp) and drawn by the
 // HIP library.  Exit status 0 when every check passes.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 
@@ -362,6 +363,75 @@ static void ClipRectLimitsDraws(hipr_context ctx) {
   EXPECT_EQ(s->Flush(), HIPR_ERROR_NOT_SUPPORTED);
 }
 
+// RRect normalisation, the reference's own known answers
+// (test/ut/geometry/rrect_test.cc:101-241,450-471), and the outline
+// Path::AddRRect builds for a simple rrect (path.cc:881-934: from the legacy
+// start index 6, line / conic pairs around the eight radii points).
+static void RRectKnownAnswers(hipr_context) {
+  const Rect rect = Rect::MakeWH(100.f, 100.f);
+  RRect r1;
+  r1.SetRectXY(rect, 0.f, 0.f);
+  EXPECT_EQ(r1.GetType(), RRect::kRect);
+  const Point zero[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  RRect r2;
+  r2.SetRectRadii(rect, zero);
+  EXPECT_EQ(r2.GetType(), RRect::kRect);
+  const Point mixed[4] = {{0, 0}, {20, 20}, {50, 50}, {20, 50}};
+  RRect r3;
+  r3.SetRectRadii(rect, mixed);
+  EXPECT_EQ(r3.GetType(), RRect::kComplex);
+  RRect oval;
+  oval.SetRectXY(rect, 50.f, 50.f);
+  EXPECT_EQ(oval.GetType(), RRect::kOval);
+  RRect simple;
+  simple.SetRectXY(rect, 20.f, 20.f);
+  EXPECT_EQ(simple.GetType(), RRect::kSimple);
+  RRect inf;
+  inf.SetRectXY(rect, INFINITY, INFINITY);
+  EXPECT_EQ(inf.GetType(), RRect::kRect);
+  RRect big;
+  big.SetRectXY(rect, 100.f, 100.f);
+  EXPECT_EQ(big.GetType(), RRect::kOval);
+  const Point wide[4] = {{50, 100}, {100, 50}, {50, 100}, {100, 50}};
+  RRect scaled;
+  scaled.SetRectRadii(rect, wide);
+  EXPECT_EQ(scaled.GetType(), RRect::kComplex);
+  Point ul = scaled.Radii(RRect::kUpperLeft);
+  EXPECT_TRUE(std::fabs(ul.x - 33.333333f) <= 1.f / 4096 && std::fabs(ul.y - 66.666666f) <= 1.f / 4096);
+  RRect neg;
+  neg.SetRectXY(rect, -10.f, -20.f);
+  EXPECT_EQ(neg.GetType(), RRect::kRect);
+  const Point one[4] = {{0, 0}, {48.8800011f, 48.8800011f}, {0, 0}, {0, 0}};
+  RRect sr;
+  sr.SetRectRadii(Rect::MakeXYWH(1.f, 1.12f, 47.7599983f, 47.7599983f), one);
+  EXPECT_EQ(sr.GetType(), RRect::kComplex);
+  // EXPECT_FLOAT_EQ: within 4 ULPs
+  auto float_eq = [](float a, float b) {
+    int32_t ia, ib;
+    std::memcpy(&ia, &a, 4);
+    std::memcpy(&ib, &b, 4);
+    return std::abs(ia - ib) <= 4;
+  };
+  EXPECT_TRUE(float_eq(sr.Radii(RRect::kUpperRight).x, 47.7599983f));
+  EXPECT_TRUE(float_eq(sr.Radii(RRect::kUpperRight).y, 47.7599983f));
+  EXPECT_TRUE(sr.Radii(RRect::kUpperLeft).x == 0.f && sr.Radii(RRect::kLowerLeft).y == 0.f);
+  // AddRoundRect(10, 20, 100, 200, 10, 10): M L C L C L C L C Z
+  Path p;
+  p.AddRoundRect(Rect::MakeLTRB(10, 20, 100, 200), 10, 10);
+  const uint8_t want_v[10] = {0, 1, 3, 1, 3, 1, 3, 1, 3, 5};
+  EXPECT_EQ(p.verbs().size(), 10u);
+  bool verbs_ok = p.verbs().size() == 10;
+  for (size_t i = 0; verbs_ok && i < 10; i++) verbs_ok = p.verbs()[i] == want_v[i];
+  EXPECT_TRUE(verbs_ok);
+  const float want_p[17][2] = {{10, 190}, {10, 30},  {10, 20},  {20, 20},  {90, 20},  {100, 20},
+                               {100, 30}, {100, 190}, {100, 200}, {90, 200}, {20, 200}, {10, 200},
+                               {10, 190}};
+  bool pts_ok = p.points().size() == 13;
+  for (size_t i = 0; pts_ok && i < 13; i++)
+    pts_ok = p.points()[i].x == want_p[i][0] && p.points()[i].y == want_p[i][1];
+  EXPECT_TRUE(pts_ok);
+}
+
 int main() {
   hipr_context ctx = nullptr;
   if (hipr_context_create(0, &ctx) != HIPR_SUCCESS) {
@@ -383,7 +453,8 @@ int main() {
                {"Shader.PixmapShaderDrawsTheImage", PixmapShaderDrawsTheImage},
                {"DisplayList.ReplayEqualsDirectDraws", DisplayListReplayEqualsDirectDraws},
                {"Canvas.GlyphRunEqualsTranslatedPaths", GlyphRunEqualsTranslatedPaths},
-               {"Canvas.ClipRectLimitsDraws", ClipRectLimitsDraws}};
+               {"Canvas.ClipRectLimitsDraws", ClipRectLimitsDraws},
+               {"RRect.KnownAnswers", RRectKnownAnswers}};
   for (auto& t : tests) {
     int before = g_failed;
     t.fn(ctx);

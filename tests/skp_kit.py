@@ -15,10 +15,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden", "skp")
 
 # DrawType numbers (module/io/src/record/draw_type.hpp:24-121)
-OPS = dict(CLIP_PATH=1, CLIP_RECT=3, CONCAT=5, DRAW_CLEAR=10, DRAW_OVAL=12, DRAW_PAINT=13,
+OPS = dict(CLIP_PATH=1, CLIP_RECT=3, CLIP_RRECT=4, CONCAT=5, DRAW_CLEAR=10, DRAW_OVAL=12, DRAW_PAINT=13,
            DRAW_PATH=14, DRAW_PICTURE=15, DRAW_POINTS=16, DRAW_RECT=21, DRAW_RRECT=22,
            RESTORE=28, ROTATE=29, SAVE=30, SCALE=32, SET_MATRIX=33, SKEW=34, TRANSLATE=35,
-           NOOP=36, DRAW_TEXT_BLOB=45, SAVE_LAYER_SAVELAYERREC=52, CONCAT44=68, SET_M44=71)
+           NOOP=36, DRAW_DRRECT=40, DRAW_TEXT_BLOB=45, SAVE_LAYER_SAVELAYERREC=52, CONCAT44=68,
+           SET_M44=71)
 
 
 def _tag(s):

@@ -60,3 +60,32 @@ def test_unsupported_recording_draws_nothing(ctx):
     assert not surf.read().any()
     surf.close()
     rec.close()
+
+
+def test_rrect_ops_match_oracle(ctx):
+    """DRAW_RRECT / DRAW_DRRECT / rounded-rect path records (conic outlines
+    under a scale + translate and a rotation) drawn by the device equal the
+    oracle."""
+    import struct
+    paints = [K.paint_bytes((0.3, 0.4, 0.8, 0.9)),
+              K.paint_bytes((0.8, 0.2, 0.1, 1.0), style=abi.STYLE_STROKE, width=3.0)]
+    rr = struct.pack("<12f", 10, 20, 100, 200, 10, 30, 25, 5, 40, 40, 0, 0)
+    inner = struct.pack("<12f", 30, 40, 80, 150, 5, 5, 5, 5, 5, 5, 5, 5)
+    rpath = struct.pack("<I", (1 << 28) | (1 << 26) | 5) + rr + struct.pack("<i", 5)
+    ops = [K.op_bytes("DRAW_RRECT", ("i", 1), rr),
+           K.op_bytes("TRANSLATE", ("f", 120.0), ("f", 10.0)),
+           K.op_bytes("DRAW_DRRECT", ("i", 2), rr, inner),
+           K.op_bytes("ROTATE", ("f", 12.0)),
+           K.op_bytes("DRAW_PATH", ("i", 1), ("i", 1))]
+    rec = K.Recording(K.skp_bytes(ops, paints, [rpath]), 256, 256, (1.2, 0, 4, 0, 1.1, 3))
+    assert rec.rc == 0, rec.error
+    surf = device.Surface(ctx, 256, 256)
+    surf.clear(0)
+    assert rec.draw(surf, abi.MODE_EXACT) == 0
+    got = surf.read()
+    surf.close()
+    want = O.render_ex(rec.scene, 256, 256, abi.ALPHA_PREMUL, abi.COLOR_RGBA)
+    rec.close()
+    mx, cnt = diff_report(got, want)
+    print(f"rrect ops: max diff {mx}, {cnt} pixels differ", flush=True)
+    assert mx == 0 and want[..., 3].any()

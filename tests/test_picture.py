@@ -206,7 +206,8 @@ def test_calls_not_drawn_make_the_recording_unsupported():
                                            ("I", 0), ("i", 0)),
         "clip path": K.op_bytes("CLIP_PATH", ("i", 1), ("I", 1), ("i", 0)),
         "save layer": K.op_bytes("SAVE_LAYER_SAVELAYERREC", ("I", 0)),
-        "rrect": K.op_bytes("DRAW_RRECT", ("i", 1), struct.pack("<12f", *range(12))),
+        "clip rrect": K.op_bytes("CLIP_RRECT", struct.pack("<12f", *range(12)), ("I", 1),
+                                 ("i", 0)),
         "text blob": K.op_bytes("DRAW_TEXT_BLOB", ("i", 1), ("i", 1), ("f", 0), ("f", 0)),
         "perspective": K.op_bytes("CONCAT", struct.pack("<9f", 1, 0, 0, 0, 1, 0, 0.001, 0, 1)),
     }
@@ -313,3 +314,44 @@ def test_truncated_and_corrupted_files_do_not_crash():
         if rec.rc == 0:
             assert 0 <= rec.scene.contents.n_paths <= 305
         rec.close()
+
+
+def test_rrect_ops_build_rrect_paths():
+    """DRAW_RRECT (SetRectRadii then Path::AddRRect from the legacy start
+    index 6: line / conic pairs around the radii points), DRAW_DRRECT (the
+    inner then the outer rrect, even-odd) and a rounded-rect path record
+    (ReadRRectPath: direction and start index) become the paths the
+    reference builds; the oracle draws them inside their bounds."""
+    paints = [K.paint_bytes((0.3, 0.4, 0.8, 1.0))]
+    rr = struct.pack("<12f", 10, 20, 100, 200, 10, 10, 10, 10, 10, 10, 10, 10)
+    inner = struct.pack("<12f", 30, 40, 80, 150, 5, 5, 5, 5, 5, 5, 5, 5)
+    # a serialized rrect path: type 1, CCW, start 2, even-odd
+    rpath = struct.pack("<I", (1 << 28) | (1 << 26) | (1 << 8) | 5) + rr + struct.pack("<i", 2)
+    paths = [rpath]
+    ops = [K.op_bytes("DRAW_RRECT", ("i", 1), rr),
+           K.op_bytes("DRAW_DRRECT", ("i", 1), rr, inner),
+           K.op_bytes("DRAW_PATH", ("i", 1), ("i", 1))]
+    rec = K.Recording(K.skp_bytes(ops, paints, paths), 128, 256)
+    assert rec.rc == 0, rec.error
+    s = rec.scene.contents
+    assert s.n_paths == 3
+    verbs = [s.verbs[i] for i in range(s.verb_offsets[0], s.verb_offsets[1])]
+    pts = [(s.points[2 * i], s.points[2 * i + 1])
+           for i in range(s.point_offsets[0], s.point_offsets[1])]
+    L, C, M, Z = abi.VERB_LINE, abi.VERB_CONIC, abi.VERB_MOVE, abi.VERB_CLOSE
+    assert verbs == [M, L, C, L, C, L, C, L, C, Z]
+    assert pts == [(10, 190), (10, 30), (10, 20), (20, 20), (90, 20), (100, 20), (100, 30),
+                   (100, 190), (100, 200), (90, 200), (20, 200), (10, 200), (10, 190)]
+    # DRRECT: two closed contours, inner first, even-odd
+    v1 = [s.verbs[i] for i in range(s.verb_offsets[1], s.verb_offsets[2])]
+    assert v1.count(M) == 2 and s.fill_types[1] == abi.FILL_EVEN_ODD
+    first = (s.points[2 * s.point_offsets[1]], s.points[2 * s.point_offsets[1] + 1])
+    assert first == (30, 145)  # the inner rrect's start point
+    # the path record: counter-clockwise from start index 2, even-odd
+    v2 = [s.verbs[i] for i in range(s.verb_offsets[2], s.verb_offsets[3])]
+    assert v2[0] == M and v2[-1] == Z and s.fill_types[2] == abi.FILL_EVEN_ODD
+    assert (s.points[2 * s.point_offsets[2]], s.points[2 * s.point_offsets[2] + 1]) == (100, 30)
+    img = _oracle(rec)
+    a = img[..., 3]
+    assert a[100, 50] > 0 and a[100, 20] > 0 and not a[:19].any() and not a[:, 101:].any()
+    rec.close()
