@@ -400,9 +400,39 @@ struct CompositeParams {
   const uint8_t* srgb;  // the sRGB filter tables (512 entries)
   const ImageDev* images;       // pixmap shader images
   const uint8_t* image_pixels;
+  // per job, the canvas tiles it touches: {tc0, tr0, tc1, tr1} (half-open;
+  // empty when tc0 >= tc1), written by JobTileRangeKernel
+  const int4* job_tiles;
   // optional: [0] += pixel blends performed, [1] += target pixels touched
   unsigned long long* counters;
 };
+
+// The tile range of every job on its canvas, once per frame: the composite
+// scans these 16 bytes per job for each tile instead of the job and coverage
+// records.  A rectangle job covers the tiles its rectangle meets; a walked
+// job the tiles of its coverage bounds (as the walk wrote its maps).
+__global__ void __launch_bounds__(256)
+    JobTileRangeKernel(const Job* __restrict__ jobs, const JobCov* __restrict__ cov,
+                       const PaintDev* __restrict__ paints, const Canvas* __restrict__ canvases,
+                       int n_jobs, int4* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  const Job job = jobs[j];
+  const Canvas& cv = canvases[job.canvas];
+  int4 r = make_int4(0, 0, 0, 0);
+  if (job.kind == kJobRect) {
+    const PaintDev& pd = paints[job.paint];
+    const int x0 = pd.L > 0 ? pd.L : 0, x1 = pd.R < cv.width ? pd.R : cv.width;
+    const int y0 = pd.T > 0 ? pd.T : 0, y1 = pd.B < cv.height ? pd.B : cv.height;
+    if (x0 < x1 && y0 < y1) r = make_int4(x0 >> 4, y0 >> 4, ((x1 - 1) >> 4) + 1, ((y1 - 1) >> 4) + 1);
+  } else {
+    const JobCov c = cov[j];
+    PixBounds pb{c.left, c.top, c.right, c.bottom, !(c.left < c.right && c.top < c.bottom)};
+    TileRange tr = JobTiles(pb, cv.width, cv.height);
+    if (tr.ntc > 0) r = make_int4(tr.tc0, tr.tr0, tr.tc0 + tr.ntc, tr.tr0 + tr.ntr);
+  }
+  out[j] = r;
+}
 
 // SWSpanBrush::BrushH for one pixel and cover (sw_span_brush.cc:108-138):
 // AlphaMulQ by the cover unless it is 255, then the target blend.
@@ -533,18 +563,12 @@ __global__ void __launch_bounds__(64) CompositeKernel(CompositeParams P) {
     int s0 = -1, s1 = -1;
     bool present = false;
     if (j < cv.job_end) {
-      const Job job = P.jobs[j];
-      if (job.kind == kJobRect) {
-        const PaintDev pd = P.paints[job.paint];
-        int x0 = tx * kTile, y0 = ty * kTile;
-        present = pd.L < x0 + kTile && pd.R > x0 && pd.T < y0 + kTile && pd.B > y0;
-      } else {
-        const JobCov c = P.cov[j];
-        PixBounds pb{c.left, c.top, c.right, c.bottom, !(c.left < c.right && c.top < c.bottom)};
-        TileRange trg = JobTiles(pb, cv.width, cv.height);
-        if (trg.ntc > 0 && tx >= trg.tc0 && tx < trg.tc0 + trg.ntc && ty >= trg.tr0 &&
-            ty < trg.tr0 + trg.ntr) {
-          int64_t mi = P.map_base[j] + (int64_t)(ty - trg.tr0) * trg.ntc + (tx - trg.tc0);
+      const int4 tb = P.job_tiles[j];
+      if (tx >= tb.x && tx < tb.z && ty >= tb.y && ty < tb.w) {
+        if (P.jobs[j].kind == kJobRect) {
+          present = true;
+        } else {
+          int64_t mi = P.map_base[j] + (int64_t)(ty - tb.y) * (tb.z - tb.x) + (tx - tb.x);
           s0 = P.map0[mi];
           if (s0 >= 0) {
             s1 = P.map1[mi];

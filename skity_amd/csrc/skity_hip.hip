@@ -315,6 +315,7 @@ struct hipr_context_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // per-draw device buffers (grow-only)
   DevBuf canvases, layer_canvases, geo, cov, edge_base, edge_cap, edges, wedges, gkeys;
+  DevBuf job_tiles;  // per-job tile ranges for the composite
   DevBuf plane_base, plane_cap, planes, map_base, map0, map1, sub_list, big_list, bound,
       gather_list;
   DevBuf wcov, qstate, q_base, snaps, snap_base, band_count;  // write-pass coverage, row bands
@@ -2122,6 +2123,15 @@ hipr_result RunPipeline(const std::vector<hipr_surface>& targets, hipr_surface s
   cp.srgb = ctx->srgb.as<uint8_t>();
   cp.images = pl->d_images.as<ImageDev>();
   cp.image_pixels = pl->image_pixels.as<uint8_t>();
+  // per-job tile ranges for the composite's scan
+  if (ctx->job_tiles.Ensure(sizeof(int4) * std::max(nj, 1)) != hipSuccess)
+    return HIPR_ERROR_OUT_OF_DEVICE_MEMORY;
+  // (ctx->canvases holds every canvas of the frame, as the walk read them)
+  hipLaunchKernelGGL(JobTileRangeKernel, dim3((nj + 255) / 256), dim3(256), 0, st, f.d_jobs,
+                     ctx->wcov.as<JobCov>(), pl->d_paints.as<PaintDev>(),
+                     ctx->canvases.as<Canvas>(), nj, ctx->job_tiles.as<int4>());
+  HIP_TRY(hipGetLastError());
+  cp.job_tiles = ctx->job_tiles.as<int4>();
   // the composite variant with colour filters and pixmap sampling only when
   // a paint of the frame needs them
   bool rich = false;
@@ -2750,7 +2760,7 @@ HIPR_API void hipr_context_destroy(hipr_context ctx) { ReleaseContext(ctx); }
 static void FreeContext(hipr_context ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  DevBuf* bufs[] = {&ctx->canvases, &ctx->layer_canvases, &ctx->geo, &ctx->cov, &ctx->edge_base,
+  DevBuf* bufs[] = {&ctx->job_tiles, &ctx->canvases, &ctx->layer_canvases, &ctx->geo, &ctx->cov, &ctx->edge_base,
                     &ctx->edge_cap, &ctx->edges, &ctx->wedges, &ctx->gkeys, &ctx->plane_base,
                     &ctx->plane_cap, &ctx->planes, &ctx->map_base, &ctx->map0, &ctx->map1,
                     &ctx->sub_list, &ctx->big_list, &ctx->bound, &ctx->gather_list, &ctx->stroke_out, &ctx->o_verbs, &ctx->o_pts, &ctx->o_w,
