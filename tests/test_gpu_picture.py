@@ -2,8 +2,9 @@
 reference's own SKP) recorded under Translate(-130, 20) and drawn on the
 device through skity_hip_display_list_draw equals the oracle's drawing of
 the same recording bit for bit, in the exact and learned forms, at the
-golden's 1000 x 1000 and scaled to 4096 x 4096 (the oracle drawing is pinned
-against the reference's golden image in tests/test_picture.py)."""
+golden's 1000 x 1000 and scaled to 4096 x 4096 and 8192 x 8192 (the oracle
+drawing is pinned against the reference's golden image in
+tests/test_picture.py)."""
 import numpy as np
 import pytest
 import torch  # noqa: F401  (the HIP runtime is initialised by torch first)
@@ -26,7 +27,8 @@ def ctx():
 
 
 @pytest.mark.parametrize("size,matrix", [(1000, (1, 0, -130, 0, 1, 20)),
-                                         (4096, (4.1, 0, -560, 0, 4.1, 60))])
+                                         (4096, (4.1, 0, -560, 0, 4.1, 60)),
+                                         (8192, (8.2, 0, -1120, 0, 8.2, 120))])
 @pytest.mark.parametrize("mode", [abi.MODE_EXACT, abi.MODE_LEARNED])
 def test_tiger_display_list_matches_oracle(ctx, size, matrix, mode):
     rec = K.Recording(K.tiger_bytes(), size, size, matrix)
