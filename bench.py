@@ -92,18 +92,41 @@ def measured_traffic(stage, form):
     return int(sum(d["fetch_mib"] + d["write_mib"] for d in hits) * 1024 * 1024)
 
 
+def _probe_scene(workload):
+    """(scene, width, height, owner) of a --pmc-probe workload: a generator
+    workload number, "tiger" (tiger.skp under Scale(4.1) on 4096^2, as the
+    tiger leg draws it) or "text" (the text leg's 2048^2 page)."""
+    if workload == "tiger":
+        from skity_amd import picture
+        rec = picture.Recording(open(TIGER_SKP, "rb").read(), 4096, 4096,
+                                (4.1, 0, -560, 0, 4.1, 60))
+        if rec.scene is None:
+            raise RuntimeError(f"tiger.skp not drawable: {rec.rc} {rec.error}")
+        return rec.scene, 4096, 4096, rec
+    if workload == "text":
+        sys.path.insert(0, os.path.join(HERE, "tests"))
+        import text_kit
+        made = text_kit.text_page(3)
+        if made is None:
+            raise RuntimeError("text probe: no TrueType font installed")
+        return made[0].scene, 2048, 2048, made[0]
+    from skity_amd import scenes
+    sc = scenes.generate(int(workload))
+    return sc.scene, sc.width, sc.height, sc
+
+
 def pmc_probe(workload=WORKLOAD):
     """The program the committed counter summaries profile: one exact frame
     and one learned frame of the workload (curves + strokes by default; 3 =
-    gradients + blur) with the walk passes launched directly, so every
-    kernel is its own dispatch, then a 256 MiB device-to-device copy whose
-    counters calibrate FETCH_SIZE / WRITE_SIZE."""
+    gradients + blur; "tiger"; "text") with the walk passes launched
+    directly, so every kernel is its own dispatch, then a 256 MiB
+    device-to-device copy whose counters calibrate FETCH_SIZE / WRITE_SIZE."""
     import torch
-    from skity_amd import abi, device, presizer, scenes
-    sc = scenes.generate(workload)
+    from skity_amd import abi, device, presizer
+    scene, width, height, owner = _probe_scene(workload)
     ctx = device.Context(0)
-    prep = device.Prepared(ctx, sc.scene)
-    surf = device.Surface(ctx, sc.width, sc.height)
+    prep = device.Prepared(ctx, scene)
+    surf = device.Surface(ctx, width, height)
     prm, margin = presizer.load()
     presizer.install(ctx, prm, margin)
     surf.clear(0)
@@ -115,6 +138,7 @@ def pmc_probe(workload=WORKLOAD):
     b = torch.empty_like(a)
     b.copy_(a)
     torch.cuda.synchronize()
+    del owner
     print("pmc probe done", flush=True)
 
 
@@ -396,8 +420,10 @@ def main():
                          "text")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short program the rocprofv3 counter summaries profile")
-    ap.add_argument("--probe-workload", type=int, default=WORKLOAD,
-                    help="workload of --pmc-probe: 2 curves + strokes, 3 gradients + blur")
+    ap.add_argument("--probe-workload", default=str(WORKLOAD),
+                    choices=["2", "3", "tiger", "text"],
+                    help="workload of --pmc-probe: 2 curves + strokes, 3 gradients + blur, "
+                         "tiger (tiger.skp at 4096^2), text (the 2048^2 text page)")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe(args.probe_workload)
